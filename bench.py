@@ -1,0 +1,217 @@
+"""TOED detection throughput on one MI355X.
+
+One JSON line on stdout.  Headline: plain single-image TOED on 2160x3840 u8
+frames (sigma = 2, threshold 2), Mpixel/s with the frame resident in HBM.
+Also reported ("forms"): the synchronous per-call form, the batched form (200
+seeded scenes of mixed size and sigma in one plan, replayed as a hipGraph)
+against a loop of plain launches over the same images, kernel durations from
+HIP events on the library's stream, the VALU/HBM roofline of the dominant
+kernel and the reference CPU path timed on this host.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload all|plain|batch]
+"""
+import argparse
+import importlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG_NAME = "third-order-edge-detector_amd"
+
+METRIC = "TOED Mpixel/s at 3840x2160: batched & fwd+bwd vs plain, same GPU; % HBM peak"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+BATCH_SIZES = [(480, 640), (480, 752), (720, 1280), (1080, 1920), (2160, 3840)]
+BATCH_SIGMAS = [1.0, 1.5, 2.0]
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def dense_flops_per_px(sigma, pkg):
+    """Minimal useful dense work of the tile kernel per input pixel: Ix and Iy
+    at the four interp phases, separable: 10*W1 + 2*W0 FMAs (2 flops each)."""
+    r0, r1 = pkg.sigma_radius(sigma)
+    w0, w1 = 2 * r0 + 1, 2 * r1 + 1
+    return 2.0 * (10 * w1 + 2 * w0)
+
+
+def time_loop(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_plain(torch, pkg, scene, steps, warmup):
+    H, W = 2160, 3840
+    img_np = scene.shapes_scene(2160 * 3840, H, W)
+    img = torch.from_numpy(img_np).cuda()
+    det = pkg.Detector(0)
+    cap = 4 * H * W
+    edges = torch.empty((cap, 4), dtype=torch.float32, device="cuda")
+    counts = torch.zeros(2, dtype=torch.int32, device="cuda")
+    sec = time_loop(torch, lambda: det.detect_into(img, edges, counts), steps, warmup)
+    n, flags = (int(v) for v in counts.cpu())
+    assert flags == 0, flags
+    # per-kernel durations (HIP events on the library stream), synchronous form
+    tiles, fin = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        det.detect_sync_into(img, edges)
+        a, b = det.last_times()
+        tiles.append(a)
+        fin.append(b)
+    sync_sec = (time.perf_counter() - t0) / steps
+    det.close()
+    return dict(H=H, W=W, sec=sec, n=n, ms_tiles=float(np.mean(tiles)), ms_final=float(np.mean(fin)),
+                sync_sec=sync_sec, img_np=img_np)
+
+
+def batch_images(scene, n=200, seed=2024):
+    rng = np.random.default_rng(seed)
+    shapes = [BATCH_SIZES[k] for k in rng.integers(0, len(BATCH_SIZES), n)]
+    sigmas = [BATCH_SIGMAS[k] for k in rng.integers(0, len(BATCH_SIGMAS), n)]
+    imgs = [scene.shapes_scene(seed * 1000 + m, H, W) for m, (H, W) in enumerate(shapes)]
+    return imgs, sigmas
+
+
+def bench_batch(torch, pkg, scene, steps, warmup, n_images):
+    imgs, sigmas = batch_images(scene, n_images)
+    dev = [torch.from_numpy(im).cuda() for im in imgs]
+    px = float(sum(im.size for im in imgs))
+    det = pkg.Detector(0, staging_fraction=0.25)
+    caps = [max(1024, (4 * im.size) // 8) for im in imgs]
+    b = pkg.BatchDetector(det, dev, sigmas, capacities=caps, persistent=True)
+    sec_graph = time_loop(torch, lambda: b.run(graph=True), steps, warmup)
+    counts = b.counts.cpu().numpy()
+    assert not b.flags.cpu().numpy().any()
+    b_plain = pkg.BatchDetector(det, dev, sigmas, capacities=caps, persistent=False)
+    sec_batch_plain = time_loop(torch, lambda: b_plain.run(graph=False), steps, warmup)
+    # loop of plain single-image launches over the same images and outputs
+    cnt = torch.zeros((len(dev), 2), dtype=torch.int32, device="cuda")
+    views = [b.edges[int(b.offsets[m]):int(b.offsets[m]) + caps[m]] for m in range(len(dev))]
+
+    def loop():
+        for m in range(len(dev)):
+            det.detect_into(dev[m], views[m], cnt[m], sigmas[m])
+
+    sec_loop = time_loop(torch, loop, max(2, steps // 4), 1)
+    loop_counts = cnt[:, 0].cpu().numpy()
+    assert np.array_equal(loop_counts, counts), "batched and per-image counts differ"
+    b.close()
+    b_plain.close()
+    det.close()
+    return dict(px=px, n_images=len(imgs), sec_graph=sec_graph, sec_batch_plain=sec_batch_plain, sec_loop=sec_loop,
+                edgels=int(counts.sum()), sigmas=sigmas, shapes=[im.shape for im in imgs])
+
+
+def cpu_baseline(img_np):
+    """The reference CPU path (cpu_toed.cpp, FP64, OpenMP) on one frame of the
+    same workload, on this host's cores; the oracle port when the reference
+    harness is not built."""
+    import oracle
+    cores = max(1, min(16, os.cpu_count() or 1))
+    H, W = img_np.shape
+    if oracle.ref_available():
+        t0 = time.perf_counter()
+        edges, tc, tn = oracle.run_ref(img_np, nthreads=cores)
+        wall = time.perf_counter() - t0
+        kind = "reference"
+        sec = tc + tn
+    else:
+        t0 = time.perf_counter()
+        oracle.detect(img_np.astype(np.float64), 2.0, nthreads=cores)
+        wall = sec = time.perf_counter() - t0
+        kind = "port"
+    return {"value": round(H * W / sec / 1e6, 4), "unit": "Mpixel/s", "cores": cores, "kind": kind,
+            "sample": "one %dx%d frame of the same workload, FP64, convolution + NMS timed at the reference's "
+                      "omp_get_wtime boundaries (%.2f s; %.2f s wall incl. list build and I/O)" % (H, W, sec, wall)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="all", choices=["all", "plain", "batch"])
+    ap.add_argument("--batch-images", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each image is independent; "
+              "see DESIGN.md for how the work would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    pkg = importlib.import_module(PKG_NAME)
+    scene = importlib.import_module(PKG_NAME + ".scene")
+    torch.cuda.init()
+    forms = {}
+    out = {"metric": METRIC, "unit": "Mpixel/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded shape scenes)"}
+    plain = None
+    if args.workload in ("all", "plain"):
+        log("plain 2160x3840 ...")
+        plain = bench_plain(torch, pkg, scene, args.steps, args.warmup)
+        px = plain["H"] * plain["W"]
+        mpx = px / plain["sec"] / 1e6
+        forms["plain"] = {"mpx_s": round(mpx, 1), "ms": round(plain["sec"] * 1e3, 4), "edgels": plain["n"],
+                          "edgels_s": round(plain["n"] / plain["sec"], 1)}
+        forms["plain_sync"] = {"mpx_s": round(px / plain["sync_sec"] / 1e6, 1),
+                               "ms": round(plain["sync_sec"] * 1e3, 4)}
+        forms["kernels_ms"] = {"tiles": round(plain["ms_tiles"], 4), "scan_finalize": round(plain["ms_final"], 4)}
+        flops = dense_flops_per_px(2.0, pkg) * px
+        ach = flops / (plain["ms_tiles"] * 1e-3) / 1e12
+        out["roofline"] = {"bound": "valu", "achieved": round(ach, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                           "kernel": "toed_tiles_kernel<uint8_t,16>",
+                           "algorithmic": "2*(10*W1+2*W0) flop/px = %d flop/px at sigma 2, %d px/launch"
+                                          % (dense_flops_per_px(2.0, pkg), px)}
+        hb = (px * 1 + 16 * plain["n"] + 4) / plain["sec"] / 1e9
+        out["hbm"] = {"achieved": round(hb, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(hb / PEAK_HBM_GBS, 5),
+                      "algorithmic_bytes": "1 B/px in + 16 B/edgel + 4 B count"}
+        out["value"] = round(mpx, 1)
+        out["ms_per_step"] = round(plain["sec"] * 1e3, 4)
+        out["config"] = {"workload": "plain TOED: one 2160x3840 u8 synthetic frame per step (96 shapes), "
+                                     "sigma 2 (R0 8 / R1 9), threshold 2, border 10, FP32 + FP64 per edgel",
+                         "global_batch": 1, "H": 2160, "W": 3840}
+        log("plain: %.1f Mpx/s, %.3f ms/frame, %d edgels, tiles %.3f ms" % (mpx, plain["sec"] * 1e3, plain["n"],
+                                                                           plain["ms_tiles"]))
+    if args.workload in ("all", "batch"):
+        log("batch of %d ..." % args.batch_images)
+        bt = bench_batch(torch, pkg, scene, max(3, args.steps // 4), 1, args.batch_images)
+        forms["batched_graph"] = {"mpx_s": round(bt["px"] / bt["sec_graph"] / 1e6, 1),
+                                  "ms": round(bt["sec_graph"] * 1e3, 3),
+                                  "edgels_s": round(bt["edgels"] / bt["sec_graph"], 1)}
+        forms["batched_plain_grid"] = {"mpx_s": round(bt["px"] / bt["sec_batch_plain"] / 1e6, 1)}
+        forms["batch_loop_of_plain"] = {"mpx_s": round(bt["px"] / bt["sec_loop"] / 1e6, 1),
+                                        "ms": round(bt["sec_loop"] * 1e3, 3)}
+        forms["batch_config"] = {"images": bt["n_images"], "mpx": round(bt["px"] / 1e6, 2), "edgels": bt["edgels"]}
+        if "value" not in out:
+            out["value"] = forms["batched_graph"]["mpx_s"]
+            out["ms_per_step"] = round(bt["sec_graph"] * 1e3, 3)
+            out["config"] = {"workload": "batched TOED: 200 mixed-size scenes, mixed sigma", "global_batch": 200}
+        log("batch: graph %.1f Mpx/s, loop-of-plain %.1f Mpx/s" % (forms["batched_graph"]["mpx_s"],
+                                                                    forms["batch_loop_of_plain"]["mpx_s"]))
+    out["forms"] = forms
+    if plain is not None and not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(plain["img_np"])
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+/*
+ * toed_capi.h -- C ABI of the MI355X (gfx950) TOED detection library
+ * (libtoed_hip.so).  Plain pointers and sizes only; device pointers are HIP
+ * device allocations owned by the caller unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   gpu_kernels.hpp:24-45   gpu_convolve(...) f32/f64 overloads  -> toed_gpu_convolve_f32
+ *   gpu_kernels.hpp:48-63   gpu_nms(...) f32/f64 overloads       -> toed_gpu_nms_f32
+ *   gpu_toed.hpp:79-135     ThirdOrderEdgeDetectionGPU ctor: device buffers,
+ *                           events                               -> toed_create / toed_plan
+ *   gpu_toed.hpp:340-441    convolve_img + non_maximum_suppresion + host
+ *                           build_edge_list (:313-338)           -> toed_detect (one fused
+ *                           launch sequence producing the raster-ordered N x 4 list on the GPU)
+ *   run_gpu_batch.sh:152-167 one process per image              -> toed_batch_* (one plan,
+ *                           one launch sequence, optionally one captured hipGraph)
+ * A C ABI cannot overload, so the f32 variants carry a suffix.
+ *
+ * Every function returns a toed_status (0 = success).  Launch functions are
+ * asynchronous on the context's stream unless named *_sync.
+ */
+#ifndef TOED_CAPI_H
+#define TOED_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TOED_OK = 0,
+    TOED_ERR_ARG = 1,          /* invalid argument (null pointer, bad size, ...) */
+    TOED_ERR_SIGMA = 2,        /* sigma outside the supported window radii */
+    TOED_ERR_ALLOC = 3,        /* device allocation failed */
+    TOED_ERR_HIP = 4,          /* a HIP runtime call failed */
+    TOED_ERR_CAPACITY = 5,     /* more edgels than the output capacity (count is still exact) */
+    TOED_ERR_STAGING = 6,      /* per-tile staging overflow (re-plan with a larger staging fraction) */
+    TOED_ERR_NOT_BUILT = 7     /* no device code for this GPU */
+} toed_status;
+
+/* input pixel formats */
+enum { TOED_U8 = 0, TOED_F32 = 1 };
+
+/* per-image status bits written by the kernels */
+enum { TOED_FLAG_CAPACITY = 1, TOED_FLAG_STAGING = 2 };
+
+typedef struct toed_ctx toed_ctx;
+typedef struct toed_batch toed_batch;
+
+/* One image of a detection call.  Edgels are written as rows of 4 floats
+ * [x, y, orientation, strength] (reference layout, indices.hpp:40) in the
+ * reference's raster order of the 2x interpolated grid (cpu_toed.cpp:510-538). */
+typedef struct {
+    int in_type;             /* TOED_U8 or TOED_F32 */
+    const void *d_img;       /* device pointer, H rows of `pitch` elements */
+    int H, W, pitch;         /* pitch in elements, >= W */
+    float sigma;             /* Gaussian scale; windows R0 = ceil(4 sigma), R1 = R0 + 1 */
+    float thresh;            /* gradient-magnitude threshold (reference: 2, cpu_toed.cpp:403) */
+    int border;              /* NMS border on the interp grid (reference: 10, cpu_toed.cpp:400) */
+    float *d_edges;          /* device output, capacity x 4 floats */
+    long long capacity;      /* rows available in d_edges */
+    int *d_index;            /* optional (may be NULL): interp-grid index row*(2W)+col of
+                                each edgel, capacity ints (no reference counterpart) */
+} toed_image_t;
+
+/* Library / device queries -------------------------------------------------- */
+int toed_version(void);
+/* sigma -> window radii as used by the kernels; TOED_ERR_SIGMA if unsupported */
+int toed_sigma_radius(float sigma, int *R0, int *R1);
+
+/* Context ------------------------------------------------------------------- */
+/* Creates a context on `device` (made current while the context is used) with
+ * its own HIP stream.  Replaces the device half of gpu_toed.hpp:79-135. */
+int toed_create(int device, toed_ctx **out);
+int toed_destroy(toed_ctx *ctx);
+/* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int toed_set_stream(toed_ctx *ctx, void *hip_stream);
+void *toed_get_stream(toed_ctx *ctx);
+/* Tile rows per workgroup (8 or 16); 0 = autotuned default. */
+int toed_set_tile_rows(toed_ctx *ctx, int tile_rows);
+/* Staging records per tile and phase as a fraction of the phase's pixels
+ * (0 < frac <= 1; 1 can never overflow).  Applies to later plans. */
+int toed_set_staging_fraction(toed_ctx *ctx, float frac);
+/* Last kernel timings of toed_detect_sync, in ms (reference time_conv /
+ * time_nms, gpu_toed.hpp:57): tile kernel, and scan + finalize. */
+int toed_last_times(toed_ctx *ctx, float *ms_tiles, float *ms_finalize);
+
+/* Single image ---------------------------------------------------------------
+ * Asynchronous: enqueues the fused pipeline; *d_count (device int) receives N
+ * and *d_flags (device int, may be NULL) the TOED_FLAG_* bits. */
+int toed_detect(toed_ctx *ctx, const toed_image_t *img, int *d_count, int *d_flags);
+/* Synchronous convenience: returns N in *n_out and maps flags to a status. */
+int toed_detect_sync(toed_ctx *ctx, const toed_image_t *img, long long *n_out);
+
+/* Batch: many images of uneven sizes and sigma in one launch sequence ------- */
+int toed_batch_plan(toed_ctx *ctx, int n, const toed_image_t *imgs, toed_batch **out);
+/* 0: one launch per kernel over all tiles of all images; 1: persistent tile
+ * kernel over a device work queue. */
+int toed_batch_set_persistent(toed_batch *b, int persistent);
+/* Enqueue; per-image counts and flags land in d_counts[n] / d_flags[n] (device). */
+int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags);
+/* Capture the launch sequence once in a hipGraph and replay it. */
+int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags);
+int toed_batch_destroy(toed_batch *b);
+
+/* Reference launch layer (unfused, for drop-in and debugging) ---------------
+ * toed_gpu_convolve_f32: gpu_kernels.hpp:24-33.  Writes the four interp planes
+ * Ix, Iy, |grad I| and TO orientation ((2h) x (2w), row-major) from a float
+ * image; the 8 tap pointers of the reference are replaced by sigma.
+ * toed_gpu_nms_f32: gpu_kernels.hpp:48-54.  Writes the subpixel position maps
+ * (zero where no edgel, reference convention) from the three planes. */
+int toed_gpu_convolve_f32(int device_id, int h, int w, int interp_h, int interp_w,
+                          const float *dev_img, float *dev_Ix, float *dev_Iy,
+                          float *dev_I_grad_mag, float *dev_I_orient, float sigma);
+int toed_gpu_nms_f32(int device_id, int interp_h, int interp_w,
+                     const float *dev_Ix, const float *dev_Iy, const float *dev_I_grad_mag,
+                     float *dev_subpix_pos_x_map, float *dev_subpix_pos_y_map,
+                     float thresh, int border);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOED_CAPI_H */

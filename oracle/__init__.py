@@ -1,0 +1,156 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes bindings of the FP64 TOED oracle.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this package, and only as the checker (or the timed CPU baseline).
+The C sources restate the reference's ``cpu_toed.cpp`` (see the header of
+``toed_oracle.c`` for the file:line map).
+"""
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def build(ref=False):
+    """Compile liboracle.so (and, with ref=True, the reference harness)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+    if ref:
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        L.toed_oracle_radius.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.toed_oracle_taps.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]
+        L.toed_oracle_convolve.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                           _dp, _dp, _dp, _dp, _dp]
+        L.toed_oracle_nms.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_int, ctypes.c_int, _dp, ctypes.c_long, _i64p, _dp, _dp]
+        L.toed_oracle_nms.restype = ctypes.c_long
+        L.toed_oracle_detect.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_int, _dp, ctypes.c_long, _i64p,
+                                         _dp, _dp, _dp, _dp, _dp, _dp]
+        L.toed_oracle_detect.restype = ctypes.c_long
+        L.toed_oracle_to_theta.argtypes = [ctypes.c_double] * 9
+        L.toed_oracle_to_theta.restype = ctypes.c_double
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def radius(sigma):
+    r0, r1 = ctypes.c_int(), ctypes.c_int()
+    if lib().toed_oracle_radius(float(sigma), ctypes.byref(r0), ctypes.byref(r1)):
+        raise ValueError("bad sigma %r" % sigma)
+    return r0.value, r1.value
+
+
+def taps(sigma):
+    """(R0, R1, taps[8, 2*R1+1]): G, G', G'', G''' at x=k-R1, then at x+1/2."""
+    r0, r1 = radius(sigma)
+    t = np.zeros((8, 2 * r1 + 1), np.float64)
+    lib().toed_oracle_taps(float(sigma), None, None, _p(t))
+    return r0, r1, t
+
+
+def convolve(img, sigma=2.0, nthreads=0, resp9=False):
+    """Ix, Iy, |grad|, theta planes (2H x 2W); optionally the nine responses."""
+    img = np.ascontiguousarray(img, np.float64)
+    H, W = img.shape
+    out = [np.empty((2 * H, 2 * W), np.float64) for _ in range(4)]
+    r9 = np.empty((9, 2 * H, 2 * W), np.float64) if resp9 else None
+    if lib().toed_oracle_convolve(_p(img), H, W, float(sigma), int(nthreads), *[_p(o) for o in out], _p(r9)):
+        raise ValueError("convolve failed")
+    res = dict(Ix=out[0], Iy=out[1], mag=out[2], orient=out[3])
+    if resp9:
+        res["resp9"] = r9
+    return res
+
+
+def nms(planes, thresh=2.0, border=10, nthreads=0, margins=False):
+    Ix, Iy, mag, orient = (np.ascontiguousarray(planes[k], np.float64) for k in ("Ix", "Iy", "mag", "orient"))
+    IH, IW = Ix.shape
+    cap = IH * IW
+    edges = np.empty((cap, 4), np.float64)
+    eidx = np.empty(cap, np.int64)
+    mm = np.empty((IH, IW), np.float64) if margins else None
+    ms = np.empty((IH, IW), np.float64) if margins else None
+    n = lib().toed_oracle_nms(_p(Ix), _p(Iy), _p(mag), _p(orient), IH // 2, IW // 2, float(thresh), int(border),
+                              int(nthreads), _p(edges), cap, _p(eidx, _i64p), _p(mm), _p(ms))
+    if n < 0:
+        raise MemoryError("oracle nms failed")
+    res = dict(edges=edges[:n].copy(), eidx=eidx[:n].copy(), n=int(n))
+    if margins:
+        res["mmag"], res["msub"] = mm, ms
+    return res
+
+
+def detect(img, sigma=2.0, nthreads=0, thresh=2.0, border=10, planes=False, margins=False):
+    """FP64 TOED on a 2-D image (any real dtype).  Returns dict with
+    ``edges`` (N x 4: x, y, theta, strength), ``eidx`` (interp index)."""
+    img = np.ascontiguousarray(img, np.float64)
+    H, W = img.shape
+    if planes:
+        pl = convolve(img, sigma, nthreads)
+        res = nms(pl, thresh, border, nthreads, margins)
+        res.update(pl)
+        return res
+    IH, IW = 2 * H, 2 * W
+    cap = IH * IW
+    edges = np.empty((cap, 4), np.float64)
+    eidx = np.empty(cap, np.int64)
+    mm = np.empty((IH, IW), np.float64) if margins else None
+    ms = np.empty((IH, IW), np.float64) if margins else None
+    n = lib().toed_oracle_detect(_p(img), H, W, float(sigma), int(nthreads), float(thresh), int(border),
+                                 _p(edges), cap, _p(eidx, _i64p), None, None, None, None, _p(mm), _p(ms))
+    if n < 0:
+        raise ValueError("oracle detect failed")
+    res = dict(edges=edges[:n].copy(), eidx=eidx[:n].copy(), n=int(n))
+    if margins:
+        res["mmag"], res["msub"] = mm, ms
+    return res
+
+
+def to_theta(*d):
+    return lib().toed_oracle_to_theta(*[float(v) for v in d])
+
+
+REF_BIN = os.path.join(HERE, "_ref", "toed_ref")
+
+
+def ref_available():
+    return os.path.exists(REF_BIN)
+
+
+def run_ref(img_u8, nthreads=1):
+    """Run the reference's own cpu_toed.cpp (compiled into oracle/_ref) on a
+    uint8 image.  Returns (edges N x 4 float64, time_conv_s, time_nms_s)."""
+    img_u8 = np.ascontiguousarray(img_u8, np.uint8)
+    H, W = img_u8.shape
+    with tempfile.TemporaryDirectory() as td:
+        raw = os.path.join(td, "img.raw")
+        out = os.path.join(td, "out.bin")
+        img_u8.tofile(raw)
+        subprocess.run([REF_BIN, raw, str(H), str(W), str(int(nthreads)), out, td],
+                       check=True, stdout=subprocess.DEVNULL)
+        with open(out, "rb") as f:
+            n = int(np.frombuffer(f.read(8), np.int64)[0])
+            tc, tn = np.frombuffer(f.read(16), np.float64)
+            edges = np.frombuffer(f.read(n * 32), np.float64).reshape(n, 4).copy()
+    return edges, float(tc), float(tn)

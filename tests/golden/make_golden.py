@@ -1,0 +1,115 @@
+"""Write the golden vectors of the TOED detection path.
+
+Runs the reference's own FP64 CPU detector -- cpu_toed.cpp compiled unmodified
+from /root/reference by oracle/Makefile into oracle/_ref/toed_ref -- on
+
+  * seeded synthetic scenes from the project's generator (64x96, 128x160 and
+    the 480x752 scene of the CPU-runnable benchmark configuration), and
+  * the reference's sample images input_images/2018.pgm (P5),
+    t_less_test_img.pgm (P5) and euroc_sample_img.png (8-bit grey PNG),
+    decoded here without OpenCV (input_images/0_colors.png is RGBA; its
+    OpenCV grey conversion is not restated, so it is left out).
+
+Each fixture stores the uint8 input and the reference's N x 4 edgel list
+[x, y, theta, strength] in float64 (full precision, not the reference's
+6-digit text).  Run from the repo root:  python tests/golden/make_golden.py
+"""
+import importlib
+import os
+import struct
+import sys
+import zlib
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import oracle  # noqa: E402
+
+scene = importlib.import_module("third-order-edge-detector_amd.scene")
+OUT = os.path.dirname(os.path.abspath(__file__))
+REF_IMAGES = "/root/reference/input_images"
+
+SCENES = [("scene_64x96_s1", 1, 64, 96), ("scene_128x160_s2", 2, 128, 160), ("scene_480x752_s3", 3, 480, 752)]
+
+
+def read_pgm_p5(path):
+    data = open(path, "rb").read()
+    fields, pos = [], 0
+    while len(fields) < 4:
+        while data[pos:pos + 1].isspace():
+            pos += 1
+        if data[pos:pos + 1] == b"#":
+            pos = data.index(b"\n", pos) + 1
+            continue
+        end = pos
+        while not data[end:end + 1].isspace():
+            end += 1
+        fields.append(data[pos:end])
+        pos = end
+    pos += 1  # exactly one whitespace byte before the raster
+    assert fields[0] == b"P5" and int(fields[3]) == 255
+    w, h = int(fields[1]), int(fields[2])
+    return np.frombuffer(data[pos:pos + w * h], np.uint8).reshape(h, w).copy()
+
+
+def read_png_gray8(path):
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, hdr = 8, b"", None
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        t = data[pos + 4:pos + 8]
+        body = data[pos + 8:pos + 8 + n]
+        if t == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", body)
+        elif t == b"IDAT":
+            idat += body
+        pos += 12 + n
+    w, h, bd, ct, _, _, il = hdr
+    assert bd == 8 and ct == 0 and il == 0, "only 8-bit grey, non-interlaced"
+    raw = zlib.decompress(idat)
+    out = np.zeros((h, w), np.uint8)
+    prev = np.zeros(w, np.int32)
+    for y in range(h):
+        f = raw[y * (w + 1)]
+        line = np.frombuffer(raw, np.uint8, w, y * (w + 1) + 1).astype(np.int32)
+        cur = np.zeros(w, np.int32)
+        for x in range(w):
+            a = cur[x - 1] if x else 0
+            b = prev[x]
+            c = prev[x - 1] if x else 0
+            if f == 0:
+                p = 0
+            elif f == 1:
+                p = a
+            elif f == 2:
+                p = b
+            elif f == 3:
+                p = (a + b) // 2
+            else:
+                pa, pb, pc = abs(b - c), abs(a - c), abs(a + b - 2 * c)
+                p = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+            cur[x] = (line[x] + p) & 255
+        out[y] = cur
+        prev = cur
+    return out
+
+
+def main():
+    oracle.build(ref=True)
+    items = [(name, scene.shapes_scene(seed, H, W)) for name, seed, H, W in SCENES]
+    if os.path.isdir(REF_IMAGES):
+        items.append(("img_2018_pgm", read_pgm_p5(os.path.join(REF_IMAGES, "2018.pgm"))))
+        items.append(("img_t_less_pgm", read_pgm_p5(os.path.join(REF_IMAGES, "t_less_test_img.pgm"))))
+        items.append(("img_euroc_png", read_png_gray8(os.path.join(REF_IMAGES, "euroc_sample_img.png"))))
+    for name, img in items:
+        edges, tc, tn = oracle.run_ref(img, nthreads=8)
+        path = os.path.join(OUT, "toed_ref_%s.npz" % name)
+        np.savez_compressed(path, img=img, edges=edges)
+        print("%-22s %4dx%-4d N=%6d conv %.3fs nms %.4fs -> %s" % (name, img.shape[0], img.shape[1], len(edges),
+                                                                    tc, tn, os.path.basename(path)))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+"""Batched form: one plan over images of uneven sizes and sigma, plain and
+persistent tile kernels, and hipGraph replays, each checked edgel for edgel
+against single-image detection and the oracle."""
+import numpy as np
+import pytest
+
+from parity import assert_parity, compare
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+SHAPES = [(480, 640), (120, 200), (64, 900), (333, 257), (20, 20), (481, 321)]
+SIGMAS = [1.0, 1.5, 2.0, 2.0, 1.5, 1.0]
+
+
+@pytest.fixture(scope="module")
+def batch_inputs(scene):
+    imgs = [scene.shapes_scene(500 + k, H, W) for k, (H, W) in enumerate(SHAPES)]
+    return imgs, [torch.from_numpy(im).cuda() for im in imgs]
+
+
+def _singles(detector, dev_imgs):
+    out = []
+    for t, s in zip(dev_imgs, SIGMAS):
+        e, i = detector.detect(t, sigma=s, with_index=True)
+        out.append((e.cpu().numpy(), i.cpu().numpy()))
+    return out
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_equals_singles(pkg, detector, batch_inputs, persistent):
+    imgs, dev = batch_inputs
+    singles = _singles(detector, dev)
+    b = pkg.BatchDetector(detector, dev, SIGMAS, persistent=persistent, with_index=True)
+    b.run()
+    res = b.results()
+    idx = b.index_results()
+    for m in range(len(dev)):
+        assert np.array_equal(idx[m].cpu().numpy(), singles[m][1]), m
+        assert np.array_equal(res[m].cpu().numpy(), singles[m][0]), m
+    b.close()
+
+
+def test_batch_graph_replays(pkg, detector, batch_inputs, oracle):
+    imgs, dev = batch_inputs
+    b = pkg.BatchDetector(detector, dev, SIGMAS, persistent=True, with_index=True)
+    b.run()
+    first = [r.cpu().numpy().copy() for r in b.results()]
+    for rep in range(4):
+        b.edges.fill_(-1.0)   # eager work between replays
+        b.counts.zero_()
+        torch.cuda.synchronize()
+        b.run(graph=True)
+        torch.cuda.synchronize()
+        now = [r.cpu().numpy() for r in b.results()]
+        for m in range(len(dev)):
+            assert np.array_equal(now[m], first[m]), (rep, m)
+    # and the batch agrees with the oracle
+    idx = [i.cpu().numpy() for i in b.index_results()]
+    for m in (0, 3):
+        ref = oracle.detect(imgs[m].astype(np.float64), SIGMAS[m], margins=True)
+        assert_parity(compare(ref, first[m], idx[m]))
+    b.close()
+
+
+def test_batch_float_images(pkg, detector, batch_inputs):
+    imgs, dev = batch_inputs
+    devf = [t.float() for t in dev[:3]]
+    b = pkg.BatchDetector(detector, devf, SIGMAS[:3])
+    b.run()
+    res = b.results()
+    b8 = pkg.BatchDetector(detector, dev[:3], SIGMAS[:3])
+    b8.run()
+    res8 = b8.results()
+    for m in range(3):
+        assert torch.equal(res[m], res8[m])
+    b.close()
+    b8.close()

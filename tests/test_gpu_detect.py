@@ -1,0 +1,210 @@
+"""GPU parity of the fused detection path against the FP64 oracle (which is
+itself pinned to the reference's compiled cpu_toed.cpp by the golden tests).
+Tolerances are stated in tests/parity.py."""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from parity import assert_parity, compare
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "toed_ref_*.npz")))
+
+
+def _run(detector, img, sigma=2.0, dtype=None, **kw):
+    t = torch.from_numpy(np.ascontiguousarray(img)).cuda()
+    if dtype is not None:
+        t = t.to(dtype)
+    e, i = detector.detect(t, sigma=sigma, with_index=True, **kw)
+    torch.cuda.synchronize()
+    return e.cpu().numpy(), i.cpu().numpy()
+
+
+def _report(name, rep):
+    print("%s: %s" % (name, rep))
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[9:-4] for p in GOLDEN])
+def test_golden_inputs(detector, oracle, path):
+    d = np.load(path)
+    img = d["img"]
+    ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
+    assert ref["n"] == len(d["edges"])
+    e, i = _run(detector, img)
+    rep = compare(ref, e, i)
+    _report(os.path.basename(path), rep)
+    assert_parity(rep)
+
+
+@pytest.mark.parametrize("sigma", [1.0, 1.5, 2.0])
+@pytest.mark.parametrize("shape", [(480, 752), (301, 413), (64, 40)])
+def test_seeded_scenes_sigma(detector, oracle, scene, sigma, shape):
+    H, W = shape
+    img = scene.shapes_scene(100 + H, H, W)
+    ref = oracle.detect(img.astype(np.float64), sigma, margins=True)
+    e, i = _run(detector, img, sigma)
+    rep = compare(ref, e, i)
+    _report("scene %dx%d sigma %g" % (H, W, sigma), rep)
+    assert_parity(rep)
+
+
+def test_float_input_matches_u8(detector, oracle, scene):
+    img = scene.shapes_scene(21, 200, 260)
+    e8, i8 = _run(detector, img)
+    ef, i_f = _run(detector, img, dtype=torch.float32)
+    assert np.array_equal(i8, i_f)
+    assert np.array_equal(e8, ef)
+
+
+def test_float_input_non_integer(detector, oracle, scene):
+    img = scene.shapes_scene(22, 160, 200).astype(np.float32) * np.float32(0.731) + np.float32(3.3)
+    ref = oracle.detect(img.astype(np.float64), 1.5, margins=True)
+    e, i = _run(detector, img, 1.5)
+    rep = compare(ref, e, i)
+    _report("float scene", rep)
+    assert_parity(rep)
+
+
+def test_strided_rows(detector, oracle, scene):
+    img = scene.shapes_scene(23, 128, 190)
+    big = torch.zeros((128, 256), dtype=torch.uint8, device="cuda")
+    big[:, :190] = torch.from_numpy(img).cuda()
+    view = big[:, :190]
+    e, i = detector.detect(view, with_index=True)
+    e2, i2 = _run(detector, img)
+    assert np.array_equal(i.cpu().numpy(), i2) and np.array_equal(e.cpu().numpy(), e2)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (5, 7), (10, 300), (11, 11), (33, 17)])
+def test_tiny_and_ragged(detector, oracle, shape):
+    rng = np.random.default_rng(shape[0] * 1000 + shape[1])
+    img = rng.integers(0, 256, shape, dtype=np.uint8)
+    ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
+    e, i = _run(detector, img)
+    rep = compare(ref, e, i)
+    _report("tiny %s" % (shape,), rep)
+    assert_parity(rep)
+    if shape[0] <= 10 or shape[1] <= 10:
+        assert len(i) == 0
+
+
+def test_constant_and_step(detector):
+    e, i = _run(detector, np.full((64, 96), 128, np.uint8))
+    assert len(i) == 0
+    img = np.zeros((64, 96), np.uint8)
+    img[:, 48:] = 200
+    e, i = _run(detector, img)
+    IW = 192
+    cols = i % IW
+    assert len(i) > 0 and np.all(np.abs(e[:, 0] - 47.0) < 0.05) and np.all(np.abs(cols - 95) <= 1)
+
+
+def test_capacity_overflow_reports_exact_count(detector, pkg, scene):
+    img = torch.from_numpy(scene.shapes_scene(5, 200, 200)).cuda()
+    full = detector.detect(img)
+    n = full.shape[0]
+    assert n > 10
+    with pytest.raises(pkg.ToedError) as ei:
+        detector.detect(img, capacity=n // 2)
+    assert ei.value.code == 5
+    # the rows that fit are the first rows of the full list
+    edges = torch.zeros((n // 2, 4), dtype=torch.float32, device="cuda")
+    counts = torch.zeros(2, dtype=torch.int32, device="cuda")
+    detector.detect_into(img, edges, counts)
+    c = counts.cpu().numpy()
+    assert c[0] == n and c[1] & 1
+    assert torch.equal(edges, full[: n // 2])
+
+
+def test_staging_overflow_is_flagged(pkg, scene):
+    det = pkg.Detector(0, staging_fraction=0.002)
+    img = torch.from_numpy(scene.shapes_scene(6, 200, 200)).cuda()
+    with pytest.raises(pkg.ToedError) as ei:
+        det.detect(img)
+    assert ei.value.code == 6
+    det.close()
+
+
+def test_tile_rows_variants_identical(pkg, scene):
+    img = torch.from_numpy(scene.shapes_scene(8, 250, 333)).cuda()
+    outs = []
+    for tr in (8, 16):
+        det = pkg.Detector(0, tile_rows=tr)
+        e, i = det.detect(img, with_index=True)
+        outs.append((e.cpu().numpy(), i.cpu().numpy()))
+        det.close()
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][0], outs[1][0])
+
+
+def test_bad_arguments(detector, pkg):
+    img = torch.zeros((32, 32), dtype=torch.uint8, device="cuda")
+    with pytest.raises(pkg.ToedError):
+        detector.detect(img, sigma=2.7)
+    with pytest.raises(TypeError):
+        detector.detect(img.to(torch.int16))
+    with pytest.raises(pkg.ToedError):
+        detector.detect(img, border=0)
+
+
+def test_reference_class_mirror(pkg, oracle, scene, tmp_path):
+    img = scene.shapes_scene(31, 120, 150)
+    cls = pkg.ThirdOrderEdgeDetectionGPU(0, 120, 150, 17, 2)
+    cls.set_output_dir(str(tmp_path))
+    cls.preprocessing(img)
+    cls.convolve_img()
+    TOED_edges = np.zeros((4 * 120 * 150, 4), np.float32)
+    n = cls.non_maximum_suppresion(TOED_edges)
+    assert n == cls.edge_pt_list_idx > 0
+    assert cls.time_conv > 0 and cls.time_nms > 0
+    txt = np.loadtxt(tmp_path / "data_final_output_gpu.txt")
+    np.testing.assert_allclose(txt, TOED_edges[:n], rtol=1e-6, atol=1e-6)
+    # the byte-stream preprocessing overload gives the same result
+    import io
+    cls2 = pkg.ThirdOrderEdgeDetectionGPU(0, 120, 150, 17, 2)
+    cls2.set_output_dir(str(tmp_path))
+    cls2.preprocessing(io.BytesIO(img.tobytes()))
+    cls2.convolve_img()
+    e2 = np.zeros_like(TOED_edges)
+    assert cls2.non_maximum_suppresion(e2) == n
+    assert np.array_equal(e2[:n], TOED_edges[:n])
+    with pytest.raises(ValueError):
+        pkg.ThirdOrderEdgeDetectionGPU(0, 10, 10, 19, 2)
+
+
+def test_unfused_launch_layer(pkg, oracle, scene):
+    """toed_gpu_convolve_f32 / toed_gpu_nms_f32 (the reference's launch layer)
+    against the oracle's planes and its subpixel maps."""
+    import ctypes
+    L = pkg._lib.lib()
+    H, W = 90, 120
+    img = scene.shapes_scene(41, H, W)
+    t = torch.from_numpy(img.astype(np.float32)).cuda()
+    planes = [torch.zeros((2 * H, 2 * W), dtype=torch.float32, device="cuda") for _ in range(4)]
+    ptr = lambda x: ctypes.c_void_p(x.data_ptr())
+    assert L.toed_gpu_convolve_f32(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in planes], 2.0) == 0
+    torch.cuda.synchronize()
+    c = oracle.convolve(img.astype(np.float64), 2.0)
+    Ix, Iy, mag, ori = (p.cpu().numpy().astype(np.float64) for p in planes)
+    assert np.abs(Ix - c["Ix"]).max() < 1e-3 and np.abs(Iy - c["Iy"]).max() < 1e-3
+    assert np.abs(mag - c["mag"]).max() < 1e-3
+    strong = c["mag"] > 2
+    d = np.abs(ori - c["orient"])[strong]
+    d = np.minimum(d, np.abs(d - 2 * np.pi))
+    assert np.nanquantile(d, 0.99) < 1e-2
+    px = torch.zeros((2 * H, 2 * W), dtype=torch.float32, device="cuda")
+    py = torch.zeros_like(px)
+    assert L.toed_gpu_nms_f32(0, 2 * H, 2 * W, ptr(planes[0]), ptr(planes[1]), ptr(planes[2]), ptr(px), ptr(py),
+                              2.0, 10) == 0
+    torch.cuda.synchronize()
+    sel = np.flatnonzero(px.cpu().numpy().reshape(-1))
+    ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
+    only = np.setxor1d(sel, ref["eidx"])
+    mm, ms = ref["mmag"].reshape(-1), ref["msub"].reshape(-1)
+    assert np.all((mm[only] < 2e-3) | (ms[only] < 2e-3))

@@ -1,0 +1,85 @@
+"""ctypes binding of libtoed_hip.so (the C ABI declared in include/toed_capi.h).
+
+The library is the only compute path: if it is missing, every entry point
+raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtoed_hip.so")
+
+TOED_U8, TOED_F32 = 0, 1
+FLAG_CAPACITY, FLAG_STAGING = 1, 2
+
+STATUS = {
+    0: "TOED_OK", 1: "TOED_ERR_ARG", 2: "TOED_ERR_SIGMA", 3: "TOED_ERR_ALLOC", 4: "TOED_ERR_HIP",
+    5: "TOED_ERR_CAPACITY", 6: "TOED_ERR_STAGING", 7: "TOED_ERR_NOT_BUILT",
+}
+
+
+class ToedError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__("%s failed: %s (%d)" % (what or "toed call", STATUS.get(code, "?"), code))
+
+
+class toed_image_t(ctypes.Structure):
+    _fields_ = [
+        ("in_type", ctypes.c_int),
+        ("d_img", ctypes.c_void_p),
+        ("H", ctypes.c_int), ("W", ctypes.c_int), ("pitch", ctypes.c_int),
+        ("sigma", ctypes.c_float), ("thresh", ctypes.c_float), ("border", ctypes.c_int),
+        ("d_edges", ctypes.c_void_p),
+        ("capacity", ctypes.c_longlong),
+        ("d_index", ctypes.c_void_p),
+    ]
+
+
+# exported symbols and their signatures: (name, restype, argtypes)
+_VP, _I, _F, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
+_PI, _PF = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
+SIGNATURES = [
+    ("toed_version", _I, []),
+    ("toed_sigma_radius", _I, [_F, _PI, _PI]),
+    ("toed_create", _I, [_I, ctypes.POINTER(_VP)]),
+    ("toed_destroy", _I, [_VP]),
+    ("toed_set_stream", _I, [_VP, _VP]),
+    ("toed_get_stream", _VP, [_VP]),
+    ("toed_set_tile_rows", _I, [_VP, _I]),
+    ("toed_set_staging_fraction", _I, [_VP, _F]),
+    ("toed_last_times", _I, [_VP, _PF, _PF]),
+    ("toed_detect", _I, [_VP, ctypes.POINTER(toed_image_t), _VP, _VP]),
+    ("toed_detect_sync", _I, [_VP, ctypes.POINTER(toed_image_t), ctypes.POINTER(_LL)]),
+    ("toed_batch_plan", _I, [_VP, _I, ctypes.POINTER(toed_image_t), ctypes.POINTER(_VP)]),
+    ("toed_batch_set_persistent", _I, [_VP, _I]),
+    ("toed_batch_run", _I, [_VP, _VP, _VP]),
+    ("toed_batch_graph_run", _I, [_VP, _VP, _VP]),
+    ("toed_batch_destroy", _I, [_VP]),
+    ("toed_gpu_convolve_f32", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F]),
+    ("toed_gpu_nms_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F, _I]),
+]
+
+_LIB = None
+
+
+def lib():
+    """Load libtoed_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtoed_hip.so is not built (run __graft_entry__.build()); "
+                               "there is no fallback compute path")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code, what=""):
+    if code != 0:
+        raise ToedError(code, what)
+    return code

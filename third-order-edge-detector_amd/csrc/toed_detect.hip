@@ -1,0 +1,1149 @@
+// TOED detection path for MI355X (gfx950): fused tile kernel, segmented scan,
+// FP64 per-edgel finalize, host orchestration and the C ABI (toed_capi.h).
+//
+// Algorithm (restated from the reference's CPU detector, cpu_toed.cpp:128-545):
+//   * four interp samples per original pixel (i, j): (2i+a, 2j+b), a, b in {0,1},
+//     each a Gaussian-derivative response with x shifted by b/2 and y by a/2;
+//     phase (0,0) uses the R0 = ceil(4 sigma) window, the others R1 = R0 + 1
+//     (cpu_toed.cpp:205-344);
+//   * |grad I| threshold, octant NMS with linear interpolation of the magnitude
+//     one step along the gradient, Fisher-Naidu parabola (cpu_toed.cpp:400-494);
+//   * third-order orientation from the nine responses (cpu_toed.cpp:227-232);
+//   * raster-ordered N x 4 list [x, y, theta, strength] (cpu_toed.cpp:510-538).
+//
+// MI355X design:
+//   K1 toed_tiles_kernel   one workgroup (4 wavefronts) per TH x 32 tile of
+//        original pixels = 2TH x 64 interp pixels.  The image tile plus halo is
+//        staged once in LDS; the 2-D filters are applied separably (row pass,
+//        then column pass) in FP32 in three rounds that reuse two LDS planes:
+//        (R0-window G, G') -> phase (0,0); + end taps -> phase (1,0);
+//        (shifted G, G') -> phases (0,1), (1,1).  Ix, Iy, |grad I| of the tile
+//        plus a one-pixel interp halo stay in LDS; each wavefront runs NMS on one
+//        64-wide interp row at a time, computes survivors' parabola in FP64 and
+//        compacts them with a ballot: per-(row, tile) counts go to HBM, records to
+//        a per-tile, per-phase staging area.
+//   K2 toed_scan_chunks / toed_scan_images   exclusive scan of the counts
+//        (row-major over (interp row, tile column)), segmented per image.
+//   K3 toed_finalize_kernel   one workgroup per tile, phases processed in turn
+//        so the filter taps are wavefront-uniform: FP64 recomputation of the
+//        nine responses at each edgel, third-order orientation, write of the
+//        edgel at offset[row, tile] + rank (raster order).
+// No MFMA: the filters are short separable stencils, not a contraction.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "toed_capi.h"
+
+namespace toed {
+
+constexpr int TW = 32;              // tile width in original pixels -> 64 interp columns
+constexpr int NT = 256;             // threads per tile workgroup (4 wavefronts)
+constexpr int MAXR1 = 9;            // largest supported R1 (sigma <= 2)
+constexpr int MAXTAPS = 2 * MAXR1 + 1;
+constexpr int CHUNK = 4096;         // scan chunk (entries)
+
+// Filter taps for one sigma.  Families: 0 G, 1 G', 2 G'', 3 G''' sampled at
+// x = k - R1 (unshifted) and x = k - R1 + 1/2 (shifted), k = 0..2R1
+// (closed forms of cpu_toed.cpp:141-162, window rule of cpu_toed.cpp:131-132).
+struct TapSet {
+    int R0, R1, pad0, pad1;
+    float fu[2][MAXTAPS];
+    float fs[2][MAXTAPS];
+    double du[4][MAXTAPS];
+    double ds[4][MAXTAPS];
+};
+
+// One image of a launch.  All offsets are in elements of their arrays.
+struct ImageJob {
+    const void *img;
+    float *out;
+    int *count_out;
+    int *flags_out;
+    const TapSet *taps;
+    long long capacity;
+    long long counts_base;     // first entry in counts/offsets (chunk aligned)
+    long long chunk_base;      // first scan chunk
+    long long stage_base;      // first staging record
+    int in_type, H, W, pitch;
+    int R0, R1, ntc, ntr;
+    int tile_begin, nchunks, border, stage_cap;   // stage_cap = records per tile and phase
+    float thresh;
+    int pad0;
+    int *index_out;            // optional interp index per edgel
+};
+
+struct Staged {
+    uint32_t key;     // local row | local col << 8 | rank-in-row << 16
+    float x, y, s;
+};
+
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+__device__ inline const ImageJob &job_for_tile(const ImageJob *jobs, int njobs, int tile) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {   // last job with tile_begin <= tile
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+    }
+    return jobs[lo];
+}
+
+__device__ inline int job_for_chunk(const ImageJob *jobs, int njobs, long long chunk) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].chunk_base <= chunk) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T> __device__ inline float load_px(const void *p, long long idx) {
+    return (float)((const T *)p)[idx];
+}
+
+// ---------------------------------------------------------------------------
+// LDS geometry of one tile
+// ---------------------------------------------------------------------------
+template <int R1, int TH> struct Geo {
+    static constexpr int R0 = R1 - 1;
+    static constexpr int NRI = TH + 2 + 2 * R1;            // staged image rows
+    static constexpr int NCI = TW + 2 + 2 * R1;            // staged image cols
+    static constexpr int LDI = NCI | 1;                    // odd pitch: conflict-free columns
+    static constexpr int NCP = TW + 2;                     // row-pass output cols
+    static constexpr int LDP = NCP | 1;
+    static constexpr int NVR = TH + 2;                     // column-pass output rows
+    static constexpr int RH = 2 * TH + 2, RW = 2 * TW + 2; // interp results incl. 1-px halo
+    static constexpr int LDR = RW;
+    static constexpr int OFF_IMG = 0;
+    static constexpr int OFF_P0 = OFF_IMG + NRI * LDI;
+    static constexpr int OFF_P1 = OFF_P0 + NRI * LDP;
+    static constexpr int OFF_IX = OFF_P1 + NRI * LDP;
+    static constexpr int OFF_IY = OFF_IX + RH * LDR;
+    static constexpr int OFF_MG = OFF_IY + RH * LDR;
+    static constexpr int TOTAL = OFF_MG + RH * LDR;
+    // row pass: items = (row, block of KH columns)
+    static constexpr int KH = 2;
+    static constexpr int NBH = NCP / KH;
+    // column pass: items = (col, block of KV rows)
+    static constexpr int KV = (NVR + 6) / 7 < 2 ? 2 : (NVR + 6) / 7;
+    static constexpr int NBV = (NVR + KV - 1) / KV;
+};
+
+template <int TH> constexpr int lds_floats() { return Geo<MAXR1, TH>::TOTAL; }
+
+// Row pass over the staged image.  mode 0: R0-window taps (fresh);
+// mode 1: add the two end taps (R0 -> R1 window); mode 2: shifted taps (fresh).
+template <int R1, int TH, int MODE>
+__device__ inline void row_pass(float *lds, const float *__restrict__ t0, const float *__restrict__ t1) {
+    using Gm = Geo<R1, TH>;
+    const float *img = lds + Gm::OFF_IMG;
+    float *P0 = lds + Gm::OFF_P0;
+    float *P1 = lds + Gm::OFF_P1;
+    constexpr int R = (MODE == 0) ? Gm::R0 : R1;
+    for (int it = threadIdx.x; it < Gm::NRI * Gm::NBH; it += NT) {
+        const int rho = it % Gm::NRI;
+        const int k0 = (it / Gm::NRI) * Gm::KH;
+        const float *row = img + rho * Gm::LDI + k0;
+        if (MODE == 1) {
+#pragma unroll
+            for (int d = 0; d < Gm::KH; ++d) {
+                const float lo = row[d], hi = row[d + 2 * R1];
+                P0[rho * Gm::LDP + k0 + d] += hi * t0[0] + lo * t0[2 * R1];
+                P1[rho * Gm::LDP + k0 + d] += hi * t1[0] + lo * t1[2 * R1];
+            }
+        } else {
+            float w[Gm::KH + 2 * R1];
+#pragma unroll
+            for (int t = 0; t < Gm::KH + 2 * R1; ++t) w[t] = row[t];
+#pragma unroll
+            for (int d = 0; d < Gm::KH; ++d) {
+                float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+                for (int q = -R; q <= R; ++q) {
+                    const float v = w[d + R1 - q];
+                    a0 = fmaf(v, t0[q + R1], a0);
+                    a1 = fmaf(v, t1[q + R1], a1);
+                }
+                P0[rho * Gm::LDP + k0 + d] = a0;
+                P1[rho * Gm::LDP + k0 + d] = a1;
+            }
+        }
+    }
+}
+
+// Column pass: for phases with y shift A (0/1) and x shift B (0/1), from the
+// current P planes: Ix = sum P1 * Y0, Iy = sum P0 * Y1; writes Ix, Iy, |grad|
+// into the interp result planes.  NPH = 1 or 2 phases from the same P planes.
+template <int R1, int TH, int R, int NPH>
+__device__ inline void col_pass(float *lds, int B,
+                                const float *__restrict__ ya0, const float *__restrict__ ya1, int A0,
+                                const float *__restrict__ yb0, const float *__restrict__ yb1, int A1) {
+    using Gm = Geo<R1, TH>;
+    const float *P0 = lds + Gm::OFF_P0;
+    const float *P1 = lds + Gm::OFF_P1;
+    float *RX = lds + Gm::OFF_IX;
+    float *RY = lds + Gm::OFF_IY;
+    float *RM = lds + Gm::OFF_MG;
+    constexpr int KV = Gm::KV;
+    constexpr int NW = KV + 2 * R1;
+    for (int it = threadIdx.x; it < Gm::NBV * Gm::NCP; it += NT) {
+        const int u = it % Gm::NCP;
+        const int v0 = (it / Gm::NCP) * KV;
+        float w0[NW], w1[NW];
+#pragma unroll
+        for (int t = 0; t < NW; ++t) {
+            const int r = min(v0 + t, Gm::NRI - 1);
+            w0[t] = P0[r * Gm::LDP + u];
+            w1[t] = P1[r * Gm::LDP + u];
+        }
+#pragma unroll
+        for (int ph = 0; ph < NPH; ++ph) {
+            const float *y0 = ph ? yb0 : ya0;
+            const float *y1 = ph ? yb1 : ya1;
+            const int A = ph ? A1 : A0;
+#pragma unroll
+            for (int d = 0; d < KV; ++d) {
+                const int v = v0 + d;
+                float ix = 0.f, iy = 0.f;
+#pragma unroll
+                for (int p = -R; p <= R; ++p) {
+                    ix = fmaf(w1[d + R1 - p], y0[p + R1], ix);
+                    iy = fmaf(w0[d + R1 - p], y1[p + R1], iy);
+                }
+                const int rr = 2 * v - 1 + A, cc = 2 * u - 1 + B;
+                if (v < Gm::NVR && rr >= 0 && rr < Gm::RH && cc >= 0 && cc < Gm::RW) {
+                    RX[rr * Gm::LDR + cc] = ix;
+                    RY[rr * Gm::LDR + cc] = iy;
+                    RM[rr * Gm::LDR + cc] = sqrtf(ix * ix + iy * iy);
+                }
+            }
+        }
+    }
+}
+
+// Octant choice and neighbour interpolation of cpu_toed.cpp:415-462 for a pixel
+// whose magnitude plane is reachable through M(dr, dc).  Returns 0 for a NaN
+// gradient (no branch applies).  T is float (selection) or double (values).
+template <typename T, typename MF>
+__device__ inline int octant(T gx, T gy, T nx, T ny, MF M, T &slope, T &fp, T &fm) {
+    if (gx >= 0 && gy >= 0) {
+        if (gx >= gy) { slope = ny / nx; fp = M(0, 1) * (1 - slope) + M(1, 1) * slope;  fm = M(0, -1) * (1 - slope) + M(-1, -1) * slope; }
+        else          { slope = nx / ny; fp = M(1, 0) * (1 - slope) + M(1, 1) * slope;  fm = M(-1, 0) * (1 - slope) + M(-1, -1) * slope; }
+    } else if (gx < 0 && gy >= 0) {
+        if (fabs(gx) < gy) { slope = -nx / ny; fp = M(1, 0) * (1 - slope) + M(1, -1) * slope; fm = M(-1, 0) * (1 - slope) + M(-1, 1) * slope; }
+        else               { slope = -ny / nx; fp = M(0, -1) * (1 - slope) + M(1, -1) * slope; fm = M(0, 1) * (1 - slope) + M(-1, 1) * slope; }
+    } else if (gx < 0 && gy < 0) {
+        if (fabs(gx) >= fabs(gy)) { slope = ny / nx; fp = M(0, -1) * (1 - slope) + M(-1, -1) * slope; fm = M(0, 1) * (1 - slope) + M(1, 1) * slope; }
+        else                      { slope = nx / ny; fp = M(-1, 0) * (1 - slope) + M(-1, -1) * slope; fm = M(1, 0) * (1 - slope) + M(1, 1) * slope; }
+    } else if (gx >= 0 && gy < 0) {
+        if (gx < fabs(gy)) { slope = -nx / ny; fp = M(-1, 0) * (1 - slope) + M(-1, 1) * slope; fm = M(1, 0) * (1 - slope) + M(1, -1) * slope; }
+        else               { slope = -ny / nx; fp = M(0, 1) * (1 - slope) + M(-1, 1) * slope; fm = M(0, -1) * (1 - slope) + M(1, -1) * slope; }
+    } else {
+        return 0;
+    }
+    return 1;
+}
+
+// NMS over the tile's 2TH x 64 interp pixels, one 64-wide row per wavefront
+// step, survivors compacted into staging (cpu_toed.cpp:400-494).
+template <int R1, int TH>
+__device__ inline void nms_tile(float *lds, const ImageJob &J, int tr, int tc, long long tile_local,
+                                int *__restrict__ counts, Staged *__restrict__ staging, int *sh_cnt) {
+    using Gm = Geo<R1, TH>;
+    const float *RX = lds + Gm::OFF_IX;
+    const float *RY = lds + Gm::OFF_IY;
+    const float *RM = lds + Gm::OFF_MG;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int IH = 2 * J.H, IW = 2 * J.W;
+    const int R0g = 2 * tr * TH, C0g = 2 * tc * TW;
+    const int C = C0g + lane;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long even = 0x5555555555555555ull;
+    const double SQ2 = 1.4142135623730951;
+    for (int r = wave; r < 2 * TH; r += NT / 64) {
+        const int R = R0g + r;
+        const int cc = lane + 1, rr = r + 1;
+        bool keep = false;
+        float xo = 0.f, yo = 0.f, so = 0.f;
+        const bool inside = R >= J.border && R < IH - J.border && C >= J.border && C < IW - J.border;
+        const float f = RM[rr * Gm::LDR + cc];
+        if (inside && f > J.thresh) {
+            const float gx = RX[rr * Gm::LDR + cc], gy = RY[rr * Gm::LDR + cc];
+            if (!(fabsf(gx) < 10e-6f && fabsf(gy) < 10e-6f)) {
+                auto Mf = [&](int dr, int dc) { return RM[(rr + dr) * Gm::LDR + cc + dc]; };
+                const float nx = gx / f, ny = gy / f;
+                float sl, fp, fm;
+                if (octant<float>(gx, gy, nx, ny, Mf, sl, fp, fm) &&
+                    ((f > fm && f > fp) || (f > fm && f >= fp) || (f >= fm && f > fp))) {
+                    // parabola in FP64 from the FP32 magnitudes (cpu_toed.cpp:465-489)
+                    auto Md = [&](int dr, int dc) { return (double)RM[(rr + dr) * Gm::LDR + cc + dc]; };
+                    const double fd = f, gxd = gx, gyd = gy;
+                    const double nxd = gxd / fd, nyd = gyd / fd;
+                    double sld, fpd, fmd;
+                    octant<double>(gxd, gyd, nxd, nyd, Md, sld, fpd, fmd);
+                    const double s = sqrt(1.0 + sld * sld);
+                    const double A = (fmd + fpd - 2.0 * fd) / (2.0 * s * s);
+                    const double Bc = (fpd - fmd) / (2.0 * s);
+                    const double ss = -Bc / (2.0 * A);
+                    const double maxf = A * ss * ss + Bc * ss + fd;
+                    if (fabs(ss) <= SQ2) {
+                        const double sgx = maxf * nxd, sgy = maxf * nyd;
+                        keep = true;
+                        so = (float)sqrt(sgx * sgx + sgy * sgy);
+                        xo = (float)(((double)C + ss * nxd - 1.0) * 0.5);
+                        yo = (float)(((double)R + ss * nyd - 1.0) * 0.5);
+                    }
+                }
+            }
+        }
+        const unsigned long long bal = __ballot(keep);
+        if (R < IH && lane == 0) counts[J.counts_base + (long long)R * J.ntc + tc] = __popcll(bal);
+        if (bal) {
+            const int a = R & 1;
+            const int ne = __popcll(bal & even), no = __popcll(bal & ~even);
+            int be = 0, bo = 0;
+            if (lane == 0) {
+                be = atomicAdd(&sh_cnt[2 * a], ne);
+                bo = atomicAdd(&sh_cnt[2 * a + 1], no);
+            }
+            be = __shfl(be, 0);
+            bo = __shfl(bo, 0);
+            if (keep) {
+                const int b = lane & 1;
+                const unsigned long long par = b ? ~even : even;
+                const int idx = (b ? bo : be) + __popcll(bal & par & lt);
+                const int phase = 2 * a + b;
+                if (idx < J.stage_cap) {
+                    Staged st;
+                    st.key = (uint32_t)r | ((uint32_t)lane << 8) | ((uint32_t)__popcll(bal & lt) << 16);
+                    st.x = xo; st.y = yo; st.s = so;
+                    staging[J.stage_base + (tile_local * 4 + phase) * J.stage_cap + idx] = st;
+                }
+            }
+        }
+    }
+}
+
+template <typename IN_T, int R1, int TH>
+__device__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
+                          int *__restrict__ counts, Staged *__restrict__ staging, int *__restrict__ tile_cnt) {
+    using Gm = Geo<R1, TH>;
+    const long long tile_local = tile - J.tile_begin;
+    const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
+    const int i0 = tr * TH, j0 = tc * TW;
+    const TapSet *__restrict__ tp = J.taps;
+
+    if (threadIdx.x < 4) sh_cnt[threadIdx.x] = 0;
+    // stage the image tile with its halo; zero outside the image (cpu_toed.cpp:207-208)
+    {
+        float *img = lds + Gm::OFF_IMG;
+        const int gi0 = i0 - 1 - R1, gj0 = j0 - 1 - R1;
+        for (int e = threadIdx.x; e < Gm::NRI * Gm::NCI; e += NT) {
+            const int rr = e / Gm::NCI, cc = e % Gm::NCI;
+            const int gi = gi0 + rr, gj = gj0 + cc;
+            float v = 0.f;
+            if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
+            img[rr * Gm::LDI + cc] = v;
+        }
+    }
+    __syncthreads();
+    // round A: R0 window, unshifted -> phase (0,0)
+    row_pass<R1, TH, 0>(lds, tp->fu[0], tp->fu[1]);
+    __syncthreads();
+    col_pass<R1, TH, R1 - 1, 1>(lds, 0, tp->fu[0], tp->fu[1], 0, tp->fu[0], tp->fu[1], 0);
+    __syncthreads();
+    // round B: R1 window, unshifted in x -> phase (1,0) with shifted y taps
+    row_pass<R1, TH, 1>(lds, tp->fu[0], tp->fu[1]);
+    __syncthreads();
+    col_pass<R1, TH, R1, 1>(lds, 0, tp->fs[0], tp->fs[1], 1, tp->fs[0], tp->fs[1], 1);
+    __syncthreads();
+    // round C: shifted in x -> phases (0,1) and (1,1)
+    row_pass<R1, TH, 2>(lds, tp->fs[0], tp->fs[1]);
+    __syncthreads();
+    col_pass<R1, TH, R1, 2>(lds, 1, tp->fu[0], tp->fu[1], 0, tp->fs[0], tp->fs[1], 1);
+    __syncthreads();
+    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, staging, sh_cnt);
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int n = sh_cnt[threadIdx.x];
+        tile_cnt[(long long)tile * 4 + threadIdx.x] = n;
+        if (n > J.stage_cap && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_STAGING);
+    }
+}
+
+template <typename IN_T, int TH>
+__device__ inline void tile_dispatch(float *lds, int *sh_cnt, const ImageJob &J, int tile,
+                                     int *counts, Staged *staging, int *tile_cnt) {
+    switch (J.R1) {
+    case 5: tile_body<IN_T, 5, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
+    case 7: tile_body<IN_T, 7, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
+    case 9: tile_body<IN_T, 9, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
+    default: break;
+    }
+}
+
+// K0: per-image flag reset (a kernel, not a memset: safe inside captured graphs)
+__global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= njobs) return;
+    const ImageJob &J = jobs ? jobs[m] : job0;
+    if (J.flags_out) *J.flags_out = 0;
+}
+
+// K1: one workgroup per tile (plain form) or a persistent grid pulling tiles
+// from a device counter (ticket reset by K3's last workgroup).
+template <typename IN_T, int TH>
+__global__ __launch_bounds__(NT) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
+                                                        int *counts, Staged *staging, int *tile_cnt,
+                                                        int ntiles, int *ticket) {
+    __shared__ float lds[lds_floats<TH>()];
+    __shared__ int sh_cnt[4];
+    __shared__ int sh_tile;
+    if (ticket == nullptr) {
+        const int tile = blockIdx.x;
+        const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
+        tile_dispatch<IN_T, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        return;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) sh_tile = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int tile = sh_tile;
+        __syncthreads();
+        if (tile >= ntiles) break;
+        const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
+        tile_dispatch<IN_T, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        __syncthreads();
+    }
+}
+
+// K2a: exclusive scan of one CHUNK of counts; chunk totals to chunk_sum.
+__global__ __launch_bounds__(256) void toed_scan_chunks(const ImageJob *jobs, int njobs, ImageJob job0,
+                                                        const int *counts, int *offsets, int *chunk_sum) {
+    __shared__ int wsum[4];
+    const long long chunk = blockIdx.x;
+    const int m = jobs ? job_for_chunk(jobs, njobs, chunk) : 0;
+    const ImageJob &J = jobs ? jobs[m] : job0;
+    const long long nvalid = (long long)(2 * J.H) * J.ntc;
+    const long long local0 = (chunk - J.chunk_base) * CHUNK;   // entry index within the image
+    constexpr int PER = CHUNK / 256;
+    int v[PER];
+    int s = 0;
+    const long long e0 = local0 + threadIdx.x * PER;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const long long e = e0 + k;
+        v[k] = (e < nvalid) ? counts[J.counts_base + e] : 0;
+        s += v[k];
+    }
+    // wavefront inclusive scan of per-thread sums
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d);
+        if (lane >= d) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < wave; ++w) wbase += wsum[w];
+    int run = wbase + inc - s;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const long long e = e0 + k;
+        if (e < nvalid) offsets[J.counts_base + e] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) chunk_sum[chunk] = wbase + inc;
+}
+
+// K2b: one workgroup per image: exclusive scan of its chunk totals; N out.
+__global__ __launch_bounds__(1024) void toed_scan_images(const ImageJob *jobs, int njobs, ImageJob job0,
+                                                         int *chunk_sum) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const ImageJob &J = jobs ? jobs[blockIdx.x] : job0;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int base = 0; base < J.nchunks; base += 1024) {
+        const int k = base + threadIdx.x;
+        const int x = (k < J.nchunks) ? chunk_sum[J.chunk_base + k] : 0;
+        int inc = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int wbase = carry;
+        for (int w = 0; w < wave; ++w) wbase += wsum[w];
+        if (k < J.nchunks) chunk_sum[J.chunk_base + k] = wbase + inc - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = wbase + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int n = carry;
+        if (J.count_out) *J.count_out = n;
+        if (n > J.capacity && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_CAPACITY);
+    }
+}
+
+// FP64 nine responses at original pixel (i, j) for phase taps X (columns) and
+// Y (rows), window radius rad <= R1; zero padding (cpu_toed.cpp:207-208).
+template <typename IN_T, int R1>
+__device__ inline double to_theta_at(const ImageJob &J, int i, int j, int rad,
+                                     const double *__restrict__ X, const double *__restrict__ Y) {
+    constexpr int N = 2 * R1 + 1;
+    double fx = 0, fy = 0, fxx = 0, fxy = 0, fyy = 0, fxxy = 0, fxyy = 0, fxxx = 0, fyyy = 0;
+    const IN_T *img = (const IN_T *)J.img;
+#pragma unroll 1
+    for (int p = -rad; p <= rad; ++p) {
+        const int ii = i - p;
+        if (ii < 0 || ii >= J.H) continue;
+        const IN_T *row = img + (long long)ii * J.pitch;
+        double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+#pragma unroll
+        for (int q = -R1; q <= R1; ++q) {
+            if (q < -rad || q > rad) continue;
+            const int jj = j - q;
+            const double v = (jj >= 0 && jj < J.W) ? (double)row[jj] : 0.0;
+            h0 = fma(v, X[0 * MAXTAPS + q + R1], h0);
+            h1 = fma(v, X[1 * MAXTAPS + q + R1], h1);
+            h2 = fma(v, X[2 * MAXTAPS + q + R1], h2);
+            h3 = fma(v, X[3 * MAXTAPS + q + R1], h3);
+        }
+        const double y0 = Y[0 * MAXTAPS + p + R1], y1 = Y[1 * MAXTAPS + p + R1];
+        const double y2 = Y[2 * MAXTAPS + p + R1], y3 = Y[3 * MAXTAPS + p + R1];
+        fx = fma(h1, y0, fx);   fy = fma(h0, y1, fy);
+        fxx = fma(h2, y0, fxx); fxy = fma(h1, y1, fxy); fyy = fma(h0, y2, fyy);
+        fxxy = fma(h2, y1, fxxy); fxyy = fma(h1, y2, fxyy);
+        fxxx = fma(h3, y0, fxxx); fyyy = fma(h0, y3, fyyy);
+    }
+    (void)N;
+    // third-order direction (cpu_toed.cpp:227-232)
+    double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+              + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+    double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+              + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+    const double m = sqrt(tx * tx + ty * ty);
+    tx /= m;
+    ty /= m;
+    return atan2(tx, -ty);
+}
+
+template <typename IN_T, int R1, int TH>
+__device__ void finalize_body(const ImageJob &J, int tile, const int *offsets, const int *chunk_pref,
+                              const Staged *staging, const int *tile_cnt) {
+    const long long tile_local = tile - J.tile_begin;
+    const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
+    const TapSet *tp = J.taps;
+    for (int phase = 0; phase < 4; ++phase) {
+        const int a = phase >> 1, b = phase & 1;
+        const double *X = b ? &tp->ds[0][0] : &tp->du[0][0];
+        const double *Y = a ? &tp->ds[0][0] : &tp->du[0][0];
+        const int rad = phase == 0 ? J.R0 : J.R1;
+        const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
+        const Staged *st = staging + J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
+        for (int e = threadIdx.x; e < n; e += NT) {
+            const Staged rec = st[e];
+            const int r = rec.key & 0xff, c = (rec.key >> 8) & 0xff, rank = rec.key >> 16;
+            const int R = 2 * tr * TH + r, C = 2 * tc * TW + c;
+            const long long ce = J.counts_base + (long long)R * J.ntc + tc;
+            const long long idx = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
+            if (idx >= J.capacity) continue;
+            const double th = to_theta_at<IN_T, R1>(J, R >> 1, C >> 1, rad, X, Y);
+            float4 o;
+            o.x = rec.x; o.y = rec.y; o.z = (float)th; o.w = rec.s;
+            reinterpret_cast<float4 *>(J.out)[idx] = o;
+            if (J.index_out) J.index_out[idx] = R * (2 * J.W) + C;
+        }
+    }
+}
+
+template <typename IN_T, int TH>
+__global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
+                                                           const int *offsets, const int *chunk_pref,
+                                                           const Staged *staging, const int *tile_cnt,
+                                                           int ntiles, int *ticket) {
+    const int tile = blockIdx.x;
+    const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
+    switch (J.R1) {
+    case 5: finalize_body<IN_T, 5, TH>(J, tile, offsets, chunk_pref, staging, tile_cnt); break;
+    case 7: finalize_body<IN_T, 7, TH>(J, tile, offsets, chunk_pref, staging, tile_cnt); break;
+    case 9: finalize_body<IN_T, 9, TH>(J, tile, offsets, chunk_pref, staging, tile_cnt); break;
+    default: break;
+    }
+    // the persistent tile kernel's ticket is re-armed once every tile is final
+    if (ticket && tile == ntiles - 1 && threadIdx.x == 0) *ticket = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Unfused reference launch layer: interp planes and NMS maps
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void toed_planes_kernel(int H, int W, const float *img, float *Ix, float *Iy,
+                                                          float *mag, float *orient, const TapSet *tp) {
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long IW = 2LL * W;
+    if (o >= 4LL * H * W) return;
+    const int R = (int)(o / IW), C = (int)(o % IW);
+    const int i = R >> 1, j = C >> 1, a = R & 1, b = C & 1;
+    const int R1 = tp->R1, rad = (a | b) ? tp->R1 : tp->R0;
+    const float *X0 = b ? tp->fs[0] : tp->fu[0];
+    const float *X1 = b ? tp->fs[1] : tp->fu[1];
+    const float *Y0 = a ? tp->fs[0] : tp->fu[0];
+    const float *Y1 = a ? tp->fs[1] : tp->fu[1];
+    const double *XD = b ? &tp->ds[0][0] : &tp->du[0][0];
+    const double *YD = a ? &tp->ds[0][0] : &tp->du[0][0];
+    float fx = 0.f, fy = 0.f;
+    float g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = -rad; p <= rad; ++p) {
+        const int ii = i - p;
+        if (ii < 0 || ii >= H) continue;
+        float h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+        for (int q = -rad; q <= rad; ++q) {
+            const int jj = j - q;
+            if (jj < 0 || jj >= W) continue;
+            const float v = img[(long long)ii * W + jj];
+            h0 = fmaf(v, X0[q + R1], h0);
+            h1 = fmaf(v, X1[q + R1], h1);
+            h2 = fmaf(v, (float)XD[2 * MAXTAPS + q + R1], h2);
+            h3 = fmaf(v, (float)XD[3 * MAXTAPS + q + R1], h3);
+        }
+        fx = fmaf(h1, Y0[p + R1], fx);
+        fy = fmaf(h0, Y1[p + R1], fy);
+        const float y2 = (float)YD[2 * MAXTAPS + p + R1], y3 = (float)YD[3 * MAXTAPS + p + R1];
+        g[2] = fmaf(h2, Y0[p + R1], g[2]);
+        g[3] = fmaf(h1, Y1[p + R1], g[3]);
+        g[4] = fmaf(h0, y2, g[4]);
+        g[5] = fmaf(h2, Y1[p + R1], g[5]);
+        g[6] = fmaf(h1, y2, g[6]);
+        g[7] = fmaf(h3, Y0[p + R1], g[7]);
+        g[8] = fmaf(h0, y3, g[8]);
+    }
+    const float fxx = g[2], fxy = g[3], fyy = g[4], fxxy = g[5], fxyy = g[6], fxxx = g[7], fyyy = g[8];
+    float tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+             + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+    float ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+             + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+    const float m = sqrtf(tx * tx + ty * ty);
+    tx /= m;
+    ty /= m;
+    Ix[o] = fx;
+    Iy[o] = fy;
+    mag[o] = sqrtf(fx * fx + fy * fy);
+    orient[o] = atan2f(tx, -ty);
+}
+
+__global__ __launch_bounds__(256) void toed_nms_maps_kernel(int IH, int IW, const float *Ix, const float *Iy,
+                                                            const float *mag, float *px, float *py, float thresh,
+                                                            int border) {
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long long)IH * IW) return;
+    const int i = (int)(o / IW), j = (int)(o % IW);
+    float ox = 0.f, oy = 0.f;
+    const float f = mag[o];
+    if (i >= border && i < IH - border && j >= border && j < IW - border && f > thresh) {
+        const float gx = Ix[o], gy = Iy[o];
+        if (!(fabsf(gx) < 10e-6f && fabsf(gy) < 10e-6f)) {
+            auto Mf = [&](int dr, int dc) { return mag[o + (long long)dr * IW + dc]; };
+            const float nx = gx / f, ny = gy / f;
+            float sl, fp, fm;
+            if (octant<float>(gx, gy, nx, ny, Mf, sl, fp, fm) &&
+                ((f > fm && f > fp) || (f > fm && f >= fp) || (f >= fm && f > fp))) {
+                auto Md = [&](int dr, int dc) { return (double)mag[o + (long long)dr * IW + dc]; };
+                const double fd = f, nxd = (double)gx / fd, nyd = (double)gy / fd;
+                double sld, fpd, fmd;
+                octant<double>((double)gx, (double)gy, nxd, nyd, Md, sld, fpd, fmd);
+                const double s = sqrt(1.0 + sld * sld);
+                const double A = (fmd + fpd - 2.0 * fd) / (2.0 * s * s);
+                const double Bc = (fpd - fmd) / (2.0 * s);
+                const double ss = -Bc / (2.0 * A);
+                if (fabs(ss) <= 1.4142135623730951) {
+                    ox = (float)(j + ss * nxd);
+                    oy = (float)(i + ss * nyd);
+                }
+            }
+        }
+    }
+    px[o] = ox;
+    py[o] = oy;
+}
+
+}  // namespace toed
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace toed;
+
+struct toed_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int tile_rows = 16;
+    float stage_frac = 1.0f;
+    // tap sets by sigma
+    std::vector<float> tap_sigmas;
+    std::vector<TapSet *> tap_dev;
+    // single-image scratch (grow-only)
+    int *d_counts = nullptr, *d_offsets = nullptr, *d_chunks = nullptr, *d_tilecnt = nullptr;
+    Staged *d_staging = nullptr;
+    size_t n_counts = 0, n_offsets = 0, n_chunks = 0, n_tiles = 0, n_staging = 0;
+    int *d_scalar = nullptr;   // [0] count, [1] flags for *_sync
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    float ms_tiles = 0.f, ms_final = 0.f;
+};
+
+struct toed_batch {
+    toed_ctx *ctx = nullptr;
+    int n = 0;
+    std::vector<ImageJob> jobs;
+    ImageJob *d_jobs = nullptr;
+    int *d_counts = nullptr, *d_offsets = nullptr, *d_chunks = nullptr, *d_tilecnt = nullptr, *d_ticket = nullptr;
+    Staged *d_staging = nullptr;
+    long long ntiles = 0, ncounts = 0, nchunks = 0, nstaging = 0;
+    int in_type = 0, tile_rows = 16, persistent = 0;
+    int grid_persistent = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int *g_counts = nullptr, *g_flags = nullptr;
+};
+
+static std::mutex g_mu;
+
+#define HIPCHK(x)                                              \
+    do {                                                       \
+        hipError_t e_ = (x);                                   \
+        if (e_ != hipSuccess) {                                \
+            std::fprintf(stderr, "toed: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return TOED_ERR_HIP;                               \
+        }                                                      \
+    } while (0)
+
+static int radius_for(float sigma, int *R0, int *R1) {
+    if (!(sigma > 0.f) || !std::isfinite(sigma)) return TOED_ERR_SIGMA;
+    const int r0 = (int)std::ceil(4.0 * (double)sigma - 1e-9);
+    const int r1 = r0 + 1;
+    if (r1 != 5 && r1 != 7 && r1 != 9) return TOED_ERR_SIGMA;
+    *R0 = r0;
+    *R1 = r1;
+    return TOED_OK;
+}
+
+static void fill_taps(double sigma, int R0, int R1, TapSet *t) {
+    std::memset(t, 0, sizeof(*t));
+    t->R0 = R0;
+    t->R1 = R1;
+    const double s2 = sigma * sigma, norm = std::sqrt(2.0 * M_PI) * sigma;
+    for (int sh = 0; sh < 2; ++sh) {
+        for (int k = 0; k <= 2 * R1; ++k) {
+            const double x = (double)(k - R1) + (sh ? 0.5 : 0.0);
+            const double g = std::exp(-x * x / (2.0 * s2)) / norm;
+            const double v[4] = {g, -x * g / s2, (x * x - s2) * g / (s2 * s2), x * (3.0 * s2 - x * x) * g / (s2 * s2 * s2)};
+            for (int f = 0; f < 4; ++f) {
+                if (sh) t->ds[f][k] = v[f]; else t->du[f][k] = v[f];
+                if (f < 2) { if (sh) t->fs[f][k] = (float)v[f]; else t->fu[f][k] = (float)v[f]; }
+            }
+        }
+    }
+}
+
+static int get_taps(toed_ctx *c, float sigma, const TapSet **out) {
+    for (size_t k = 0; k < c->tap_sigmas.size(); ++k)
+        if (c->tap_sigmas[k] == sigma) { *out = c->tap_dev[k]; return TOED_OK; }
+    int R0, R1;
+    int st = radius_for(sigma, &R0, &R1);
+    if (st) return st;
+    TapSet h;
+    fill_taps(sigma, R0, R1, &h);
+    TapSet *d = nullptr;
+    if (hipMalloc(&d, sizeof(TapSet)) != hipSuccess) return TOED_ERR_ALLOC;
+    HIPCHK(hipMemcpy(d, &h, sizeof(TapSet), hipMemcpyHostToDevice));
+    c->tap_sigmas.push_back(sigma);
+    c->tap_dev.push_back(d);
+    *out = d;
+    return TOED_OK;
+}
+
+static int check_image(const toed_image_t *im) {
+    if (!im || !im->d_img || im->H <= 0 || im->W <= 0 || im->pitch < im->W) return TOED_ERR_ARG;
+    if (im->in_type != TOED_U8 && im->in_type != TOED_F32) return TOED_ERR_ARG;
+    if (im->border < 1 || im->capacity < 0 || (im->capacity > 0 && !im->d_edges)) return TOED_ERR_ARG;
+    if ((long long)im->H * im->W > (1LL << 28)) return TOED_ERR_ARG;   // 2H*2W must fit int32 indices
+    return TOED_OK;
+}
+
+// fills geometry fields of a job; returns number of tiles/count entries/chunks
+static void job_geometry(ImageJob &J, int TH, int stage_cap) {
+    J.ntr = ceil_div(J.H, TH);
+    J.ntc = ceil_div(J.W, TW);
+    const long long ncount = (long long)(2 * J.H) * J.ntc;
+    J.nchunks = (int)((ncount + CHUNK - 1) / CHUNK);
+    J.stage_cap = stage_cap;
+}
+
+static int stage_cap_for(float frac, int TH) {
+    int cap = (int)std::ceil((double)frac * TH * TW);
+    if (cap < 1) cap = 1;
+    if (cap > TH * TW) cap = TH * TW;
+    return cap;
+}
+
+template <typename F> static void for_tile_rows(int TH, F f) {
+    if (TH == 8) f(std::integral_constant<int, 8>());
+    else f(std::integral_constant<int, 16>());
+}
+
+// enqueue the four kernels for jobs (device table) or job0 (single image)
+static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, int njobs, const ImageJob &job0,
+                   long long ntiles, long long nchunks, int *counts, int *offsets, int *chunks, Staged *staging,
+                   int *tilecnt, int *ticket, int grid_persistent, hipEvent_t *ev) {
+    if (ntiles <= 0) return TOED_OK;
+    hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0);
+    if (ev) HIPCHK(hipEventRecord(ev[0], s));
+    for_tile_rows(TH, [&](auto THc) {
+        constexpr int T = decltype(THc)::value;
+        const dim3 grid(ticket ? grid_persistent : (unsigned)ntiles);
+        if (in_type == TOED_U8)
+            hipLaunchKernelGGL((toed_tiles_kernel<uint8_t, T>), grid, dim3(NT), 0, s, d_jobs, njobs, job0, counts,
+                               staging, tilecnt, (int)ntiles, ticket);
+        else
+            hipLaunchKernelGGL((toed_tiles_kernel<float, T>), grid, dim3(NT), 0, s, d_jobs, njobs, job0, counts,
+                               staging, tilecnt, (int)ntiles, ticket);
+    });
+    if (ev) HIPCHK(hipEventRecord(ev[1], s));
+    hipLaunchKernelGGL(toed_scan_chunks, dim3((unsigned)nchunks), dim3(256), 0, s, d_jobs, njobs, job0, counts,
+                       offsets, chunks);
+    hipLaunchKernelGGL(toed_scan_images, dim3(njobs), dim3(1024), 0, s, d_jobs, njobs, job0, chunks);
+    for_tile_rows(TH, [&](auto THc) {
+        constexpr int T = decltype(THc)::value;
+        if (in_type == TOED_U8)
+            hipLaunchKernelGGL((toed_finalize_kernel<uint8_t, T>), dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs,
+                               njobs, job0, offsets, chunks, staging, tilecnt, (int)ntiles, ticket);
+        else
+            hipLaunchKernelGGL((toed_finalize_kernel<float, T>), dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs,
+                               njobs, job0, offsets, chunks, staging, tilecnt, (int)ntiles, ticket);
+    });
+    if (ev) HIPCHK(hipEventRecord(ev[2], s));
+    HIPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+template <typename T> static int grow(T **p, size_t *have, size_t need) {
+    if (*have >= need && *p) return TOED_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    if (hipMalloc((void **)p, std::max<size_t>(need, 1) * sizeof(T)) != hipSuccess) { *p = nullptr; return TOED_ERR_ALLOC; }
+    *have = need;
+    return TOED_OK;
+}
+
+extern "C" {
+
+int toed_version(void) { return 100; }
+
+int toed_sigma_radius(float sigma, int *R0, int *R1) {
+    if (!R0 || !R1) return TOED_ERR_ARG;
+    return radius_for(sigma, R0, R1);
+}
+
+int toed_create(int device, toed_ctx **out) {
+    if (!out) return TOED_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(device));
+    toed_ctx *c = new toed_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TOED_ERR_HIP; }
+    c->stream = c->own_stream;
+    for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
+    if (hipMalloc(&c->d_scalar, 2 * sizeof(int)) != hipSuccess) { delete c; return TOED_ERR_ALLOC; }
+    *out = c;
+    return TOED_OK;
+}
+
+int toed_destroy(toed_ctx *c) {
+    if (!c) return TOED_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto *t : c->tap_dev) (void)hipFree(t);
+    (void)hipFree(c->d_counts); (void)hipFree(c->d_offsets); (void)hipFree(c->d_chunks);
+    (void)hipFree(c->d_tilecnt); (void)hipFree(c->d_staging); (void)hipFree(c->d_scalar);
+    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return TOED_OK;
+}
+
+int toed_set_stream(toed_ctx *c, void *s) {
+    if (!c) return TOED_ERR_ARG;
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return TOED_OK;
+}
+
+void *toed_get_stream(toed_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int toed_set_tile_rows(toed_ctx *c, int tr) {
+    if (!c) return TOED_ERR_ARG;
+    if (tr == 0) tr = 16;
+    if (tr != 8 && tr != 16) return TOED_ERR_ARG;
+    c->tile_rows = tr;
+    return TOED_OK;
+}
+
+int toed_set_staging_fraction(toed_ctx *c, float f) {
+    if (!c || !(f > 0.f) || f > 1.f) return TOED_ERR_ARG;
+    c->stage_frac = f;
+    return TOED_OK;
+}
+
+int toed_last_times(toed_ctx *c, float *a, float *b) {
+    if (!c) return TOED_ERR_ARG;
+    if (a) *a = c->ms_tiles;
+    if (b) *b = c->ms_final;
+    return TOED_OK;
+}
+
+static int detect_impl(toed_ctx *c, const toed_image_t *im, int *d_count, int *d_flags, bool timed) {
+    if (!c || !d_count) return TOED_ERR_ARG;
+    int st = check_image(im);
+    if (st) return st;
+    HIPCHK(hipSetDevice(c->device));
+    const TapSet *taps = nullptr;
+    if ((st = get_taps(c, im->sigma, &taps))) return st;
+    const int TH = c->tile_rows;
+    ImageJob J;
+    std::memset(&J, 0, sizeof J);
+    J.img = im->d_img; J.out = im->d_edges; J.count_out = d_count; J.flags_out = d_flags; J.taps = taps;
+    J.index_out = im->d_index;
+    J.capacity = im->capacity; J.in_type = im->in_type; J.H = im->H; J.W = im->W; J.pitch = im->pitch;
+    radius_for(im->sigma, &J.R0, &J.R1);
+    J.border = im->border; J.thresh = im->thresh;
+    job_geometry(J, TH, stage_cap_for(c->stage_frac, TH));
+    const long long ntiles = (long long)J.ntr * J.ntc;
+    const long long ncounts = (long long)J.nchunks * CHUNK;
+    if ((st = grow(&c->d_counts, &c->n_counts, (size_t)ncounts))) return st;
+    if ((st = grow(&c->d_offsets, &c->n_offsets, (size_t)ncounts))) return st;
+    if ((st = grow(&c->d_chunks, &c->n_chunks, (size_t)J.nchunks))) return st;
+    if ((st = grow(&c->d_tilecnt, &c->n_tiles, (size_t)ntiles * 4))) return st;
+    if ((st = grow(&c->d_staging, &c->n_staging, (size_t)ntiles * 4 * J.stage_cap))) return st;
+    J.counts_base = 0; J.chunk_base = 0; J.stage_base = 0; J.tile_begin = 0;
+    return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, ntiles, J.nchunks, c->d_counts, c->d_offsets,
+                   c->d_chunks, c->d_staging, c->d_tilecnt, nullptr, 0, timed ? c->ev : nullptr);
+}
+
+int toed_detect(toed_ctx *c, const toed_image_t *im, int *d_count, int *d_flags) {
+    return detect_impl(c, im, d_count, d_flags, false);
+}
+
+int toed_detect_sync(toed_ctx *c, const toed_image_t *im, long long *n_out) {
+    if (!c || !n_out) return TOED_ERR_ARG;
+    int st = detect_impl(c, im, c->d_scalar, c->d_scalar + 1, true);
+    if (st) return st;
+    int h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(h, c->d_scalar, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipEventElapsedTime(&c->ms_tiles, c->ev[0], c->ev[1]));
+    HIPCHK(hipEventElapsedTime(&c->ms_final, c->ev[1], c->ev[2]));
+    *n_out = h[0];
+    if (h[1] & TOED_FLAG_STAGING) return TOED_ERR_STAGING;
+    if (h[1] & TOED_FLAG_CAPACITY) return TOED_ERR_CAPACITY;
+    return TOED_OK;
+}
+
+int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **out) {
+    if (!c || n <= 0 || !imgs || !out) return TOED_ERR_ARG;
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    const int in_type = imgs[0].in_type;
+    const int TH = c->tile_rows;
+    const int scap = stage_cap_for(c->stage_frac, TH);
+    toed_batch *b = new toed_batch();
+    b->ctx = c; b->n = n; b->in_type = in_type; b->tile_rows = TH;
+    b->jobs.resize(n);
+    long long tiles = 0, chunks = 0;
+    for (int m = 0; m < n; ++m) {
+        const toed_image_t *im = &imgs[m];
+        int st = check_image(im);
+        if (!st && im->in_type != in_type) st = TOED_ERR_ARG;
+        const TapSet *taps = nullptr;
+        if (!st) st = get_taps(c, im->sigma, &taps);
+        if (st) { delete b; return st; }
+        ImageJob &J = b->jobs[m];
+        std::memset(&J, 0, sizeof J);
+        J.img = im->d_img; J.out = im->d_edges; J.taps = taps; J.capacity = im->capacity;
+        J.index_out = im->d_index;
+        J.in_type = in_type; J.H = im->H; J.W = im->W; J.pitch = im->pitch;
+        radius_for(im->sigma, &J.R0, &J.R1);
+        J.border = im->border; J.thresh = im->thresh;
+        job_geometry(J, TH, scap);
+        J.tile_begin = (int)tiles;
+        J.chunk_base = chunks;
+        J.counts_base = chunks * CHUNK;
+        J.stage_base = tiles * 4 * scap;
+        tiles += (long long)J.ntr * J.ntc;
+        chunks += J.nchunks;
+    }
+    if (tiles >= (1LL << 31)) { delete b; return TOED_ERR_ARG; }
+    b->ntiles = tiles; b->nchunks = chunks; b->ncounts = chunks * CHUNK; b->nstaging = tiles * 4 * scap;
+    bool ok = hipMalloc(&b->d_jobs, sizeof(ImageJob) * n) == hipSuccess &&
+              hipMalloc(&b->d_counts, sizeof(int) * b->ncounts) == hipSuccess &&
+              hipMalloc(&b->d_offsets, sizeof(int) * b->ncounts) == hipSuccess &&
+              hipMalloc(&b->d_chunks, sizeof(int) * std::max<long long>(chunks, 1)) == hipSuccess &&
+              hipMalloc(&b->d_tilecnt, sizeof(int) * tiles * 4) == hipSuccess &&
+              hipMalloc(&b->d_ticket, sizeof(int)) == hipSuccess &&
+              hipMalloc(&b->d_staging, sizeof(Staged) * std::max<long long>(b->nstaging, 1)) == hipSuccess;
+    if (!ok) { toed_batch_destroy(b); return TOED_ERR_ALLOC; }
+    if (hipMemset(b->d_ticket, 0, sizeof(int)) != hipSuccess) { toed_batch_destroy(b); return TOED_ERR_HIP; }
+    int dev_cu = 0;
+    (void)hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+    int occ = 0;
+    for_tile_rows(TH, [&](auto THc) {
+        constexpr int T = decltype(THc)::value;
+        if (in_type == TOED_U8)
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, toed_tiles_kernel<uint8_t, T>, NT, 0);
+        else
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, toed_tiles_kernel<float, T>, NT, 0);
+    });
+    b->grid_persistent = std::max(1, dev_cu * std::max(occ, 1));
+    *out = b;
+    return TOED_OK;
+}
+
+int toed_batch_set_persistent(toed_batch *b, int p) {
+    if (!b) return TOED_ERR_ARG;
+    b->persistent = p ? 1 : 0;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    return TOED_OK;
+}
+
+static int batch_upload(toed_batch *b, int *d_counts, int *d_flags) {
+    for (int m = 0; m < b->n; ++m) {
+        b->jobs[m].count_out = d_counts ? d_counts + m : nullptr;
+        b->jobs[m].flags_out = d_flags ? d_flags + m : nullptr;
+    }
+    HIPCHK(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(ImageJob) * b->n, hipMemcpyHostToDevice));
+    return TOED_OK;
+}
+
+static int batch_enqueue(toed_batch *b, hipStream_t s) {
+    ImageJob dummy;
+    std::memset(&dummy, 0, sizeof dummy);
+    return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->ntiles, b->nchunks, b->d_counts,
+                   b->d_offsets, b->d_chunks, b->d_staging, b->d_tilecnt, b->persistent ? b->d_ticket : nullptr,
+                   b->grid_persistent, nullptr);
+}
+
+int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags) {
+    if (!b) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (b->g_counts != d_counts || b->g_flags != d_flags || !b->gexec) {
+        HIPCHK(hipStreamSynchronize(b->ctx->stream));
+        int st = batch_upload(b, d_counts, d_flags);
+        if (st) return st;
+        if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+        b->g_counts = d_counts; b->g_flags = d_flags;
+    }
+    return batch_enqueue(b, b->ctx->stream);
+}
+
+int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags) {
+    if (!b) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    if (!b->gexec || b->g_counts != d_counts || b->g_flags != d_flags) {
+        HIPCHK(hipStreamSynchronize(s));
+        int st = batch_upload(b, d_counts, d_flags);
+        if (st) return st;
+        if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        st = batch_enqueue(b, s);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s, &g);
+        if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+        HIPCHK(e);
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+        b->g_counts = d_counts; b->g_flags = d_flags;
+    }
+    HIPCHK(hipGraphLaunch(b->gexec, s));
+    return TOED_OK;
+}
+
+int toed_batch_destroy(toed_batch *b) {
+    if (!b) return TOED_ERR_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    (void)hipFree(b->d_jobs); (void)hipFree(b->d_counts); (void)hipFree(b->d_offsets); (void)hipFree(b->d_chunks);
+    (void)hipFree(b->d_tilecnt); (void)hipFree(b->d_ticket); (void)hipFree(b->d_staging);
+    delete b;
+    return TOED_OK;
+}
+
+// ---- unfused launch layer --------------------------------------------------
+static TapSet *compat_taps(float sigma, int *st) {
+    static std::vector<std::pair<float, TapSet *>> cache;
+    std::lock_guard<std::mutex> lk(g_mu);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    for (auto &kv : cache) if (kv.first == sigma) return kv.second;   // single-device use
+    int R0, R1;
+    if ((*st = radius_for(sigma, &R0, &R1))) return nullptr;
+    TapSet h;
+    fill_taps(sigma, R0, R1, &h);
+    TapSet *d = nullptr;
+    if (hipMalloc(&d, sizeof(TapSet)) != hipSuccess) { *st = TOED_ERR_ALLOC; return nullptr; }
+    if (hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) { *st = TOED_ERR_HIP; return nullptr; }
+    cache.emplace_back(sigma, d);
+    return d;
+}
+
+int toed_gpu_convolve_f32(int device_id, int h, int w, int ih, int iw, const float *img, float *Ix, float *Iy,
+                          float *mag, float *orient, float sigma) {
+    if (h <= 0 || w <= 0 || ih != 2 * h || iw != 2 * w || !img || !Ix || !Iy || !mag || !orient) return TOED_ERR_ARG;
+    if ((long long)h * w > (1LL << 28)) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(device_id));
+    int st = 0;
+    TapSet *tp = compat_taps(sigma, &st);
+    if (!tp) return st;
+    const long long n = 4LL * h * w;
+    hipLaunchKernelGGL(toed_planes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, h, w, img, Ix,
+                       Iy, mag, orient, tp);
+    HIPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+int toed_gpu_nms_f32(int device_id, int ih, int iw, const float *Ix, const float *Iy, const float *mag, float *px,
+                     float *py, float thresh, int border) {
+    if (ih <= 0 || iw <= 0 || !Ix || !Iy || !mag || !px || !py || border < 1) return TOED_ERR_ARG;
+    if ((long long)ih * iw > (1LL << 30)) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(device_id));
+    const long long n = (long long)ih * iw;
+    hipLaunchKernelGGL(toed_nms_maps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, ih, iw, Ix,
+                       Iy, mag, px, py, thresh, border);
+    HIPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+}  // extern "C"
