@@ -74,9 +74,11 @@ int toed_sigma_radius(float sigma, int *R0, int *R1);
  * its own HIP stream.  Replaces the device half of gpu_toed.hpp:79-135. */
 int toed_create(int device, toed_ctx **out);
 int toed_destroy(toed_ctx *ctx);
-/* Use an external HIP stream (hipStream_t passed as void*); NULL restores the
- * context's own stream. */
+/* Launch on an external HIP stream (hipStream_t passed as void*); NULL selects
+ * the legacy default stream.  toed_use_own_stream switches back to the
+ * context's own (blocking) stream. */
 int toed_set_stream(toed_ctx *ctx, void *hip_stream);
+int toed_use_own_stream(toed_ctx *ctx);
 void *toed_get_stream(toed_ctx *ctx);
 /* Tile rows per workgroup (8 or 16); 0 = autotuned default. */
 int toed_set_tile_rows(toed_ctx *ctx, int tile_rows);

@@ -46,6 +46,7 @@ SIGNATURES = [
     ("toed_destroy", _I, [_VP]),
     ("toed_set_stream", _I, [_VP, _VP]),
     ("toed_get_stream", _VP, [_VP]),
+    ("toed_use_own_stream", _I, [_VP]),
     ("toed_set_tile_rows", _I, [_VP, _I]),
     ("toed_set_staging_fraction", _I, [_VP, _F]),
     ("toed_last_times", _I, [_VP, _PF, _PF]),

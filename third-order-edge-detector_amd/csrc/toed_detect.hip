@@ -870,7 +870,9 @@ int toed_create(int device, toed_ctx **out) {
     HIPCHK(hipSetDevice(device));
     toed_ctx *c = new toed_ctx();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TOED_ERR_HIP; }
+    // blocking stream: orders with the legacy default stream, so work launched
+    // here (e.g. graph replays) is ordered with a caller on the null stream
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault) != hipSuccess) { delete c; return TOED_ERR_HIP; }
     c->stream = c->own_stream;
     for (auto &e : c->ev) HIPCHK(hipEventCreate(&e));
     if (hipMalloc(&c->d_scalar, 2 * sizeof(int)) != hipSuccess) { delete c; return TOED_ERR_ALLOC; }
@@ -893,7 +895,13 @@ int toed_destroy(toed_ctx *c) {
 
 int toed_set_stream(toed_ctx *c, void *s) {
     if (!c) return TOED_ERR_ARG;
-    c->stream = s ? (hipStream_t)s : c->own_stream;
+    c->stream = (hipStream_t)s;
+    return TOED_OK;
+}
+
+int toed_use_own_stream(toed_ctx *c) {
+    if (!c) return TOED_ERR_ARG;
+    c->stream = c->own_stream;
     return TOED_OK;
 }
 
@@ -1068,7 +1076,9 @@ int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags) {
 int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags) {
     if (!b) return TOED_ERR_ARG;
     HIPCHK(hipSetDevice(b->ctx->device));
-    hipStream_t s = b->ctx->stream;
+    // the legacy default stream cannot be captured: capture and replay on the
+    // context's own (blocking) stream, which orders with the default stream
+    hipStream_t s = b->ctx->stream ? b->ctx->stream : b->ctx->own_stream;
     if (!b->gexec || b->g_counts != d_counts || b->g_flags != d_flags) {
         HIPCHK(hipStreamSynchronize(s));
         int st = batch_upload(b, d_counts, d_flags);
