@@ -80,9 +80,11 @@ struct ImageJob {
     int *index_out;            // optional interp index per edgel
 };
 
-struct Staged {
-    uint32_t key;     // local row | local col << 8 | rank-in-row << 16
-    float x, y, s;
+// Survivors of a tile, per phase: the finished edgel [x, y, theta, strength]
+// and its raster key (local row | local col << 8 | rank-in-row << 16).
+struct StageBuf {
+    float4 *rec;
+    uint32_t *key;
 };
 
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -122,13 +124,18 @@ template <int R1, int TH> struct Geo {
     static constexpr int NVR = TH + 2;                     // column-pass output rows
     static constexpr int RH = 2 * TH + 2, RW = 2 * TW + 2; // interp results incl. 1-px halo
     static constexpr int LDR = RW;
+    // LDS layout (floats): staged image | Ix, Iy, |grad| with 1-px halo |
+    // two row-pass planes, later reused for the survivor queue + FP64 taps
     static constexpr int OFF_IMG = 0;
-    static constexpr int OFF_P0 = OFF_IMG + NRI * LDI;
-    static constexpr int OFF_P1 = OFF_P0 + NRI * LDP;
-    static constexpr int OFF_IX = OFF_P1 + NRI * LDP;
+    static constexpr int OFF_IX = (NRI * LDI + 1) & ~1;
     static constexpr int OFF_IY = OFF_IX + RH * LDR;
     static constexpr int OFF_MG = OFF_IY + RH * LDR;
-    static constexpr int TOTAL = OFF_MG + RH * LDR;
+    static constexpr int OFF_P0 = (OFF_MG + RH * LDR + 1) & ~1;      // 8-byte aligned
+    static constexpr int OFF_P1 = OFF_P0 + NRI * LDP;
+    static constexpr int QMAX = 2 * TH * 2 * TW;                     // survivor queue (ints)
+    static constexpr int P_END = OFF_P1 + NRI * LDP;
+    static constexpr int Q_END = OFF_P0 + QMAX + 2 * 12 * (2 * R1 + 1);
+    static constexpr int TOTAL = P_END > Q_END ? P_END : Q_END;
     // row pass: items = (row, block of KH columns)
     static constexpr int KH = 2;
     static constexpr int NBH = NCP / KH;
@@ -137,7 +144,9 @@ template <int R1, int TH> struct Geo {
     static constexpr int NBV = (NVR + KV - 1) / KV;
 };
 
-template <int TH> constexpr int lds_floats() { return Geo<MAXR1, TH>::TOTAL; }
+template <int TH> constexpr int lds_floats() {
+    return std::max(Geo<5, TH>::TOTAL, std::max(Geo<7, TH>::TOTAL, Geo<9, TH>::TOTAL));
+}
 
 // Row pass over the staged image.  mode 0: R0-window taps (fresh);
 // mode 1: add the two end taps (R0 -> R1 window); mode 2: shifted taps (fresh).
@@ -256,7 +265,7 @@ __device__ inline int octant(T gx, T gy, T nx, T ny, MF M, T &slope, T &fp, T &f
 // step, survivors compacted into staging (cpu_toed.cpp:400-494).
 template <int R1, int TH>
 __device__ inline void nms_tile(float *lds, const ImageJob &J, int tr, int tc, long long tile_local,
-                                int *__restrict__ counts, Staged *__restrict__ staging, int *sh_cnt) {
+                                int *__restrict__ counts, StageBuf stg, int *sh_cnt, int *queue, int *qcnt) {
     using Gm = Geo<R1, TH>;
     const float *RX = lds + Gm::OFF_IX;
     const float *RY = lds + Gm::OFF_IY;
@@ -317,32 +326,108 @@ __device__ inline void nms_tile(float *lds, const ImageJob &J, int tr, int tc, l
             }
             be = __shfl(be, 0);
             bo = __shfl(bo, 0);
-            if (keep) {
-                const int b = lane & 1;
-                const unsigned long long par = b ? ~even : even;
-                const int idx = (b ? bo : be) + __popcll(bal & par & lt);
-                const int phase = 2 * a + b;
-                if (idx < J.stage_cap) {
-                    Staged st;
-                    st.key = (uint32_t)r | ((uint32_t)lane << 8) | ((uint32_t)__popcll(bal & lt) << 16);
-                    st.x = xo; st.y = yo; st.s = so;
-                    staging[J.stage_base + (tile_local * 4 + phase) * J.stage_cap + idx] = st;
-                }
+            const int b = lane & 1;
+            const unsigned long long par = b ? ~even : even;
+            const int idx = (b ? bo : be) + __popcll(bal & par & lt);
+            const int phase = 2 * a + b;
+            const bool stored = keep && idx < J.stage_cap;
+            const int slot = phase * J.stage_cap + idx;     // within the tile
+            if (stored) {
+                const long long g = J.stage_base + tile_local * 4 * J.stage_cap + slot;
+                stg.key[g] = (uint32_t)r | ((uint32_t)lane << 8) | ((uint32_t)__popcll(bal & lt) << 16);
+                stg.rec[g] = make_float4(xo, yo, 0.f, so);
             }
+            // queue the survivor for the orientation pass
+            const unsigned long long bq = __ballot(stored);
+            int qb = 0;
+            if (lane == 0 && bq) qb = atomicAdd(qcnt, __popcll(bq));
+            qb = __shfl(qb, 0);
+            if (stored) queue[qb + __popcll(bq & lt)] = r | (lane << 6) | (slot << 12);
+        }
+    }
+}
+
+// Third-order orientation of the tile's survivors (cpu_toed.cpp:227-232) in
+// FP64 from the LDS image tile: 16 lanes per edgel, each lane one or two rows
+// p of the (2R1+1)^2 window, row sums for G..G''' in x then the nine products
+// with the row's y taps, reduced across the 16 lanes.  Phase (0,0) uses the
+// R0 window through taps whose end values are zero.
+template <int R1, int TH>
+__device__ inline void theta_tile(float *lds, const ImageJob &J, long long tile_local, StageBuf stg,
+                                  const int *queue, int qn, const double *tapd) {
+    using Gm = Geo<R1, TH>;
+    constexpr int N = 2 * R1 + 1;
+    const float *img = lds + Gm::OFF_IMG;
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    for (int e0 = 0; e0 < qn; e0 += NT / 16) {
+        const int e = e0 + g;
+        const bool valid = e < qn;
+        const int q = valid ? queue[e] : 0;
+        const int r = q & 63, c = (q >> 6) & 63, slot = q >> 12;
+        const int a = r & 1, b = c & 1, phase = 2 * a + b;
+        // tap families: 0 unshifted, 1 unshifted with zero end taps, 2 shifted
+        const double *X = tapd + (b ? 2 : (phase == 0 ? 1 : 0)) * 4 * N;
+        const double *Y = tapd + (a ? 2 : (phase == 0 ? 1 : 0)) * 4 * N;
+        const int ci = (r >> 1) + 1 + R1, cj = (c >> 1) + 1 + R1;
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+        for (int k = 0; k < 2; ++k) {
+            const int p = -R1 + l + 16 * k;
+            if (valid && p <= R1) {
+                const float *row = img + (ci - p) * Gm::LDI + cj;
+                double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+#pragma unroll 1
+                for (int qq = -R1; qq <= R1; ++qq) {
+                    const double v = (double)row[-qq];
+                    h0 = fma(v, X[0 * N + qq + R1], h0);
+                    h1 = fma(v, X[1 * N + qq + R1], h1);
+                    h2 = fma(v, X[2 * N + qq + R1], h2);
+                    h3 = fma(v, X[3 * N + qq + R1], h3);
+                }
+                const double y0 = Y[0 * N + p + R1], y1 = Y[1 * N + p + R1];
+                const double y2 = Y[2 * N + p + R1], y3 = Y[3 * N + p + R1];
+                acc[0] = fma(h1, y0, acc[0]);   // fx
+                acc[1] = fma(h0, y1, acc[1]);   // fy
+                acc[2] = fma(h2, y0, acc[2]);   // fxx
+                acc[3] = fma(h1, y1, acc[3]);   // fxy
+                acc[4] = fma(h0, y2, acc[4]);   // fyy
+                acc[5] = fma(h2, y1, acc[5]);   // fxxy
+                acc[6] = fma(h1, y2, acc[6]);   // fxyy
+                acc[7] = fma(h3, y0, acc[7]);   // fxxx
+                acc[8] = fma(h0, y3, acc[8]);   // fyyy
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 9; ++d) {
+#pragma unroll
+            for (int m = 8; m >= 1; m >>= 1) acc[d] += __shfl_xor(acc[d], m, 16);
+        }
+        if (valid && l == 0) {
+            const double fx = acc[0], fy = acc[1], fxx = acc[2], fxy = acc[3], fyy = acc[4];
+            const double fxxy = acc[5], fxyy = acc[6], fxxx = acc[7], fyyy = acc[8];
+            double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+                      + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+            double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+                      + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+            const double m = sqrt(tx * tx + ty * ty);
+            tx /= m;
+            ty /= m;
+            const long long gi = J.stage_base + tile_local * 4 * J.stage_cap + slot;
+            stg.rec[gi].z = (float)atan2(tx, -ty);
         }
     }
 }
 
 template <typename IN_T, int R1, int TH>
 __device__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
-                          int *__restrict__ counts, Staged *__restrict__ staging, int *__restrict__ tile_cnt) {
+                          int *__restrict__ counts, StageBuf stg, int *__restrict__ tile_cnt) {
     using Gm = Geo<R1, TH>;
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
     const int i0 = tr * TH, j0 = tc * TW;
     const TapSet *__restrict__ tp = J.taps;
 
-    if (threadIdx.x < 4) sh_cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 5) sh_cnt[threadIdx.x] = 0;   // 4 phase counters + queue length
     // stage the image tile with its halo; zero outside the image (cpu_toed.cpp:207-208)
     {
         float *img = lds + Gm::OFF_IMG;
@@ -371,8 +456,22 @@ __device__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
     __syncthreads();
     col_pass<R1, TH, R1, 2>(lds, 1, tp->fu[0], tp->fu[1], 0, tp->fs[0], tp->fs[1], 1);
     __syncthreads();
-    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, staging, sh_cnt);
+    // the row-pass planes are dead now: the survivor queue and the FP64 taps
+    // of the orientation pass live in their place
+    int *queue = (int *)(lds + Gm::OFF_P0);
+    double *tapd = (double *)(lds + Gm::OFF_P0 + Gm::QMAX);
+    {
+        constexpr int N = 2 * R1 + 1;
+        for (int k = threadIdx.x; k < 12 * N; k += NT) {
+            const int fam = k / (4 * N), f = (k / N) & 3, t = k % N;
+            double v = (fam == 2) ? tp->ds[f][t] : tp->du[f][t];
+            if (fam == 1 && (t == 0 || t == N - 1)) v = 0.0;
+            tapd[k] = v;
+        }
+    }
+    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, queue, &sh_cnt[4]);
     __syncthreads();
+    theta_tile<R1, TH>(lds, J, tile_local, stg, queue, sh_cnt[4], tapd);
     if (threadIdx.x < 4) {
         const int n = sh_cnt[threadIdx.x];
         tile_cnt[(long long)tile * 4 + threadIdx.x] = n;
@@ -382,7 +481,7 @@ __device__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
 
 template <typename IN_T, int TH>
 __device__ inline void tile_dispatch(float *lds, int *sh_cnt, const ImageJob &J, int tile,
-                                     int *counts, Staged *staging, int *tile_cnt) {
+                                     int *counts, StageBuf staging, int *tile_cnt) {
     switch (J.R1) {
     case 5: tile_body<IN_T, 5, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
     case 7: tile_body<IN_T, 7, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
@@ -403,10 +502,10 @@ __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0
 // from a device counter (ticket reset by K3's last workgroup).
 template <typename IN_T, int TH>
 __global__ __launch_bounds__(NT) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
-                                                        int *counts, Staged *staging, int *tile_cnt,
+                                                        int *counts, StageBuf staging, int *tile_cnt,
                                                         int ntiles, int *ticket) {
     __shared__ float lds[lds_floats<TH>()];
-    __shared__ int sh_cnt[4];
+    __shared__ int sh_cnt[5];
     __shared__ int sh_tile;
     if (ticket == nullptr) {
         const int tile = blockIdx.x;
@@ -501,90 +600,30 @@ __global__ __launch_bounds__(1024) void toed_scan_images(const ImageJob *jobs, i
     }
 }
 
-// FP64 nine responses at original pixel (i, j) for phase taps X (columns) and
-// Y (rows), window radius rad <= R1; zero padding (cpu_toed.cpp:207-208).
-template <typename IN_T, int R1>
-__device__ inline double to_theta_at(const ImageJob &J, int i, int j, int rad,
-                                     const double *__restrict__ X, const double *__restrict__ Y) {
-    constexpr int N = 2 * R1 + 1;
-    double fx = 0, fy = 0, fxx = 0, fxy = 0, fyy = 0, fxxy = 0, fxyy = 0, fxxx = 0, fyyy = 0;
-    const IN_T *img = (const IN_T *)J.img;
-#pragma unroll 1
-    for (int p = -rad; p <= rad; ++p) {
-        const int ii = i - p;
-        if (ii < 0 || ii >= J.H) continue;
-        const IN_T *row = img + (long long)ii * J.pitch;
-        double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
-#pragma unroll
-        for (int q = -R1; q <= R1; ++q) {
-            if (q < -rad || q > rad) continue;
-            const int jj = j - q;
-            const double v = (jj >= 0 && jj < J.W) ? (double)row[jj] : 0.0;
-            h0 = fma(v, X[0 * MAXTAPS + q + R1], h0);
-            h1 = fma(v, X[1 * MAXTAPS + q + R1], h1);
-            h2 = fma(v, X[2 * MAXTAPS + q + R1], h2);
-            h3 = fma(v, X[3 * MAXTAPS + q + R1], h3);
-        }
-        const double y0 = Y[0 * MAXTAPS + p + R1], y1 = Y[1 * MAXTAPS + p + R1];
-        const double y2 = Y[2 * MAXTAPS + p + R1], y3 = Y[3 * MAXTAPS + p + R1];
-        fx = fma(h1, y0, fx);   fy = fma(h0, y1, fy);
-        fxx = fma(h2, y0, fxx); fxy = fma(h1, y1, fxy); fyy = fma(h0, y2, fyy);
-        fxxy = fma(h2, y1, fxxy); fxyy = fma(h1, y2, fxyy);
-        fxxx = fma(h3, y0, fxxx); fyyy = fma(h0, y3, fyyy);
-    }
-    (void)N;
-    // third-order direction (cpu_toed.cpp:227-232)
-    double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
-              + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
-    double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
-              + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
-    const double m = sqrt(tx * tx + ty * ty);
-    tx /= m;
-    ty /= m;
-    return atan2(tx, -ty);
-}
-
-template <typename IN_T, int R1, int TH>
-__device__ void finalize_body(const ImageJob &J, int tile, const int *offsets, const int *chunk_pref,
-                              const Staged *staging, const int *tile_cnt) {
+// K3: scatter every staged edgel to offset[row, tile column] + rank, the
+// reference's raster order of the interp grid (cpu_toed.cpp:510-538).
+__global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0, int TH,
+                                                           const int *offsets, const int *chunk_pref,
+                                                           StageBuf stg, const int *tile_cnt, int ntiles,
+                                                           int *ticket) {
+    const int tile = blockIdx.x;
+    const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
-    const TapSet *tp = J.taps;
+    const int IW = 2 * J.W;
     for (int phase = 0; phase < 4; ++phase) {
-        const int a = phase >> 1, b = phase & 1;
-        const double *X = b ? &tp->ds[0][0] : &tp->du[0][0];
-        const double *Y = a ? &tp->ds[0][0] : &tp->du[0][0];
-        const int rad = phase == 0 ? J.R0 : J.R1;
         const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
-        const Staged *st = staging + J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
+        const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
         for (int e = threadIdx.x; e < n; e += NT) {
-            const Staged rec = st[e];
-            const int r = rec.key & 0xff, c = (rec.key >> 8) & 0xff, rank = rec.key >> 16;
+            const uint32_t key = stg.key[base + e];
+            const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
             const int R = 2 * tr * TH + r, C = 2 * tc * TW + c;
             const long long ce = J.counts_base + (long long)R * J.ntc + tc;
             const long long idx = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
             if (idx >= J.capacity) continue;
-            const double th = to_theta_at<IN_T, R1>(J, R >> 1, C >> 1, rad, X, Y);
-            float4 o;
-            o.x = rec.x; o.y = rec.y; o.z = (float)th; o.w = rec.s;
-            reinterpret_cast<float4 *>(J.out)[idx] = o;
-            if (J.index_out) J.index_out[idx] = R * (2 * J.W) + C;
+            reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[base + e];
+            if (J.index_out) J.index_out[idx] = R * IW + C;
         }
-    }
-}
-
-template <typename IN_T, int TH>
-__global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
-                                                           const int *offsets, const int *chunk_pref,
-                                                           const Staged *staging, const int *tile_cnt,
-                                                           int ntiles, int *ticket) {
-    const int tile = blockIdx.x;
-    const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
-    switch (J.R1) {
-    case 5: finalize_body<IN_T, 5, TH>(J, tile, offsets, chunk_pref, staging, tile_cnt); break;
-    case 7: finalize_body<IN_T, 7, TH>(J, tile, offsets, chunk_pref, staging, tile_cnt); break;
-    case 9: finalize_body<IN_T, 9, TH>(J, tile, offsets, chunk_pref, staging, tile_cnt); break;
-    default: break;
     }
     // the persistent tile kernel's ticket is re-armed once every tile is final
     if (ticket && tile == ntiles - 1 && threadIdx.x == 0) *ticket = 0;
@@ -700,8 +739,9 @@ struct toed_ctx {
     std::vector<TapSet *> tap_dev;
     // single-image scratch (grow-only)
     int *d_counts = nullptr, *d_offsets = nullptr, *d_chunks = nullptr, *d_tilecnt = nullptr;
-    Staged *d_staging = nullptr;
-    size_t n_counts = 0, n_offsets = 0, n_chunks = 0, n_tiles = 0, n_staging = 0;
+    float4 *d_srec = nullptr;
+    uint32_t *d_skey = nullptr;
+    size_t n_counts = 0, n_offsets = 0, n_chunks = 0, n_tiles = 0, n_srec = 0, n_skey = 0;
     int *d_scalar = nullptr;   // [0] count, [1] flags for *_sync
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     float ms_tiles = 0.f, ms_final = 0.f;
@@ -713,7 +753,8 @@ struct toed_batch {
     std::vector<ImageJob> jobs;
     ImageJob *d_jobs = nullptr;
     int *d_counts = nullptr, *d_offsets = nullptr, *d_chunks = nullptr, *d_tilecnt = nullptr, *d_ticket = nullptr;
-    Staged *d_staging = nullptr;
+    float4 *d_srec = nullptr;
+    uint32_t *d_skey = nullptr;
     long long ntiles = 0, ncounts = 0, nchunks = 0, nstaging = 0;
     int in_type = 0, tile_rows = 16, persistent = 0;
     int grid_persistent = 0;
@@ -809,7 +850,7 @@ template <typename F> static void for_tile_rows(int TH, F f) {
 
 // enqueue the four kernels for jobs (device table) or job0 (single image)
 static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, int njobs, const ImageJob &job0,
-                   long long ntiles, long long nchunks, int *counts, int *offsets, int *chunks, Staged *staging,
+                   long long ntiles, long long nchunks, int *counts, int *offsets, int *chunks, StageBuf staging,
                    int *tilecnt, int *ticket, int grid_persistent, hipEvent_t *ev) {
     if (ntiles <= 0) return TOED_OK;
     hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0);
@@ -828,15 +869,8 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
     hipLaunchKernelGGL(toed_scan_chunks, dim3((unsigned)nchunks), dim3(256), 0, s, d_jobs, njobs, job0, counts,
                        offsets, chunks);
     hipLaunchKernelGGL(toed_scan_images, dim3(njobs), dim3(1024), 0, s, d_jobs, njobs, job0, chunks);
-    for_tile_rows(TH, [&](auto THc) {
-        constexpr int T = decltype(THc)::value;
-        if (in_type == TOED_U8)
-            hipLaunchKernelGGL((toed_finalize_kernel<uint8_t, T>), dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs,
-                               njobs, job0, offsets, chunks, staging, tilecnt, (int)ntiles, ticket);
-        else
-            hipLaunchKernelGGL((toed_finalize_kernel<float, T>), dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs,
-                               njobs, job0, offsets, chunks, staging, tilecnt, (int)ntiles, ticket);
-    });
+    hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs, njobs, job0, TH,
+                       offsets, chunks, staging, tilecnt, (int)ntiles, ticket);
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     HIPCHK(hipGetLastError());
     return TOED_OK;
@@ -886,7 +920,7 @@ int toed_destroy(toed_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto *t : c->tap_dev) (void)hipFree(t);
     (void)hipFree(c->d_counts); (void)hipFree(c->d_offsets); (void)hipFree(c->d_chunks);
-    (void)hipFree(c->d_tilecnt); (void)hipFree(c->d_staging); (void)hipFree(c->d_scalar);
+    (void)hipFree(c->d_tilecnt); (void)hipFree(c->d_srec); (void)hipFree(c->d_skey); (void)hipFree(c->d_scalar);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -950,10 +984,11 @@ static int detect_impl(toed_ctx *c, const toed_image_t *im, int *d_count, int *d
     if ((st = grow(&c->d_offsets, &c->n_offsets, (size_t)ncounts))) return st;
     if ((st = grow(&c->d_chunks, &c->n_chunks, (size_t)J.nchunks))) return st;
     if ((st = grow(&c->d_tilecnt, &c->n_tiles, (size_t)ntiles * 4))) return st;
-    if ((st = grow(&c->d_staging, &c->n_staging, (size_t)ntiles * 4 * J.stage_cap))) return st;
+    if ((st = grow(&c->d_srec, &c->n_srec, (size_t)ntiles * 4 * J.stage_cap))) return st;
+    if ((st = grow(&c->d_skey, &c->n_skey, (size_t)ntiles * 4 * J.stage_cap))) return st;
     J.counts_base = 0; J.chunk_base = 0; J.stage_base = 0; J.tile_begin = 0;
     return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, ntiles, J.nchunks, c->d_counts, c->d_offsets,
-                   c->d_chunks, c->d_staging, c->d_tilecnt, nullptr, 0, timed ? c->ev : nullptr);
+                   c->d_chunks, StageBuf{c->d_srec, c->d_skey}, c->d_tilecnt, nullptr, 0, timed ? c->ev : nullptr);
 }
 
 int toed_detect(toed_ctx *c, const toed_image_t *im, int *d_count, int *d_flags) {
@@ -1016,7 +1051,8 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
               hipMalloc(&b->d_chunks, sizeof(int) * std::max<long long>(chunks, 1)) == hipSuccess &&
               hipMalloc(&b->d_tilecnt, sizeof(int) * tiles * 4) == hipSuccess &&
               hipMalloc(&b->d_ticket, sizeof(int)) == hipSuccess &&
-              hipMalloc(&b->d_staging, sizeof(Staged) * std::max<long long>(b->nstaging, 1)) == hipSuccess;
+              hipMalloc(&b->d_srec, sizeof(float4) * std::max<long long>(b->nstaging, 1)) == hipSuccess &&
+              hipMalloc(&b->d_skey, sizeof(uint32_t) * std::max<long long>(b->nstaging, 1)) == hipSuccess;
     if (!ok) { toed_batch_destroy(b); return TOED_ERR_ALLOC; }
     if (hipMemset(b->d_ticket, 0, sizeof(int)) != hipSuccess) { toed_batch_destroy(b); return TOED_ERR_HIP; }
     int dev_cu = 0;
@@ -1055,7 +1091,8 @@ static int batch_enqueue(toed_batch *b, hipStream_t s) {
     ImageJob dummy;
     std::memset(&dummy, 0, sizeof dummy);
     return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->ntiles, b->nchunks, b->d_counts,
-                   b->d_offsets, b->d_chunks, b->d_staging, b->d_tilecnt, b->persistent ? b->d_ticket : nullptr,
+                   b->d_offsets, b->d_chunks, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
+                   b->persistent ? b->d_ticket : nullptr,
                    b->grid_persistent, nullptr);
 }
 
@@ -1106,7 +1143,7 @@ int toed_batch_destroy(toed_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     (void)hipFree(b->d_jobs); (void)hipFree(b->d_counts); (void)hipFree(b->d_offsets); (void)hipFree(b->d_chunks);
-    (void)hipFree(b->d_tilecnt); (void)hipFree(b->d_ticket); (void)hipFree(b->d_staging);
+    (void)hipFree(b->d_tilecnt); (void)hipFree(b->d_ticket); (void)hipFree(b->d_srec); (void)hipFree(b->d_skey);
     delete b;
     return TOED_OK;
 }
