@@ -38,6 +38,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "toed_capi.h"
@@ -49,6 +50,7 @@ constexpr int NT = 256;             // threads per tile workgroup (4 wavefronts)
 constexpr int MAXR1 = 9;            // largest supported R1 (sigma <= 2)
 constexpr int MAXTAPS = 2 * MAXR1 + 1;
 constexpr int CHUNK = 4096;         // scan chunk (entries)
+constexpr int NGROUPS = 3;          // supported window radii R1 = 5, 7, 9 (sigma 1, 1.5, 2)
 
 // Filter taps for one sigma.  Families: 0 G, 1 G', 2 G'', 3 G''' sampled at
 // x = k - R1 (unshifted) and x = k - R1 + 1/2 (shifted), k = 0..2R1
@@ -88,6 +90,30 @@ struct StageBuf {
 };
 
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Read-only tables (jobs, taps) are read through the constant address space so
+// that wavefront-uniform reads become scalar loads into SGPRs.
+#define TOED_CONST __attribute__((address_space(4)))
+template <typename T> __device__ __forceinline__ const TOED_CONST T *as_const(const T *p) {
+    return (const TOED_CONST T *)p;
+}
+
+// The job of a tile: the kernel-argument job for single images, else the table
+// entry found by binary search over tile_begin (uniform across the workgroup).
+__device__ __forceinline__ ImageJob load_job(const ImageJob *jobs, int njobs, const ImageJob &job0, int tile) {
+    if (!jobs) return job0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const TOED_CONST ImageJob *cj = as_const(jobs);
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cj[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+    }
+    return cj[lo];
+#else
+    return job0;
+#endif
+}
 
 __device__ inline const ImageJob &job_for_tile(const ImageJob *jobs, int njobs, int tile) {
     int lo = 0, hi = njobs - 1;
@@ -135,7 +161,9 @@ template <int R1, int TH> struct Geo {
     static constexpr int QMAX = 2 * TH * 2 * TW;                     // survivor queue (ints)
     static constexpr int P_END = OFF_P1 + NRI * LDP;
     static constexpr int Q_END = OFF_P0 + QMAX + 2 * 12 * (2 * R1 + 1);
-    static constexpr int TOTAL = P_END > Q_END ? P_END : Q_END;
+    static constexpr int OFF_CAND = ((P_END > Q_END ? P_END : Q_END) + 1) & ~1;   // near-peak queue (ints)
+    static constexpr int OFF_MASK = OFF_CAND + QMAX;                              // 2TH u64 row masks
+    static constexpr int TOTAL = OFF_MASK + 2 * (2 * TH);
     // row pass: items = (row, block of KH columns)
     static constexpr int KH = 2;
     static constexpr int NBH = NCP / KH;
@@ -144,14 +172,11 @@ template <int R1, int TH> struct Geo {
     static constexpr int NBV = (NVR + KV - 1) / KV;
 };
 
-template <int TH> constexpr int lds_floats() {
-    return std::max(Geo<5, TH>::TOTAL, std::max(Geo<7, TH>::TOTAL, Geo<9, TH>::TOTAL));
-}
 
 // Row pass over the staged image.  mode 0: R0-window taps (fresh);
 // mode 1: add the two end taps (R0 -> R1 window); mode 2: shifted taps (fresh).
 template <int R1, int TH, int MODE>
-__device__ inline void row_pass(float *lds, const float *__restrict__ t0, const float *__restrict__ t1) {
+__device__ __forceinline__ void row_pass(float *lds, const TOED_CONST float *t0, const TOED_CONST float *t1) {
     using Gm = Geo<R1, TH>;
     const float *img = lds + Gm::OFF_IMG;
     float *P0 = lds + Gm::OFF_P0;
@@ -192,9 +217,9 @@ __device__ inline void row_pass(float *lds, const float *__restrict__ t0, const 
 // current P planes: Ix = sum P1 * Y0, Iy = sum P0 * Y1; writes Ix, Iy, |grad|
 // into the interp result planes.  NPH = 1 or 2 phases from the same P planes.
 template <int R1, int TH, int R, int NPH>
-__device__ inline void col_pass(float *lds, int B,
-                                const float *__restrict__ ya0, const float *__restrict__ ya1, int A0,
-                                const float *__restrict__ yb0, const float *__restrict__ yb1, int A1) {
+__device__ __forceinline__ void col_pass(float *lds, int B,
+                                const TOED_CONST float *ya0, const TOED_CONST float *ya1, int A0,
+                                const TOED_CONST float *yb0, const TOED_CONST float *yb1, int A1) {
     using Gm = Geo<R1, TH>;
     const float *P0 = lds + Gm::OFF_P0;
     const float *P1 = lds + Gm::OFF_P1;
@@ -215,8 +240,8 @@ __device__ inline void col_pass(float *lds, int B,
         }
 #pragma unroll
         for (int ph = 0; ph < NPH; ++ph) {
-            const float *y0 = ph ? yb0 : ya0;
-            const float *y1 = ph ? yb1 : ya1;
+            const TOED_CONST float *y0 = ph ? yb0 : ya0;
+            const TOED_CONST float *y1 = ph ? yb1 : ya1;
             const int A = ph ? A1 : A0;
 #pragma unroll
             for (int d = 0; d < KV; ++d) {
@@ -242,7 +267,7 @@ __device__ inline void col_pass(float *lds, int B,
 // whose magnitude plane is reachable through M(dr, dc).  Returns 0 for a NaN
 // gradient (no branch applies).  T is float (selection) or double (values).
 template <typename T, typename MF>
-__device__ inline int octant(T gx, T gy, T nx, T ny, MF M, T &slope, T &fp, T &fm) {
+__device__ __forceinline__ int octant(T gx, T gy, T nx, T ny, MF M, T &slope, T &fp, T &fm) {
     if (gx >= 0 && gy >= 0) {
         if (gx >= gy) { slope = ny / nx; fp = M(0, 1) * (1 - slope) + M(1, 1) * slope;  fm = M(0, -1) * (1 - slope) + M(-1, -1) * slope; }
         else          { slope = nx / ny; fp = M(1, 0) * (1 - slope) + M(1, 1) * slope;  fm = M(-1, 0) * (1 - slope) + M(-1, -1) * slope; }
@@ -261,11 +286,22 @@ __device__ inline int octant(T gx, T gy, T nx, T ny, MF M, T &slope, T &fp, T &f
     return 1;
 }
 
-// NMS over the tile's 2TH x 64 interp pixels, one 64-wide row per wavefront
-// step, survivors compacted into staging (cpu_toed.cpp:400-494).
+// NMS over the tile's 2TH x 64 interp pixels (cpu_toed.cpp:400-494), in
+// three steps so that the FP64 work runs on compacted lanes:
+//  1. dense FP32 filter, one 64-wide row per wavefront step: threshold, then
+//     the peak test against the magnitudes interpolated one step along the
+//     gradient, with a 1e-5 relative slack and a fast reciprocal for the slope;
+//     near-peaks are queued (ballot compaction);
+//  2. every queued pixel re-decided exactly in FP64 on the FP32 magnitudes
+//     (octant rules, peak test, Fisher-Naidu parabola, |s*| <= sqrt 2);
+//     survivors take a staging slot of their phase, join the orientation
+//     queue and set their bit in the row's 64-bit mask;
+//  3. per-(row, tile) counts from the masks (the raster ranks are taken from
+//     the same masks by the orientation pass).
 template <int R1, int TH>
-__device__ inline void nms_tile(float *lds, const ImageJob &J, int tr, int tc, long long tile_local,
-                                int *__restrict__ counts, StageBuf stg, int *sh_cnt, int *queue, int *qcnt) {
+__device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, int tc, long long tile_local,
+                                int *__restrict__ counts, StageBuf stg, int *sh_cnt, int *cand, int *queue,
+                                unsigned long long *rowmask) {
     using Gm = Geo<R1, TH>;
     const float *RX = lds + Gm::OFF_IX;
     const float *RY = lds + Gm::OFF_IY;
@@ -274,76 +310,77 @@ __device__ inline void nms_tile(float *lds, const ImageJob &J, int tr, int tc, l
     const int wave = threadIdx.x >> 6;
     const int IH = 2 * J.H, IW = 2 * J.W;
     const int R0g = 2 * tr * TH, C0g = 2 * tc * TW;
-    const int C = C0g + lane;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const unsigned long long even = 0x5555555555555555ull;
-    const double SQ2 = 1.4142135623730951;
+    // step 1: FP32 filter
     for (int r = wave; r < 2 * TH; r += NT / 64) {
-        const int R = R0g + r;
-        const int cc = lane + 1, rr = r + 1;
-        bool keep = false;
-        float xo = 0.f, yo = 0.f, so = 0.f;
+        const int R = R0g + r, C = C0g + lane;
+        const int rr = r + 1, cc = lane + 1;
+        bool cnd = false;
         const bool inside = R >= J.border && R < IH - J.border && C >= J.border && C < IW - J.border;
         const float f = RM[rr * Gm::LDR + cc];
         if (inside && f > J.thresh) {
             const float gx = RX[rr * Gm::LDR + cc], gy = RY[rr * Gm::LDR + cc];
-            if (!(fabsf(gx) < 10e-6f && fabsf(gy) < 10e-6f)) {
-                auto Mf = [&](int dr, int dc) { return RM[(rr + dr) * Gm::LDR + cc + dc]; };
-                const float nx = gx / f, ny = gy / f;
-                float sl, fp, fm;
-                if (octant<float>(gx, gy, nx, ny, Mf, sl, fp, fm) &&
-                    ((f > fm && f > fp) || (f > fm && f >= fp) || (f >= fm && f > fp))) {
-                    // parabola in FP64 from the FP32 magnitudes (cpu_toed.cpp:465-489)
-                    auto Md = [&](int dr, int dc) { return (double)RM[(rr + dr) * Gm::LDR + cc + dc]; };
-                    const double fd = f, gxd = gx, gyd = gy;
-                    const double nxd = gxd / fd, nyd = gyd / fd;
-                    double sld, fpd, fmd;
-                    octant<double>(gxd, gyd, nxd, nyd, Md, sld, fpd, fmd);
-                    const double s = sqrt(1.0 + sld * sld);
-                    const double A = (fmd + fpd - 2.0 * fd) / (2.0 * s * s);
-                    const double Bc = (fpd - fmd) / (2.0 * s);
-                    const double ss = -Bc / (2.0 * A);
-                    const double maxf = A * ss * ss + Bc * ss + fd;
-                    if (fabs(ss) <= SQ2) {
-                        const double sgx = maxf * nxd, sgy = maxf * nyd;
-                        keep = true;
-                        so = (float)sqrt(sgx * sgx + sgy * sgy);
-                        xo = (float)(((double)C + ss * nxd - 1.0) * 0.5);
-                        yo = (float)(((double)R + ss * nyd - 1.0) * 0.5);
-                    }
-                }
+            const float ax = fabsf(gx), ay = fabsf(gy);
+            if (!(ax < 10e-6f && ay < 10e-6f)) {
+                // x-major iff |gx| >= |gy|; step signs from the gradient signs
+                const int sx = gx >= 0.f ? 1 : -1, sy = gy >= 0.f ? 1 : -1;
+                const bool xm = ax >= ay;
+                const float sl = (xm ? ay : ax) * __builtin_amdgcn_rcpf(xm ? ax : ay);
+                const int pr = xm ? 0 : sy, pc = xm ? sx : 0;        // primary neighbour
+                const float mp = RM[(rr + pr) * Gm::LDR + cc + pc], md = RM[(rr + sy) * Gm::LDR + cc + sx];
+                const float np = RM[(rr - pr) * Gm::LDR + cc - pc], nd = RM[(rr - sy) * Gm::LDR + cc - sx];
+                const float fp = mp + sl * (md - mp), fm = np + sl * (nd - np);
+                cnd = f >= fmaxf(fp, fm) * (1.0f - 1e-5f);
             }
         }
-        const unsigned long long bal = __ballot(keep);
-        if (R < IH && lane == 0) counts[J.counts_base + (long long)R * J.ntc + tc] = __popcll(bal);
+        const unsigned long long bal = __ballot(cnd);
         if (bal) {
-            const int a = R & 1;
-            const int ne = __popcll(bal & even), no = __popcll(bal & ~even);
-            int be = 0, bo = 0;
-            if (lane == 0) {
-                be = atomicAdd(&sh_cnt[2 * a], ne);
-                bo = atomicAdd(&sh_cnt[2 * a + 1], no);
-            }
-            be = __shfl(be, 0);
-            bo = __shfl(bo, 0);
-            const int b = lane & 1;
-            const unsigned long long par = b ? ~even : even;
-            const int idx = (b ? bo : be) + __popcll(bal & par & lt);
-            const int phase = 2 * a + b;
-            const bool stored = keep && idx < J.stage_cap;
-            const int slot = phase * J.stage_cap + idx;     // within the tile
-            if (stored) {
-                const long long g = J.stage_base + tile_local * 4 * J.stage_cap + slot;
-                stg.key[g] = (uint32_t)r | ((uint32_t)lane << 8) | ((uint32_t)__popcll(bal & lt) << 16);
-                stg.rec[g] = make_float4(xo, yo, 0.f, so);
-            }
-            // queue the survivor for the orientation pass
-            const unsigned long long bq = __ballot(stored);
-            int qb = 0;
-            if (lane == 0 && bq) qb = atomicAdd(qcnt, __popcll(bq));
-            qb = __shfl(qb, 0);
-            if (stored) queue[qb + __popcll(bq & lt)] = r | (lane << 6) | (slot << 12);
+            int b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&sh_cnt[5], __popcll(bal));
+            b0 = __shfl(b0, 0);
+            if (cnd) cand[b0 + __popcll(bal & lt)] = r | (lane << 6);
         }
+    }
+    __syncthreads();
+    // step 2: exact FP64 decision and parabola on the queued pixels
+    const int ncand = sh_cnt[5];
+    const double SQ2 = 1.4142135623730951;
+    for (int k = threadIdx.x; k < ncand; k += NT) {
+        const int q = cand[k];
+        const int r = q & 63, c = q >> 6;
+        const int rr = r + 1, cc = c + 1;
+        const double fd = RM[rr * Gm::LDR + cc];
+        const double gxd = RX[rr * Gm::LDR + cc], gyd = RY[rr * Gm::LDR + cc];
+        const double nxd = gxd / fd, nyd = gyd / fd;
+        auto Md = [&](int dr, int dc) { return (double)RM[(rr + dr) * Gm::LDR + cc + dc]; };
+        double sld, fpd, fmd;
+        if (!octant<double>(gxd, gyd, nxd, nyd, Md, sld, fpd, fmd)) continue;
+        if (!((fd > fmd && fd > fpd) || (fd > fmd && fd >= fpd) || (fd >= fmd && fd > fpd))) continue;
+        const double s = sqrt(1.0 + sld * sld);
+        const double A = (fmd + fpd - 2.0 * fd) / (2.0 * s * s);
+        const double Bc = (fpd - fmd) / (2.0 * s);
+        const double ss = -Bc / (2.0 * A);
+        const double maxf = A * ss * ss + Bc * ss + fd;
+        if (!(fabs(ss) <= SQ2)) continue;
+        const double sgx = maxf * nxd, sgy = maxf * nyd;
+        const int R = R0g + r, C = C0g + c;
+        const int phase = 2 * (r & 1) + (c & 1);
+        atomicOr(&rowmask[r], 1ull << c);
+        const int idx = atomicAdd(&sh_cnt[phase], 1);
+        if (idx < J.stage_cap) {
+            const int slot = phase * J.stage_cap + idx;
+            const long long g = J.stage_base + tile_local * 4 * J.stage_cap + slot;
+            stg.rec[g] = make_float4((float)(((double)C + ss * nxd - 1.0) * 0.5),
+                                     (float)(((double)R + ss * nyd - 1.0) * 0.5), 0.f,
+                                     (float)sqrt(sgx * sgx + sgy * sgy));
+            queue[atomicAdd(&sh_cnt[4], 1)] = r | (c << 6) | (slot << 12);
+        }
+    }
+    __syncthreads();
+    // step 3: survivors per (interp row, tile column)
+    for (int r = threadIdx.x; r < 2 * TH; r += NT) {
+        const int R = R0g + r;
+        if (R < IH) counts[J.counts_base + (long long)R * J.ntc + tc] = __popcll(rowmask[r]);
     }
 }
 
@@ -353,8 +390,8 @@ __device__ inline void nms_tile(float *lds, const ImageJob &J, int tr, int tc, l
 // with the row's y taps, reduced across the 16 lanes.  Phase (0,0) uses the
 // R0 window through taps whose end values are zero.
 template <int R1, int TH>
-__device__ inline void theta_tile(float *lds, const ImageJob &J, long long tile_local, StageBuf stg,
-                                  const int *queue, int qn, const double *tapd) {
+__device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long long tile_local, StageBuf stg,
+                                  const int *queue, int qn, const double *tapd, const unsigned long long *rowmask) {
     using Gm = Geo<R1, TH>;
     constexpr int N = 2 * R1 + 1;
     const float *img = lds + Gm::OFF_IMG;
@@ -414,30 +451,62 @@ __device__ inline void theta_tile(float *lds, const ImageJob &J, long long tile_
             ty /= m;
             const long long gi = J.stage_base + tile_local * 4 * J.stage_cap + slot;
             stg.rec[gi].z = (float)atan2(tx, -ty);
+            const int rank = __popcll(rowmask[r] & ((1ull << c) - 1ull));
+            stg.key[gi] = (uint32_t)r | ((uint32_t)c << 8) | ((uint32_t)rank << 16);
         }
     }
 }
 
 template <typename IN_T, int R1, int TH>
-__device__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
+__device__ __forceinline__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
                           int *__restrict__ counts, StageBuf stg, int *__restrict__ tile_cnt) {
     using Gm = Geo<R1, TH>;
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
     const int i0 = tr * TH, j0 = tc * TW;
-    const TapSet *__restrict__ tp = J.taps;
+    const TOED_CONST TapSet *tp = as_const(J.taps);
 
-    if (threadIdx.x < 5) sh_cnt[threadIdx.x] = 0;   // 4 phase counters + queue length
+    // 4 phase counters, orientation-queue length, near-peak count; row masks
+    if (threadIdx.x < 6) sh_cnt[threadIdx.x] = 0;
+    unsigned long long *rowmask = (unsigned long long *)(lds + Gm::OFF_MASK);
+    if (threadIdx.x < 2 * TH) rowmask[threadIdx.x] = 0ull;
     // stage the image tile with its halo; zero outside the image (cpu_toed.cpp:207-208)
     {
         float *img = lds + Gm::OFF_IMG;
         const int gi0 = i0 - 1 - R1, gj0 = j0 - 1 - R1;
-        for (int e = threadIdx.x; e < Gm::NRI * Gm::NCI; e += NT) {
-            const int rr = e / Gm::NCI, cc = e % Gm::NCI;
-            const int gi = gi0 + rr, gj = gj0 + cc;
-            float v = 0.f;
-            if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
-            img[rr * Gm::LDI + cc] = v;
+        if (std::is_same<IN_T, uint8_t>::value && (J.pitch & 3) == 0 && ((uintptr_t)J.img & 3) == 0) {
+            // aligned dwords of 4 pixels; partial dwords at the image edges byte by byte
+            const uint8_t *base = (const uint8_t *)J.img;
+            const int gc0 = gj0 & ~3;
+            const int nd = (gj0 + Gm::NCI - gc0 + 3) >> 2;
+            for (int e = threadIdx.x; e < Gm::NRI * nd; e += NT) {
+                const int rr = e / nd, k = e - (e / nd) * nd;
+                const int gi = gi0 + rr, col0 = gc0 + 4 * k;
+                uint32_t w = 0;
+                if (gi >= 0 && gi < J.H) {
+                    const uint8_t *row = base + (long long)gi * J.pitch;
+                    if (col0 >= 0 && col0 + 3 < J.W) {
+                        w = *(const uint32_t *)(row + col0);
+                    } else {
+#pragma unroll
+                        for (int b = 0; b < 4; ++b)
+                            if (col0 + b >= 0 && col0 + b < J.W) w |= (uint32_t)row[col0 + b] << (8 * b);
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int lc = col0 + b - gj0;
+                    if (lc >= 0 && lc < Gm::NCI) img[rr * Gm::LDI + lc] = (float)((w >> (8 * b)) & 255u);
+                }
+            }
+        } else {
+            for (int e = threadIdx.x; e < Gm::NRI * Gm::NCI; e += NT) {
+                const int rr = e / Gm::NCI, cc = e % Gm::NCI;
+                const int gi = gi0 + rr, gj = gj0 + cc;
+                float v = 0.f;
+                if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
+                img[rr * Gm::LDI + cc] = v;
+            }
         }
     }
     __syncthreads();
@@ -469,24 +538,12 @@ __device__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
             tapd[k] = v;
         }
     }
-    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, queue, &sh_cnt[4]);
-    __syncthreads();
-    theta_tile<R1, TH>(lds, J, tile_local, stg, queue, sh_cnt[4], tapd);
+    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
+    theta_tile<R1, TH>(lds, J, tile_local, stg, queue, sh_cnt[4], tapd, rowmask);
     if (threadIdx.x < 4) {
         const int n = sh_cnt[threadIdx.x];
         tile_cnt[(long long)tile * 4 + threadIdx.x] = n;
         if (n > J.stage_cap && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_STAGING);
-    }
-}
-
-template <typename IN_T, int TH>
-__device__ inline void tile_dispatch(float *lds, int *sh_cnt, const ImageJob &J, int tile,
-                                     int *counts, StageBuf staging, int *tile_cnt) {
-    switch (J.R1) {
-    case 5: tile_body<IN_T, 5, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
-    case 7: tile_body<IN_T, 7, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
-    case 9: tile_body<IN_T, 9, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt); break;
-    default: break;
     }
 }
 
@@ -499,28 +556,30 @@ __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0
 }
 
 // K1: one workgroup per tile (plain form) or a persistent grid pulling tiles
-// from a device counter (ticket reset by K3's last workgroup).
-template <typename IN_T, int TH>
+// from a device counter (re-armed by K3's last workgroup).  One
+// instantiation per window radius R1; a launch covers the tiles
+// [tile_begin, tile_end) of the images with that radius.
+template <typename IN_T, int TH, int R1>
 __global__ __launch_bounds__(NT) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
                                                         int *counts, StageBuf staging, int *tile_cnt,
-                                                        int ntiles, int *ticket) {
-    __shared__ float lds[lds_floats<TH>()];
-    __shared__ int sh_cnt[5];
+                                                        int tile_begin, int tile_end, int *ticket) {
+    __shared__ float lds[Geo<R1, TH>::TOTAL];
+    __shared__ int sh_cnt[6];
     __shared__ int sh_tile;
     if (ticket == nullptr) {
-        const int tile = blockIdx.x;
-        const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
-        tile_dispatch<IN_T, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        const int tile = tile_begin + blockIdx.x;
+        const ImageJob J = load_job(jobs, njobs, job0, tile);
+        tile_body<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
         return;
     }
     for (;;) {
-        if (threadIdx.x == 0) sh_tile = atomicAdd(ticket, 1);
+        if (threadIdx.x == 0) sh_tile = tile_begin + atomicAdd(ticket, 1);
         __syncthreads();
         const int tile = sh_tile;
         __syncthreads();
-        if (tile >= ntiles) break;
-        const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
-        tile_dispatch<IN_T, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        if (tile >= tile_end) break;
+        const ImageJob J = load_job(jobs, njobs, job0, tile);
+        tile_body<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
         __syncthreads();
     }
 }
@@ -607,7 +666,7 @@ __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs,
                                                            StageBuf stg, const int *tile_cnt, int ntiles,
                                                            int *ticket) {
     const int tile = blockIdx.x;
-    const ImageJob &J = jobs ? job_for_tile(jobs, njobs, tile) : job0;
+    const ImageJob J = load_job(jobs, njobs, job0, tile);
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
     const int IW = 2 * J.W;
@@ -625,8 +684,8 @@ __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs,
             if (J.index_out) J.index_out[idx] = R * IW + C;
         }
     }
-    // the persistent tile kernel's ticket is re-armed once every tile is final
-    if (ticket && tile == ntiles - 1 && threadIdx.x == 0) *ticket = 0;
+    // the persistent tile kernels' tickets are re-armed once every tile is final
+    if (ticket && tile == ntiles - 1 && threadIdx.x < NGROUPS) ticket[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -728,6 +787,11 @@ __global__ __launch_bounds__(256) void toed_nms_maps_kernel(int IH, int IW, cons
 // ===========================================================================
 using namespace toed;
 
+// tiles of the images with one window radius (contiguous in the tile order)
+struct TileGroup {
+    int R1 = 0, tile_begin = 0, tile_end = 0, grid_persistent = 0;
+};
+
 struct toed_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -757,7 +821,8 @@ struct toed_batch {
     uint32_t *d_skey = nullptr;
     long long ntiles = 0, ncounts = 0, nchunks = 0, nstaging = 0;
     int in_type = 0, tile_rows = 16, persistent = 0;
-    int grid_persistent = 0;
+    std::vector<int> order;
+    std::vector<TileGroup> groups;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int *g_counts = nullptr, *g_flags = nullptr;
@@ -848,29 +913,48 @@ template <typename F> static void for_tile_rows(int TH, F f) {
     else f(std::integral_constant<int, 16>());
 }
 
-// enqueue the four kernels for jobs (device table) or job0 (single image)
+template <typename F> static void for_radius(int R1, F f) {
+    if (R1 == 5) f(std::integral_constant<int, 5>());
+    else if (R1 == 7) f(std::integral_constant<int, 7>());
+    else f(std::integral_constant<int, 9>());
+}
+
+static int group_of(int R1) { return R1 == 5 ? 0 : (R1 == 7 ? 1 : 2); }
+
+
+template <typename IN_T, int T, int R>
+static const void *tiles_kernel_ptr() { return (const void *)toed_tiles_kernel<IN_T, T, R>; }
+
+// enqueue the kernels for jobs (device table) or job0 (single image)
 static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, int njobs, const ImageJob &job0,
-                   long long ntiles, long long nchunks, int *counts, int *offsets, int *chunks, StageBuf staging,
-                   int *tilecnt, int *ticket, int grid_persistent, hipEvent_t *ev) {
+                   const std::vector<TileGroup> &groups, long long ntiles, long long nchunks, int *counts,
+                   int *offsets, int *chunks, StageBuf staging, int *tilecnt, int *tickets, hipEvent_t *ev) {
     if (ntiles <= 0) return TOED_OK;
     hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
-    for_tile_rows(TH, [&](auto THc) {
-        constexpr int T = decltype(THc)::value;
-        const dim3 grid(ticket ? grid_persistent : (unsigned)ntiles);
-        if (in_type == TOED_U8)
-            hipLaunchKernelGGL((toed_tiles_kernel<uint8_t, T>), grid, dim3(NT), 0, s, d_jobs, njobs, job0, counts,
-                               staging, tilecnt, (int)ntiles, ticket);
-        else
-            hipLaunchKernelGGL((toed_tiles_kernel<float, T>), grid, dim3(NT), 0, s, d_jobs, njobs, job0, counts,
-                               staging, tilecnt, (int)ntiles, ticket);
-    });
+    for (const TileGroup &g : groups) {
+        if (g.tile_end <= g.tile_begin) continue;
+        int *ticket = tickets ? tickets + group_of(g.R1) : nullptr;
+        const dim3 grid(ticket ? (unsigned)g.grid_persistent : (unsigned)(g.tile_end - g.tile_begin));
+        for_tile_rows(TH, [&](auto THc) {
+            constexpr int T = decltype(THc)::value;
+            for_radius(g.R1, [&](auto Rc) {
+                constexpr int R = decltype(Rc)::value;
+                if (in_type == TOED_U8)
+                    hipLaunchKernelGGL((toed_tiles_kernel<uint8_t, T, R>), grid, dim3(NT), 0, s, d_jobs, njobs, job0,
+                                       counts, staging, tilecnt, g.tile_begin, g.tile_end, ticket);
+                else
+                    hipLaunchKernelGGL((toed_tiles_kernel<float, T, R>), grid, dim3(NT), 0, s, d_jobs, njobs, job0,
+                                       counts, staging, tilecnt, g.tile_begin, g.tile_end, ticket);
+            });
+        });
+    }
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(toed_scan_chunks, dim3((unsigned)nchunks), dim3(256), 0, s, d_jobs, njobs, job0, counts,
                        offsets, chunks);
     hipLaunchKernelGGL(toed_scan_images, dim3(njobs), dim3(1024), 0, s, d_jobs, njobs, job0, chunks);
     hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs, njobs, job0, TH,
-                       offsets, chunks, staging, tilecnt, (int)ntiles, ticket);
+                       offsets, chunks, staging, tilecnt, (int)ntiles, tickets);
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     HIPCHK(hipGetLastError());
     return TOED_OK;
@@ -987,8 +1071,11 @@ static int detect_impl(toed_ctx *c, const toed_image_t *im, int *d_count, int *d
     if ((st = grow(&c->d_srec, &c->n_srec, (size_t)ntiles * 4 * J.stage_cap))) return st;
     if ((st = grow(&c->d_skey, &c->n_skey, (size_t)ntiles * 4 * J.stage_cap))) return st;
     J.counts_base = 0; J.chunk_base = 0; J.stage_base = 0; J.tile_begin = 0;
-    return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, ntiles, J.nchunks, c->d_counts, c->d_offsets,
-                   c->d_chunks, StageBuf{c->d_srec, c->d_skey}, c->d_tilecnt, nullptr, 0, timed ? c->ev : nullptr);
+    TileGroup g;
+    g.R1 = J.R1; g.tile_begin = 0; g.tile_end = (int)ntiles;
+    return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, std::vector<TileGroup>{g}, ntiles, J.nchunks,
+                   c->d_counts, c->d_offsets, c->d_chunks, StageBuf{c->d_srec, c->d_skey}, c->d_tilecnt, nullptr,
+                   timed ? c->ev : nullptr);
 }
 
 int toed_detect(toed_ctx *c, const toed_image_t *im, int *d_count, int *d_flags) {
@@ -1020,15 +1107,27 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
     toed_batch *b = new toed_batch();
     b->ctx = c; b->n = n; b->in_type = in_type; b->tile_rows = TH;
     b->jobs.resize(n);
-    long long tiles = 0, chunks = 0;
+    // jobs ordered by window radius so each radius' tiles are contiguous and
+    // run in one specialised kernel; order[k] is the caller's image index
+    b->order.resize(n);
+    for (int m = 0; m < n; ++m) b->order[m] = m;
+    std::vector<int> r1s(n, 0);
     for (int m = 0; m < n; ++m) {
+        int r0, r1;
+        if (radius_for(imgs[m].sigma, &r0, &r1)) { delete b; return TOED_ERR_SIGMA; }
+        r1s[m] = r1;
+    }
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int x, int y) { return r1s[x] < r1s[y]; });
+    long long tiles = 0, chunks = 0;
+    for (int k = 0; k < n; ++k) {
+        const int m = b->order[k];
         const toed_image_t *im = &imgs[m];
         int st = check_image(im);
         if (!st && im->in_type != in_type) st = TOED_ERR_ARG;
         const TapSet *taps = nullptr;
         if (!st) st = get_taps(c, im->sigma, &taps);
         if (st) { delete b; return st; }
-        ImageJob &J = b->jobs[m];
+        ImageJob &J = b->jobs[k];
         std::memset(&J, 0, sizeof J);
         J.img = im->d_img; J.out = im->d_edges; J.taps = taps; J.capacity = im->capacity;
         J.index_out = im->d_index;
@@ -1042,6 +1141,13 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
         J.stage_base = tiles * 4 * scap;
         tiles += (long long)J.ntr * J.ntc;
         chunks += J.nchunks;
+        if (b->groups.empty() || b->groups.back().R1 != J.R1) {
+            TileGroup g;
+            g.R1 = J.R1;
+            g.tile_begin = J.tile_begin;
+            b->groups.push_back(g);
+        }
+        b->groups.back().tile_end = (int)tiles;
     }
     if (tiles >= (1LL << 31)) { delete b; return TOED_ERR_ARG; }
     b->ntiles = tiles; b->nchunks = chunks; b->ncounts = chunks * CHUNK; b->nstaging = tiles * 4 * scap;
@@ -1050,22 +1156,27 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
               hipMalloc(&b->d_offsets, sizeof(int) * b->ncounts) == hipSuccess &&
               hipMalloc(&b->d_chunks, sizeof(int) * std::max<long long>(chunks, 1)) == hipSuccess &&
               hipMalloc(&b->d_tilecnt, sizeof(int) * tiles * 4) == hipSuccess &&
-              hipMalloc(&b->d_ticket, sizeof(int)) == hipSuccess &&
+              hipMalloc(&b->d_ticket, sizeof(int) * NGROUPS) == hipSuccess &&
               hipMalloc(&b->d_srec, sizeof(float4) * std::max<long long>(b->nstaging, 1)) == hipSuccess &&
               hipMalloc(&b->d_skey, sizeof(uint32_t) * std::max<long long>(b->nstaging, 1)) == hipSuccess;
     if (!ok) { toed_batch_destroy(b); return TOED_ERR_ALLOC; }
-    if (hipMemset(b->d_ticket, 0, sizeof(int)) != hipSuccess) { toed_batch_destroy(b); return TOED_ERR_HIP; }
+    if (hipMemset(b->d_ticket, 0, sizeof(int) * NGROUPS) != hipSuccess) { toed_batch_destroy(b); return TOED_ERR_HIP; }
     int dev_cu = 0;
     (void)hipDeviceGetAttribute(&dev_cu, hipDeviceAttributeMultiprocessorCount, c->device);
-    int occ = 0;
-    for_tile_rows(TH, [&](auto THc) {
-        constexpr int T = decltype(THc)::value;
-        if (in_type == TOED_U8)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, toed_tiles_kernel<uint8_t, T>, NT, 0);
-        else
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, toed_tiles_kernel<float, T>, NT, 0);
-    });
-    b->grid_persistent = std::max(1, dev_cu * std::max(occ, 1));
+    for (TileGroup &g : b->groups) {
+        int occ = 0;
+        for_tile_rows(TH, [&](auto THc) {
+            constexpr int T = decltype(THc)::value;
+            for_radius(g.R1, [&](auto Rc) {
+                constexpr int R = decltype(Rc)::value;
+                if (in_type == TOED_U8)
+                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, toed_tiles_kernel<uint8_t, T, R>, NT, 0);
+                else
+                    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, toed_tiles_kernel<float, T, R>, NT, 0);
+            });
+        });
+        g.grid_persistent = std::max(1, std::min(dev_cu * std::max(occ, 1), g.tile_end - g.tile_begin));
+    }
     *out = b;
     return TOED_OK;
 }
@@ -1079,9 +1190,10 @@ int toed_batch_set_persistent(toed_batch *b, int p) {
 }
 
 static int batch_upload(toed_batch *b, int *d_counts, int *d_flags) {
-    for (int m = 0; m < b->n; ++m) {
-        b->jobs[m].count_out = d_counts ? d_counts + m : nullptr;
-        b->jobs[m].flags_out = d_flags ? d_flags + m : nullptr;
+    for (int k = 0; k < b->n; ++k) {
+        const int m = b->order[k];
+        b->jobs[k].count_out = d_counts ? d_counts + m : nullptr;
+        b->jobs[k].flags_out = d_flags ? d_flags + m : nullptr;
     }
     HIPCHK(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(ImageJob) * b->n, hipMemcpyHostToDevice));
     return TOED_OK;
@@ -1090,10 +1202,9 @@ static int batch_upload(toed_batch *b, int *d_counts, int *d_flags) {
 static int batch_enqueue(toed_batch *b, hipStream_t s) {
     ImageJob dummy;
     std::memset(&dummy, 0, sizeof dummy);
-    return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->ntiles, b->nchunks, b->d_counts,
-                   b->d_offsets, b->d_chunks, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
-                   b->persistent ? b->d_ticket : nullptr,
-                   b->grid_persistent, nullptr);
+    return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->groups, b->ntiles, b->nchunks,
+                   b->d_counts, b->d_offsets, b->d_chunks, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
+                   b->persistent ? b->d_ticket : nullptr, nullptr);
 }
 
 int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags) {
