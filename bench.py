@@ -118,6 +118,34 @@ def bench_batch(torch, pkg, scene, steps, warmup, n_images):
                 edgels=int(counts.sum()), sigmas=sigmas, shapes=[im.shape for im in imgs])
 
 
+def bench_fwd_bwd(torch, pkg, scene, steps, warmup, B=64, H=512, W=512, sigma=1.5):
+    """Differentiable form: B float32 scenes, loss sum(a x + b y + c theta + d s)
+    with (a, b, c, d) ~ N(0, 1) fixed from the seed; gradients w.r.t. every pixel
+    and sigma.  Reports forward-only and forward+backward throughput."""
+    imgs = np.stack([scene.shapes_scene(9000 + b, H, W) for b in range(B)]).astype(np.float32)
+    x = torch.from_numpy(imgs).cuda().requires_grad_(True)
+    sig = torch.tensor(sigma, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(scene.loss_weights(12345), dtype=torch.float32, device="cuda")
+    det = pkg.Detector(0, staging_fraction=0.5)
+    state = {}
+
+    def fwd():
+        with torch.no_grad():
+            e, off = pkg.detect_differentiable(x, sig, detector=det)
+        state["n"] = e.shape[0]
+
+    def fwd_bwd():
+        x.grad = None
+        sig.grad = None
+        e, off = pkg.detect_differentiable(x, sig, detector=det)
+        (e * w).sum().backward()
+
+    t_f = time_loop(torch, fwd, steps, warmup)
+    t_fb = time_loop(torch, fwd_bwd, steps, warmup)
+    det.close()
+    return dict(px=float(B * H * W), t_f=t_f, t_fb=t_fb, n=state["n"])
+
+
 def cpu_baseline(img_np):
     """The reference CPU path (cpu_toed.cpp, FP64, OpenMP) on one frame of the
     same workload, on this host's cores; the oracle port when the reference
@@ -146,7 +174,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="all", choices=["all", "plain", "batch"])
+    ap.add_argument("--workload", default="all", choices=["all", "plain", "batch", "fwdbwd"])
     ap.add_argument("--batch-images", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -206,6 +234,18 @@ def main():
             out["config"] = {"workload": "batched TOED: 200 mixed-size scenes, mixed sigma", "global_batch": 200}
         log("batch: graph %.1f Mpx/s, loop-of-plain %.1f Mpx/s" % (forms["batched_graph"]["mpx_s"],
                                                                     forms["batch_loop_of_plain"]["mpx_s"]))
+    if args.workload in ("all", "fwdbwd"):
+        log("forward+backward 64 x 512x512 ...")
+        fb = bench_fwd_bwd(torch, pkg, scene, max(3, args.steps // 4), 1)
+        forms["fwd_only_64x512x512"] = {"mpx_s": round(fb["px"] / fb["t_f"] / 1e6, 1), "ms": round(fb["t_f"] * 1e3, 3)}
+        forms["fwd_bwd_64x512x512"] = {"mpx_s": round(fb["px"] / fb["t_fb"] / 1e6, 1), "ms": round(fb["t_fb"] * 1e3, 3),
+                                       "edgels": fb["n"], "ratio_to_fwd": round(fb["t_fb"] / fb["t_f"], 3)}
+        if "value" not in out:
+            out["value"] = forms["fwd_bwd_64x512x512"]["mpx_s"]
+            out["ms_per_step"] = round(fb["t_fb"] * 1e3, 3)
+            out["config"] = {"workload": "differentiable TOED fwd+bwd, 64 x 512x512 f32, sigma 1.5", "global_batch": 64}
+        log("fwd+bwd: %.1f Mpx/s (fwd %.1f), ratio %.2f" % (forms["fwd_bwd_64x512x512"]["mpx_s"],
+                                                            forms["fwd_only_64x512x512"]["mpx_s"], fb["t_fb"] / fb["t_f"]))
     out["forms"] = forms
     if plain is not None and not args.no_cpu:
         log("cpu baseline ...")

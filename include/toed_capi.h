@@ -80,6 +80,7 @@ int toed_destroy(toed_ctx *ctx);
 int toed_set_stream(toed_ctx *ctx, void *hip_stream);
 int toed_use_own_stream(toed_ctx *ctx);
 void *toed_get_stream(toed_ctx *ctx);
+int toed_get_device(toed_ctx *ctx);
 /* Tile rows per workgroup (8 or 16); 0 = autotuned default. */
 int toed_set_tile_rows(toed_ctx *ctx, int tile_rows);
 /* Staging records per tile and phase as a fraction of the phase's pixels
@@ -106,6 +107,41 @@ int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags);
 /* Capture the launch sequence once in a hipGraph and replay it. */
 int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags);
 int toed_batch_destroy(toed_batch *b);
+
+/* Differentiable form -------------------------------------------------------
+ * No reference counterpart (the reference has no differentiation).  Adjoint
+ * of a forward detection with its edgel set frozen: for each image, the
+ * forward's interp indices and count, and the upstream gradient of every
+ * output row dL/d[x, y, orientation, strength]; accumulates (+=) dL/dimage
+ * into d_gimg and dL/dsigma into *d_gsigma (window radii held fixed).
+ * Everything is recomputed from the image in FP64 (no stored planes). */
+typedef struct {
+    int in_type;             /* TOED_U8 or TOED_F32, as in the forward */
+    const void *d_img;
+    int H, W, pitch;
+    float sigma;
+    const int *d_index;      /* forward's d_index (interp index per edgel) */
+    const int *d_count;      /* forward's edgel count (device int) */
+    int max_edgels;          /* launch bound, >= the count */
+    const float *d_gout;     /* max_edgels x 4 upstream gradient rows */
+    float *d_gimg;           /* H rows of gpitch floats, accumulated */
+    int gpitch;
+    double *d_gsigma;        /* accumulated, may be NULL */
+} toed_grad_t;
+int toed_backward(toed_ctx *ctx, int n, const toed_grad_t *items);
+
+/* The same adjoint for the last forward run of a batch plan, tile by tile from
+ * the plan's own staging (no index list; LDS-accumulated gradient patches).
+ * outs[m] is image m's: upstream rows (its d_edges layout), dL/dimage
+ * (accumulated), dL/dsigma (accumulated, may be NULL).  Valid until the plan
+ * runs again.  No reference counterpart. */
+typedef struct {
+    const float *d_gout;
+    float *d_gimg;
+    int gpitch;
+    double *d_gsigma;
+} toed_grad_out_t;
+int toed_batch_backward(toed_batch *b, const toed_grad_out_t *outs);
 
 /* Reference launch layer (unfused, for drop-in and debugging) ---------------
  * toed_gpu_convolve_f32: gpu_kernels.hpp:24-33.  Writes the four interp planes

@@ -46,6 +46,13 @@ def lib():
         L.toed_oracle_detect.restype = ctypes.c_long
         L.toed_oracle_to_theta.argtypes = [ctypes.c_double] * 9
         L.toed_oracle_to_theta.restype = ctypes.c_double
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.toed_oracle_taps_frozen.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.toed_oracle_frozen_loss.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, _i64p, ctypes.c_long, _dp, _ip, _dp]
+        L.toed_oracle_frozen_loss.restype = ctypes.c_double
+        L.toed_oracle_frozen_grad.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, _i64p, ctypes.c_long, _dp, _ip, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -154,3 +161,59 @@ def run_ref(img_u8, nthreads=1):
             tc, tn = np.frombuffer(f.read(16), np.float64)
             edges = np.frombuffer(f.read(n * 32), np.float64).reshape(n, 4).copy()
     return edges, float(tc), float(tn)
+
+
+# ---- differentiable form (toed_grad_oracle.c) ------------------------------
+
+def taps_frozen(sigma, R0, R1):
+    """(taps[8, 2R1+1], d taps / d sigma) at sigma with frozen window radii."""
+    n = 2 * R1 + 1
+    t = np.zeros((8, n), np.float64)
+    dt = np.zeros((8, n), np.float64)
+    if lib().toed_oracle_taps_frozen(float(sigma), int(R0), int(R1), _p(t), _p(dt)):
+        raise ValueError("bad taps request")
+    return t, dt
+
+
+def _weights(w, n):
+    w = np.asarray(w, np.float64)
+    if w.ndim == 1:
+        w = np.tile(w.reshape(1, 4), (n, 1))
+    return np.ascontiguousarray(w.reshape(n, 4))
+
+
+def frozen_loss(img, sigma, eidx, w, R0=None, R1=None, octants=None, outputs=False):
+    """L = sum_e w_e . [x, y, theta, s] on the frozen edgel set eidx (interp
+    indices) with frozen octants (decided here when None) and frozen window
+    radii.  Returns (L, octants[, outputs])."""
+    img = np.ascontiguousarray(img, np.float64)
+    H, W = img.shape
+    if R0 is None:
+        R0, R1 = radius(sigma)
+    eidx = np.ascontiguousarray(eidx, np.int64)
+    n = len(eidx)
+    oc = np.zeros(n, np.int32) if octants is None else np.ascontiguousarray(octants, np.int32).copy()
+    outs = np.zeros((n, 4), np.float64)
+    L = lib().toed_oracle_frozen_loss(_p(img), H, W, float(sigma), int(R0), int(R1), _p(eidx, _i64p), n,
+                                      _p(_weights(w, n)), oc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      _p(outs))
+    return (L, oc, outs) if outputs else (L, oc)
+
+
+def frozen_grad(img, sigma, eidx, w, octants, R0=None, R1=None):
+    """(dL/dimg, dL/dsigma) of frozen_loss by the independent route of
+    toed_grad_oracle.c (local central differences + transposed filters)."""
+    img = np.ascontiguousarray(img, np.float64)
+    H, W = img.shape
+    if R0 is None:
+        R0, R1 = radius(sigma)
+    eidx = np.ascontiguousarray(eidx, np.int64)
+    n = len(eidx)
+    g = np.zeros((H, W), np.float64)
+    gs = ctypes.c_double()
+    oc = np.ascontiguousarray(octants, np.int32)
+    if lib().toed_oracle_frozen_grad(_p(img), H, W, float(sigma), int(R0), int(R1), _p(eidx, _i64p), n,
+                                     _p(_weights(w, n)), oc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(g),
+                                     ctypes.byref(gs)):
+        raise ValueError("frozen_grad failed")
+    return g, gs.value

@@ -1,0 +1,107 @@
+"""Differentiable form on the GPU: gradients of the edgel loss
+sum(a x + b y + c theta + d strength) w.r.t. the pixels and sigma, with the
+edgel set frozen, against (1) FP64 central finite differences of the oracle's
+frozen-set loss and (2) the oracle's independent FP64 gradient."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _forward_backward(pkg, imgs_np, sigma, weights):
+    imgs = torch.from_numpy(np.stack(imgs_np).astype(np.float32)).cuda().requires_grad_(True)
+    sig = torch.tensor(float(sigma), dtype=torch.float64, requires_grad=True)
+    edges, offsets = pkg.detect_differentiable(imgs, sig)
+    w = torch.tensor(weights, dtype=torch.float32, device="cuda")
+    loss = (edges * w).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    return edges.detach().cpu().numpy(), offsets.numpy(), imgs.grad.cpu().numpy(), float(sig.grad)
+
+
+@pytest.mark.parametrize("sigma", [1.0, 1.5, 2.0])
+def test_grad_matches_oracle_and_fd(pkg, detector, scene, oracle, sigma):
+    rng = np.random.default_rng(7)
+    w = rng.normal(size=4)
+    imgs = [scene.shapes_scene(60 + k, 96, 128) for k in range(2)]
+    edges, off, g, gs = _forward_backward(pkg, imgs, sigma, w)
+    gs_ref = 0.0
+    for b, im in enumerate(imgs):
+        n = off[b + 1] - off[b]
+        assert n > 50
+        # frozen set = the GPU's edgels (interp indices from the forward)
+        det = oracle.detect(im.astype(np.float64), sigma)
+        x = edges[off[b]:off[b + 1]]
+        # match GPU rows to oracle rows by position (the sets are identical here)
+        eidx = det["eidx"]
+        assert len(eidx) == n and np.abs(det["edges"][:, :2] - x[:, :2]).max() < 1e-3
+        L, oc = oracle.frozen_loss(im.astype(np.float64), sigma, eidx, w)
+        g_ref, s_ref = oracle.frozen_grad(im.astype(np.float64), sigma, eidx, w, oc)
+        gs_ref += s_ref
+        scale = np.abs(g_ref).max()
+        err = np.abs(g[b] - g_ref).max() / scale
+        print("sigma %g image %d: %d edgels, max |g - g_oracle| / max|g| = %.2e" % (sigma, b, n, err))
+        assert err < 1e-4
+        # central finite differences on 64 pixels, step 1e-2 grey levels
+        cand = np.flatnonzero(np.abs(g_ref) > 1e-2 * scale)
+        pix = np.random.default_rng(b).choice(cand, size=min(64, len(cand)), replace=False)
+        base = im.astype(np.float64)
+        h = 1e-2
+        for k in pix:
+            i, j = divmod(int(k), im.shape[1])
+            p, m = base.copy(), base.copy()
+            p[i, j] += h
+            m[i, j] -= h
+            fd = (oracle.frozen_loss(p, sigma, eidx, w, octants=oc)[0] -
+                  oracle.frozen_loss(m, sigma, eidx, w, octants=oc)[0]) / (2 * h)
+            assert abs(fd - g[b][i, j]) <= 1e-4 * scale + 1e-4 * abs(fd), (i, j, fd, g[b][i, j])
+    # sigma: FD with the window radii frozen, step 1e-4
+    R0, R1 = oracle.radius(sigma)
+    fd_s = 0.0
+    for b, im in enumerate(imgs):
+        det = oracle.detect(im.astype(np.float64), sigma)
+        L, oc = oracle.frozen_loss(im.astype(np.float64), sigma, det["eidx"], w)
+        fd_s += (oracle.frozen_loss(im.astype(np.float64), sigma + 1e-4, det["eidx"], w, R0, R1, octants=oc)[0] -
+                 oracle.frozen_loss(im.astype(np.float64), sigma - 1e-4, det["eidx"], w, R0, R1, octants=oc)[0]) / 2e-4
+    print("sigma %g: dL/dsigma gpu %.6f oracle %.6f fd %.6f" % (sigma, gs, gs_ref, fd_s))
+    assert gs == pytest.approx(gs_ref, rel=1e-4)
+    assert gs == pytest.approx(fd_s, rel=1e-4)
+
+
+def test_grad_linear_in_upstream(pkg, detector, scene):
+    imgs = [scene.shapes_scene(70, 80, 100)]
+    w = np.array([0.3, -1.2, 0.7, 2.0])
+    _, _, g1, s1 = _forward_backward(pkg, imgs, 1.5, w)
+    _, _, g2, s2 = _forward_backward(pkg, imgs, 1.5, 3 * w)
+    np.testing.assert_allclose(g2, 3 * g1, rtol=1e-4, atol=1e-6 * np.abs(g1).max())
+    assert s2 == pytest.approx(3 * s1, rel=1e-5)
+
+
+def test_grad_zero_without_edgels(pkg, detector):
+    imgs = [np.full((64, 64), 100, np.uint8)]
+    e, off, g, gs = _forward_backward(pkg, imgs, 2.0, np.ones(4))
+    assert len(e) == 0 and np.all(g == 0) and gs == 0.0
+
+
+def test_tile_and_index_backward_agree(pkg, detector, scene):
+    """The tile-driven backward (plan staging, LDS patches) and the
+    index-driven one (used when the plan has run again since the forward)
+    give the same gradients."""
+    x = torch.from_numpy(np.stack([scene.shapes_scene(80 + k, 120, 150) for k in range(3)]).astype(np.float32))
+    x = x.cuda().requires_grad_(True)
+    sig = torch.tensor(2.0, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor([0.4, -0.8, 1.1, 0.6], device="cuda")
+    e1, _ = pkg.detect_differentiable(x, sig)
+    (e1 * w).sum().backward()
+    g_tile, s_tile = x.grad.clone(), float(sig.grad)
+    x.grad = None
+    sig.grad = None
+    e2, _ = pkg.detect_differentiable(x, sig)
+    e3, _ = pkg.detect_differentiable(x, sig)     # the plan runs again: e2's backward takes the index path
+    (e2 * w).sum().backward()
+    g_idx, s_idx = x.grad.clone(), float(sig.grad)
+    assert torch.equal(e1, e2)
+    scale = g_tile.abs().max().item()
+    assert (g_tile - g_idx).abs().max().item() <= 1e-5 * scale
+    assert s_tile == pytest.approx(s_idx, rel=1e-9)

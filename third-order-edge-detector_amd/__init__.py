@@ -10,9 +10,12 @@ in the package's own HIP kernels.
                                 sequence (optionally a persistent tile kernel and a
                                 captured hipGraph)
     ThirdOrderEdgeDetectionGPU  mirror of the reference class (gpu_toed.hpp:21-74)
+    detect_differentiable       torch.autograd.Function: gradients of the edgel
+                                rows w.r.t. the pixels and sigma (edgel set frozen)
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -20,7 +23,7 @@ from . import _lib
 from ._lib import FLAG_CAPACITY, FLAG_STAGING, TOED_F32, TOED_U8, ToedError, check, lib
 
 __all__ = ["Detector", "BatchDetector", "ThirdOrderEdgeDetectionGPU", "ToedError", "sigma_radius",
-           "DEFAULT_SIGMA", "DEFAULT_THRESH", "DEFAULT_BORDER"]
+           "detect_differentiable", "DEFAULT_SIGMA", "DEFAULT_THRESH", "DEFAULT_BORDER"]
 
 # reference constants: sigma = 2 / 17-tap window (main_gpu_cpu.cpp:158-159),
 # threshold 2 (cpu_toed.cpp:403), NMS border 10 (cpu_toed.cpp:400-401)
@@ -81,10 +84,12 @@ class Detector:
         if tile_rows:
             check(lib().toed_set_tile_rows(self._ctx, int(tile_rows)), "toed_set_tile_rows")
         check(lib().toed_set_staging_fraction(self._ctx, float(staging_fraction)), "toed_set_staging_fraction")
-        self._scalars = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self._batches = weakref.WeakSet()   # plans made on this context, destroyed before it
 
     def close(self):
         if getattr(self, "_ctx", None):
+            for b in list(self._batches):
+                b.close()
             lib().toed_destroy(self._ctx)
             self._ctx = None
 
@@ -182,6 +187,7 @@ class BatchDetector:
         self._b = ctypes.c_void_p()
         detector._bind_stream()
         check(lib().toed_batch_plan(detector._ctx, n, arr, ctypes.byref(self._b)), "toed_batch_plan")
+        detector._batches.add(self)
         if persistent:
             check(lib().toed_batch_set_persistent(self._b, 1), "toed_batch_set_persistent")
 
@@ -203,6 +209,21 @@ class BatchDetector:
             o = int(self.offsets[m])
             out.append(self.edges[o:o + int(counts[m])])
         return out
+
+    def backward(self, gouts, gimgs, gsigma=None):
+        """Adjoint of the last run (edgel set frozen): gouts[m] holds image m's
+        dL/d[x, y, orientation, strength] rows in its output layout, gimgs[m]
+        receives dL/dimage (accumulated), gsigma (float64 device tensor, one
+        element, optional) dL/dsigma summed over the images."""
+        n = len(self.images)
+        arr = (_lib.toed_grad_out_t * n)()
+        for m in range(n):
+            arr[m].d_gout = gouts[m].data_ptr()
+            arr[m].d_gimg = gimgs[m].data_ptr()
+            arr[m].gpitch = int(gimgs[m].stride(0))
+            arr[m].d_gsigma = gsigma.data_ptr() if gsigma is not None else None
+        self.det._bind_stream()
+        check(lib().toed_batch_backward(self._b, arr), "toed_batch_backward")
 
     def index_results(self):
         counts = self.counts.cpu().numpy()
@@ -314,3 +335,10 @@ class ThirdOrderEdgeDetectionGPU:
         flat = rd_data.reshape(-1)
         k = min(flat.size, vals.size, first_dim * second_dim)
         flat[:k] = vals[:k]
+
+
+def detect_differentiable(images, sigma, detector=None, thresh=DEFAULT_THRESH, border=DEFAULT_BORDER, capacity=None):
+    """Differentiable batched detection: returns (edges [N, 4], offsets [B+1]);
+    see the autograd module."""
+    from .autograd import detect_differentiable as _dd
+    return _dd(images, sigma, detector, thresh, border, capacity)

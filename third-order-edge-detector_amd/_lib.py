@@ -36,6 +36,31 @@ class toed_image_t(ctypes.Structure):
     ]
 
 
+class toed_grad_t(ctypes.Structure):
+    _fields_ = [
+        ("in_type", ctypes.c_int),
+        ("d_img", ctypes.c_void_p),
+        ("H", ctypes.c_int), ("W", ctypes.c_int), ("pitch", ctypes.c_int),
+        ("sigma", ctypes.c_float),
+        ("d_index", ctypes.c_void_p),
+        ("d_count", ctypes.c_void_p),
+        ("max_edgels", ctypes.c_int),
+        ("d_gout", ctypes.c_void_p),
+        ("d_gimg", ctypes.c_void_p),
+        ("gpitch", ctypes.c_int),
+        ("d_gsigma", ctypes.c_void_p),
+    ]
+
+
+class toed_grad_out_t(ctypes.Structure):
+    _fields_ = [
+        ("d_gout", ctypes.c_void_p),
+        ("d_gimg", ctypes.c_void_p),
+        ("gpitch", ctypes.c_int),
+        ("d_gsigma", ctypes.c_void_p),
+    ]
+
+
 # exported symbols and their signatures: (name, restype, argtypes)
 _VP, _I, _F, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
 _PI, _PF = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
@@ -47,6 +72,9 @@ SIGNATURES = [
     ("toed_set_stream", _I, [_VP, _VP]),
     ("toed_get_stream", _VP, [_VP]),
     ("toed_use_own_stream", _I, [_VP]),
+    ("toed_get_device", _I, [_VP]),
+    ("toed_backward", _I, [_VP, _I, ctypes.POINTER(toed_grad_t)]),
+    ("toed_batch_backward", _I, [_VP, ctypes.POINTER(toed_grad_out_t)]),
     ("toed_set_tile_rows", _I, [_VP, _I]),
     ("toed_set_staging_fraction", _I, [_VP, _F]),
     ("toed_last_times", _I, [_VP, _PF, _PF]),

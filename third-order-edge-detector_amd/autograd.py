@@ -1,0 +1,162 @@
+"""Differentiable TOED: a torch.autograd.Function over the C ABI.
+
+    edges, offsets = detect_differentiable(images, sigma)
+
+``images`` is a float32 [B, H, W] device tensor, ``sigma`` a float or a 0-dim
+tensor (which may require grad).  The forward is the batched FP32 detection
+(one plan over the B images); ``edges`` is the concatenation of the B
+raster-ordered N_b x 4 lists and ``offsets`` (int64, B+1, on the host) marks
+where each image's rows start.  The backward holds the edgel set fixed (each
+edgel keeps its interp pixel, and its octant is re-decided from the FP64
+gradient, which only differs at exact ties) and back-propagates
+dL/d[x, y, orientation, strength] to every pixel and to sigma (window radii
+held fixed) through toed_backward: FP64 recomputation and hand-written adjoint
+per edgel, transposed filters scattered into dL/dimage.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class _Plan:
+    """A cached forward plan for one (images tensor, sigma) pair."""
+
+    def __init__(self, det, images, sigma, thresh, border, cap):
+        from . import BatchDetector
+        B = images.shape[0]
+        self.images = images
+        self.gen = 0    # forward runs of this plan; the tile backward needs the latest
+        self.bd = BatchDetector(det, [images[b] for b in range(B)], [sigma] * B, thresh, border,
+                                capacities=[cap] * B, with_index=True)
+
+
+_PLANS = {}
+
+
+def _plan_for(det, images, sigma, thresh, border, cap):
+    key = (id(det), images.data_ptr(), tuple(images.shape), tuple(images.stride()), float(sigma), float(thresh),
+           int(border), int(cap))
+    p = _PLANS.get(key)
+    if p is None or p.images is not images:
+        if len(_PLANS) > 16:
+            _PLANS.clear()
+        p = _Plan(det, images, sigma, thresh, border, cap)
+        _PLANS[key] = p
+    return p
+
+
+def _make_function():
+    torch = _torch()
+
+    class _Detect(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, images, sigma_t, det, thresh, border, cap):
+            sigma = float(sigma_t)
+            plan = _plan_for(det, images, sigma, thresh, border, cap)
+            bd = plan.bd
+            bd.run()
+            plan.gen += 1
+            ctx.gen = plan.gen
+            counts = bd.counts.cpu().numpy().astype(np.int64)
+            flags = bd.flags.cpu().numpy()
+            if flags.any():
+                raise _lib.ToedError(6 if (flags & _lib.FLAG_STAGING).any() else 5, "differentiable forward")
+            offsets = np.concatenate([[0], np.cumsum(counts)])
+            parts = [bd.edges[int(bd.offsets[b]):int(bd.offsets[b]) + int(counts[b])] for b in range(len(counts))]
+            out = torch.cat(parts, 0) if parts else bd.edges[:0]
+            # compact copies, so a later forward on the same plan cannot change this backward
+            iparts = [bd.index[int(bd.offsets[b]):int(bd.offsets[b]) + int(counts[b])] for b in range(len(counts))]
+            ctx.index = torch.cat(iparts, 0) if iparts else bd.index[:0].clone()
+            ctx.counts_dev = bd.counts.clone()
+            ctx.plan = plan
+            ctx.counts = counts
+            ctx.offsets = offsets
+            ctx.sigma = sigma
+            ctx.det = det
+            ctx.sigma_shape = sigma_t.shape if torch.is_tensor(sigma_t) else None
+            ctx.sigma_dtype = sigma_t.dtype if torch.is_tensor(sigma_t) else None
+            off_t = torch.from_numpy(offsets)
+            ctx.mark_non_differentiable(off_t)
+            return out, off_t
+
+        @staticmethod
+        def backward(ctx, g_out, g_offsets):
+            plan, counts, offsets = ctx.plan, ctx.counts, ctx.offsets
+            images = plan.images
+            B, H, W = images.shape
+            g_out = g_out.contiguous().float()
+            gimg = torch.zeros_like(images, dtype=torch.float32)
+            gsig = torch.zeros(1, dtype=torch.float64, device=images.device)
+            if plan.gen == ctx.gen:
+                # the plan's staging still holds this forward's edgels: tile-driven backward
+                rows = [g_out[int(offsets[b]):] if counts[b] else g_out for b in range(B)]
+                plan.bd.backward([r if r.numel() else torch.zeros(4, device=images.device) for r in rows],
+                                 [gimg[b] for b in range(B)], gsig)
+                gs = None
+                if ctx.sigma_shape is not None:
+                    gs = gsig.reshape(()).to(ctx.sigma_dtype).reshape(ctx.sigma_shape)
+                return gimg, gs, None, None, None, None
+            arr = (_lib.toed_grad_t * B)()
+            for b in range(B):
+                it = arr[b]
+                it.in_type = _lib.TOED_F32
+                it.d_img = images[b].data_ptr()
+                it.H, it.W, it.pitch = H, W, int(images.stride(1))
+                it.sigma = ctx.sigma
+                it.d_index = ctx.index[int(offsets[b]):].data_ptr() if counts[b] else ctx.counts_dev.data_ptr()
+                it.d_count = ctx.counts_dev[b:].data_ptr()
+                it.max_edgels = int(counts[b])
+                it.d_gout = g_out[int(offsets[b]):].data_ptr() if counts[b] else g_out.data_ptr()
+                it.d_gimg = gimg[b].data_ptr()
+                it.gpitch = W
+                it.d_gsigma = gsig.data_ptr()
+            ctx.det._bind_stream()
+            check(lib().toed_backward(ctx.det._ctx, B, arr), "toed_backward")
+            gs = None
+            if ctx.sigma_shape is not None:
+                gs = gsig.reshape(()).to(ctx.sigma_dtype).reshape(ctx.sigma_shape)
+            return gimg, gs, None, None, None, None
+
+    return _Detect
+
+
+_FN = None
+
+
+def detect_differentiable(images, sigma, detector=None, thresh=2.0, border=10, capacity=None):
+    """Differentiable batched detection; see the module docstring."""
+    global _FN
+    torch = _torch()
+    if _FN is None:
+        _FN = _make_function()
+    if images.dim() == 2:
+        images = images.unsqueeze(0)
+    if images.dtype != torch.float32 or not images.is_cuda or images.dim() != 3:
+        raise TypeError("images must be a float32 [B, H, W] device tensor")
+    if images.stride(2) != 1:
+        images = images.contiguous()
+    det = detector or _default_detector(images.device)
+    B, H, W = images.shape
+    cap = int(capacity) if capacity else max(1024, H * W)
+    sig = sigma if torch.is_tensor(sigma) else torch.tensor(float(sigma), dtype=torch.float64)
+    return _FN.apply(images, sig, det, float(thresh), int(border), cap)
+
+
+_DET = {}
+
+
+def _default_detector(device):
+    from . import Detector
+    key = device.index or 0
+    if key not in _DET:
+        _DET[key] = Detector(key, staging_fraction=0.5)
+    return _DET[key]
+

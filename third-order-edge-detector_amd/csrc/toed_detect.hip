@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "toed_capi.h"
+#include "toed_adjoint.hpp"
 
 namespace toed {
 
@@ -61,6 +62,16 @@ struct TapSet {
     float fs[2][MAXTAPS];
     double du[4][MAXTAPS];
     double ds[4][MAXTAPS];
+    double dsu[4][MAXTAPS];     // d/dsigma of du (window radii held fixed)
+    double dss[4][MAXTAPS];     // d/dsigma of ds
+};
+
+// Per-image outputs of a backward pass over a batch plan.
+struct GradOut {
+    const float *gout;         // dL/d[x, y, theta, strength] per output row
+    float *gimg;               // dL/dimage (accumulated)
+    double *gsigma;            // dL/dsigma (accumulated), may be null
+    int gpitch, pad;
 };
 
 // One image of a launch.  All offsets are in elements of their arrays.
@@ -96,6 +107,23 @@ __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; 
 #define TOED_CONST __attribute__((address_space(4)))
 template <typename T> __device__ __forceinline__ const TOED_CONST T *as_const(const T *p) {
     return (const TOED_CONST T *)p;
+}
+
+// Index of the job owning a tile (0 for single images).
+__device__ __forceinline__ int job_index(const ImageJob *jobs, int njobs, int tile) {
+    if (!jobs) return 0;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const TOED_CONST ImageJob *cj = as_const(jobs);
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cj[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+#else
+    (void)njobs; (void)tile;
+    return 0;
+#endif
 }
 
 // The job of a tile: the kernel-argument job for single images, else the table
@@ -689,6 +717,96 @@ __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs,
 }
 
 // ---------------------------------------------------------------------------
+// Backward of a batch plan's last forward, tile by tile (differentiable form)
+// ---------------------------------------------------------------------------
+// One workgroup per forward tile: the tile's image plus a (R1+1)-pixel halo is
+// staged in LDS once, every survivor of the tile (from the plan's staging, so
+// no index list is needed) runs the shared FP64 reverse pass with 16 lanes,
+// its transposed filters are accumulated with LDS atomics into a
+// (TH + 2R1 + 2) x (32 + 2R1 + 2) gradient patch, and the patch is added to
+// dL/dimage once per tile.  dL/dsigma is summed per workgroup.
+template <typename IN_T, int TH, int R1>
+__global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
+                                                                const GradOut *gouts, StageBuf stg,
+                                                                const int *tile_cnt, const int *offsets,
+                                                                const int *chunk_pref, int tile_begin) {
+    constexpr int N = 2 * R1 + 1;
+    constexpr int U = R1 + 1;
+    constexpr int PR = TH + 2 * U, PC = TW + 2 * U, LDG = PC | 1;
+    __shared__ float simg[PR * LDG];
+    __shared__ float sgrad[PR * LDG];
+    __shared__ double st[3][4][N];
+    __shared__ double sd[3][4][N];
+    __shared__ double sgs[NT / toedadj::G];
+    const int tile = tile_begin + blockIdx.x;
+    const int m = job_index(jobs, njobs, tile);
+    const ImageJob J = load_job(jobs, njobs, job0, tile);
+    const GradOut O = gouts[m];
+    const long long tile_local = tile - J.tile_begin;
+    const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
+    const int i0 = tr * TH, j0 = tc * TW;
+    const int gi0 = i0 - U, gj0 = j0 - U;        // patch origin (image rows/cols)
+    const TOED_CONST TapSet *tp = as_const(J.taps);
+    for (int e = threadIdx.x; e < PR * PC; e += NT) {
+        const int r = e / PC, c = e % PC;
+        const int gi = gi0 + r, gj = gj0 + c;
+        float v = 0.f;
+        if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
+        simg[r * LDG + c] = v;
+        sgrad[r * LDG + c] = 0.f;
+    }
+    for (int k = threadIdx.x; k < 12 * N; k += NT) {
+        const int v = k / (4 * N), f = (k / N) & 3, t = k % N;
+        double a = (v == 2) ? tp->ds[f][t] : tp->du[f][t];
+        double b = (v == 2) ? tp->dss[f][t] : tp->dsu[f][t];
+        if (v == 1 && (t == 0 || t == N - 1)) { a = 0.0; b = 0.0; }
+        st[v][f][t] = a;
+        sd[v][f][t] = b;
+    }
+    const int g = threadIdx.x / toedadj::G, l = threadIdx.x % toedadj::G;
+    if (l == 0) sgs[g] = 0.0;
+    __syncthreads();
+    struct LdsImg {
+        const float *p; int gi0, gj0;
+        __device__ double operator()(int i, int j) const { return (double)p[(i - gi0) * LDG + (j - gj0)]; }
+    } img{simg, gi0, gj0};
+    struct LdsSink {
+        float *p; int gi0, gj0;
+        __device__ void operator()(int i, int j, float v) const { atomicAdd(&p[(i - gi0) * LDG + (j - gj0)], v); }
+    } sink{sgrad, gi0, gj0};
+    for (int phase = 0; phase < 4; ++phase) {
+        const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
+        const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
+        for (int e = g; e < n; e += NT / toedadj::G) {
+            const uint32_t key = stg.key[base + e];
+            const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
+            const int R = 2 * i0 + r, C = 2 * j0 + c;
+            const long long ce = J.counts_base + (long long)R * J.ntc + tc;
+            const long long idx =
+                (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
+            if (idx >= J.capacity) continue;
+            const float4 up = reinterpret_cast<const float4 *>(O.gout)[idx];
+            double gs = 0.0;
+            toedadj::edgel_backward<R1>(R, C, up, l, J.H, J.W, st, sd, img, sink, &gs);
+            if (l == 0) sgs[g] += gs;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < PR * PC; e += NT) {
+        const int r = e / PC, c = e % PC;
+        const int gi = gi0 + r, gj = gj0 + c;
+        const float v = sgrad[r * LDG + c];
+        if (v != 0.f && gi >= 0 && gi < J.H && gj >= 0 && gj < J.W)
+            unsafeAtomicAdd(O.gimg + (long long)gi * O.gpitch + gj, v);
+    }
+    if (threadIdx.x == 0 && O.gsigma) {
+        double t = 0.0;
+        for (int k = 0; k < NT / toedadj::G; ++k) t += sgs[k];
+        if (t != 0.0) unsafeAtomicAdd(O.gsigma, t);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Unfused reference launch layer: interp planes and NMS maps
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void toed_planes_kernel(int H, int W, const float *img, float *Ix, float *Iy,
@@ -823,6 +941,7 @@ struct toed_batch {
     int in_type = 0, tile_rows = 16, persistent = 0;
     std::vector<int> order;
     std::vector<TileGroup> groups;
+    GradOut *d_gouts = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int *g_counts = nullptr, *g_flags = nullptr;
@@ -859,7 +978,14 @@ static void fill_taps(double sigma, int R0, int R1, TapSet *t) {
             const double x = (double)(k - R1) + (sh ? 0.5 : 0.0);
             const double g = std::exp(-x * x / (2.0 * s2)) / norm;
             const double v[4] = {g, -x * g / s2, (x * x - s2) * g / (s2 * s2), x * (3.0 * s2 - x * x) * g / (s2 * s2 * s2)};
+            // d/dsigma: f = P(x, s) g with dg/ds = g (x^2/s^3 - 1/s)
+            const double s1 = sigma, dg = g * (x * x / (s2 * s1) - 1.0 / s1);
+            const double P[4] = {1.0, -x / s2, (x * x - s2) / (s2 * s2), x * (3.0 * s2 - x * x) / (s2 * s2 * s2)};
+            const double dP[4] = {0.0, 2.0 * x / (s2 * s1), -4.0 * x * x / (s2 * s2 * s1) + 2.0 / (s2 * s1),
+                                  -12.0 * x / (s2 * s2 * s1) + 6.0 * x * x * x / (s2 * s2 * s2 * s1)};
             for (int f = 0; f < 4; ++f) {
+                const double dv = dP[f] * g + P[f] * dg;
+                if (sh) t->dss[f][k] = dv; else t->dsu[f][k] = dv;
                 if (sh) t->ds[f][k] = v[f]; else t->du[f][k] = v[f];
                 if (f < 2) { if (sh) t->fs[f][k] = (float)v[f]; else t->fu[f][k] = (float)v[f]; }
             }
@@ -1024,6 +1150,8 @@ int toed_use_own_stream(toed_ctx *c) {
 }
 
 void *toed_get_stream(toed_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int toed_get_device(toed_ctx *c) { return c ? c->device : -1; }
 
 int toed_set_tile_rows(toed_ctx *c, int tr) {
     if (!c) return TOED_ERR_ARG;
@@ -1247,6 +1375,46 @@ int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags) {
     return TOED_OK;
 }
 
+int toed_batch_backward(toed_batch *b, const toed_grad_out_t *outs) {
+    if (!b || !outs) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    std::vector<GradOut> h(b->n);
+    for (int k = 0; k < b->n; ++k) {
+        const toed_grad_out_t &o = outs[b->order[k]];
+        if (!o.d_gout || !o.d_gimg || o.gpitch < b->jobs[k].W) return TOED_ERR_ARG;
+        h[k].gout = o.d_gout; h[k].gimg = o.d_gimg; h[k].gsigma = o.d_gsigma; h[k].gpitch = o.gpitch;
+    }
+    if (!b->d_gouts) {
+        if (hipMalloc(&b->d_gouts, sizeof(GradOut) * b->n) != hipSuccess) { b->d_gouts = nullptr; return TOED_ERR_ALLOC; }
+    }
+    HIPCHK(hipMemcpyAsync(b->d_gouts, h.data(), sizeof(GradOut) * b->n, hipMemcpyHostToDevice, s));
+    for (const TileGroup &g : b->groups) {
+        if (g.tile_end <= g.tile_begin) continue;
+        const dim3 grid((unsigned)(g.tile_end - g.tile_begin));
+        ImageJob dummy;
+        std::memset(&dummy, 0, sizeof dummy);
+        for_tile_rows(b->tile_rows, [&](auto THc) {
+            constexpr int T = decltype(THc)::value;
+            for_radius(g.R1, [&](auto Rc) {
+                constexpr int R = decltype(Rc)::value;
+                if (b->in_type == TOED_U8)
+                    hipLaunchKernelGGL((toed_tile_backward_kernel<uint8_t, T, R>), grid, dim3(NT), 0, s, b->d_jobs,
+                                       b->n, dummy, b->d_gouts, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
+                                       b->d_offsets, b->d_chunks, g.tile_begin);
+                else
+                    hipLaunchKernelGGL((toed_tile_backward_kernel<float, T, R>), grid, dim3(NT), 0, s, b->d_jobs,
+                                       b->n, dummy, b->d_gouts, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
+                                       b->d_offsets, b->d_chunks, g.tile_begin);
+            });
+        });
+    }
+    HIPCHK(hipGetLastError());
+    // the host array must outlive the asynchronous copy
+    HIPCHK(hipStreamSynchronize(s));
+    return TOED_OK;
+}
+
 int toed_batch_destroy(toed_batch *b) {
     if (!b) return TOED_ERR_ARG;
     (void)hipSetDevice(b->ctx->device);
@@ -1255,6 +1423,7 @@ int toed_batch_destroy(toed_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     (void)hipFree(b->d_jobs); (void)hipFree(b->d_counts); (void)hipFree(b->d_offsets); (void)hipFree(b->d_chunks);
     (void)hipFree(b->d_tilecnt); (void)hipFree(b->d_ticket); (void)hipFree(b->d_srec); (void)hipFree(b->d_skey);
+    (void)hipFree(b->d_gouts);
     delete b;
     return TOED_OK;
 }
