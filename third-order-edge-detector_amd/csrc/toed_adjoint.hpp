@@ -1,6 +1,6 @@
 // Per-edgel reverse pass of the TOED outputs with the edgel set frozen, shared
 // by the index-driven backward (toed_backward.hip) and the tile-driven
-// backward of a batch plan (toed_detect.hip).  FP64 throughout; one 16-lane
+// backward of a batch plan (toed_detect.hip).  FP64 throughout; one G-lane
 // group per edgel (lanes split the window rows, totals by __shfl_xor).
 //
 // Differentiated forward arithmetic: the nine responses at a pixel
@@ -13,11 +13,11 @@
 
 namespace toedadj {
 
-constexpr int G = 16;   // lanes per edgel
+constexpr int G = 8;    // lanes per edgel (window rows split over the group)
 
-__device__ __forceinline__ double red16(double v) {
+__device__ __forceinline__ double redG(double v) {
 #pragma unroll
-    for (int m = 8; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
+    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
     return v;
 }
 
@@ -29,44 +29,64 @@ __host__ __device__ constexpr int YF(int k) { return k == 1 || k == 3 || k == 5 
 // One edgel at interp pixel (R, C) of an H x W image, upstream row `up` =
 // dL/d[x, y, orientation, strength], group lane l.  st / sd: FP64 taps and
 // their sigma-derivatives, variants 0 unshifted, 1 unshifted with zero end
-// taps (window R0 of phase (0,0)), 2 shifted.  img(i, j) returns the pixel
+// taps (window R0 of phase (0,0)), 2 shifted; stf the FP32 copy of st used by
+// the scatter (its contributions are rounded to FP32 for the atomics anyway).
+// img(i, j) returns the pixel
 // (0 outside the image); sink(i, j, v) adds v to dL/dimage.  Lane 0 writes
 // the edgel's dL/dsigma to *gs_out.
 template <int R1, class Img, class Sink>
 __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, int H, int W,
                                                const double (*st)[4][2 * R1 + 1], const double (*sd)[4][2 * R1 + 1],
-                                               const Img &img, const Sink &sink, double *gs_out) {
+                                               const float (*stf)[4][2 * R1 + 1], const Img &img, const Sink &sink,
+                                               double *gs_out) {
     constexpr int U = R1 + 1;          // patch half-width around the edgel (neighbours are +-1 interp px)
+
+    // Each lane owns window rows p = -R1 + l + G*rr (NRR of them); the column
+    // loop is outermost so every tap read from LDS serves all of the lane's rows.
+    constexpr int NRR = (2 * R1 + G) / G;
 
     // ---- 1a. nine responses at the edgel and their sigma-derivatives
     double v9[9], s9[9];
     {
         const int i = R >> 1, j = C >> 1, a = R & 1, b = C & 1, ph = 2 * a + b;
         const int vx = b ? 2 : (ph == 0 ? 1 : 0), vy = a ? 2 : (ph == 0 ? 1 : 0);
+        double h[NRR][4], hs[NRR][4];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { v9[k] = 0.0; s9[k] = 0.0; }
-        for (int rr = 0; rr < 2; ++rr) {
-            const int p = -R1 + l + G * rr;
-            if (p > R1) break;
-            double h[4] = {0, 0, 0, 0}, hs[4] = {0, 0, 0, 0};
+        for (int rr = 0; rr < NRR; ++rr)
+#pragma unroll
+            for (int f = 0; f < 4; ++f) { h[rr][f] = 0.0; hs[rr][f] = 0.0; }
 #pragma unroll 1
-            for (int q = -R1; q <= R1; ++q) {
+        for (int q = -R1; q <= R1; ++q) {
+            double xt[4], xs[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) { xt[f] = st[vx][f][q + R1]; xs[f] = sd[vx][f][q + R1]; }
+#pragma unroll
+            for (int rr = 0; rr < NRR; ++rr) {
+                const int p = -R1 + l + G * rr;
+                if (p > R1) continue;
                 const double px = img(i - p, j - q);
 #pragma unroll
                 for (int f = 0; f < 4; ++f) {
-                    h[f] = fma(px, st[vx][f][q + R1], h[f]);
-                    hs[f] = fma(px, sd[vx][f][q + R1], hs[f]);
+                    h[rr][f] = fma(px, xt[f], h[rr][f]);
+                    hs[rr][f] = fma(px, xs[f], hs[rr][f]);
                 }
-            }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const double y = st[vy][YF(k)][p + R1], ys = sd[vy][YF(k)][p + R1];
-                v9[k] = fma(h[XF(k)], y, v9[k]);
-                s9[k] = fma(hs[XF(k)], y, fma(h[XF(k)], ys, s9[k]));
             }
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { v9[k] = red16(v9[k]); s9[k] = red16(s9[k]); }
+        for (int k = 0; k < 9; ++k) { v9[k] = 0.0; s9[k] = 0.0; }
+#pragma unroll
+        for (int rr = 0; rr < NRR; ++rr) {
+            const int p = -R1 + l + G * rr;
+            if (p > R1) continue;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const double y = st[vy][YF(k)][p + R1], ys = sd[vy][YF(k)][p + R1];
+                v9[k] = fma(h[rr][XF(k)], y, v9[k]);
+                s9[k] = fma(hs[rr][XF(k)], y, fma(h[rr][XF(k)], ys, s9[k]));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { v9[k] = redG(v9[k]); s9[k] = redG(s9[k]); }
     }
     // octant of the recomputed gradient (cpu_toed.cpp:415-462): x-major iff
     // |gx| >= |gy|; neighbours one step along +-gradient, axis and diagonal
@@ -77,33 +97,43 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
 
     // ---- 1b. Ix, Iy at the neighbours (+ sigma-derivatives)
     double nv[4][2], ns[4][2];
-#pragma unroll 1
+#pragma unroll
     for (int m = 0; m < 4; ++m) {
         const int Rn = R + nbr[m][0], Cn = C + nbr[m][1];
         const int i = Rn >> 1, j = Cn >> 1, a = Rn & 1, b = Cn & 1, ph = 2 * a + b;
         const int vx = b ? 2 : (ph == 0 ? 1 : 0), vy = a ? 2 : (ph == 0 ? 1 : 0);
-        double ix = 0, iy = 0, ixs = 0, iys = 0;
-        for (int rr = 0; rr < 2; ++rr) {
-            const int p = -R1 + l + G * rr;
-            if (p > R1) break;
-            double h0 = 0, h1 = 0, hs0 = 0, hs1 = 0;
+        double h0[NRR], h1[NRR], hs0[NRR], hs1[NRR];
+#pragma unroll
+        for (int rr = 0; rr < NRR; ++rr) { h0[rr] = 0; h1[rr] = 0; hs0[rr] = 0; hs1[rr] = 0; }
 #pragma unroll 1
-            for (int q = -R1; q <= R1; ++q) {
+        for (int q = -R1; q <= R1; ++q) {
+            const double x0 = st[vx][0][q + R1], x1 = st[vx][1][q + R1];
+            const double xs0 = sd[vx][0][q + R1], xs1 = sd[vx][1][q + R1];
+#pragma unroll
+            for (int rr = 0; rr < NRR; ++rr) {
+                const int p = -R1 + l + G * rr;
+                if (p > R1) continue;
                 const double px = img(i - p, j - q);
-                h0 = fma(px, st[vx][0][q + R1], h0);
-                h1 = fma(px, st[vx][1][q + R1], h1);
-                hs0 = fma(px, sd[vx][0][q + R1], hs0);
-                hs1 = fma(px, sd[vx][1][q + R1], hs1);
+                h0[rr] = fma(px, x0, h0[rr]);
+                h1[rr] = fma(px, x1, h1[rr]);
+                hs0[rr] = fma(px, xs0, hs0[rr]);
+                hs1[rr] = fma(px, xs1, hs1[rr]);
             }
+        }
+        double ix = 0, iy = 0, ixs = 0, iys = 0;
+#pragma unroll
+        for (int rr = 0; rr < NRR; ++rr) {
+            const int p = -R1 + l + G * rr;
+            if (p > R1) continue;
             const double y0 = st[vy][0][p + R1], y1 = st[vy][1][p + R1];
             const double ys0 = sd[vy][0][p + R1], ys1 = sd[vy][1][p + R1];
-            ix = fma(h1, y0, ix);
-            iy = fma(h0, y1, iy);
-            ixs = fma(hs1, y0, fma(h1, ys0, ixs));
-            iys = fma(hs0, y1, fma(h0, ys1, iys));
+            ix = fma(h1[rr], y0, ix);
+            iy = fma(h0[rr], y1, iy);
+            ixs = fma(hs1[rr], y0, fma(h1[rr], ys0, ixs));
+            iys = fma(hs0[rr], y1, fma(h0[rr], ys1, iys));
         }
-        nv[m][0] = red16(ix); nv[m][1] = red16(iy);
-        ns[m][0] = red16(ixs); ns[m][1] = red16(iys);
+        nv[m][0] = redG(ix); nv[m][1] = redG(iy);
+        ns[m][0] = redG(ixs); ns[m][1] = redG(iys);
     }
 
     // ---- 2. reverse pass (every lane holds the totals; all compute it)
@@ -187,43 +217,60 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
         *gs_out = gs;
     }
 
-    // ---- 4. scatter the transposed filters into dL/dimage
+    // ---- 4. scatter the transposed filters into dL/dimage: lane rows of the
+    // (2U+1)^2 patch around the edgel; per point and patch column the four x
+    // taps are read once and applied to all of the lane's rows
+    constexpr int NSR = (2 * U + G) / G;
     const int ic = R >> 1, jc = C >> 1;
-    for (int rr = 0; rr < 2; ++rr) {
-        const int o = -U + l + G * rr;            // patch row offset from the edgel's original row
-        if (o > U) break;
-        const int ii = ic + o;
-        if (ii < 0 || ii >= H) continue;
-        float row[2 * U + 1];
+    float adjf[17];
 #pragma unroll
-        for (int c = 0; c <= 2 * U; ++c) row[c] = 0.f;
-        // point m = -1: the edgel (nine responses); m >= 0: neighbour m (Ix, Iy)
-        for (int m = -1; m < 4; ++m) {
-            const int Rp = m < 0 ? R : R + nbr[m][0], Cp = m < 0 ? C : C + nbr[m][1];
-            const int ip = Rp >> 1, jp = Cp >> 1, a = Rp & 1, b = Cp & 1, ph = 2 * a + b;
-            const int vx = b ? 2 : (ph == 0 ? 1 : 0), vy = a ? 2 : (ph == 0 ? 1 : 0);
-            const int p = ip - ii;                 // tap row index: image row = ip - p
+    for (int k = 0; k < 17; ++k) adjf[k] = (float)adj[k];
+    float row[NSR][2 * U + 1];
+#pragma unroll
+    for (int rr = 0; rr < NSR; ++rr)
+#pragma unroll
+        for (int c = 0; c <= 2 * U; ++c) row[rr][c] = 0.f;
+    // point m = -1: the edgel (nine responses); m >= 0: neighbour m (Ix, Iy)
+#pragma unroll
+    for (int m = -1; m < 4; ++m) {
+        const int Rp = m < 0 ? R : R + nbr[m][0], Cp = m < 0 ? C : C + nbr[m][1];
+        const int ip = Rp >> 1, jp = Cp >> 1, a = Rp & 1, b = Cp & 1, ph = 2 * a + b;
+        const int vx = b ? 2 : (ph == 0 ? 1 : 0), vy = a ? 2 : (ph == 0 ? 1 : 0);
+        float cf[NSR][4];                           // per lane row, per x-family coefficient
+#pragma unroll
+        for (int rr = 0; rr < NSR; ++rr) {
+            const int p = ip - (ic - U + l + G * rr);   // tap row index: image row = ip - p
+#pragma unroll
+            for (int f = 0; f < 4; ++f) cf[rr][f] = 0.f;
             if (p < -R1 || p > R1) continue;
-            double cf[4] = {0, 0, 0, 0};           // per x-family coefficient of this row
             if (m < 0) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) cf[XF(k)] = fma(adj[k], st[vy][YF(k)][p + R1], cf[XF(k)]);
+                for (int k = 0; k < 9; ++k) cf[rr][XF(k)] = fmaf(adjf[k], stf[vy][YF(k)][p + R1], cf[rr][XF(k)]);
             } else {
-                cf[1] = adj[9 + 2 * m] * st[vy][0][p + R1];     // Ix: G' in x, G in y
-                cf[0] = adj[10 + 2 * m] * st[vy][1][p + R1];    // Iy: G in x, G' in y
-            }
-#pragma unroll
-            for (int c = 0; c <= 2 * U; ++c) {
-                const int qq = jp - (jc - U + c);  // tap column index: image col = jp - q
-                if (qq < -R1 || qq > R1) continue;
-                row[c] += (float)(cf[0] * st[vx][0][qq + R1] + cf[1] * st[vx][1][qq + R1] +
-                                  cf[2] * st[vx][2][qq + R1] + cf[3] * st[vx][3][qq + R1]);
+                cf[rr][1] = adjf[9 + 2 * m] * stf[vy][0][p + R1];     // Ix: G' in x, G in y
+                cf[rr][0] = adjf[10 + 2 * m] * stf[vy][1][p + R1];    // Iy: G in x, G' in y
             }
         }
 #pragma unroll
         for (int c = 0; c <= 2 * U; ++c) {
+            const int qq = jp - (jc - U + c);          // tap column index: image col = jp - q
+            if (qq < -R1 || qq > R1) continue;
+            const float x0 = stf[vx][0][qq + R1], x1 = stf[vx][1][qq + R1];
+            const float x2 = stf[vx][2][qq + R1], x3 = stf[vx][3][qq + R1];
+#pragma unroll
+            for (int rr = 0; rr < NSR; ++rr)
+                row[rr][c] = fmaf(cf[rr][0], x0, fmaf(cf[rr][1], x1, fmaf(cf[rr][2], x2, fmaf(cf[rr][3], x3, row[rr][c]))));
+        }
+    }
+#pragma unroll
+    for (int rr = 0; rr < NSR; ++rr) {
+        const int o = -U + l + G * rr;
+        const int ii = ic + o;
+        if (o > U || ii < 0 || ii >= H) continue;
+#pragma unroll
+        for (int c = 0; c <= 2 * U; ++c) {
             const int jj = jc - U + c;
-            if (jj >= 0 && jj < W && row[c] != 0.f) sink(ii, jj, row[c]);
+            if (jj >= 0 && jj < W && row[rr][c] != 0.f) sink(ii, jj, row[rr][c]);
         }
     }
 }

@@ -3,7 +3,7 @@
 // is cpu_toed.cpp:194-232 and :411-489).
 //
 // For every edgel of a forward pass (its interp pixel (R, C) from the forward's
-// index output) one 16-lane group of toed_backward_kernel:
+// index output) one group of toedadj::G lanes of toed_backward_kernel:
 //   1. recomputes in FP64 from the image the nine responses at the edgel and
 //      Ix, Iy at its four interpolation neighbours (octant from the recomputed
 //      gradient), each with its derivative with respect to sigma (window radii
@@ -93,17 +93,18 @@ struct GlobalSink {
     __device__ void operator()(int i, int j, float v) const { unsafeAtomicAdd(g + (long long)i * pitch + j, v); }
 };
 
-// One edgel of job J (16-lane group, lane l): see toed_adjoint.hpp.
+// One edgel of job J (lane group, lane l): see toed_adjoint.hpp.
 template <typename IN_T, int R1>
 __device__ __forceinline__ void edgel_adjoint(const BwJob &J, int e, int l, const double (*st)[4][2 * R1 + 1],
-                                              const double (*sd)[4][2 * R1 + 1], double *gs_out) {
+                                              const double (*sd)[4][2 * R1 + 1], const float (*stf)[4][2 * R1 + 1],
+                                              double *gs_out) {
     const int IW = 2 * J.W;
     const int idx = J.index[e];
     const int R = idx / IW, C = idx - (idx / IW) * IW;
     const float4 up = reinterpret_cast<const float4 *>(J.gout)[e];
     GlobalImg<IN_T> img{&J};
     GlobalSink sink{J.gimg, J.gpitch};
-    toedadj::edgel_backward<R1>(R, C, up, l, J.H, J.W, st, sd, img, sink, gs_out);
+    toedadj::edgel_backward<R1>(R, C, up, l, J.H, J.W, st, sd, stf, img, sink, gs_out);
 }
 
 template <typename IN_T, int R1>
@@ -111,6 +112,7 @@ __global__ __launch_bounds__(NT) void toed_backward_kernel(const BwJob *jobs, in
     constexpr int N = 2 * R1 + 1;
     __shared__ double st[3][4][N];     // taps
     __shared__ double sd[3][4][N];     // d taps / d sigma
+    __shared__ float stf[3][4][N];     // FP32 taps for the scatter
     __shared__ double sgs[NT / G];     // per-group dL/dsigma, reduced once per workgroup
     const int wg = blockIdx.x;
     const BwJob J = load_bwjob(jobs, njobs, wg);
@@ -118,13 +120,14 @@ __global__ __launch_bounds__(NT) void toed_backward_kernel(const BwJob *jobs, in
         const int v = k / (4 * N), f = (k / N) & 3, t = k % N;
         st[v][f][t] = J.taps->t[v][f][t];
         sd[v][f][t] = J.taps->d[v][f][t];
+        stf[v][f][t] = (float)J.taps->t[v][f][t];
     }
     __syncthreads();
     const int n = min(*J.count, J.max_edgels);
     const int g = threadIdx.x / G, l = threadIdx.x % G;
     const int e = (wg - J.wg_begin) * (NT / G) + g;
     if (l == 0) sgs[g] = 0.0;
-    if (e < n) edgel_adjoint<IN_T, R1>(J, e, l, st, sd, &sgs[g]);
+    if (e < n) edgel_adjoint<IN_T, R1>(J, e, l, st, sd, stf, &sgs[g]);
     __syncthreads();
     if (threadIdx.x == 0 && J.gsigma) {
         double t = 0.0;

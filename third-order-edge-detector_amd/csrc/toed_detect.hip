@@ -52,6 +52,7 @@ constexpr int MAXR1 = 9;            // largest supported R1 (sigma <= 2)
 constexpr int MAXTAPS = 2 * MAXR1 + 1;
 constexpr int CHUNK = 4096;         // scan chunk (entries)
 constexpr int NGROUPS = 3;          // supported window radii R1 = 5, 7, 9 (sigma 1, 1.5, 2)
+constexpr int BW_CHUNK = 256;       // edgels per pre-pass chunk of the tile backward
 
 // Filter taps for one sigma.  Families: 0 G, 1 G', 2 G'', 3 G''' sampled at
 // x = k - R1 (unshifted) and x = k - R1 + 1/2 (shifted), k = 0..2R1
@@ -721,7 +722,7 @@ __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs,
 // ---------------------------------------------------------------------------
 // One workgroup per forward tile: the tile's image plus a (R1+1)-pixel halo is
 // staged in LDS once, every survivor of the tile (from the plan's staging, so
-// no index list is needed) runs the shared FP64 reverse pass with 16 lanes,
+// no index list is needed) runs the shared FP64 reverse pass on a lane group,
 // its transposed filters are accumulated with LDS atomics into a
 // (TH + 2R1 + 2) x (32 + 2R1 + 2) gradient patch, and the patch is added to
 // dL/dimage once per tile.  dL/dsigma is summed per workgroup.
@@ -737,7 +738,10 @@ __global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *
     __shared__ float sgrad[PR * LDG];
     __shared__ double st[3][4][N];
     __shared__ double sd[3][4][N];
+    __shared__ float stf[3][4][N];
     __shared__ double sgs[NT / toedadj::G];
+    __shared__ int s_rc[BW_CHUNK];           // interp (row << 16 | col) of a chunk's edgels, -1: no output row
+    __shared__ float4 s_up[BW_CHUNK];        // their upstream gradient rows
     const int tile = tile_begin + blockIdx.x;
     const int m = job_index(jobs, njobs, tile);
     const ImageJob J = load_job(jobs, njobs, job0, tile);
@@ -762,6 +766,7 @@ __global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *
         if (v == 1 && (t == 0 || t == N - 1)) { a = 0.0; b = 0.0; }
         st[v][f][t] = a;
         sd[v][f][t] = b;
+        stf[v][f][t] = (float)a;
     }
     const int g = threadIdx.x / toedadj::G, l = threadIdx.x % toedadj::G;
     if (l == 0) sgs[g] = 0.0;
@@ -774,22 +779,41 @@ __global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *
         float *p; int gi0, gj0;
         __device__ void operator()(int i, int j, float v) const { atomicAdd(&p[(i - gi0) * LDG + (j - gj0)], v); }
     } sink{sgrad, gi0, gj0};
-    for (int phase = 0; phase < 4; ++phase) {
-        const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
-        const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
-        for (int e = g; e < n; e += NT / toedadj::G) {
-            const uint32_t key = stg.key[base + e];
+    // All four phases form one work list, processed in chunks: a cooperative
+    // pre-pass resolves each edgel's interp pixel, output row and upstream
+    // gradient into LDS (no global-load chains inside the per-edgel work);
+    // the groups of one wavefront then take edgels spread across the chunk
+    // (their patches overlap less, so their LDS atomics collide less).
+    int nph[4];
+#pragma unroll
+    for (int ph = 0; ph < 4; ++ph) nph[ph] = min(tile_cnt[(long long)tile * 4 + ph], J.stage_cap);
+    const int total = nph[0] + nph[1] + nph[2] + nph[3];
+    constexpr int GPW = 64 / toedadj::G;                 // groups per wavefront
+    for (int c0 = 0; c0 < total; c0 += BW_CHUNK) {
+        const int cn = min(BW_CHUNK, total - c0);
+        for (int k = threadIdx.x; k < cn; k += NT) {
+            int e = c0 + k, phase = 0;
+            while (e >= nph[phase]) { e -= nph[phase]; ++phase; }
+            const uint32_t key = stg.key[J.stage_base + (tile_local * 4 + phase) * J.stage_cap + e];
             const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
             const int R = 2 * i0 + r, C = 2 * j0 + c;
             const long long ce = J.counts_base + (long long)R * J.ntc + tc;
             const long long idx =
                 (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
-            if (idx >= J.capacity) continue;
-            const float4 up = reinterpret_cast<const float4 *>(O.gout)[idx];
+            s_rc[k] = idx < J.capacity ? (R << 16) | C : -1;
+            s_up[k] = idx < J.capacity ? reinterpret_cast<const float4 *>(O.gout)[idx] : make_float4(0, 0, 0, 0);
+        }
+        __syncthreads();
+        const int S = (cn + GPW - 1) / GPW;
+        for (int t = g; t < S * GPW; t += NT / toedadj::G) {
+            const int k = (t % GPW) * S + t / GPW;
+            if (k >= cn || s_rc[k] < 0) continue;
+            const int R = s_rc[k] >> 16, C = s_rc[k] & 0xffff;
             double gs = 0.0;
-            toedadj::edgel_backward<R1>(R, C, up, l, J.H, J.W, st, sd, img, sink, &gs);
+            toedadj::edgel_backward<R1>(R, C, s_up[k], l, J.H, J.W, st, sd, stf, img, sink, &gs);
             if (l == 0) sgs[g] += gs;
         }
+        __syncthreads();
     }
     __syncthreads();
     for (int e = threadIdx.x; e < PR * PC; e += NT) {
