@@ -143,6 +143,40 @@ typedef struct {
 } toed_grad_out_t;
 int toed_batch_backward(toed_batch *b, const toed_grad_out_t *outs);
 
+/* Subpixel third-order correction of dense edge maps ------------------------
+ * Replaces SubpixelTOCorrectionCPU<T> (subpix_TO_correction/
+ * subpix_to_correction.hpp:20-96) with interp_n = 0: set_edgemap (:86),
+ * set_orientation (:87) or estimate_orientation (:88), then run (:95), which
+ * returns the N x 4 list [x, y, orientation, strength] and fills TO_edgemap /
+ * TO_orientation (:71-72).  Many maps of any sizes go through one plan and
+ * one launch sequence; sigma and the isolation radius are per plan, the NMS
+ * threshold per map. */
+typedef struct toed_subpix toed_subpix;
+typedef struct {
+    const float *d_map;        /* H rows of `pitch` floats: dense edge-strength map */
+    const float *d_orient;     /* orientation map in radians (rows of `opitch`), or
+                                  NULL: estimated from the map (:307-375) */
+    int H, W, pitch, opitch;
+    double thresh;             /* NMS threshold on the map (reference default 0.2) */
+    float *d_edges;            /* device output, capacity x 4 floats */
+    long long capacity;
+    int *d_index;              /* optional: map pixel i*W+j of each edgel (no reference
+                                  counterpart) */
+    float *d_to_edgemap;       /* optional H rows of `tpitch` floats (TO_edgemap) */
+    float *d_to_orientation;   /* optional, same layout (TO_orientation) */
+    int tpitch;
+} toed_subpix_map_t;
+/* sigma -> filter radius ceil(4 sigma) (:107); TOED_ERR_SIGMA above 12 */
+int toed_subpix_radius(double sigma, int *R);
+/* nbr_radius: drop edgels with no other edgel within this distance (:558-621,
+ * reference default 2; <= 0 keeps all; at most 5). */
+int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double sigma, double nbr_radius,
+                     toed_subpix **out);
+/* Enqueue on the context's stream; per-map counts and TOED_FLAG_* bits land in
+ * d_counts[n] / d_flags[n] (device, either may be NULL). */
+int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags);
+int toed_subpix_destroy(toed_subpix *p);
+
 /* Reference launch layer (unfused, for drop-in and debugging) ---------------
  * toed_gpu_convolve_f32: gpu_kernels.hpp:24-33.  Writes the four interp planes
  * Ix, Iy, |grad I| and TO orientation ((2h) x (2w), row-major) from a float

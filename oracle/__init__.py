@@ -3,7 +3,8 @@
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
 may import this package, and only as the checker (or the timed CPU baseline).
 The C sources restate the reference's ``cpu_toed.cpp`` (see the header of
-``toed_oracle.c`` for the file:line map).
+``toed_oracle.c`` for the file:line map) and its
+``subpix_TO_correction/subpix_to_correction.cpp`` (``subpix_oracle.c``).
 """
 import ctypes
 import os
@@ -53,6 +54,14 @@ def lib():
         L.toed_oracle_frozen_loss.restype = ctypes.c_double
         L.toed_oracle_frozen_grad.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                               ctypes.c_int, _i64p, ctypes.c_long, _dp, _ip, _dp, _dp]
+        L.toed_subpix_taps.argtypes = [ctypes.c_double, _ip, _dp, _dp, _dp]
+        L.toed_subpix_estimate_orientation.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.toed_subpix_conv_circ.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.toed_subpix_interp_cubic.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.toed_subpix_interp_cubic.restype = ctypes.c_double
+        L.toed_subpix_oracle.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int, _dp, ctypes.c_long, _i64p, _dp, _dp]
+        L.toed_subpix_oracle.restype = ctypes.c_long
         _LIB = L
     return _LIB
 
@@ -217,3 +226,92 @@ def frozen_grad(img, sigma, eidx, w, octants, R0=None, R1=None):
                                      ctypes.byref(gs)):
         raise ValueError("frozen_grad failed")
     return g, gs.value
+
+
+# ---- subpixel third-order correction (subpix_oracle.c) ---------------------
+
+def subpix_taps(sigma):
+    """(radius, G, G', G'') at integer offsets -r..r, r = ceil(4 sigma)."""
+    G, dG, d2G = (np.zeros(129, np.float64) for _ in range(3))
+    r = ctypes.c_int()
+    if lib().toed_subpix_taps(float(sigma), ctypes.byref(r), _p(G), _p(dG), _p(d2G)):
+        raise ValueError("bad sigma %r" % sigma)
+    n = 2 * r.value + 1
+    return r.value, G[:n].copy(), dG[:n].copy(), d2G[:n].copy()
+
+
+def subpix_estimate_orientation(E):
+    E = np.ascontiguousarray(E, np.float64)
+    th = np.empty_like(E)
+    lib().toed_subpix_estimate_orientation(_p(E), E.shape[0], E.shape[1], _p(th))
+    return th
+
+
+def subpix_conv_circ(src, kx, ky):
+    src = np.ascontiguousarray(src, np.float64)
+    kx = np.ascontiguousarray(kx, np.float64)
+    ky = np.ascontiguousarray(ky, np.float64)
+    assert len(kx) == len(ky) and len(kx) % 2 == 1
+    dst = np.empty_like(src)
+    lib().toed_subpix_conv_circ(_p(src), _p(dst), src.shape[0], src.shape[1], len(kx) // 2, _p(kx), _p(ky))
+    return dst
+
+
+def subpix_interp_cubic(src, x, y):
+    src = np.ascontiguousarray(src, np.float64)
+    return lib().toed_subpix_interp_cubic(_p(src), src.shape[0], src.shape[1], float(x), float(y))
+
+
+def subpix(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthreads=0, maps=False):
+    """FP64 subpixel third-order correction of an edge map.  Returns dict with
+    ``edges`` (N x 4: x, y, orientation, strength), ``idx`` (map pixel i*W+j
+    of each edgel) and, with maps=True, ``TO_E`` / ``TO_O``."""
+    E = np.ascontiguousarray(E, np.float64)
+    h, w = E.shape
+    th = None if theta is None else np.ascontiguousarray(theta, np.float64)
+    cap = h * w
+    edges = np.empty((cap, 4), np.float64)
+    idx = np.empty(cap, np.int64)
+    te = np.empty((h, w), np.float64) if maps else None
+    to = np.empty((h, w), np.float64) if maps else None
+    n = lib().toed_subpix_oracle(_p(E), h, w, _p(th), float(thresh), float(sigma), float(nbr_radius),
+                                 int(nthreads), _p(edges), cap, _p(idx, _i64p), _p(te), _p(to))
+    if n < 0:
+        raise ValueError("subpix oracle failed")
+    res = dict(edges=edges[:n].copy(), idx=idx[:n].copy(), n=int(n))
+    if maps:
+        res["TO_E"], res["TO_O"] = te, to
+    return res
+
+
+SUBPIX_REF_BIN = os.path.join(HERE, "_ref", "subpix_ref")
+
+
+def subpix_ref_available():
+    return os.path.exists(SUBPIX_REF_BIN)
+
+
+def run_subpix_ref(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthreads=1):
+    """Run the reference's own subpix_to_correction.cpp (compiled into
+    oracle/_ref).  Returns dict(edges, TO_E, TO_O, times=(nms, conv, correct,
+    total) seconds)."""
+    E = np.ascontiguousarray(E, np.float64)
+    h, w = E.shape
+    with tempfile.TemporaryDirectory() as td:
+        mp = os.path.join(td, "map.f64")
+        E.tofile(mp)
+        op = "-"
+        if theta is not None:
+            op = os.path.join(td, "orient.f64")
+            np.ascontiguousarray(theta, np.float64).tofile(op)
+        out = os.path.join(td, "out.bin")
+        subprocess.run([SUBPIX_REF_BIN, mp, op, str(h), str(w), repr(float(thresh)), repr(float(sigma)),
+                        str(int(nthreads)), repr(float(nbr_radius)), out, td], check=True,
+                       stdout=subprocess.DEVNULL)
+        with open(out, "rb") as f:
+            n = int(np.frombuffer(f.read(8), np.int64)[0])
+            times = tuple(float(t) for t in np.frombuffer(f.read(32), np.float64))
+            edges = np.frombuffer(f.read(n * 32), np.float64).reshape(n, 4).copy()
+            te = np.frombuffer(f.read(h * w * 8), np.float64).reshape(h, w).copy()
+            to = np.frombuffer(f.read(h * w * 8), np.float64).reshape(h, w).copy()
+    return dict(edges=edges, TO_E=te, TO_O=to, times=times, n=n)

@@ -1,0 +1,340 @@
+/*
+ * TEST INFRASTRUCTURE ONLY -- FP64 oracle of the subpixel third-order
+ * correction path (reference: subpix_TO_correction/subpix_to_correction.cpp,
+ * class SubpixelTOCorrectionCPU<double>, interp_n = 0).
+ *
+ *   estimate_orientation  :307-375  triangle smoothing (r = 4, half-sample
+ *                                   symmetric boundary, :47-55), central
+ *                                   differences (one-sided at the borders),
+ *                                   theta = mod(atan(Oyy sgn(-Oxy) / (Oxx + 1e-5)), pi)
+ *   compute_dir_from_theta :296-305 dir = (cos, sin)(theta mod pi), :216-222
+ *   nms_token(margin 3)   :413-519  octant NMS on the map along dir, parabola,
+ *                                   |A| < 1e-16 rejected, strength |max_f|
+ *   run                   :623-736  Hx = E dirx, Hy = E diry; 8 circular
+ *                                   separable convolutions (:377-411); Keys
+ *                                   cubic (a = -1/2) sampling with clamped
+ *                                   borders (:57-67, :521-556) of 10 planes at
+ *                                   each edgel; F = (Fx, Fy); orientation
+ *                                   atan2(Fx, -Fy)/|F| with the fallback
+ *                                   atan2(dirx, -diry) when |F| < 1e-12;
+ *                                   isolated-edgel removal (:558-621); map
+ *                                   scatter, last writer wins (:708-729).
+ *
+ * Pinned by golden vectors of the reference's own subpix_to_correction.cpp
+ * compiled from its sources (oracle/_ref, tests/golden) and by the checks in
+ * tests/test_subpix_oracle.py (circular convolution = FFT convolution, Keys
+ * cubic reproduces cubics, orientation of straight ridges).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static const double SP_PI = 3.14159265358979323846;
+
+static int wrap_idx(int i, int n) {
+    int r = i % n;
+    if (r < 0) r += n;
+    return r;
+}
+
+static int wrap_sym(int i, int n) {
+    if (n <= 1) return 0;
+    while (i < 0 || i >= n) {
+        if (i < 0) i = -i - 1;
+        else i = 2 * n - i - 1;
+    }
+    return i;
+}
+
+static double wrap_mod_pi(double a) {
+    double r = fmod(a, SP_PI);
+    if (r < 0) r += SP_PI;
+    return r;
+}
+
+static double cubic_w(double s) {
+    const double a = -0.5;
+    const double x = fabs(s);
+    if (x <= 1.0) return ((a + 2) * x - (a + 3)) * x * x + 1;
+    if (x < 2.0) return ((a * x - 5 * a) * x + 8 * a) * x - 4 * a;
+    return 0.0;
+}
+
+/* S1 taps: G, G', G'' of radius ceil(4 sigma) at integer offsets (:194-214) */
+int toed_subpix_taps(double sigma, int *rad, double *G, double *dG, double *d2G) {
+    if (!(sigma > 0)) return -1;
+    const int r = (int)ceil(sigma * 4);
+    const double s2 = sigma * sigma, s3 = s2 * sigma, s5 = s2 * s3, nrm = sqrt(2.0 * SP_PI);
+    for (int k = -r; k <= r; ++k) {
+        const double e = exp(-((double)k * k) / (2 * s2));
+        G[k + r] = e / (nrm * sigma);
+        dG[k + r] = -k * e / (nrm * s3);
+        d2G[k + r] = ((double)k * k - s2) * e / (nrm * s5);
+    }
+    *rad = r;
+    return 0;
+}
+
+/* S2 */
+void toed_subpix_estimate_orientation(const double *E, int h, int w, double *theta) {
+    const int r = 4;
+    double tri[9], tsum = 0;
+    for (int k = -r; k <= r; ++k) { tri[k + r] = r + 1 - abs(k); tsum += tri[k + r]; }
+    for (int k = 0; k < 9; ++k) tri[k] /= tsum;
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)h * w);
+    double *sm = (double *)malloc(sizeof(double) * (size_t)h * w);
+    double *Ox = (double *)malloc(sizeof(double) * (size_t)h * w);
+    double *Oy = (double *)malloc(sizeof(double) * (size_t)h * w);
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            double acc = 0;
+            for (int q = -r; q <= r; ++q) acc += E[(size_t)i * w + wrap_sym(j - q, w)] * tri[q + r];
+            tmp[(size_t)i * w + j] = acc;
+        }
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            double acc = 0;
+            for (int p = -r; p <= r; ++p) acc += tmp[(size_t)wrap_sym(i - p, h) * w + j] * tri[p + r];
+            sm[(size_t)i * w + j] = acc;
+        }
+#define DX(a, i, j) ((j) == 0 ? (a)[(size_t)(i) * w + 1] - (a)[(size_t)(i) * w] : \
+                     (j) == w - 1 ? (a)[(size_t)(i) * w + w - 1] - (a)[(size_t)(i) * w + w - 2] : \
+                     0.5 * ((a)[(size_t)(i) * w + (j) + 1] - (a)[(size_t)(i) * w + (j) - 1]))
+#define DY(a, i, j) ((i) == 0 ? (a)[(size_t)w + (j)] - (a)[(j)] : \
+                     (i) == h - 1 ? (a)[(size_t)(h - 1) * w + (j)] - (a)[(size_t)(h - 2) * w + (j)] : \
+                     0.5 * ((a)[(size_t)((i) + 1) * w + (j)] - (a)[(size_t)((i) - 1) * w + (j)]))
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            Ox[(size_t)i * w + j] = DX(sm, i, j);
+            Oy[(size_t)i * w + j] = DY(sm, i, j);
+        }
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            const double oxx = DX(Ox, i, j), oxy = DX(Oy, i, j), oyy = DY(Oy, i, j);
+            const double sgn = oxy < 0 ? 1.0 : (oxy > 0 ? -1.0 : 0.0);
+            theta[(size_t)i * w + j] = wrap_mod_pi(atan(oyy * sgn / (oxx + 1e-5)));
+        }
+#undef DX
+#undef DY
+    free(tmp); free(sm); free(Ox); free(Oy);
+}
+
+/* S5: circular separable convolution, x pass then y pass (:377-411) */
+static void conv_circ(const double *src, double *dst, double *tmp, int h, int w, int rad, const double *kx,
+                      const double *ky) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            double acc = 0;
+            for (int q = -rad; q <= rad; ++q) acc += src[(size_t)i * w + wrap_idx(j - q, w)] * kx[q + rad];
+            tmp[(size_t)i * w + j] = acc;
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            double acc = 0;
+            for (int p = -rad; p <= rad; ++p) acc += tmp[(size_t)wrap_idx(i - p, h) * w + j] * ky[p + rad];
+            dst[(size_t)i * w + j] = acc;
+        }
+}
+
+void toed_subpix_conv_circ(const double *src, double *dst, int h, int w, int rad, const double *kx, const double *ky) {
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)h * w);
+    conv_circ(src, dst, tmp, h, w, rad, kx, ky);
+    free(tmp);
+}
+
+/* S6: Keys cubic sample with clamped borders (:521-556) */
+double toed_subpix_interp_cubic(const double *src, int h, int w, double x, double y) {
+    const int x0 = (int)floor(x), y0 = (int)floor(y);
+    const double fx = x - x0, fy = y - y0;
+    const double wx[4] = {cubic_w(fx + 1), cubic_w(fx), cubic_w(fx - 1), cubic_w(fx - 2)};
+    const double wy[4] = {cubic_w(fy + 1), cubic_w(fy), cubic_w(fy - 1), cubic_w(fy - 2)};
+    double acc = 0;
+    for (int p = 0; p < 4; ++p) {
+        int ii = y0 - 1 + p;
+        ii = ii < 0 ? 0 : (ii >= h ? h - 1 : ii);
+        double row = 0;
+        for (int q = 0; q < 4; ++q) {
+            int jj = x0 - 1 + q;
+            jj = jj < 0 ? 0 : (jj >= w ? w - 1 : jj);
+            row += src[(size_t)ii * w + jj] * wx[q];
+        }
+        acc += row * wy[p];
+    }
+    return acc;
+}
+
+/*
+ * Whole path.  E: h x w map; theta: given orientation or NULL (estimated).
+ * out: N x 4 [x, y, orientation, strength] after isolated-edgel removal;
+ * nms_idx (optional): the map pixel i*w+j of each output edgel; TO_E / TO_O
+ * (optional) h x w maps.  Returns N (rows beyond cap counted, not written).
+ */
+long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, double thresh, double sigma,
+                        double nbr_radius, int nthreads, double *out, long cap, int64_t *nms_idx, double *TO_E,
+                        double *TO_O) {
+    if (h <= 0 || w <= 0 || !E || !(sigma > 0)) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const size_t n = (size_t)h * w;
+    int rad;
+    double G[129], dG[129], d2G[129];
+    if ((int)ceil(sigma * 4) > 64 || toed_subpix_taps(sigma, &rad, G, dG, d2G)) return -1;
+    double *theta = (double *)malloc(sizeof(double) * n);
+    double *dirx = (double *)malloc(sizeof(double) * n), *diry = (double *)malloc(sizeof(double) * n);
+    double *P[10];
+    for (int k = 0; k < 10; ++k) P[k] = (double *)malloc(sizeof(double) * n);
+    double *tmp = (double *)malloc(sizeof(double) * n);
+    double *nx = (double *)calloc(n, sizeof(double)), *ny = (double *)calloc(n, sizeof(double));
+    double *nm = (double *)calloc(n, sizeof(double));
+    unsigned char *nv = (unsigned char *)calloc(n, 1);
+    if (theta_in) memcpy(theta, theta_in, sizeof(double) * n);
+    else toed_subpix_estimate_orientation(E, h, w, theta);
+    for (size_t k = 0; k < n; ++k) {
+        const double a = wrap_mod_pi(theta[k]);
+        dirx[k] = cos(a);
+        diry[k] = sin(a);
+    }
+    /* S4: nms_token(3): rows/cols [4, n - 5) */
+    const int margin = 3;
+    const int j0 = margin + 1, j1 = w - (margin + 2), i0 = margin + 1, i1 = h - (margin + 2);
+#define EM(a, b) E[(size_t)(a) * w + (b)]
+#pragma omp parallel for schedule(dynamic)
+    for (int j = j0; j < j1; ++j) {
+        for (int i = i0; i < i1; ++i) {
+            if (EM(i, j) <= thresh) continue;
+            const double gx = dirx[(size_t)i * w + j], gy = diry[(size_t)i * w + j];
+            if (fabs(gx) < 1e-5 && fabs(gy) < 1e-5) continue;
+            const double md = sqrt(gx * gx + gy * gy);
+            const double ndx = gx / md, ndy = gy / md, f = EM(i, j);
+            double slope, fp, fm;
+            if (gx >= 0 && gy >= 0) {
+                if (gx >= gy) { slope = ndy / ndx; fp = EM(i, j + 1) * (1 - slope) + EM(i + 1, j + 1) * slope; fm = EM(i, j - 1) * (1 - slope) + EM(i - 1, j - 1) * slope; }
+                else { slope = ndx / ndy; fp = EM(i + 1, j) * (1 - slope) + EM(i + 1, j + 1) * slope; fm = EM(i - 1, j) * (1 - slope) + EM(i - 1, j - 1) * slope; }
+            } else if (gx < 0 && gy >= 0) {
+                if (fabs(gx) < gy) { slope = -ndx / ndy; fp = EM(i + 1, j) * (1 - slope) + EM(i + 1, j - 1) * slope; fm = EM(i - 1, j) * (1 - slope) + EM(i - 1, j + 1) * slope; }
+                else { slope = -ndy / ndx; fp = EM(i, j - 1) * (1 - slope) + EM(i + 1, j - 1) * slope; fm = EM(i, j + 1) * (1 - slope) + EM(i - 1, j + 1) * slope; }
+            } else if (gx < 0 && gy < 0) {
+                if (fabs(gx) >= fabs(gy)) { slope = ndy / ndx; fp = EM(i, j - 1) * (1 - slope) + EM(i - 1, j - 1) * slope; fm = EM(i, j + 1) * (1 - slope) + EM(i + 1, j + 1) * slope; }
+                else { slope = ndx / ndy; fp = EM(i - 1, j) * (1 - slope) + EM(i - 1, j - 1) * slope; fm = EM(i + 1, j) * (1 - slope) + EM(i + 1, j + 1) * slope; }
+            } else {
+                if (gx < fabs(gy)) { slope = -ndx / ndy; fp = EM(i - 1, j) * (1 - slope) + EM(i - 1, j + 1) * slope; fm = EM(i + 1, j) * (1 - slope) + EM(i + 1, j - 1) * slope; }
+                else { slope = -ndy / ndx; fp = EM(i, j + 1) * (1 - slope) + EM(i - 1, j + 1) * slope; fm = EM(i, j - 1) * (1 - slope) + EM(i + 1, j - 1) * slope; }
+            }
+            if (!((f > fm && f > fp) || (f > fm && f >= fp) || (f >= fm && f > fp))) continue;
+            const double s = sqrt(1 + slope * slope);
+            const double A = (fm + fp - 2 * f) / (2 * s * s);
+            const double B = (fp - fm) / (2 * s);
+            if (fabs(A) < 1e-16) continue;
+            const double ss = -B / (2 * A);
+            const double maxf = A * ss * ss + B * ss + f;
+            if (fabs(ss) <= sqrt(2.0)) {
+                nx[(size_t)i * w + j] = j + ss * ndx;
+                ny[(size_t)i * w + j] = i + ss * ndy;
+                nm[(size_t)i * w + j] = fabs(maxf);
+                nv[(size_t)i * w + j] = 1;
+            }
+        }
+    }
+#undef EM
+    /* S5: P[0..9] = Px, Py, Pxx, Pyy, Pxy, Hx, Hxx, Hxy, Hyy, Hy */
+    double *Hxm = P[5], *Hym = P[9];
+    for (size_t k = 0; k < n; ++k) { Hxm[k] = E[k] * dirx[k]; Hym[k] = E[k] * diry[k]; }
+    conv_circ(E, P[0], tmp, h, w, rad, dG, G);
+    conv_circ(E, P[1], tmp, h, w, rad, G, dG);
+    conv_circ(E, P[2], tmp, h, w, rad, d2G, G);
+    conv_circ(E, P[4], tmp, h, w, rad, dG, dG);
+    conv_circ(E, P[3], tmp, h, w, rad, G, d2G);
+    conv_circ(Hxm, P[6], tmp, h, w, rad, dG, G);
+    conv_circ(Hym, P[8], tmp, h, w, rad, G, dG);
+    conv_circ(Hym, P[7], tmp, h, w, rad, dG, G);
+    /* S6: per edgel in raster order */
+    long ne = 0;
+    for (size_t k = 0; k < n; ++k) ne += nv[k];
+    double *ex = (double *)malloc(sizeof(double) * (ne + 1) * 4);
+    int64_t *eidx = (int64_t *)malloc(sizeof(int64_t) * (ne + 1));
+    long m = 0;
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            const size_t o = (size_t)i * w + j;
+            if (!nv[o]) continue;
+            const double sx = nx[o], sy = ny[o];
+            double v[10];
+            for (int k = 0; k < 10; ++k) v[k] = toed_subpix_interp_cubic(P[k], h, w, sx, sy);
+            const double Px_ = v[0], Py_ = v[1], Pxx_ = v[2], Pyy_ = v[3], Pxy_ = v[4];
+            const double Hx_ = v[5], Hxx_ = v[6], Hxy_ = v[7], Hyy_ = v[8], Hy_ = v[9];
+            double Fx = Px_ * Hxx_ + Pxx_ * Hx_ + Py_ * Hxy_ + Pxy_ * Hy_;
+            double Fy = Px_ * Hxy_ + Pxy_ * Hx_ + Py_ * Hyy_ + Pyy_ * Hy_;
+            const double Fm = sqrt(Fx * Fx + Fy * Fy);
+            double ori;
+            if (Fm < 1e-12) ori = atan2(dirx[o], -diry[o]);
+            else { Fx /= Fm; Fy /= Fm; ori = atan2(Fx, -Fy); }
+            ex[4 * m + 0] = sx; ex[4 * m + 1] = sy; ex[4 * m + 2] = ori; ex[4 * m + 3] = nm[o];
+            eidx[m] = (int64_t)o;
+            ++m;
+        }
+    /* S7: keep edgels with another edgel within nbr_radius (grid of cells of size r) */
+    char *keep = (char *)malloc(ne + 1);
+    if (nbr_radius > 0 && ne >= 2) {
+        const double r2 = nbr_radius * nbr_radius, inv = 1.0 / nbr_radius;
+        /* brute force over cells: sort by cell via a simple spatial hash on a dense cell grid */
+        const int cw = (int)floor((w + 2) * inv) + 3, ch = (int)floor((h + 2) * inv) + 3;
+        int *head = (int *)malloc(sizeof(int) * (size_t)cw * ch), *next = (int *)malloc(sizeof(int) * ne);
+        int *ccx = (int *)malloc(sizeof(int) * ne), *ccy = (int *)malloc(sizeof(int) * ne);
+        for (size_t k = 0; k < (size_t)cw * ch; ++k) head[k] = -1;
+        for (long a = 0; a < ne; ++a) {
+            ccx[a] = (int)floor(ex[4 * a] * inv);
+            ccy[a] = (int)floor(ex[4 * a + 1] * inv);
+            const int cx = ccx[a] + 1, cy = ccy[a] + 1;
+            next[a] = head[(size_t)cy * cw + cx];
+            head[(size_t)cy * cw + cx] = (int)a;
+        }
+        for (long a = 0; a < ne; ++a) {
+            int found = 0;
+            for (int dy = -1; dy <= 1 && !found; ++dy)
+                for (int dx = -1; dx <= 1 && !found; ++dx) {
+                    const int cx = ccx[a] + 1 + dx, cy = ccy[a] + 1 + dy;
+                    if (cx < 0 || cy < 0 || cx >= cw || cy >= ch) continue;
+                    for (int b = head[(size_t)cy * cw + cx]; b >= 0; b = next[b]) {
+                        if (b == a) continue;
+                        const double ddx = ex[4 * a] - ex[4 * b], ddy = ex[4 * a + 1] - ex[4 * b + 1];
+                        if (ddx * ddx + ddy * ddy <= r2) { found = 1; break; }
+                    }
+                }
+            keep[a] = (char)found;
+        }
+        free(head); free(next); free(ccx); free(ccy);
+    } else {
+        for (long a = 0; a < ne; ++a) keep[a] = 1;
+    }
+    /* S8 */
+    if (TO_E) memset(TO_E, 0, sizeof(double) * n);
+    if (TO_O) memset(TO_O, 0, sizeof(double) * n);
+    long N = 0;
+    for (long a = 0; a < ne; ++a) {
+        if (!keep[a]) continue;
+        if (TO_E || TO_O) {
+            int X = (int)llround(ex[4 * a]), Y = (int)llround(ex[4 * a + 1]);
+            X = X < 0 ? 0 : (X > w - 1 ? w - 1 : X);
+            Y = Y < 0 ? 0 : (Y > h - 1 ? h - 1 : Y);
+            if (TO_E) TO_E[(size_t)Y * w + X] = ex[4 * a + 3];
+            if (TO_O) TO_O[(size_t)Y * w + X] = ex[4 * a + 2];
+        }
+        if (N < cap && out) memcpy(out + 4 * N, ex + 4 * a, 4 * sizeof(double));
+        if (N < cap && nms_idx) nms_idx[N] = eidx[a];
+        ++N;
+    }
+    free(keep); free(ex); free(eidx);
+    free(theta); free(dirx); free(diry); free(tmp); free(nx); free(ny); free(nm); free(nv);
+    for (int k = 0; k < 10; ++k) free(P[k]);
+    return N;
+}

@@ -12,7 +12,17 @@ from /root/reference by oracle/Makefile into oracle/_ref/toed_ref -- on
 
 Each fixture stores the uint8 input and the reference's N x 4 edgel list
 [x, y, theta, strength] in float64 (full precision, not the reference's
-6-digit text).  Run from the repo root:  python tests/golden/make_golden.py
+6-digit text).
+
+The subpixel third-order correction path (subpix_TO_correction/) gets its own
+fixtures: the reference's subpix_to_correction.cpp compiled unmodified into
+oracle/_ref/subpix_ref, run on seeded Bezier edge maps from the project's
+generator (the reference ships no edge maps of its own), with the orientation
+estimated from the map or given as a seeded random field.  Each stores the
+float32 map, the orientation field if given, the parameters, the N x 4 list
+and the TO_edgemap / TO_orientation maps in float64.
+
+Run from the repo root:  python tests/golden/make_golden.py
 """
 import importlib
 import os
@@ -31,6 +41,13 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 REF_IMAGES = "/root/reference/input_images"
 
 SCENES = [("scene_64x96_s1", 1, 64, 96), ("scene_128x160_s2", 2, 128, 160), ("scene_480x752_s3", 3, 480, 752)]
+
+# name, seed, H, W, curves, thresh, sigma, nbr_radius, orientation given
+SUBPIX = [("bezier_96x128_s1", 1, 96, 128, 12, 0.1, 2.0, 2.0, False),
+          ("bezier_128x160_s4_orient", 4, 128, 160, 12, 0.1, 2.0, 2.0, True),
+          ("bezier_160x200_s2_sig1.5", 2, 160, 200, 12, 0.1, 1.5, 2.0, False),
+          ("bezier_256x256_s3_t0.2_r0", 3, 256, 256, 12, 0.2, 2.0, 0.0, False),
+          ("bezier_512x512_s5", 5, 512, 512, 40, 0.1, 2.0, 2.0, False)]
 
 
 def read_pgm_p5(path):
@@ -109,6 +126,15 @@ def main():
         np.savez_compressed(path, img=img, edges=edges)
         print("%-22s %4dx%-4d N=%6d conv %.3fs nms %.4fs -> %s" % (name, img.shape[0], img.shape[1], len(edges),
                                                                     tc, tn, os.path.basename(path)))
+    for name, seed, H, W, nc, thr, sig, nbr, given in SUBPIX:
+        E = scene.bezier_map(seed, H, W, nc)
+        theta = np.random.default_rng(seed).uniform(-4.0, 4.0, (H, W)) if given else None
+        r = oracle.run_subpix_ref(E.astype(np.float64), theta, thresh=thr, sigma=sig, nbr_radius=nbr, nthreads=8)
+        path = os.path.join(OUT, "subpix_ref_%s.npz" % name)
+        extra = {} if theta is None else {"theta": theta}
+        np.savez_compressed(path, E=E, params=np.array([thr, sig, nbr]), edges=r["edges"], TO_E=r["TO_E"],
+                            TO_O=r["TO_O"], **extra)
+        print("%-28s %4dx%-4d N=%6d total %.3fs -> %s" % (name, H, W, r["n"], r["times"][3], os.path.basename(path)))
 
 
 if __name__ == "__main__":

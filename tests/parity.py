@@ -66,3 +66,33 @@ def assert_parity(rep, max_out_of_tol_frac=0.0):
     assert rep["unexplained_set"] == 0, rep
     unexpl = rep["out_of_tol"] - rep["out_of_tol_explained"]
     assert unexpl <= max_out_of_tol_frac * max(rep["matched"], 1), rep
+
+
+def compare_subpix(ref, gpu_edges, gpu_index):
+    """Subpixel correction path: GPU list vs oracle.subpix() result, matched
+    by map pixel (i * W + j).  Same tolerances as the detection path.  The
+    edgel set is decided in FP64 on both sides (orientation estimate, NMS,
+    isolation distances), so it must be identical."""
+    g = np.asarray(gpu_edges, np.float64)
+    gi = np.asarray(gpu_index, np.int64)
+    oi = ref["idx"]
+    rep = {"n_oracle": int(len(oi)), "n_gpu": int(len(gi))}
+    rep["gpu_sorted"] = bool(np.all(np.diff(gi) > 0)) if len(gi) > 1 else True
+    common, ia, ib = np.intersect1d(oi, gi, return_indices=True)
+    rep["only_oracle"] = int(len(oi) - len(common))
+    rep["only_gpu"] = int(len(gi) - len(common))
+    a, b = ref["edges"][ia], g[ib]
+    dxy = np.abs(a[:, :2] - b[:, :2]).max(1) if len(a) else np.zeros(0)
+    dth = np.abs(np.angle(np.exp(1j * (a[:, 2] - b[:, 2])))) if len(a) else np.zeros(0)
+    ds = np.abs(a[:, 3] - b[:, 3]) / np.maximum(np.abs(a[:, 3]), 1e-30) if len(a) else np.zeros(0)
+    rep["max_dxy"] = float(dxy.max()) if len(dxy) else 0.0
+    rep["max_dtheta"] = float(dth.max()) if len(dth) else 0.0
+    rep["max_ds_rel"] = float(ds.max()) if len(ds) else 0.0
+    rep["n_bad"] = int(((dxy > TOL_XY) | (dth > TOL_THETA) | (ds > TOL_S_REL)).sum())
+    return rep
+
+
+def assert_subpix_parity(rep):
+    assert rep["gpu_sorted"], rep
+    assert rep["only_oracle"] == 0 and rep["only_gpu"] == 0, rep
+    assert rep["n_bad"] == 0, rep

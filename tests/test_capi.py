@@ -45,3 +45,12 @@ def test_sigma_radius_host_only(pkg):
     for bad in [0.0, -1.0, 2.5, float("nan")]:
         with pytest.raises(pkg.ToedError):
             pkg.sigma_radius(bad)
+
+
+def test_subpix_radius_host_only(pkg):
+    assert pkg.subpix_radius(2.0) == 8
+    assert pkg.subpix_radius(1.5) == 6
+    assert pkg.subpix_radius(3.0) == 12
+    for bad in [0.0, -1.0, 3.1, float("nan")]:
+        with pytest.raises(pkg.ToedError):
+            pkg.subpix_radius(bad)

@@ -1,0 +1,130 @@
+"""Subpixel third-order correction path on the GPU (SubpixBatch over the C
+ABI) against the FP64 oracle, which itself equals the reference's own
+subpix_to_correction.cpp bit for bit on the golden maps."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from parity import assert_subpix_parity, compare_subpix
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "subpix_ref_*.npz")))
+
+
+def _run(pkg, detector, maps, orients=None, thresh=0.1, sigma=2.0, nbr=2.0, with_maps=False, caps=None):
+    dev = [torch.from_numpy(np.ascontiguousarray(E, np.float32)).cuda() for E in maps]
+    ors = None
+    if orients is not None:
+        ors = [None if o is None else torch.from_numpy(np.ascontiguousarray(o, np.float32)).cuda() for o in orients]
+    b = pkg.SubpixBatch(detector, dev, ors, thresh, sigma, nbr, capacities=caps, with_index=True,
+                        with_maps=with_maps)
+    b.run()
+    return b
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[11:-4] for p in GOLDEN])
+def test_subpix_golden(pkg, detector, oracle, path):
+    z = np.load(path)
+    thr, sig, nbr = (float(v) for v in z["params"])
+    theta = z["theta"] if "theta" in z.files else None
+    # the GPU takes float32 orientation maps: the oracle gets the same rounded values
+    th32 = None if theta is None else theta.astype(np.float32)
+    ref = oracle.subpix(z["E"].astype(np.float64), None if th32 is None else th32.astype(np.float64), thr, sig,
+                        nbr, maps=True)
+    if th32 is None:
+        assert np.array_equal(ref["edges"], z["edges"])
+    print("map %s sigma %g thresh %g r %g" % (z["E"].shape, sig, thr, nbr), flush=True)
+    b = _run(pkg, detector, [z["E"]], None if th32 is None else [th32], thr, sig, nbr, with_maps=True)
+    e = b.results()[0].cpu().numpy()
+    i = b.index_results()[0].cpu().numpy()
+    rep = compare_subpix(ref, e, i)
+    print(rep)
+    assert_subpix_parity(rep)
+    te = b.to_edgemap[0].cpu().numpy()
+    to = b.to_orientation[0].cpu().numpy()
+    assert np.array_equal(te != 0, ref["TO_E"] != 0)
+    np.testing.assert_allclose(te, ref["TO_E"], rtol=1e-4, atol=1e-7)
+    nz = ref["TO_E"] != 0
+    assert np.abs(np.angle(np.exp(1j * (to[nz] - ref["TO_O"][nz])))).max() < 1e-3
+    b.close()
+
+
+def test_subpix_mixed_batch(pkg, detector, oracle, scene):
+    shapes = [(96, 128), (200, 300), (64, 517), (333, 257), (12, 12)]
+    maps = [scene.bezier_map(40 + k, H, W, 12) for k, (H, W) in enumerate(shapes)]
+    rng = np.random.default_rng(7)
+    orients = [None, rng.uniform(-3.0, 3.0, shapes[1]).astype(np.float32), None, None, None]
+    thr = [0.1, 0.2, 0.15, 0.1, 0.1]
+    print("mixed batch", shapes, flush=True)
+    b = _run(pkg, detector, maps, orients, thr, 1.5, 2.0)
+    res, idx = b.results(), b.index_results()
+    for m in range(len(maps)):
+        ref = oracle.subpix(maps[m].astype(np.float64),
+                            None if orients[m] is None else orients[m].astype(np.float64), thr[m], 1.5, 2.0)
+        rep = compare_subpix(ref, res[m].cpu().numpy(), idx[m].cpu().numpy())
+        print(m, rep)
+        assert_subpix_parity(rep)
+    assert len(res[-1]) == 0          # 12 x 12: no NMS rows
+    b.close()
+
+
+@pytest.mark.parametrize("sigma", [1.0, 3.0])
+def test_subpix_sigma(pkg, detector, oracle, scene, sigma):
+    E = scene.bezier_map(11, 256, 320, 16)
+    print("sigma", sigma, flush=True)
+    b = _run(pkg, detector, [E], None, 0.1, sigma, 2.0)
+    ref = oracle.subpix(E.astype(np.float64), None, 0.1, sigma, 2.0)
+    rep = compare_subpix(ref, b.results()[0].cpu().numpy(), b.index_results()[0].cpu().numpy())
+    print(rep)
+    assert_subpix_parity(rep)
+    b.close()
+
+
+def test_subpix_full_size_map(pkg, detector, oracle, scene):
+    # one map of the benchmark shape: 2048 x 2048, 40 curves, thresh 0.1, sigma 2
+    E = scene.bezier_map(2024, 2048, 2048, 40)
+    print("2048 x 2048", flush=True)
+    b = _run(pkg, detector, [E], None, 0.1, 2.0, 2.0)
+    ref = oracle.subpix(E.astype(np.float64), None, 0.1, 2.0, 2.0, nthreads=8)
+    rep = compare_subpix(ref, b.results()[0].cpu().numpy(), b.index_results()[0].cpu().numpy())
+    print(rep)
+    assert rep["n_oracle"] > 50000
+    assert_subpix_parity(rep)
+    # repeated runs are bit-identical
+    e1 = b.results()[0].clone()
+    b.run()
+    assert torch.equal(e1, b.results()[0])
+    b.close()
+
+
+def test_subpix_capacity_flag(pkg, detector, scene):
+    E = scene.bezier_map(5, 128, 128, 12)
+    b = _run(pkg, detector, [E], caps=[10])
+    with pytest.raises(pkg.ToedError):
+        b.results()
+    assert int(b.counts[0]) > 10          # the count stays exact
+    b.close()
+
+
+def test_subpix_empty_and_flat_maps(pkg, detector):
+    b = _run(pkg, detector, [np.zeros((64, 64), np.float32), np.full((40, 50), 0.5, np.float32)])
+    assert [len(r) for r in b.results()] == [0, 0]
+    b.close()
+
+
+def test_subpix_class_mirror(pkg, detector, oracle, scene):
+    E = scene.bezier_map(3, 120, 150, 10)
+    c = pkg.SubpixelTOCorrectionGPU(120, 150, 0.1, 2.0, 0)
+    c.set_edgemap(E)
+    c.estimate_orientation()
+    out = np.zeros((120 * 150, 4), np.float32)
+    n = c.run(out)
+    ref = oracle.subpix(E.astype(np.float64), None, 0.1, 2.0, 2.0, maps=True)
+    assert n == ref["n"] == c.edge_pt_list_idx
+    np.testing.assert_allclose(out[:n, :2], ref["edges"][:, :2], atol=1e-3)
+    np.testing.assert_allclose(c.TO_edgemap, ref["TO_E"], rtol=1e-4, atol=1e-7)
+    c.close()

@@ -12,6 +12,9 @@ in the package's own HIP kernels.
     ThirdOrderEdgeDetectionGPU  mirror of the reference class (gpu_toed.hpp:21-74)
     detect_differentiable       torch.autograd.Function: gradients of the edgel
                                 rows w.r.t. the pixels and sigma (edgel set frozen)
+    SubpixBatch                 subpixel third-order correction of dense edge maps
+    SubpixelTOCorrectionGPU     mirror of SubpixelTOCorrectionCPU<T>
+                                (subpix_to_correction.hpp:20-96)
 """
 import ctypes
 import os
@@ -23,7 +26,8 @@ from . import _lib
 from ._lib import FLAG_CAPACITY, FLAG_STAGING, TOED_F32, TOED_U8, ToedError, check, lib
 
 __all__ = ["Detector", "BatchDetector", "ThirdOrderEdgeDetectionGPU", "ToedError", "sigma_radius",
-           "detect_differentiable", "DEFAULT_SIGMA", "DEFAULT_THRESH", "DEFAULT_BORDER"]
+           "detect_differentiable", "DEFAULT_SIGMA", "DEFAULT_THRESH", "DEFAULT_BORDER", "SubpixBatch",
+           "SubpixelTOCorrectionGPU", "subpix_radius"]
 
 # reference constants: sigma = 2 / 17-tap window (main_gpu_cpu.cpp:158-159),
 # threshold 2 (cpu_toed.cpp:403), NMS border 10 (cpu_toed.cpp:400-401)
@@ -342,3 +346,6 @@ def detect_differentiable(images, sigma, detector=None, thresh=DEFAULT_THRESH, b
     see the autograd module."""
     from .autograd import detect_differentiable as _dd
     return _dd(images, sigma, detector, thresh, border, capacity)
+
+
+from .subpix import SubpixBatch, SubpixelTOCorrectionGPU, subpix_radius  # noqa: E402

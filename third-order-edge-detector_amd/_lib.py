@@ -61,6 +61,21 @@ class toed_grad_out_t(ctypes.Structure):
     ]
 
 
+class toed_subpix_map_t(ctypes.Structure):
+    _fields_ = [
+        ("d_map", ctypes.c_void_p),
+        ("d_orient", ctypes.c_void_p),
+        ("H", ctypes.c_int), ("W", ctypes.c_int), ("pitch", ctypes.c_int), ("opitch", ctypes.c_int),
+        ("thresh", ctypes.c_double),
+        ("d_edges", ctypes.c_void_p),
+        ("capacity", ctypes.c_longlong),
+        ("d_index", ctypes.c_void_p),
+        ("d_to_edgemap", ctypes.c_void_p),
+        ("d_to_orientation", ctypes.c_void_p),
+        ("tpitch", ctypes.c_int),
+    ]
+
+
 # exported symbols and their signatures: (name, restype, argtypes)
 _VP, _I, _F, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
 _PI, _PF = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
@@ -85,6 +100,11 @@ SIGNATURES = [
     ("toed_batch_run", _I, [_VP, _VP, _VP]),
     ("toed_batch_graph_run", _I, [_VP, _VP, _VP]),
     ("toed_batch_destroy", _I, [_VP]),
+    ("toed_subpix_radius", _I, [ctypes.c_double, _PI]),
+    ("toed_subpix_plan", _I, [_VP, _I, ctypes.POINTER(toed_subpix_map_t), ctypes.c_double, ctypes.c_double,
+                              ctypes.POINTER(_VP)]),
+    ("toed_subpix_run", _I, [_VP, _VP, _VP]),
+    ("toed_subpix_destroy", _I, [_VP]),
     ("toed_gpu_convolve_f32", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F]),
     ("toed_gpu_nms_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F, _I]),
 ]

@@ -21,7 +21,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 HIP_LIBS = {
-    "libtoed_hip.so": ["toed_detect.hip", "toed_backward.hip"],
+    "libtoed_hip.so": ["toed_detect.hip", "toed_backward.hip", "toed_subpix.hip"],
 }
 
 

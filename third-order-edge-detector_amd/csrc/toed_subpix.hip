@@ -1,0 +1,841 @@
+// Subpixel third-order correction of dense edge maps on MI355X (gfx950):
+// the GPU form of the reference's SubpixelTOCorrectionCPU<T>
+// (subpix_TO_correction/subpix_to_correction.cpp, interp_n = 0).
+//
+// Per map (any number of maps of any sizes in one plan, one sigma):
+//   A  sp_orient_nms_kernel   one workgroup per 16 x 32 tile.  Orientation:
+//        given, or estimated from the map (:307-375): triangle smoothing r = 4
+//        with half-sample-symmetric borders (row pass, column pass, FP64, same
+//        summation order), central differences (one-sided at the map border),
+//        theta = mod(atan(Oyy sgn(-Oxy) / (Oxx + 1e-5)), pi), direction
+//        (cos, sin)(theta) (:296-305).  Writes the planes Hx = E dirx,
+//        Hy = E diry (FP32) and runs the octant NMS with the parabola in FP64
+//        (:413-519); survivors write their FP64 position and strength into
+//        dense per-pixel planes and set a bit in a per-row 32-bit mask word.
+//   B  sp_correct_kernel<R>   one workgroup per tile with survivors:
+//        isolated-edgel removal (:558-621) by looking up the survivor masks of
+//        the pixels within floor(r + 2 sqrt 2) (edgels lie within sqrt 2 of
+//        their pixel) -> kept mask words; then, for tiles with kept edgels,
+//        the eight circular separable Gaussian-derivative convolutions
+//        (:377-411, :641-648) of the tile plus a 3-pixel halo in LDS (FP32,
+//        x pass then y pass), Keys cubic sampling of the ten planes at every
+//        kept edgel (FP64, :521-556) and the third-order orientation
+//        (:663-700, FP64).
+//   C  sp_scan_chunks / sp_scan_maps   exclusive scan of the kept-mask
+//        popcounts in raster order (chunks of 4096 words, then per map).
+//   D  sp_scatter_kernel      edgel k of raster order -> row k of the N x 4
+//        list [x, y, orientation, strength] (:702-729).
+//   E  sp_maps_kernel         optional TO_edgemap / TO_orientation: each pixel
+//        gathers the kept edgels of its 3 x 3 neighbourhood whose rounded
+//        position is the pixel and takes the last in list order (:708-715).
+// No MFMA: short separable stencils.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "toed_capi.h"
+
+namespace toedsp {
+
+constexpr int TH = 16;          // tile rows
+constexpr int TW = 32;          // tile columns = one mask word per tile row
+constexpr int NT = 256;         // threads per workgroup
+constexpr int MAXR = 12;        // largest filter radius ceil(4 sigma)
+constexpr int MAXK = 8;         // largest isolation reach in pixels
+constexpr int CHUNKW = 4096;    // mask words per scan chunk
+constexpr int HALO = 3;         // cubic stencil reach around an edgel pixel
+
+struct SpTaps {
+    float k[3][2 * MAXR + 1];   // G, G', G'' at integer offsets -R..R (:194-214)
+    double r2;                  // isolation radius squared
+    int R, K, keep_all, pad;    // K = isolation reach; keep_all when radius <= 0
+};
+
+struct SpJob {
+    const float *map;
+    const float *orient;        // null: estimated
+    float *hx, *hy;             // H x W planes
+    double2 *pos;               // H x W: FP64 subpixel position of survivors
+    float2 *val;                // H x W: strength, orientation of survivors
+    uint32_t *vmask, *kmask;    // H x nw survivor / kept words
+    int *nsurv;                 // survivors of the whole map
+    float *edges;
+    int *index;
+    float *to_e, *to_o;
+    int *count_out, *flags_out;
+    long long capacity;
+    double thresh;
+    int H, W, pitch, opitch;
+    int nw, ntx, nty, tile_begin;
+    int chunk_begin, nchunks, tpitch, pad;
+};
+
+#define SP_CONST __attribute__((address_space(4)))
+
+__device__ __forceinline__ SpJob load_job(const SpJob *jobs, int njobs, int tile) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const SP_CONST SpJob *cj = (const SP_CONST SpJob *)jobs;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cj[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+    }
+    return cj[lo];
+#else
+    (void)njobs; (void)tile;
+    return jobs[0];
+#endif
+}
+
+__device__ __forceinline__ SpJob load_job_chunk(const SpJob *jobs, int njobs, int chunk) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const SP_CONST SpJob *cj = (const SP_CONST SpJob *)jobs;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cj[mid].chunk_begin <= chunk) lo = mid; else hi = mid - 1;
+    }
+    return cj[lo];
+#else
+    (void)njobs; (void)chunk;
+    return jobs[0];
+#endif
+}
+
+// half-sample-symmetric index (:47-55) and periodic index (:40-45)
+__device__ __forceinline__ int wrap_sym(int i, int n) {
+    if (n <= 1) return 0;
+    while (i < 0 || i >= n) i = (i < 0) ? -i - 1 : 2 * n - i - 1;
+    return i;
+}
+__device__ __forceinline__ int wrap_circ(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ double mod_pi(double a) {
+    double r = fmod(a, M_PI);
+    if (r < 0) r += M_PI;
+    return r;
+}
+
+// direction (cos, sin)(mod(atan(t), pi)) without trigonometry: atan(t) lies
+// in (-pi/2, pi/2); a negative angle moves up by pi, which flips both signs.
+__device__ __forceinline__ void dir_from_slope(double t, double &dx, double &dy) {
+    if (fabs(t) <= 1e150 && !(t < 0.0 && t > -1e-6)) {
+        const double inv = 1.0 / sqrt(1.0 + t * t);
+        if (t < 0.0) {
+            dx = -inv;
+            dy = -t * inv;
+        } else {
+            dx = inv;
+            dy = t * inv;
+        }
+        return;
+    }
+    // infinities, NaN, and small negative slopes, where atan(t) + pi may round
+    // to pi and the angle wraps to 0: the trigonometric route of the reference
+    const double a = mod_pi(mod_pi(atan(t)));
+    dx = cos(a);
+    dy = sin(a);
+}
+
+__device__ __forceinline__ double tri_w(int q) { return (double)(5 - (q < 0 ? -q : q)) / 25.0; }
+
+// ---------------------------------------------------------------------------
+// A: orientation, H planes, NMS
+constexpr int AE_R = TH + 12, AE_C = TW + 12;   // map rows/cols, tile +- 6
+constexpr int AT_C = TW + 4;                    // row-pass columns, tile +- 2
+constexpr int AS_R = TH + 4;                    // smoothed rows, tile +- 2
+
+__global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, int njobs) {
+    __shared__ float Es[AE_R][AE_C];
+    __shared__ double Tr[AE_R][AT_C];
+    __shared__ double Ss[AS_R][AT_C];
+    const int tile = blockIdx.x;
+    const SpJob J = load_job(jobs, njobs, tile);
+    const int lt = tile - J.tile_begin;
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int H = J.H, W = J.W;
+    const bool est = (J.orient == nullptr);
+
+    for (int idx = threadIdx.x; idx < AE_R * AE_C; idx += NT) {
+        const int r = idx / AE_C, c = idx - r * AE_C;
+        const int gy = wrap_sym(y0 - 6 + r, H), gx = wrap_sym(x0 - 6 + c, W);
+        Es[r][c] = J.map[(long long)gy * J.pitch + gx];
+    }
+    __syncthreads();
+    if (est) {
+        // row pass (:329-336): tmp(i, j) = sum_q E(i, sym(j - q)) tri[q]
+        for (int idx = threadIdx.x; idx < AE_R * AT_C; idx += NT) {
+            const int r = idx / AT_C, c = idx - r * AT_C;
+            double acc = 0.0;
+#pragma unroll
+            for (int q = -4; q <= 4; ++q) acc += (double)Es[r][c + 4 - q] * tri_w(q);
+            Tr[r][c] = acc;
+        }
+        __syncthreads();
+        // column pass (:337-344)
+        for (int idx = threadIdx.x; idx < AS_R * AT_C; idx += NT) {
+            const int r = idx / AT_C, c = idx - r * AT_C;
+            double acc = 0.0;
+#pragma unroll
+            for (int p = -4; p <= 4; ++p) acc += Tr[r + 4 - p][c] * tri_w(p);
+            Ss[r][c] = acc;
+        }
+        __syncthreads();
+    }
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        const int li = p / TW, lj = p - li * TW;
+        const int gi = y0 + li, gj = x0 + lj;
+        bool surv = false;
+        if (gi < H && gj < W) {
+            double dx, dy;
+            if (est) {
+                // smoothed map S(i, j) for in-map (i, j) of the tile +- 2 window
+                auto S = [&](int i, int j) -> double { return Ss[i - y0 + 2][j - x0 + 2]; };
+                // gradient2 (:346-357): central differences, one-sided at the border
+                auto Ox = [&](int i, int j) -> double {
+                    if (j == 0) return S(i, 1) - S(i, 0);
+                    if (j == W - 1) return S(i, W - 1) - S(i, W - 2);
+                    return 0.5 * (S(i, j + 1) - S(i, j - 1));
+                };
+                auto Oy = [&](int i, int j) -> double {
+                    if (i == 0) return S(1, j) - S(0, j);
+                    if (i == H - 1) return S(H - 1, j) - S(H - 2, j);
+                    return 0.5 * (S(i + 1, j) - S(i - 1, j));
+                };
+                double oxx, oxy, oyy;
+                if (gj == 0) { oxx = Ox(gi, 1) - Ox(gi, 0); oxy = Oy(gi, 1) - Oy(gi, 0); }
+                else if (gj == W - 1) { oxx = Ox(gi, W - 1) - Ox(gi, W - 2); oxy = Oy(gi, W - 1) - Oy(gi, W - 2); }
+                else { oxx = 0.5 * (Ox(gi, gj + 1) - Ox(gi, gj - 1)); oxy = 0.5 * (Oy(gi, gj + 1) - Oy(gi, gj - 1)); }
+                if (gi == 0) oyy = Oy(1, gj) - Oy(0, gj);
+                else if (gi == H - 1) oyy = Oy(H - 1, gj) - Oy(H - 2, gj);
+                else oyy = 0.5 * (Oy(gi + 1, gj) - Oy(gi - 1, gj));
+                const double sgn = (oxy < 0) ? 1.0 : ((oxy > 0) ? -1.0 : 0.0);
+                dir_from_slope(oyy * sgn / (oxx + 1e-5), dx, dy);
+            } else {
+                const double a = mod_pi((double)J.orient[(long long)gi * J.opitch + gj]);
+                dx = cos(a);
+                dy = sin(a);
+            }
+            const double e = (double)Es[li + 6][lj + 6];
+            const long long o = (long long)gi * W + gj;
+            J.hx[o] = (float)(e * dx);
+            J.hy[o] = (float)(e * dy);
+            // NMS (:446-516): rows/cols [margin + 1, n - margin - 2), margin 3
+            if (gi >= 4 && gi < H - 5 && gj >= 4 && gj < W - 5 && !(e <= J.thresh) &&
+                !(fabs(dx) < 1e-5 && fabs(dy) < 1e-5)) {
+                auto M = [&](int di, int dj) -> double { return (double)Es[li + 6 + di][lj + 6 + dj]; };
+                const double md = sqrt(dx * dx + dy * dy);
+                const double nx = dx / md, ny = dy / md;
+                double slope, fp, fm;
+                if (dx >= 0 && dy >= 0) {
+                    if (dx >= dy) { slope = ny / nx; fp = M(0, 1) * (1 - slope) + M(1, 1) * slope; fm = M(0, -1) * (1 - slope) + M(-1, -1) * slope; }
+                    else { slope = nx / ny; fp = M(1, 0) * (1 - slope) + M(1, 1) * slope; fm = M(-1, 0) * (1 - slope) + M(-1, -1) * slope; }
+                } else if (dx < 0 && dy >= 0) {
+                    if (fabs(dx) < dy) { slope = -nx / ny; fp = M(1, 0) * (1 - slope) + M(1, -1) * slope; fm = M(-1, 0) * (1 - slope) + M(-1, 1) * slope; }
+                    else { slope = -ny / nx; fp = M(0, -1) * (1 - slope) + M(1, -1) * slope; fm = M(0, 1) * (1 - slope) + M(-1, 1) * slope; }
+                } else if (dx < 0 && dy < 0) {
+                    if (fabs(dx) >= fabs(dy)) { slope = ny / nx; fp = M(0, -1) * (1 - slope) + M(-1, -1) * slope; fm = M(0, 1) * (1 - slope) + M(1, 1) * slope; }
+                    else { slope = nx / ny; fp = M(-1, 0) * (1 - slope) + M(-1, -1) * slope; fm = M(1, 0) * (1 - slope) + M(1, 1) * slope; }
+                } else {
+                    if (dx < fabs(dy)) { slope = -nx / ny; fp = M(-1, 0) * (1 - slope) + M(-1, 1) * slope; fm = M(1, 0) * (1 - slope) + M(1, -1) * slope; }
+                    else { slope = -ny / nx; fp = M(0, 1) * (1 - slope) + M(-1, 1) * slope; fm = M(0, -1) * (1 - slope) + M(1, -1) * slope; }
+                }
+                const double f = e;
+                if ((f > fm && f > fp) || (f > fm && f >= fp) || (f >= fm && f > fp)) {
+                    const double s = sqrt(1 + slope * slope);
+                    const double A = (fm + fp - 2 * f) / (2 * s * s);
+                    const double B = (fp - fm) / (2 * s);
+                    if (!(fabs(A) < 1e-16)) {
+                        const double ss = -B / (2 * A);
+                        const double mf = A * ss * ss + B * ss + f;
+                        if (fabs(ss) <= sqrt(2.0)) {
+                            J.pos[o] = make_double2(gj + ss * nx, gi + ss * ny);
+                            J.val[o] = make_float2((float)fabs(mf), (float)atan2(dx, -dy));
+                            surv = true;
+                        }
+                    }
+                }
+            }
+        }
+        const unsigned long long b = __ballot(surv);
+        if ((lane & 31) == 0) {
+            const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
+            if (row < H) J.vmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
+        }
+        if (lane == 0 && b) atomicAdd(J.nsurv, (int)__popcll(b));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// B: isolation, circular convolutions, cubic sampling, third-order orientation
+__device__ __forceinline__ double keys(double s) {
+    const double a = -0.5;
+    const double x = fabs(s);
+    if (x <= 1.0) return ((a + 2) * x - (a + 3)) * x * x + 1;
+    if (x < 2.0) return ((a * x - 5 * a) * x + 8 * a) * x - 4 * a;
+    return 0.0;
+}
+
+template <int R> struct CGeo {
+    static constexpr int NK = 2 * R + 1;
+    static constexpr int PR = TH + 2 * HALO, PC = TW + 2 * HALO;   // plane region
+    static constexpr int IR = PR + 2 * R, IC = PC + 2 * R;         // input region
+    static constexpr int IN = 3 * IR * IC;                         // E, Hx, Hy
+    static constexpr int XN = 3 * IR * PC;                         // x-pass, one group
+    static constexpr int YN = 8 * PR * PC;                         // 8 planes
+    static constexpr int FLOATS = IN + XN + YN;
+};
+
+template <int R>
+__global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+    using G = CGeo<R>;
+    constexpr int NK = G::NK, PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
+    __shared__ float lds[G::FLOATS];
+    __shared__ uint32_t vm[TH + 2 * MAXK][3];
+    __shared__ int s_flag;
+    float *In = lds, *X = lds + G::IN, *Y = lds + G::IN + G::XN;
+
+    const int tile = blockIdx.x;
+    const SpJob J = load_job(jobs, njobs, tile);
+    const int lt = tile - J.tile_begin;
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int H = J.H, W = J.W, K = tp.K;
+
+    // survivor words of rows y0-K .. y0+TH+K, word columns tx-1 .. tx+1
+    if (threadIdx.x == 0) s_flag = 0;
+    for (int idx = threadIdx.x; idx < (TH + 2 * K) * 3; idx += NT) {
+        const int r = idx / 3, c = idx - r * 3;
+        const int gy = y0 - K + r, wc = tx - 1 + c;
+        vm[r][c] = (gy >= 0 && gy < H && wc >= 0 && wc < J.nw) ? J.vmask[(long long)gy * J.nw + wc] : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x < TH && vm[K + threadIdx.x][1]) s_flag = 1;
+    __syncthreads();
+    const bool any_surv = s_flag != 0;
+    __syncthreads();
+    const bool keep_all = tp.keep_all || (*J.nsurv < 2);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    // isolated-edgel removal: another survivor within the radius
+    int keptm = 0;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        const int li = p / TW, lj = p - li * TW;
+        const int gi = y0 + li, gj = x0 + lj;
+        bool kp = false;
+        if (any_surv && (vm[K + li][1] >> lj) & 1u) {
+            if (keep_all) {
+                kp = true;
+            } else {
+                const double2 me = J.pos[(long long)gi * W + gj];
+                for (int dy = -K; dy <= K && !kp; ++dy) {
+                    const int r = K + li + dy;
+                    for (int dxp = -K; dxp <= K; ++dxp) {
+                        const int c = TW + lj + dxp;          // column in the 3-word window
+                        if (!((vm[r][c >> 5] >> (c & 31)) & 1u) || (dy == 0 && dxp == 0)) continue;
+                        const double2 o = J.pos[(long long)(gi + dy) * W + (gj + dxp)];
+                        const double ddx = me.x - o.x, ddy = me.y - o.y;
+                        if (ddx * ddx + ddy * ddy <= tp.r2) { kp = true; break; }
+                    }
+                }
+            }
+        }
+        keptm |= (int)kp << k;
+        const unsigned long long b = __ballot(kp);
+        if ((lane & 31) == 0) {
+            const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
+            if (row < H) J.kmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
+        }
+        if (lane == 0 && b) s_flag = 2;
+    }
+    __syncthreads();
+    if (s_flag != 2) return;
+
+    // load E, Hx, Hy over the plane region + R, periodic borders
+    const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
+    for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
+        const int r = idx / IC, c = idx - r * IC;
+        const int gy = wrap_circ(iy0 + r, H), gx = wrap_circ(ix0 + c, W);
+        const long long o = (long long)gy * W + gx;
+        In[idx] = J.map[(long long)gy * J.pitch + gx];
+        In[IR * IC + idx] = J.hx[o];
+        In[2 * IR * IC + idx] = J.hy[o];
+    }
+    __syncthreads();
+    // group 0: E * (G', G, G'') along x -> Px, Pxx, Py, Pxy, Pyy
+    // group 1: Hx * G', Hy * G, Hy * G' along x -> Hxx, Hyy, Hxy
+    // planes Y: 0 Px, 1 Py, 2 Pxx, 3 Pyy, 4 Pxy, 5 Hxx, 6 Hxy, 7 Hyy
+#pragma unroll 1
+    for (int grp = 0; grp < 2; ++grp) {
+        for (int idx = threadIdx.x; idx < IR * PC; idx += NT) {
+            const int r = idx / PC, c = idx - r * PC;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+            if (grp == 0) {
+                const float *src = In + r * IC + c;
+#pragma unroll
+                for (int q = -R; q <= R; ++q) {
+                    const float v = src[R - q];
+                    a0 = fmaf(v, tp.k[1][q + R], a0);
+                    a1 = fmaf(v, tp.k[0][q + R], a1);
+                    a2 = fmaf(v, tp.k[2][q + R], a2);
+                }
+            } else {
+                const float *sx = In + IR * IC + r * IC + c;
+                const float *sy = In + 2 * IR * IC + r * IC + c;
+#pragma unroll
+                for (int q = -R; q <= R; ++q) {
+                    const float vx = sx[R - q], vy = sy[R - q];
+                    a0 = fmaf(vx, tp.k[1][q + R], a0);
+                    a1 = fmaf(vy, tp.k[0][q + R], a1);
+                    a2 = fmaf(vy, tp.k[1][q + R], a2);
+                }
+            }
+            X[idx] = a0;
+            X[IR * PC + idx] = a1;
+            X[2 * IR * PC + idx] = a2;
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < PR * PC; idx += NT) {
+            const int r = idx / PC, c = idx - r * PC;
+            const float *x0p = X + r * PC + c, *x1p = X + IR * PC + r * PC + c, *x2p = X + 2 * IR * PC + r * PC + c;
+            if (grp == 0) {
+                float px = 0.f, py = 0.f, pxx = 0.f, pxy = 0.f, pyy = 0.f;
+#pragma unroll
+                for (int p = -R; p <= R; ++p) {
+                    const int o = (R - p) * PC;
+                    const float dgx = x0p[o], gx = x1p[o], d2gx = x2p[o];
+                    px = fmaf(dgx, tp.k[0][p + R], px);
+                    pxy = fmaf(dgx, tp.k[1][p + R], pxy);
+                    py = fmaf(gx, tp.k[1][p + R], py);
+                    pyy = fmaf(gx, tp.k[2][p + R], pyy);
+                    pxx = fmaf(d2gx, tp.k[0][p + R], pxx);
+                }
+                Y[0 * PR * PC + idx] = px;
+                Y[1 * PR * PC + idx] = py;
+                Y[2 * PR * PC + idx] = pxx;
+                Y[3 * PR * PC + idx] = pyy;
+                Y[4 * PR * PC + idx] = pxy;
+            } else {
+                float hxx = 0.f, hyy = 0.f, hxy = 0.f;
+#pragma unroll
+                for (int p = -R; p <= R; ++p) {
+                    const int o = (R - p) * PC;
+                    hxx = fmaf(x0p[o], tp.k[0][p + R], hxx);
+                    hyy = fmaf(x1p[o], tp.k[1][p + R], hyy);
+                    hxy = fmaf(x2p[o], tp.k[0][p + R], hxy);
+                }
+                Y[5 * PR * PC + idx] = hxx;
+                Y[6 * PR * PC + idx] = hxy;
+                Y[7 * PR * PC + idx] = hyy;
+            }
+        }
+        __syncthreads();
+    }
+
+    // cubic sampling of the ten planes and the orientation, one edgel per thread
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        if (!((keptm >> k) & 1)) continue;
+        const int p = threadIdx.x + k * NT;
+        const int li = p / TW, lj = p - li * TW;
+        const int gi = y0 + li, gj = x0 + lj;
+        const long long o = (long long)gi * W + gj;
+        const double2 ps = J.pos[o];
+        const int xf = (int)floor(ps.x), yf = (int)floor(ps.y);
+        const double fx = ps.x - xf, fy = ps.y - yf;
+        const double wx[4] = {keys(fx + 1), keys(fx), keys(fx - 1), keys(fx - 2)};
+        const double wy[4] = {keys(fy + 1), keys(fy), keys(fy - 1), keys(fy - 2)};
+        double v[10];
+#pragma unroll
+        for (int m = 0; m < 10; ++m) v[m] = 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int ii = min(max(yf - 1 + a, 0), H - 1) - (y0 - HALO);
+            double row[10];
+#pragma unroll
+            for (int m = 0; m < 10; ++m) row[m] = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int jj = min(max(xf - 1 + b, 0), W - 1) - (x0 - HALO);
+                const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
+                const double w = wx[b];
+                row[0] += (double)Y[0 * PR * PC + yo] * w;   // Px
+                row[1] += (double)Y[1 * PR * PC + yo] * w;   // Py
+                row[2] += (double)Y[2 * PR * PC + yo] * w;   // Pxx
+                row[3] += (double)Y[3 * PR * PC + yo] * w;   // Pyy
+                row[4] += (double)Y[4 * PR * PC + yo] * w;   // Pxy
+                row[5] += (double)In[IR * IC + io] * w;      // Hx
+                row[6] += (double)Y[5 * PR * PC + yo] * w;   // Hxx
+                row[7] += (double)Y[6 * PR * PC + yo] * w;   // Hxy
+                row[8] += (double)Y[7 * PR * PC + yo] * w;   // Hyy
+                row[9] += (double)In[2 * IR * IC + io] * w;  // Hy
+            }
+#pragma unroll
+            for (int m = 0; m < 10; ++m) v[m] += row[m] * wy[a];
+        }
+        double Fx = v[0] * v[6] + v[2] * v[5] + v[1] * v[7] + v[4] * v[9];
+        double Fy = v[0] * v[7] + v[4] * v[5] + v[1] * v[8] + v[3] * v[9];
+        const double Fm = sqrt(Fx * Fx + Fy * Fy);
+        if (!(Fm < 1e-12)) {
+            Fx /= Fm;
+            Fy /= Fm;
+            J.val[o].y = (float)atan2(Fx, -Fy);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// C: scan of kept-word popcounts (raster order), chunk totals, per-map totals
+__global__ __launch_bounds__(256) void sp_scan_chunks(const SpJob *jobs, int njobs, int *chunk_sum) {
+    __shared__ int wsum[4];
+    const int chunk = blockIdx.x;
+    const SpJob J = load_job_chunk(jobs, njobs, chunk);
+    const long long nwords = (long long)J.H * J.nw;
+    const long long w0 = (long long)(chunk - J.chunk_begin) * CHUNKW + threadIdx.x * (CHUNKW / 256);
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < CHUNKW / 256; ++k)
+        if (w0 + k < nwords) s += __popc(J.kmask[w0 + k]);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) chunk_sum[chunk] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chunk_sum) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    const SpJob &J = jobs[blockIdx.x];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int base = 0; base < J.nchunks; base += 1024) {
+        const int k = base + threadIdx.x;
+        const int x = (k < J.nchunks) ? chunk_sum[J.chunk_begin + k] : 0;
+        int inc = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int wb = carry;
+        for (int w = 0; w < wave; ++w) wb += wsum[w];
+        if (k < J.nchunks) chunk_sum[J.chunk_begin + k] = wb + inc - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = wb + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (J.count_out) *J.count_out = carry;
+        if (carry > J.capacity && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_CAPACITY);
+    }
+}
+
+// D: scatter in raster order
+__global__ __launch_bounds__(256) void sp_scatter_kernel(const SpJob *jobs, int njobs, const int *chunk_off) {
+    __shared__ int wsum[4];
+    const int chunk = blockIdx.x;
+    const SpJob J = load_job_chunk(jobs, njobs, chunk);
+    const long long nwords = (long long)J.H * J.nw;
+    constexpr int PER = CHUNKW / 256;
+    const long long w0 = (long long)(chunk - J.chunk_begin) * CHUNKW + threadIdx.x * PER;
+    uint32_t wv[PER];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        wv[k] = (w0 + k < nwords) ? J.kmask[w0 + k] : 0u;
+        s += __popc(wv[k]);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d);
+        if (lane >= d) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int rank = chunk_off[chunk];
+    for (int w = 0; w < wave; ++w) rank += wsum[w];
+    rank += inc - s;
+    float4 *out = reinterpret_cast<float4 *>(J.edges);
+#pragma unroll 1
+    for (int k = 0; k < PER; ++k) {
+        uint32_t bits = wv[k];
+        if (!bits) continue;
+        const long long w = w0 + k;
+        const int gi = (int)(w / J.nw), gj0 = (int)(w - (long long)gi * J.nw) * 32;
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            if (rank < J.capacity) {
+                const long long o = (long long)gi * J.W + gj0 + b;
+                const double2 ps = J.pos[o];
+                const float2 vv = J.val[o];
+                out[rank] = make_float4((float)ps.x, (float)ps.y, vv.y, vv.x);
+                if (J.index) J.index[rank] = (int)o;
+            }
+            ++rank;
+        }
+    }
+}
+
+// E: TO_edgemap / TO_orientation by gather
+__global__ __launch_bounds__(NT) void sp_maps_kernel(const SpJob *jobs, int njobs) {
+    const int tile = blockIdx.x;
+    const SpJob J = load_job(jobs, njobs, tile);
+    if (!J.to_e && !J.to_o) return;
+    const int lt = tile - J.tile_begin;
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    const int H = J.H, W = J.W;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        const int Y = ty * TH + p / TW, X = tx * TW + (p % TW);
+        if (Y >= H || X >= W) continue;
+        long long win = -1;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int i = Y + dy;
+            if (i < 0 || i >= H) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int j = X + dx;
+                if (j < 0 || j >= W) continue;
+                if (!((J.kmask[(long long)i * J.nw + (j >> 5)] >> (j & 31)) & 1u)) continue;
+                const long long o = (long long)i * W + j;
+                const double2 ps = J.pos[o];
+                const int rx = min(max((int)round(ps.x), 0), W - 1), ry = min(max((int)round(ps.y), 0), H - 1);
+                if (rx == X && ry == Y && o > win) win = o;
+            }
+        }
+        float e = 0.f, th = 0.f;
+        if (win >= 0) {
+            const float2 vv = J.val[win];
+            e = vv.x;
+            th = vv.y;
+        }
+        const long long t = (long long)Y * J.tpitch + X;
+        if (J.to_e) J.to_e[t] = e;
+        if (J.to_o) J.to_o[t] = th;
+    }
+}
+
+// zero per-map counters and flags (a kernel, so the sequence can be captured)
+__global__ void sp_reset_kernel(const SpJob *jobs, int njobs) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= njobs) return;
+    *jobs[m].nsurv = 0;
+    if (jobs[m].flags_out) *jobs[m].flags_out = 0;
+    if (jobs[m].count_out) *jobs[m].count_out = 0;
+}
+
+}  // namespace toedsp
+
+using namespace toedsp;
+
+struct toed_subpix {
+    toed_ctx *ctx = nullptr;
+    int device = 0, n = 0, R = 0;
+    SpTaps taps{};
+    std::vector<SpJob> jobs;
+    SpJob *d_jobs = nullptr;
+    char *d_work = nullptr;
+    int *d_chunk = nullptr;
+    int *d_nsurv = nullptr;
+    long long ntiles = 0, nchunks = 0;
+};
+
+#define SPCHK(x)                                                                                         \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) {                                                                          \
+            std::fprintf(stderr, "toed: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return TOED_ERR_HIP;                                                                         \
+        }                                                                                                \
+    } while (0)
+
+template <int R> static void launch_correct(hipStream_t s, const toed_subpix *p) {
+    hipLaunchKernelGGL(sp_correct_kernel<R>, dim3((unsigned)p->ntiles), dim3(NT), 0, s, p->d_jobs, p->n, p->taps);
+}
+
+static int launch_correct_r(hipStream_t s, const toed_subpix *p) {
+    switch (p->R) {
+#define SP_CASE(r) case r: launch_correct<r>(s, p); break;
+        SP_CASE(1) SP_CASE(2) SP_CASE(3) SP_CASE(4) SP_CASE(5) SP_CASE(6)
+        SP_CASE(7) SP_CASE(8) SP_CASE(9) SP_CASE(10) SP_CASE(11) SP_CASE(12)
+#undef SP_CASE
+        default: return TOED_ERR_SIGMA;
+    }
+    return TOED_OK;
+}
+
+static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+extern "C" {
+
+int toed_subpix_radius(double sigma, int *R) {
+    if (!R) return TOED_ERR_ARG;
+    if (!(sigma > 0) || !std::isfinite(sigma)) return TOED_ERR_SIGMA;
+    const int r = (int)std::ceil(sigma * 4);
+    if (r < 1 || r > MAXR) return TOED_ERR_SIGMA;
+    *R = r;
+    return TOED_OK;
+}
+
+int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double sigma, double nbr_radius,
+                     toed_subpix **out) {
+    if (!ctx || !out || n <= 0 || !maps) return TOED_ERR_ARG;
+    *out = nullptr;
+    int R = 0;
+    const int st = toed_subpix_radius(sigma, &R);
+    if (st) return st;
+    if (!(nbr_radius == nbr_radius)) return TOED_ERR_ARG;
+    const int K = nbr_radius > 0 ? (int)std::floor(nbr_radius + 2.8284271247461903 + 1e-9) : 0;
+    if (K > MAXK) return TOED_ERR_ARG;
+    for (int m = 0; m < n; ++m) {
+        const toed_subpix_map_t &M = maps[m];
+        if (!M.d_map || M.H < 2 || M.W < 2 || M.pitch < M.W || !M.d_edges || M.capacity < 0) return TOED_ERR_ARG;
+        if (M.d_orient && M.opitch < M.W) return TOED_ERR_ARG;
+        if ((M.d_to_edgemap || M.d_to_orientation) && M.tpitch < M.W) return TOED_ERR_ARG;
+        if ((long long)M.H * M.W >= (1LL << 31)) return TOED_ERR_ARG;
+    }
+    toed_subpix *p = new toed_subpix();
+    p->ctx = ctx;
+    p->device = toed_get_device(ctx);
+    p->n = n;
+    p->R = R;
+    // taps (:194-214), FP64 closed forms rounded once to FP32
+    {
+        const double s2 = sigma * sigma, s3 = s2 * sigma, s5 = s2 * s3, nrm = std::sqrt(2.0 * M_PI);
+        for (int k = -R; k <= R; ++k) {
+            const double e = std::exp(-((double)k * k) / (2 * s2));
+            p->taps.k[0][k + R] = (float)(e / (nrm * sigma));
+            p->taps.k[1][k + R] = (float)(-k * e / (nrm * s3));
+            p->taps.k[2][k + R] = (float)(((double)k * k - s2) * e / (nrm * s5));
+        }
+        p->taps.R = R;
+        p->taps.K = K;
+        p->taps.keep_all = nbr_radius > 0 ? 0 : 1;
+        p->taps.r2 = nbr_radius * nbr_radius;
+    }
+    // work planes per map
+    size_t total = 0;
+    std::vector<size_t> off(n);
+    for (int m = 0; m < n; ++m) {
+        const size_t px = (size_t)maps[m].H * maps[m].W, nw = (size_t)(maps[m].W + 31) / 32;
+        off[m] = total;
+        total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(maps[m].H * nw * 4) * 2;
+    }
+    if (hipSetDevice(p->device) != hipSuccess) { delete p; return TOED_ERR_HIP; }
+    if (hipMalloc(&p->d_work, total) != hipSuccess) { delete p; return TOED_ERR_ALLOC; }
+    if (hipMalloc(&p->d_nsurv, sizeof(int) * n) != hipSuccess) { toed_subpix_destroy(p); return TOED_ERR_ALLOC; }
+    long long tiles = 0, chunks = 0;
+    p->jobs.resize(n);
+    for (int m = 0; m < n; ++m) {
+        const toed_subpix_map_t &M = maps[m];
+        SpJob &J = p->jobs[m];
+        std::memset(&J, 0, sizeof(J));
+        const size_t px = (size_t)M.H * M.W;
+        char *b = p->d_work + off[m];
+        J.hx = (float *)b; b += align_up(px * 4);
+        J.hy = (float *)b; b += align_up(px * 4);
+        J.pos = (double2 *)b; b += align_up(px * 16);
+        J.val = (float2 *)b; b += align_up(px * 8);
+        J.nw = (M.W + 31) / 32;
+        J.vmask = (uint32_t *)b; b += align_up((size_t)M.H * J.nw * 4);
+        J.kmask = (uint32_t *)b;
+        J.nsurv = p->d_nsurv + m;
+        J.map = M.d_map;
+        J.orient = M.d_orient;
+        J.edges = M.d_edges;
+        J.index = M.d_index;
+        J.to_e = M.d_to_edgemap;
+        J.to_o = M.d_to_orientation;
+        J.capacity = M.capacity;
+        J.thresh = M.thresh;
+        J.H = M.H;
+        J.W = M.W;
+        J.pitch = M.pitch;
+        J.opitch = M.d_orient ? M.opitch : M.W;
+        J.tpitch = (M.d_to_edgemap || M.d_to_orientation) ? M.tpitch : M.W;
+        J.ntx = (M.W + TW - 1) / TW;
+        J.nty = (M.H + TH - 1) / TH;
+        J.tile_begin = (int)tiles;
+        tiles += (long long)J.ntx * J.nty;
+        J.nchunks = (int)(((long long)M.H * J.nw + CHUNKW - 1) / CHUNKW);
+        J.chunk_begin = (int)chunks;
+        chunks += J.nchunks;
+    }
+    if (tiles >= (1LL << 31) || chunks >= (1LL << 31)) { toed_subpix_destroy(p); return TOED_ERR_ARG; }
+    p->ntiles = tiles;
+    p->nchunks = chunks;
+    if (hipMalloc(&p->d_chunk, sizeof(int) * (size_t)chunks) != hipSuccess ||
+        hipMalloc(&p->d_jobs, sizeof(SpJob) * (size_t)n) != hipSuccess) {
+        toed_subpix_destroy(p);
+        return TOED_ERR_ALLOC;
+    }
+    if (hipMemcpy(p->d_jobs, p->jobs.data(), sizeof(SpJob) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        toed_subpix_destroy(p);
+        return TOED_ERR_HIP;
+    }
+    *out = p;
+    return TOED_OK;
+}
+
+int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
+    if (!p) return TOED_ERR_ARG;
+    SPCHK(hipSetDevice(p->device));
+    bool changed = false;
+    for (int m = 0; m < p->n; ++m) {
+        int *c = d_counts ? d_counts + m : nullptr, *f = d_flags ? d_flags + m : nullptr;
+        if (p->jobs[m].count_out != c || p->jobs[m].flags_out != f) changed = true;
+        p->jobs[m].count_out = c;
+        p->jobs[m].flags_out = f;
+    }
+    hipStream_t s = (hipStream_t)toed_get_stream(p->ctx);
+    if (changed) SPCHK(hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(SpJob) * p->n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(sp_reset_kernel, dim3((p->n + 255) / 256), dim3(256), 0, s, p->d_jobs, p->n);
+    hipLaunchKernelGGL(sp_orient_nms_kernel, dim3((unsigned)p->ntiles), dim3(NT), 0, s, p->d_jobs, p->n);
+    const int st = launch_correct_r(s, p);
+    if (st) return st;
+    hipLaunchKernelGGL(sp_scan_chunks, dim3((unsigned)p->nchunks), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
+    hipLaunchKernelGGL(sp_scan_maps, dim3(p->n), dim3(1024), 0, s, p->d_jobs, p->d_chunk);
+    hipLaunchKernelGGL(sp_scatter_kernel, dim3((unsigned)p->nchunks), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
+    bool maps = false;
+    for (const auto &J : p->jobs) maps = maps || J.to_e || J.to_o;
+    if (maps) hipLaunchKernelGGL(sp_maps_kernel, dim3((unsigned)p->ntiles), dim3(NT), 0, s, p->d_jobs, p->n);
+    SPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+int toed_subpix_destroy(toed_subpix *p) {
+    if (!p) return TOED_ERR_ARG;
+    (void)hipSetDevice(p->device);
+    (void)hipStreamSynchronize((hipStream_t)toed_get_stream(p->ctx));
+    (void)hipFree(p->d_work);
+    (void)hipFree(p->d_chunk);
+    (void)hipFree(p->d_nsurv);
+    (void)hipFree(p->d_jobs);
+    delete p;
+    return TOED_OK;
+}
+
+}  // extern "C"

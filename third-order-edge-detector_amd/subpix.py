@@ -1,0 +1,211 @@
+"""Subpixel third-order correction of dense edge maps (GPU form of the
+reference's subpix_TO_correction/ path) over the C ABI of libtoed_hip.so.
+
+    SubpixBatch               many float32 maps of any sizes, one plan and one
+                              launch sequence -> per-map raster-ordered N x 4
+                              lists [x, y, orientation, strength], optional
+                              TO_edgemap / TO_orientation maps
+    SubpixelTOCorrectionGPU   mirror of SubpixelTOCorrectionCPU<T>
+                              (subpix_to_correction.hpp:20-96)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import FLAG_CAPACITY, ToedError, check, lib
+
+# reference defaults (main_subpix_to_correction.cpp:147-151)
+DEFAULT_THRESH = 0.2
+DEFAULT_SIGMA = 2.0
+DEFAULT_NBR_RADIUS = 2.0
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def subpix_radius(sigma):
+    """Filter radius ceil(4 sigma) of the correction path (subpix_to_correction.cpp:107)."""
+    r = ctypes.c_int()
+    check(lib().toed_subpix_radius(float(sigma), ctypes.byref(r)), "toed_subpix_radius")
+    return r.value
+
+
+class SubpixBatch:
+    """Edge maps (2-D float32 device tensors, row stride >= width, unit column
+    stride) corrected by one plan.  ``orients`` (optional, same shapes, radians)
+    gives the orientation of every map, or None per map to estimate it from
+    the map.  Outputs: one device buffer of sum(capacity) x 4 floats; map m's
+    edgels start at row ``offsets[m]``, ``counts[m]`` of them."""
+
+    def __init__(self, detector, maps, orients=None, thresh=DEFAULT_THRESH, sigma=DEFAULT_SIGMA,
+                 nbr_radius=DEFAULT_NBR_RADIUS, capacities=None, with_index=False, with_maps=False):
+        torch = _torch()
+        self.det = detector
+        self.maps = list(maps)
+        n = len(self.maps)
+        if n == 0:
+            raise ValueError("empty batch")
+        ors = list(orients) if orients is not None else [None] * n
+        if len(ors) != n:
+            raise ValueError("one orientation entry per map")
+        thr = list(thresh) if np.ndim(thresh) else [float(thresh)] * n
+        for m, E in enumerate(self.maps):
+            if E.dim() != 2 or not E.is_cuda or E.dtype != torch.float32 or E.stride(1) != 1:
+                raise TypeError("maps must be 2-D float32 device tensors with unit column stride")
+            if ors[m] is not None and (ors[m].shape != E.shape or ors[m].dtype != torch.float32
+                                       or ors[m].stride(1) != 1):
+                raise TypeError("orientation maps must match their edge maps (float32)")
+        caps = capacities or [int(E.shape[0]) * int(E.shape[1]) for E in self.maps]
+        self.capacities = [int(c) for c in caps]
+        self.offsets = np.concatenate([[0], np.cumsum(self.capacities)]).astype(np.int64)
+        dev = self.maps[0].device
+        self.edges = torch.empty((max(1, int(self.offsets[-1])), 4), dtype=torch.float32, device=dev)
+        self.index = torch.empty(max(1, int(self.offsets[-1])), dtype=torch.int32, device=dev) if with_index else None
+        self.counts = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.flags = torch.zeros(n, dtype=torch.int32, device=dev)
+        self.to_edgemap = [torch.empty_like(E, memory_format=torch.contiguous_format) for E in self.maps] \
+            if with_maps else None
+        self.to_orientation = [torch.empty_like(E, memory_format=torch.contiguous_format) for E in self.maps] \
+            if with_maps else None
+        arr = (_lib.toed_subpix_map_t * n)()
+        for m, E in enumerate(self.maps):
+            it = arr[m]
+            o = int(self.offsets[m])
+            it.d_map = E.data_ptr()
+            it.H, it.W, it.pitch = int(E.shape[0]), int(E.shape[1]), int(E.stride(0))
+            if ors[m] is not None:
+                it.d_orient = ors[m].data_ptr()
+                it.opitch = int(ors[m].stride(0))
+            it.thresh = float(thr[m])
+            it.d_edges = self.edges[o:].data_ptr()
+            it.capacity = self.capacities[m]
+            it.d_index = self.index[o:].data_ptr() if with_index else None
+            if with_maps:
+                it.d_to_edgemap = self.to_edgemap[m].data_ptr()
+                it.d_to_orientation = self.to_orientation[m].data_ptr()
+                it.tpitch = int(E.shape[1])
+        self._orients = ors
+        self._arr = arr
+        self._p = ctypes.c_void_p()
+        detector._bind_stream()
+        check(lib().toed_subpix_plan(detector._ctx, n, arr, float(sigma), float(nbr_radius), ctypes.byref(self._p)),
+              "toed_subpix_plan")
+        detector._batches.add(self)
+
+    def run(self):
+        self.det._bind_stream()
+        check(lib().toed_subpix_run(self._p, ctypes.c_void_p(self.counts.data_ptr()),
+                                    ctypes.c_void_p(self.flags.data_ptr())), "toed_subpix_run")
+
+    def _counts(self):
+        counts = self.counts.cpu().numpy()
+        if (self.flags.cpu().numpy() & FLAG_CAPACITY).any():
+            raise ToedError(5, "subpix batch (edgel capacity)")
+        return counts
+
+    def results(self):
+        """Synchronise and return the per-map N x 4 device tensors."""
+        counts = self._counts()
+        return [self.edges[int(self.offsets[m]):int(self.offsets[m]) + int(counts[m])] for m in range(len(counts))]
+
+    def index_results(self):
+        counts = self._counts()
+        return [self.index[int(self.offsets[m]):int(self.offsets[m]) + int(counts[m])] for m in range(len(counts))]
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().toed_subpix_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SubpixelTOCorrectionGPU:
+    """Mirror of the reference's SubpixelTOCorrectionCPU<T>
+    (subpix_to_correction.hpp:20-96, driven as main_subpix_to_correction.cpp:
+    279-293): ctor (H, W, threshold, g_sigma, device, n), set_edgemap,
+    set_orientation or estimate_orientation, run(edginfo_out) -> N, public
+    TO_edgemap / TO_orientation / subpix_edge_pts_final (host arrays),
+    set_neighbor_radius, set_output_dir, write_array_to_file.  Only n = 0 (no
+    bilinear upsampling of the map, :226-232) is supported."""
+
+    num_of_edge_data = 4
+
+    def __init__(self, H, W, threshold=DEFAULT_THRESH, g_sigma=DEFAULT_SIGMA, device=0, n=0):
+        if n not in (0, None):
+            raise NotImplementedError("interp_n > 0 (upsampled work grid) is not part of the GPU path")
+        from . import Detector
+        torch = _torch()
+        self.img_height, self.img_width = int(H), int(W)
+        self.thresh, self.sigma = float(threshold), float(g_sigma)
+        subpix_radius(self.sigma)   # validates sigma
+        self.neighbor_radius = DEFAULT_NBR_RADIUS
+        self.output_dir = "./output_files"
+        self._det = Detector(device)
+        self._dev = torch.device("cuda", device)
+        self._map = torch.zeros((H, W), dtype=torch.float32, device=self._dev)
+        self._orient = None
+        self.TO_edgemap = np.zeros((H, W), np.float32)
+        self.TO_orientation = np.zeros((H, W), np.float32)
+        self.subpix_edge_pts_final = np.zeros((0, 4), np.float32)
+        self.edge_pt_list_idx = 0
+        self.time_nms = self.time_conv = self.time_correct = 0.0
+
+    def set_edgemap(self, data):
+        self._map.copy_(_torch().as_tensor(np.asarray(data, np.float32).reshape(self.img_height, self.img_width)))
+
+    def set_orientation(self, data):
+        torch = _torch()
+        self._orient = torch.as_tensor(np.asarray(data, np.float32).reshape(self.img_height, self.img_width)).to(
+            self._dev)
+
+    def estimate_orientation(self):
+        self._orient = None
+
+    def set_neighbor_radius(self, radius):
+        self.neighbor_radius = float(radius)
+
+    def set_output_dir(self, d):
+        self.output_dir = d or "./output_files"
+
+    def run(self, edginfo_out=None):
+        torch = _torch()
+        b = SubpixBatch(self._det, [self._map], [self._orient], self.thresh, self.sigma, self.neighbor_radius,
+                        with_maps=True)
+        try:
+            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record()
+            b.run()
+            stop.record()
+            edges = b.results()[0].cpu().numpy()
+            self.time_correct = start.elapsed_time(stop)
+            self.TO_edgemap = b.to_edgemap[0].cpu().numpy()
+            self.TO_orientation = b.to_orientation[0].cpu().numpy()
+        finally:
+            b.close()
+        self.subpix_edge_pts_final = edges
+        self.edge_pt_list_idx = len(edges)
+        if edginfo_out is not None:
+            edginfo_out.reshape(-1)[:edges.size] = edges.reshape(-1)
+        return self.edge_pt_list_idx
+
+    def write_array_to_file(self, filename, wr_data, first_dim, second_dim):
+        """Tab-separated text as subpix_to_correction.cpp:741-775."""
+        if wr_data is None or first_dim <= 0 or second_dim <= 0:
+            return
+        os.makedirs(self.output_dir, exist_ok=True)
+        a = np.asarray(wr_data).reshape(-1)[:first_dim * second_dim].reshape(first_dim, second_dim)
+        with open(os.path.join(self.output_dir, filename), "w") as f:
+            for row in a:
+                f.write("".join("%.6g\t" % v for v in row) + "\n")
+
+    def close(self):
+        self._det.close()
