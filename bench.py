@@ -4,11 +4,13 @@ One JSON line on stdout.  Headline: plain single-image TOED on 2160x3840 u8
 frames (sigma = 2, threshold 2), Mpixel/s with the frame resident in HBM.
 Also reported ("forms"): the synchronous per-call form, the batched form (200
 seeded scenes of mixed size and sigma in one plan, replayed as a hipGraph)
-against a loop of plain launches over the same images, kernel durations from
-HIP events on the library's stream, the VALU/HBM roofline of the dominant
-kernel and the reference CPU path timed on this host.
+against a loop of plain launches over the same images, the differentiable
+forward+backward form, the subpixel third-order correction of 32 Bezier edge
+maps of 2048x2048, kernel durations from HIP events on the library's stream,
+the VALU/HBM roofline of the dominant kernel and the reference CPU paths timed
+on this host.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload all|plain|batch]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload all|plain|batch|fwdbwd|subpix]
 """
 import argparse
 import importlib
@@ -146,6 +148,40 @@ def bench_fwd_bwd(torch, pkg, scene, steps, warmup, B=64, H=512, W=512, sigma=1.
     return dict(px=float(B * H * W), t_f=t_f, t_fb=t_fb, n=state["n"])
 
 
+def bench_subpix(torch, pkg, scene, steps, warmup, B=32, S=2048):
+    """Subpixel third-order correction on B float32 Bezier edge maps of S x S
+    (40 curves, thresh 0.1, sigma 2, orientation estimated, isolation radius
+    2) in one plan."""
+    maps = [torch.from_numpy(scene.bezier_map(5000 + b, S, S, 40)).cuda() for b in range(B)]
+    det = pkg.Detector(0)
+    sp = pkg.SubpixBatch(det, maps, None, 0.1, 2.0, 2.0, capacities=[S * S // 8] * B)
+    sec = time_loop(torch, sp.run, steps, warmup)
+    counts = sp.counts.cpu().numpy()
+    assert not sp.flags.cpu().numpy().any()
+    map0 = maps[0].cpu().numpy()
+    sp.close()
+    det.close()
+    return dict(px=float(B * S * S), sec=sec, edgels=int(counts.sum()), map0=map0, B=B, S=S)
+
+
+def cpu_baseline_subpix(map_np):
+    """The reference's subpix_to_correction.cpp (FP64, OpenMP) on one map of the
+    subpix workload; the oracle port when the reference harness is not built."""
+    import oracle
+    cores = max(1, min(16, os.cpu_count() or 1))
+    H, W = map_np.shape
+    if oracle.subpix_ref_available():
+        r = oracle.run_subpix_ref(map_np.astype(np.float64), None, 0.1, 2.0, 2.0, nthreads=cores)
+        sec, kind = r["times"][3], "reference"
+    else:
+        t0 = time.perf_counter()
+        oracle.subpix(map_np.astype(np.float64), None, 0.1, 2.0, 2.0, nthreads=cores)
+        sec, kind = time.perf_counter() - t0, "port"
+    return {"value": round(H * W / sec / 1e6, 4), "unit": "Mpixel/s", "cores": cores, "kind": kind,
+            "sample": "one %dx%d Bezier map, FP64, orientation estimate through the edgel list (%.2f s)"
+                      % (H, W, sec)}
+
+
 def cpu_baseline(img_np):
     """The reference CPU path (cpu_toed.cpp, FP64, OpenMP) on one frame of the
     same workload, on this host's cores; the oracle port when the reference
@@ -174,7 +210,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="all", choices=["all", "plain", "batch", "fwdbwd"])
+    ap.add_argument("--workload", default="all", choices=["all", "plain", "batch", "fwdbwd", "subpix"])
     ap.add_argument("--batch-images", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -246,6 +282,20 @@ def main():
             out["config"] = {"workload": "differentiable TOED fwd+bwd, 64 x 512x512 f32, sigma 1.5", "global_batch": 64}
         log("fwd+bwd: %.1f Mpx/s (fwd %.1f), ratio %.2f" % (forms["fwd_bwd_64x512x512"]["mpx_s"],
                                                             forms["fwd_only_64x512x512"]["mpx_s"], fb["t_fb"] / fb["t_f"]))
+    if args.workload in ("all", "subpix"):
+        log("subpix 32 x 2048x2048 ...")
+        sp = bench_subpix(torch, pkg, scene, max(3, args.steps // 4), 1)
+        forms["subpix_32x2048x2048"] = {"mpx_s": round(sp["px"] / sp["sec"] / 1e6, 1), "ms": round(sp["sec"] * 1e3, 3),
+                                        "edgels": sp["edgels"], "edgels_s": round(sp["edgels"] / sp["sec"], 1)}
+        if not args.no_cpu:
+            log("subpix cpu baseline ...")
+            forms["subpix_32x2048x2048"]["cpu_baseline"] = cpu_baseline_subpix(sp["map0"])
+        if "value" not in out:
+            out["value"] = forms["subpix_32x2048x2048"]["mpx_s"]
+            out["ms_per_step"] = round(sp["sec"] * 1e3, 3)
+            out["config"] = {"workload": "subpixel TO correction, 32 x 2048x2048 f32 Bezier maps, thresh 0.1, "
+                                         "sigma 2, orientation estimated", "global_batch": 32}
+        log("subpix: %.1f Mpx/s, %.3f ms per batch" % (forms["subpix_32x2048x2048"]["mpx_s"], sp["sec"] * 1e3))
     out["forms"] = forms
     if plain is not None and not args.no_cpu:
         log("cpu baseline ...")

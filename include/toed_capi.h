@@ -176,6 +176,12 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
  * d_counts[n] / d_flags[n] (device, either may be NULL). */
 int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags);
 int toed_subpix_destroy(toed_subpix *p);
+/* Per-stage timing of later runs with HIP events (the reference's time_nms,
+ * time_conv + time_correct, subpix_to_correction.hpp:76-78), in ms:
+ * orientation + NMS kernel, isolation + convolution + correction kernel, and
+ * list scan + scatter + maps. */
+int toed_subpix_set_timing(toed_subpix *p, int on);
+int toed_subpix_last_times(toed_subpix *p, float *ms_orient_nms, float *ms_correct, float *ms_list);
 
 /* Reference launch layer (unfused, for drop-in and debugging) ---------------
  * toed_gpu_convolve_f32: gpu_kernels.hpp:24-33.  Writes the four interp planes

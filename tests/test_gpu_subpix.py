@@ -54,11 +54,11 @@ def test_subpix_golden(pkg, detector, oracle, path):
 
 
 def test_subpix_mixed_batch(pkg, detector, oracle, scene):
-    shapes = [(96, 128), (200, 300), (64, 517), (333, 257), (12, 12)]
+    shapes = [(96, 128), (200, 300), (64, 517), (333, 257), (12, 12), (9, 9)]
     maps = [scene.bezier_map(40 + k, H, W, 12) for k, (H, W) in enumerate(shapes)]
     rng = np.random.default_rng(7)
-    orients = [None, rng.uniform(-3.0, 3.0, shapes[1]).astype(np.float32), None, None, None]
-    thr = [0.1, 0.2, 0.15, 0.1, 0.1]
+    orients = [None, rng.uniform(-3.0, 3.0, shapes[1]).astype(np.float32), None, None, None, None]
+    thr = [0.1, 0.2, 0.15, 0.1, 0.1, 0.1]
     print("mixed batch", shapes, flush=True)
     b = _run(pkg, detector, maps, orients, thr, 1.5, 2.0)
     res, idx = b.results(), b.index_results()
@@ -68,7 +68,7 @@ def test_subpix_mixed_batch(pkg, detector, oracle, scene):
         rep = compare_subpix(ref, res[m].cpu().numpy(), idx[m].cpu().numpy())
         print(m, rep)
         assert_subpix_parity(rep)
-    assert len(res[-1]) == 0          # 12 x 12: no NMS rows
+    assert len(res[-1]) == 0          # 9 x 9: NMS rows [4, 4) are empty
     b.close()
 
 

@@ -105,6 +105,8 @@ SIGNATURES = [
                               ctypes.POINTER(_VP)]),
     ("toed_subpix_run", _I, [_VP, _VP, _VP]),
     ("toed_subpix_destroy", _I, [_VP]),
+    ("toed_subpix_set_timing", _I, [_VP, _I]),
+    ("toed_subpix_last_times", _I, [_VP, _PF, _PF, _PF]),
     ("toed_gpu_convolve_f32", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F]),
     ("toed_gpu_nms_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F, _I]),
 ]

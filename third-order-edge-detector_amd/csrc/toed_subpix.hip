@@ -78,32 +78,12 @@ struct SpJob {
 
 #define SP_CONST __attribute__((address_space(4)))
 
-__device__ __forceinline__ SpJob load_job(const SpJob *jobs, int njobs, int tile) {
+// Grids are (tiles or chunks of the largest map, maps): the map of a
+// workgroup is blockIdx.y, read with scalar loads.
+__device__ __forceinline__ SpJob load_job(const SpJob *jobs) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    const SP_CONST SpJob *cj = (const SP_CONST SpJob *)jobs;
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cj[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
-    }
-    return cj[lo];
+    return ((const SP_CONST SpJob *)jobs)[blockIdx.y];
 #else
-    (void)njobs; (void)tile;
-    return jobs[0];
-#endif
-}
-
-__device__ __forceinline__ SpJob load_job_chunk(const SpJob *jobs, int njobs, int chunk) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const SP_CONST SpJob *cj = (const SP_CONST SpJob *)jobs;
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (cj[mid].chunk_begin <= chunk) lo = mid; else hi = mid - 1;
-    }
-    return cj[lo];
-#else
-    (void)njobs; (void)chunk;
     return jobs[0];
 #endif
 }
@@ -158,18 +138,27 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
     __shared__ float Es[AE_R][AE_C];
     __shared__ double Tr[AE_R][AT_C];
     __shared__ double Ss[AS_R][AT_C];
-    const int tile = blockIdx.x;
-    const SpJob J = load_job(jobs, njobs, tile);
-    const int lt = tile - J.tile_begin;
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if (lt >= J.ntx * J.nty) return;
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
     const bool est = (J.orient == nullptr);
 
-    for (int idx = threadIdx.x; idx < AE_R * AE_C; idx += NT) {
-        const int r = idx / AE_C, c = idx - r * AE_C;
-        const int gy = wrap_sym(y0 - 6 + r, H), gx = wrap_sym(x0 - 6 + c, W);
-        Es[r][c] = J.map[(long long)gy * J.pitch + gx];
+    if (y0 >= 6 && y0 + TH + 6 <= H && x0 >= 6 && x0 + TW + 6 <= W) {
+        // interior tile: direct indexing, no border folding
+        const float *src = J.map + (long long)(y0 - 6) * J.pitch + (x0 - 6);
+        for (int idx = threadIdx.x; idx < AE_R * AE_C; idx += NT) {
+            const int r = idx / AE_C, c = idx - r * AE_C;
+            Es[r][c] = src[(long long)r * J.pitch + c];
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < AE_R * AE_C; idx += NT) {
+            const int r = idx / AE_C, c = idx - r * AE_C;
+            const int gy = wrap_sym(y0 - 6 + r, H), gx = wrap_sym(x0 - 6 + c, W);
+            Es[r][c] = J.map[(long long)gy * J.pitch + gx];
+        }
     }
     __syncthreads();
     if (est) {
@@ -309,9 +298,9 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     __shared__ int s_flag;
     float *In = lds, *X = lds + G::IN, *Y = lds + G::IN + G::XN;
 
-    const int tile = blockIdx.x;
-    const SpJob J = load_job(jobs, njobs, tile);
-    const int lt = tile - J.tile_begin;
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if (lt >= J.ntx * J.nty) return;
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W, K = tp.K;
@@ -369,80 +358,141 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
 
     // load E, Hx, Hy over the plane region + R, periodic borders
     const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
-    for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
-        const int r = idx / IC, c = idx - r * IC;
-        const int gy = wrap_circ(iy0 + r, H), gx = wrap_circ(ix0 + c, W);
-        const long long o = (long long)gy * W + gx;
-        In[idx] = J.map[(long long)gy * J.pitch + gx];
-        In[IR * IC + idx] = J.hx[o];
-        In[2 * IR * IC + idx] = J.hy[o];
+    if (iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W) {
+        constexpr int U = 4;
+        for (int base = 0; base < IR * IC; base += U * NT) {
+            float v[3][U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = base + threadIdx.x + u * NT;
+                const int r = idx / IC, c = idx - r * IC;
+                const bool ok = idx < IR * IC;
+                const long long o = (long long)(iy0 + r) * W + (ix0 + c);
+                v[0][u] = ok ? J.map[(long long)(iy0 + r) * J.pitch + (ix0 + c)] : 0.f;
+                v[1][u] = ok ? J.hx[o] : 0.f;
+                v[2][u] = ok ? J.hy[o] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = base + threadIdx.x + u * NT;
+                if (idx < IR * IC) {
+                    In[idx] = v[0][u];
+                    In[IR * IC + idx] = v[1][u];
+                    In[2 * IR * IC + idx] = v[2][u];
+                }
+            }
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
+            const int r = idx / IC, c = idx - r * IC;
+            const int gy = wrap_circ(iy0 + r, H), gx = wrap_circ(ix0 + c, W);
+            const long long o = (long long)gy * W + gx;
+            In[idx] = J.map[(long long)gy * J.pitch + gx];
+            In[IR * IC + idx] = J.hx[o];
+            In[2 * IR * IC + idx] = J.hy[o];
+        }
     }
     __syncthreads();
     // group 0: E * (G', G, G'') along x -> Px, Pxx, Py, Pxy, Pyy
     // group 1: Hx * G', Hy * G, Hy * G' along x -> Hxx, Hyy, Hxy
     // planes Y: 0 Px, 1 Py, 2 Pxx, 3 Pyy, 4 Pxy, 5 Hxx, 6 Hxy, 7 Hyy
+// Two outputs per thread (adjacent columns in the x pass, adjacent rows in
+    // the y pass) share 2R + 2 loaded values.  Output u of a pass sums
+    // window[m + u] * k[2R - m], m = 0..2R (true convolution).
+    static_assert(PC % 2 == 0 && PR % 2 == 0, "two outputs per thread");
 #pragma unroll 1
     for (int grp = 0; grp < 2; ++grp) {
-        for (int idx = threadIdx.x; idx < IR * PC; idx += NT) {
-            const int r = idx / PC, c = idx - r * PC;
-            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        for (int idx = threadIdx.x; idx < IR * (PC / 2); idx += NT) {
+            const int r = idx / (PC / 2), c = 2 * (idx - r * (PC / 2));
+            float a[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
             if (grp == 0) {
                 const float *src = In + r * IC + c;
 #pragma unroll
-                for (int q = -R; q <= R; ++q) {
-                    const float v = src[R - q];
-                    a0 = fmaf(v, tp.k[1][q + R], a0);
-                    a1 = fmaf(v, tp.k[0][q + R], a1);
-                    a2 = fmaf(v, tp.k[2][q + R], a2);
+                for (int m = 0; m <= 2 * R + 1; ++m) {
+                    const float v = src[m];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const int t = 2 * R - (m - u);
+                        if (t < 0 || t > 2 * R) continue;
+                        a[0][u] = fmaf(v, tp.k[1][t], a[0][u]);
+                        a[1][u] = fmaf(v, tp.k[0][t], a[1][u]);
+                        a[2][u] = fmaf(v, tp.k[2][t], a[2][u]);
+                    }
                 }
             } else {
                 const float *sx = In + IR * IC + r * IC + c;
                 const float *sy = In + 2 * IR * IC + r * IC + c;
 #pragma unroll
-                for (int q = -R; q <= R; ++q) {
-                    const float vx = sx[R - q], vy = sy[R - q];
-                    a0 = fmaf(vx, tp.k[1][q + R], a0);
-                    a1 = fmaf(vy, tp.k[0][q + R], a1);
-                    a2 = fmaf(vy, tp.k[1][q + R], a2);
+                for (int m = 0; m <= 2 * R + 1; ++m) {
+                    const float vx = sx[m], vy = sy[m];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const int t = 2 * R - (m - u);
+                        if (t < 0 || t > 2 * R) continue;
+                        a[0][u] = fmaf(vx, tp.k[1][t], a[0][u]);
+                        a[1][u] = fmaf(vy, tp.k[0][t], a[1][u]);
+                        a[2][u] = fmaf(vy, tp.k[1][t], a[2][u]);
+                    }
                 }
             }
-            X[idx] = a0;
-            X[IR * PC + idx] = a1;
-            X[2 * IR * PC + idx] = a2;
+            const int o = r * PC + c;
+#pragma unroll
+            for (int g = 0; g < 3; ++g) {
+                X[g * IR * PC + o] = a[g][0];
+                X[g * IR * PC + o + 1] = a[g][1];
+            }
         }
         __syncthreads();
-        for (int idx = threadIdx.x; idx < PR * PC; idx += NT) {
-            const int r = idx / PC, c = idx - r * PC;
+        for (int idx = threadIdx.x; idx < (PR / 2) * PC; idx += NT) {
+            const int r = 2 * (idx / PC), c = idx - (idx / PC) * PC;
             const float *x0p = X + r * PC + c, *x1p = X + IR * PC + r * PC + c, *x2p = X + 2 * IR * PC + r * PC + c;
             if (grp == 0) {
-                float px = 0.f, py = 0.f, pxx = 0.f, pxy = 0.f, pyy = 0.f;
+                float px[2] = {0.f, 0.f}, py[2] = {0.f, 0.f}, pxx[2] = {0.f, 0.f}, pxy[2] = {0.f, 0.f},
+                      pyy[2] = {0.f, 0.f};
 #pragma unroll
-                for (int p = -R; p <= R; ++p) {
-                    const int o = (R - p) * PC;
-                    const float dgx = x0p[o], gx = x1p[o], d2gx = x2p[o];
-                    px = fmaf(dgx, tp.k[0][p + R], px);
-                    pxy = fmaf(dgx, tp.k[1][p + R], pxy);
-                    py = fmaf(gx, tp.k[1][p + R], py);
-                    pyy = fmaf(gx, tp.k[2][p + R], pyy);
-                    pxx = fmaf(d2gx, tp.k[0][p + R], pxx);
+                for (int m = 0; m <= 2 * R + 1; ++m) {
+                    const float dgx = x0p[m * PC], gx = x1p[m * PC], d2gx = x2p[m * PC];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const int t = 2 * R - (m - u);
+                        if (t < 0 || t > 2 * R) continue;
+                        px[u] = fmaf(dgx, tp.k[0][t], px[u]);
+                        pxy[u] = fmaf(dgx, tp.k[1][t], pxy[u]);
+                        py[u] = fmaf(gx, tp.k[1][t], py[u]);
+                        pyy[u] = fmaf(gx, tp.k[2][t], pyy[u]);
+                        pxx[u] = fmaf(d2gx, tp.k[0][t], pxx[u]);
+                    }
                 }
-                Y[0 * PR * PC + idx] = px;
-                Y[1 * PR * PC + idx] = py;
-                Y[2 * PR * PC + idx] = pxx;
-                Y[3 * PR * PC + idx] = pyy;
-                Y[4 * PR * PC + idx] = pxy;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int o = (r + u) * PC + c;
+                    Y[0 * PR * PC + o] = px[u];
+                    Y[1 * PR * PC + o] = py[u];
+                    Y[2 * PR * PC + o] = pxx[u];
+                    Y[3 * PR * PC + o] = pyy[u];
+                    Y[4 * PR * PC + o] = pxy[u];
+                }
             } else {
-                float hxx = 0.f, hyy = 0.f, hxy = 0.f;
+                float hxx[2] = {0.f, 0.f}, hyy[2] = {0.f, 0.f}, hxy[2] = {0.f, 0.f};
 #pragma unroll
-                for (int p = -R; p <= R; ++p) {
-                    const int o = (R - p) * PC;
-                    hxx = fmaf(x0p[o], tp.k[0][p + R], hxx);
-                    hyy = fmaf(x1p[o], tp.k[1][p + R], hyy);
-                    hxy = fmaf(x2p[o], tp.k[0][p + R], hxy);
+                for (int m = 0; m <= 2 * R + 1; ++m) {
+                    const float v0 = x0p[m * PC], v1 = x1p[m * PC], v2 = x2p[m * PC];
+#pragma unroll
+                    for (int u = 0; u < 2; ++u) {
+                        const int t = 2 * R - (m - u);
+                        if (t < 0 || t > 2 * R) continue;
+                        hxx[u] = fmaf(v0, tp.k[0][t], hxx[u]);
+                        hyy[u] = fmaf(v1, tp.k[1][t], hyy[u]);
+                        hxy[u] = fmaf(v2, tp.k[0][t], hxy[u]);
+                    }
                 }
-                Y[5 * PR * PC + idx] = hxx;
-                Y[6 * PR * PC + idx] = hxy;
-                Y[7 * PR * PC + idx] = hyy;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int o = (r + u) * PC + c;
+                    Y[5 * PR * PC + o] = hxx[u];
+                    Y[6 * PR * PC + o] = hxy[u];
+                    Y[7 * PR * PC + o] = hyy[u];
+                }
             }
         }
         __syncthreads();
@@ -504,10 +554,11 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
 // C: scan of kept-word popcounts (raster order), chunk totals, per-map totals
 __global__ __launch_bounds__(256) void sp_scan_chunks(const SpJob *jobs, int njobs, int *chunk_sum) {
     __shared__ int wsum[4];
-    const int chunk = blockIdx.x;
-    const SpJob J = load_job_chunk(jobs, njobs, chunk);
+    const SpJob J = load_job(jobs);
+    if ((int)blockIdx.x >= J.nchunks) return;
+    const int chunk = J.chunk_begin + blockIdx.x;
     const long long nwords = (long long)J.H * J.nw;
-    const long long w0 = (long long)(chunk - J.chunk_begin) * CHUNKW + threadIdx.x * (CHUNKW / 256);
+    const long long w0 = (long long)blockIdx.x * CHUNKW + threadIdx.x * (CHUNKW / 256);
     int s = 0;
 #pragma unroll
     for (int k = 0; k < CHUNKW / 256; ++k)
@@ -553,11 +604,12 @@ __global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chu
 // D: scatter in raster order
 __global__ __launch_bounds__(256) void sp_scatter_kernel(const SpJob *jobs, int njobs, const int *chunk_off) {
     __shared__ int wsum[4];
-    const int chunk = blockIdx.x;
-    const SpJob J = load_job_chunk(jobs, njobs, chunk);
+    const SpJob J = load_job(jobs);
+    if ((int)blockIdx.x >= J.nchunks) return;
+    const int chunk = J.chunk_begin + blockIdx.x;
     const long long nwords = (long long)J.H * J.nw;
     constexpr int PER = CHUNKW / 256;
-    const long long w0 = (long long)(chunk - J.chunk_begin) * CHUNKW + threadIdx.x * PER;
+    const long long w0 = (long long)blockIdx.x * CHUNKW + threadIdx.x * PER;
     uint32_t wv[PER];
     int s = 0;
 #pragma unroll
@@ -601,10 +653,9 @@ __global__ __launch_bounds__(256) void sp_scatter_kernel(const SpJob *jobs, int 
 
 // E: TO_edgemap / TO_orientation by gather
 __global__ __launch_bounds__(NT) void sp_maps_kernel(const SpJob *jobs, int njobs) {
-    const int tile = blockIdx.x;
-    const SpJob J = load_job(jobs, njobs, tile);
-    if (!J.to_e && !J.to_o) return;
-    const int lt = tile - J.tile_begin;
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if ((!J.to_e && !J.to_o) || lt >= J.ntx * J.nty) return;
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int H = J.H, W = J.W;
 #pragma unroll 1
@@ -661,6 +712,9 @@ struct toed_subpix {
     int *d_chunk = nullptr;
     int *d_nsurv = nullptr;
     long long ntiles = 0, nchunks = 0;
+    int max_tiles = 0, max_chunks = 0;
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 #define SPCHK(x)                                                                                         \
@@ -673,7 +727,7 @@ struct toed_subpix {
     } while (0)
 
 template <int R> static void launch_correct(hipStream_t s, const toed_subpix *p) {
-    hipLaunchKernelGGL(sp_correct_kernel<R>, dim3((unsigned)p->ntiles), dim3(NT), 0, s, p->d_jobs, p->n, p->taps);
+    hipLaunchKernelGGL(sp_correct_kernel<R>, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n, p->taps);
 }
 
 static int launch_correct_r(hipStream_t s, const toed_subpix *p) {
@@ -702,7 +756,7 @@ int toed_subpix_radius(double sigma, int *R) {
 
 int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double sigma, double nbr_radius,
                      toed_subpix **out) {
-    if (!ctx || !out || n <= 0 || !maps) return TOED_ERR_ARG;
+    if (!ctx || !out || n <= 0 || n > 65535 || !maps) return TOED_ERR_ARG;
     *out = nullptr;
     int R = 0;
     const int st = toed_subpix_radius(sigma, &R);
@@ -783,6 +837,8 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
         J.nchunks = (int)(((long long)M.H * J.nw + CHUNKW - 1) / CHUNKW);
         J.chunk_begin = (int)chunks;
         chunks += J.nchunks;
+        p->max_tiles = std::max(p->max_tiles, J.ntx * J.nty);
+        p->max_chunks = std::max(p->max_chunks, J.nchunks);
     }
     if (tiles >= (1LL << 31) || chunks >= (1LL << 31)) { toed_subpix_destroy(p); return TOED_ERR_ARG; }
     p->ntiles = tiles;
@@ -813,16 +869,42 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     hipStream_t s = (hipStream_t)toed_get_stream(p->ctx);
     if (changed) SPCHK(hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(SpJob) * p->n, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(sp_reset_kernel, dim3((p->n + 255) / 256), dim3(256), 0, s, p->d_jobs, p->n);
-    hipLaunchKernelGGL(sp_orient_nms_kernel, dim3((unsigned)p->ntiles), dim3(NT), 0, s, p->d_jobs, p->n);
+    if (p->timing) SPCHK(hipEventRecord(p->ev[0], s));
+    hipLaunchKernelGGL(sp_orient_nms_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
+    if (p->timing) SPCHK(hipEventRecord(p->ev[1], s));
     const int st = launch_correct_r(s, p);
     if (st) return st;
-    hipLaunchKernelGGL(sp_scan_chunks, dim3((unsigned)p->nchunks), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
+    if (p->timing) SPCHK(hipEventRecord(p->ev[2], s));
+    hipLaunchKernelGGL(sp_scan_chunks, dim3(p->max_chunks, p->n), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
     hipLaunchKernelGGL(sp_scan_maps, dim3(p->n), dim3(1024), 0, s, p->d_jobs, p->d_chunk);
-    hipLaunchKernelGGL(sp_scatter_kernel, dim3((unsigned)p->nchunks), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
+    hipLaunchKernelGGL(sp_scatter_kernel, dim3(p->max_chunks, p->n), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
     bool maps = false;
     for (const auto &J : p->jobs) maps = maps || J.to_e || J.to_o;
-    if (maps) hipLaunchKernelGGL(sp_maps_kernel, dim3((unsigned)p->ntiles), dim3(NT), 0, s, p->d_jobs, p->n);
+    if (maps) hipLaunchKernelGGL(sp_maps_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
+    if (p->timing) SPCHK(hipEventRecord(p->ev[3], s));
     SPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+int toed_subpix_set_timing(toed_subpix *p, int on) {
+    if (!p) return TOED_ERR_ARG;
+    SPCHK(hipSetDevice(p->device));
+    if (on && !p->ev[0])
+        for (auto &e : p->ev) SPCHK(hipEventCreate(&e));
+    p->timing = on != 0;
+    return TOED_OK;
+}
+
+int toed_subpix_last_times(toed_subpix *p, float *ms_orient_nms, float *ms_correct, float *ms_list) {
+    if (!p || !p->ev[0]) return TOED_ERR_ARG;
+    SPCHK(hipEventSynchronize(p->ev[3]));
+    float a = 0.f, b = 0.f, c = 0.f;
+    SPCHK(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
+    SPCHK(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
+    SPCHK(hipEventElapsedTime(&c, p->ev[2], p->ev[3]));
+    if (ms_orient_nms) *ms_orient_nms = a;
+    if (ms_correct) *ms_correct = b;
+    if (ms_list) *ms_list = c;
     return TOED_OK;
 }
 
@@ -834,6 +916,8 @@ int toed_subpix_destroy(toed_subpix *p) {
     (void)hipFree(p->d_chunk);
     (void)hipFree(p->d_nsurv);
     (void)hipFree(p->d_jobs);
+    for (auto &e : p->ev)
+        if (e) (void)hipEventDestroy(e);
     delete p;
     return TOED_OK;
 }

@@ -101,6 +101,17 @@ class SubpixBatch:
         check(lib().toed_subpix_run(self._p, ctypes.c_void_p(self.counts.data_ptr()),
                                     ctypes.c_void_p(self.flags.data_ptr())), "toed_subpix_run")
 
+    def set_timing(self, on=True):
+        check(lib().toed_subpix_set_timing(self._p, int(bool(on))), "toed_subpix_set_timing")
+
+    def last_times(self):
+        """(orientation + NMS, isolation + convolutions + correction, list) in
+        ms for the last run with timing on."""
+        a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        check(lib().toed_subpix_last_times(self._p, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+              "toed_subpix_last_times")
+        return a.value, b.value, c.value
+
     def _counts(self):
         counts = self.counts.cpu().numpy()
         if (self.flags.cpu().numpy() & FLAG_CAPACITY).any():
@@ -181,12 +192,10 @@ class SubpixelTOCorrectionGPU:
         b = SubpixBatch(self._det, [self._map], [self._orient], self.thresh, self.sigma, self.neighbor_radius,
                         with_maps=True)
         try:
-            start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            start.record()
+            b.set_timing(True)
             b.run()
-            stop.record()
             edges = b.results()[0].cpu().numpy()
-            self.time_correct = start.elapsed_time(stop)
+            self.time_nms, self.time_conv, self.time_correct = b.last_times()
             self.TO_edgemap = b.to_edgemap[0].cpu().numpy()
             self.TO_orientation = b.to_orientation[0].cpu().numpy()
         finally:
