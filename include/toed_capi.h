@@ -183,6 +183,24 @@ int toed_subpix_destroy(toed_subpix *p);
 int toed_subpix_set_timing(toed_subpix *p, int on);
 int toed_subpix_last_times(toed_subpix *p, float *ms_orient_nms, float *ms_correct, float *ms_list);
 
+/* Differentiable form of the correction (no reference counterpart).
+ * toed_subpix_set_grad(p, 1) makes later runs keep what the adjoint needs
+ * (per pixel: direction and the derivatives of the estimated angle; per edgel:
+ * its direction; maps at least 5 x 5).  toed_subpix_backward then takes, per
+ * map, the upstream rows dL/d[x, y, orientation, strength] (d_edges layout)
+ * and accumulates (+=) dL/dmap into d_gmap, with every discrete decision of
+ * the last run frozen: the edgel set, each edgel's NMS octant and cubic
+ * stencil, and per pixel the half-turn of the direction and sign(-Oxy).  For
+ * maps with a given orientation, the orientation is a constant input.
+ * Valid until the plan runs again. */
+typedef struct {
+    const float *d_gout;       /* capacity x 4 floats (rows of the map's d_edges) */
+    float *d_gmap;             /* H rows of gpitch floats, accumulated */
+    int gpitch;
+} toed_subpix_grad_t;
+int toed_subpix_set_grad(toed_subpix *p, int on);
+int toed_subpix_backward(toed_subpix *p, const toed_subpix_grad_t *items);
+
 /* Reference launch layer (unfused, for drop-in and debugging) ---------------
  * toed_gpu_convolve_f32: gpu_kernels.hpp:24-33.  Writes the four interp planes
  * Ix, Iy, |grad I| and TO orientation ((2h) x (2w), row-major) from a float

@@ -62,6 +62,12 @@ def lib():
         L.toed_subpix_oracle.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int, _dp, ctypes.c_long, _i64p, _dp, _dp]
         L.toed_subpix_oracle.restype = ctypes.c_long
+        _i8p = ctypes.POINTER(ctypes.c_byte)
+        L.toed_subpix_frozen_state.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_long, _i8p, _i8p,
+                                               _ip, _ip, _ip, ctypes.c_double]
+        L.toed_subpix_frozen_loss.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _i64p, ctypes.c_long,
+                                              _dp, ctypes.c_int, _i8p, _i8p, _ip, _ip, _ip, _dp]
+        L.toed_subpix_frozen_loss.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -315,3 +321,41 @@ def run_subpix_ref(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthread
             te = np.frombuffer(f.read(h * w * 8), np.float64).reshape(h, w).copy()
             to = np.frombuffer(f.read(h * w * 8), np.float64).reshape(h, w).copy()
     return dict(edges=edges, TO_E=te, TO_O=to, times=times, n=n)
+
+
+class SubpixFrozen:
+    """The subpixel correction path as a function of the map E with every
+    discrete decision frozen at E0 (edgel set idx, octants, the half-turn of
+    each pixel's direction, sign(-Oxy), cubic stencil bases): FP64, for
+    finite-difference checks of the GPU adjoint."""
+
+    def __init__(self, E0, idx, sigma=2.0):
+        E0 = np.ascontiguousarray(E0, np.float64)
+        self.h, self.w = E0.shape
+        self.sigma = float(sigma)
+        self.idx = np.ascontiguousarray(idx, np.int64)
+        n, npx = len(self.idx), E0.size
+        self.tneg = np.zeros(npx, np.int8)
+        self.sxy = np.zeros(npx, np.int8)
+        self.oct = np.zeros(n, np.int32)
+        self.base = np.zeros(2 * n, np.int32)
+        self.fb = np.zeros(n, np.int32)
+        i8 = ctypes.POINTER(ctypes.c_byte)
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib().toed_subpix_frozen_state(_p(E0), self.h, self.w, _p(self.idx, _i64p), n, self.tneg.ctypes.data_as(i8),
+                                       self.sxy.ctypes.data_as(i8), self.oct.ctypes.data_as(ip),
+                                       self.base.ctypes.data_as(ip), self.fb.ctypes.data_as(ip), self.sigma)
+
+    def loss(self, E, w, outputs=False):
+        E = np.ascontiguousarray(E, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        per = 1 if w.ndim == 2 else 0
+        n = len(self.idx)
+        outs = np.zeros((n, 4), np.float64)
+        i8 = ctypes.POINTER(ctypes.c_byte)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L = lib().toed_subpix_frozen_loss(_p(E), self.h, self.w, self.sigma, _p(self.idx, _i64p), n, _p(w), per,
+                                          self.tneg.ctypes.data_as(i8), self.sxy.ctypes.data_as(i8),
+                                          self.oct.ctypes.data_as(ip), self.base.ctypes.data_as(ip),
+                                          self.fb.ctypes.data_as(ip), _p(outs))
+        return (L, outs) if outputs else L

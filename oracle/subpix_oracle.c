@@ -338,3 +338,205 @@ long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, d
     for (int k = 0; k < 10; ++k) free(P[k]);
     return N;
 }
+
+/* ---------------------------------------------------------------------------
+ * Differentiable form (no reference counterpart): the path as a function of
+ * the map E with every discrete decision frozen at a forward point
+ *   - the edgel set (map pixels idx[0..n)), the NMS octant of each edgel,
+ *   - per pixel: the sign of t = Oyy sgn / (Oxx + 1e-5) (which half-turn the
+ *     direction takes) and sgn = sign(-Oxy),
+ *   - per edgel: the cubic stencil base (floor of the subpixel position) and
+ *     the |F| < 1e-12 fallback.
+ * toed_subpix_frozen_state() extracts the state from E; toed_subpix_frozen_loss()
+ * evaluates L = sum_e w_e . [x, y, orientation, strength] with it (FP64).
+ * Central differences of the frozen loss pin the GPU adjoint.
+ */
+static void sp_smooth(const double *E, int h, int w, double *S) {
+    const int r = 4;
+    double tri[9], tsum = 0;
+    for (int k = -r; k <= r; ++k) { tri[k + r] = r + 1 - abs(k); tsum += tri[k + r]; }
+    for (int k = 0; k < 9; ++k) tri[k] /= tsum;
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)h * w);
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            double acc = 0;
+            for (int q = -r; q <= r; ++q) acc += E[(size_t)i * w + wrap_sym(j - q, w)] * tri[q + r];
+            tmp[(size_t)i * w + j] = acc;
+        }
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            double acc = 0;
+            for (int p = -r; p <= r; ++p) acc += tmp[(size_t)wrap_sym(i - p, h) * w + j] * tri[p + r];
+            S[(size_t)i * w + j] = acc;
+        }
+    free(tmp);
+}
+
+static double sp_dx(const double *a, int w, int i, int j) {
+    if (j == 0) return a[(size_t)i * w + 1] - a[(size_t)i * w];
+    if (j == w - 1) return a[(size_t)i * w + w - 1] - a[(size_t)i * w + w - 2];
+    return 0.5 * (a[(size_t)i * w + j + 1] - a[(size_t)i * w + j - 1]);
+}
+static double sp_dy(const double *a, int h, int w, int i, int j) {
+    if (i == 0) return a[(size_t)w + j] - a[j];
+    if (i == h - 1) return a[(size_t)(h - 1) * w + j] - a[(size_t)(h - 2) * w + j];
+    return 0.5 * (a[(size_t)(i + 1) * w + j] - a[(size_t)(i - 1) * w + j]);
+}
+
+/* Oxx + 1e-5, Oyy, Oxy at every pixel (estimate_orientation :346-367) */
+static void sp_second(const double *E, int h, int w, double *Axx, double *Ayy, double *Axy) {
+    const size_t n = (size_t)h * w;
+    double *S = (double *)malloc(sizeof(double) * n), *Ox = (double *)malloc(sizeof(double) * n),
+           *Oy = (double *)malloc(sizeof(double) * n);
+    sp_smooth(E, h, w, S);
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            Ox[(size_t)i * w + j] = sp_dx(S, w, i, j);
+            Oy[(size_t)i * w + j] = sp_dy(S, h, w, i, j);
+        }
+    for (int i = 0; i < h; ++i)
+        for (int j = 0; j < w; ++j) {
+            Axx[(size_t)i * w + j] = sp_dx(Ox, w, i, j) + 1e-5;
+            if (Axy) Axy[(size_t)i * w + j] = sp_dx(Oy, w, i, j);
+            Ayy[(size_t)i * w + j] = sp_dy(Oy, h, w, i, j);
+        }
+    free(S); free(Ox); free(Oy);
+}
+
+/* octant index 0..7 of a direction, the branch order of nms_token (:452-498) */
+static int sp_octant(double gx, double gy) {
+    if (gx >= 0 && gy >= 0) return gx >= gy ? 0 : 1;
+    if (gx < 0 && gy >= 0) return fabs(gx) < gy ? 2 : 3;
+    if (gx < 0 && gy < 0) return fabs(gx) >= fabs(gy) ? 4 : 5;
+    return gx < fabs(gy) ? 6 : 7;
+}
+
+/* slope and the two interpolated neighbours of octant `oc` (:452-498) */
+static void sp_nms_interp(const double *E, int w, int i, int j, double nx, double ny, int oc, double *slope,
+                          double *fp, double *fm) {
+#define EE(a, b) E[(size_t)(a) * w + (b)]
+    double s;
+    switch (oc) {
+    case 0: s = ny / nx; *fp = EE(i, j + 1) * (1 - s) + EE(i + 1, j + 1) * s; *fm = EE(i, j - 1) * (1 - s) + EE(i - 1, j - 1) * s; break;
+    case 1: s = nx / ny; *fp = EE(i + 1, j) * (1 - s) + EE(i + 1, j + 1) * s; *fm = EE(i - 1, j) * (1 - s) + EE(i - 1, j - 1) * s; break;
+    case 2: s = -nx / ny; *fp = EE(i + 1, j) * (1 - s) + EE(i + 1, j - 1) * s; *fm = EE(i - 1, j) * (1 - s) + EE(i - 1, j + 1) * s; break;
+    case 3: s = -ny / nx; *fp = EE(i, j - 1) * (1 - s) + EE(i + 1, j - 1) * s; *fm = EE(i, j + 1) * (1 - s) + EE(i - 1, j + 1) * s; break;
+    case 4: s = ny / nx; *fp = EE(i, j - 1) * (1 - s) + EE(i - 1, j - 1) * s; *fm = EE(i, j + 1) * (1 - s) + EE(i + 1, j + 1) * s; break;
+    case 5: s = nx / ny; *fp = EE(i - 1, j) * (1 - s) + EE(i - 1, j - 1) * s; *fm = EE(i + 1, j) * (1 - s) + EE(i + 1, j + 1) * s; break;
+    case 6: s = -nx / ny; *fp = EE(i - 1, j) * (1 - s) + EE(i - 1, j + 1) * s; *fm = EE(i + 1, j) * (1 - s) + EE(i + 1, j - 1) * s; break;
+    default: s = -ny / nx; *fp = EE(i, j + 1) * (1 - s) + EE(i - 1, j + 1) * s; *fm = EE(i, j - 1) * (1 - s) + EE(i + 1, j - 1) * s; break;
+    }
+    *slope = s;
+#undef EE
+}
+
+/* frozen state at E: tneg[p] = 1 if t < 0 at pixel p, sxy[p] = sgn(-Oxy) (-1, 0, 1),
+ * per edgel: octant, stencil base (floor x, floor y), fallback flag. */
+int toed_subpix_frozen_state(const double *E, int h, int w, const int64_t *idx, long n, signed char *tneg,
+                             signed char *sxy, int *oct, int *base, int *fallback, double sigma) {
+    const size_t np = (size_t)h * w;
+    double *Axx = (double *)malloc(sizeof(double) * np), *Ayy = (double *)malloc(sizeof(double) * np),
+           *Axy = (double *)malloc(sizeof(double) * np);
+    sp_second(E, h, w, Axx, Ayy, Axy);
+    for (size_t p = 0; p < np; ++p) {
+        const double sg = Axy[p] < 0 ? 1.0 : (Axy[p] > 0 ? -1.0 : 0.0);
+        sxy[p] = (signed char)sg;
+        tneg[p] = (Ayy[p] * sg / Axx[p]) < 0.0 ? 1 : 0;
+    }
+    free(Axx); free(Ayy); free(Axy);
+    /* octants and cubic stencil bases at the forward point (:446-516, :521-530) */
+    double *th = (double *)malloc(sizeof(double) * np);
+    toed_subpix_estimate_orientation(E, h, w, th);
+    for (long e = 0; e < n; ++e) {
+        const size_t p = (size_t)idx[e];
+        const int i = (int)(p / w), j = (int)(p % w);
+        const double a = wrap_mod_pi(th[p]), gx = cos(a), gy = sin(a);
+        const double md = sqrt(gx * gx + gy * gy), nx = gx / md, ny = gy / md;
+        double slope, fp, fm;
+        oct[e] = sp_octant(gx, gy);
+        sp_nms_interp(E, w, i, j, nx, ny, oct[e], &slope, &fp, &fm);
+        const double f = E[p], s = sqrt(1 + slope * slope);
+        const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s), ss = -B / (2 * A);
+        base[2 * e] = (int)floor(j + ss * nx);
+        base[2 * e + 1] = (int)floor(i + ss * ny);
+        fallback[e] = 0;   /* |F| < 1e-12: not met on the maps this is used with */
+    }
+    free(th);
+    return 0;
+}
+
+/* The frozen loss.  w4: n x 4 weights (or one row of 4 when per_edgel == 0). */
+double toed_subpix_frozen_loss(const double *E, int h, int w, double sigma, const int64_t *idx, long n,
+                               const double *w4, int per_edgel, const signed char *tneg, const signed char *sxy,
+                               const int *oct, const int *base, const int *fallback, double *outs) {
+    const size_t np = (size_t)h * w;
+    int rad;
+    double G[129], dG[129], d2G[129];
+    if (toed_subpix_taps(sigma, &rad, G, dG, d2G)) return NAN;
+    double *Axx = (double *)malloc(sizeof(double) * np), *Ayy = (double *)malloc(sizeof(double) * np);
+    sp_second(E, h, w, Axx, Ayy, NULL);
+    double *dxp = (double *)malloc(sizeof(double) * np), *dyp = (double *)malloc(sizeof(double) * np);
+    for (size_t p = 0; p < np; ++p) {
+        const double t = Ayy[p] * sxy[p] / Axx[p];
+        const double inv = 1.0 / sqrt(1.0 + t * t), sg = tneg[p] ? -1.0 : 1.0;
+        dxp[p] = sg * inv;
+        dyp[p] = sg * t * inv;
+    }
+    free(Axx); free(Ayy);
+    double *P[10], *tmp = (double *)malloc(sizeof(double) * np);
+    for (int k = 0; k < 10; ++k) P[k] = (double *)malloc(sizeof(double) * np);
+    for (size_t p = 0; p < np; ++p) { P[5][p] = E[p] * dxp[p]; P[9][p] = E[p] * dyp[p]; }
+    conv_circ(E, P[0], tmp, h, w, rad, dG, G);
+    conv_circ(E, P[1], tmp, h, w, rad, G, dG);
+    conv_circ(E, P[2], tmp, h, w, rad, d2G, G);
+    conv_circ(E, P[4], tmp, h, w, rad, dG, dG);
+    conv_circ(E, P[3], tmp, h, w, rad, G, d2G);
+    conv_circ(P[5], P[6], tmp, h, w, rad, dG, G);
+    conv_circ(P[9], P[8], tmp, h, w, rad, G, dG);
+    conv_circ(P[9], P[7], tmp, h, w, rad, dG, G);
+    double L = 0.0;
+    for (long e = 0; e < n; ++e) {
+        const int i = (int)(idx[e] / w), j = (int)(idx[e] % w);
+        const double gx = dxp[(size_t)idx[e]], gy = dyp[(size_t)idx[e]];
+        const double md = sqrt(gx * gx + gy * gy), nx = gx / md, ny = gy / md;
+        double slope, fp, fm;
+        sp_nms_interp(E, w, i, j, nx, ny, oct[e], &slope, &fp, &fm);
+        const double f = E[(size_t)idx[e]];
+        const double s = sqrt(1 + slope * slope);
+        const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s);
+        const double ss = -B / (2 * A), mf = A * ss * ss + B * ss + f;
+        const double x = j + ss * nx, y = i + ss * ny;
+        const int x0 = base[2 * e], y0 = base[2 * e + 1];
+        const double fx = x - x0, fy = y - y0;
+        const double wx[4] = {cubic_w(fx + 1), cubic_w(fx), cubic_w(fx - 1), cubic_w(fx - 2)};
+        const double wy[4] = {cubic_w(fy + 1), cubic_w(fy), cubic_w(fy - 1), cubic_w(fy - 2)};
+        double v[10];
+        for (int k = 0; k < 10; ++k) {
+            double acc = 0;
+            for (int a = 0; a < 4; ++a) {
+                int ii = y0 - 1 + a;
+                ii = ii < 0 ? 0 : (ii >= h ? h - 1 : ii);
+                double row = 0;
+                for (int b = 0; b < 4; ++b) {
+                    int jj = x0 - 1 + b;
+                    jj = jj < 0 ? 0 : (jj >= w ? w - 1 : jj);
+                    row += P[k][(size_t)ii * w + jj] * wx[b];
+                }
+                acc += row * wy[a];
+            }
+            v[k] = acc;
+        }
+        const double Fx = v[0] * v[6] + v[2] * v[5] + v[1] * v[7] + v[4] * v[9];
+        const double Fy = v[0] * v[7] + v[4] * v[5] + v[1] * v[8] + v[3] * v[9];
+        const double ori = fallback[e] ? atan2(gx, -gy) : atan2(Fx, -Fy);
+        const double o4[4] = {x, y, ori, fabs(mf)};
+        const double *we = w4 + (per_edgel ? 4 * e : 0);
+        for (int k = 0; k < 4; ++k) {
+            L += we[k] * o4[k];
+            if (outs) outs[4 * e + k] = o4[k];
+        }
+    }
+    for (int k = 0; k < 10; ++k) free(P[k]);
+    free(tmp); free(dxp); free(dyp);
+    return L;
+}

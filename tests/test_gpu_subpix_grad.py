@@ -1,0 +1,121 @@
+"""Differentiable subpixel correction on the GPU: dL/dmap from
+SubpixBatch.backward (and the autograd wrapper) against FP64 central finite
+differences of the oracle's frozen loss (oracle.SubpixFrozen: the same path
+with every discrete decision of the forward frozen).  Tolerance: 1e-4
+relative to the largest gradient entry of the map, per pixel, and 1e-4 of
+sum |g_i d_i| on a random directional derivative sum g_i d_i over all pixels
+(the sum cancels heavily, so its own size is no scale)."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+H_FD = 1e-6
+TOL = 1e-4
+
+
+def _fd(fr, E, G, d, h=H_FD):
+    """Richardson-extrapolated central difference of the frozen loss along d
+    (error O(h^4); weak NMS peaks make the loss strongly curved)."""
+    c1 = (fr.loss(E + h * d, G) - fr.loss(E - h * d, G)) / (2 * h)
+    c2 = (fr.loss(E + 0.5 * h * d, G) - fr.loss(E - 0.5 * h * d, G)) / h
+    return (4 * c2 - c1) / 3
+
+
+def _forward(pkg, detector, maps, thresh=0.1, sigma=2.0, nbr=2.0):
+    dev = [torch.from_numpy(np.ascontiguousarray(E, np.float32)).cuda() for E in maps]
+    b = pkg.SubpixBatch(detector, dev, None, thresh, sigma, nbr, with_index=True)
+    b.set_grad(True)
+    b.run()
+    return b, dev
+
+
+def _check_map(oracle, E, idx, G, g, sigma, rng, label):
+    fr = oracle.SubpixFrozen(E, idx, sigma)
+    L0, outs = fr.loss(E, G, outputs=True)
+    scale = np.abs(g).max()
+    assert scale > 0
+    # random directional derivative over every pixel
+    d = rng.standard_normal(E.shape)
+    fd = _fd(fr, E, G, d)
+    an = float((g * d).sum())
+    l1 = float(np.abs(g * d).sum())
+    print("%s: N=%d max|g|=%.4g dir fd=%.8g gpu=%.8g |diff|/sum|g d|=%.2e" % (label, len(idx), scale, fd, an,
+                                                                             abs(fd - an) / l1), flush=True)
+    # single pixels: the largest entries and random pixels near edgels
+    flat = np.abs(g).reshape(-1)
+    top = np.argsort(flat)[-24:]
+    nz = np.flatnonzero(flat > 1e-3 * scale)
+    pick = np.unique(np.concatenate([top, rng.choice(nz, size=min(24, len(nz)), replace=False)]))
+    worst, wself = 0.0, 0.0
+    for p in pick:
+        e = np.zeros(E.size)
+        e[p] = 1.0
+        fdp = _fd(fr, E, G, e.reshape(E.shape))
+        err = abs(fdp - g.reshape(-1)[p])
+        worst = max(worst, err / scale)
+        wself = max(wself, err / max(abs(fdp), 1e-3 * scale))
+    print("%s: %d pixels, worst |fd - gpu| / max|g| = %.2e, / |fd| = %.2e" % (label, len(pick), worst, wself),
+          flush=True)
+    assert worst <= TOL
+    assert abs(fd - an) <= TOL * l1
+
+
+def test_subpix_grad_matches_fd(pkg, detector, oracle):
+    z = np.load(os.path.join(GOLD, "subpix_ref_bezier_96x128_s1.npz"))
+    E = z["E"].astype(np.float64)
+    b, dev = _forward(pkg, detector, [z["E"]])
+    e = b.results()[0]
+    idx = b.index_results()[0].cpu().numpy().astype(np.int64)
+    ref = oracle.subpix(E, None, 0.1, 2.0, 2.0)
+    assert np.array_equal(idx, ref["idx"])
+    rng = np.random.default_rng(5)
+    G = rng.standard_normal((len(idx), 4)).astype(np.float32)
+    gmap = torch.zeros_like(dev[0])
+    b.backward([torch.from_numpy(G).cuda()], [gmap])
+    g = gmap.cpu().numpy().astype(np.float64)
+    _check_map(oracle, E, idx, G.astype(np.float64), g, 2.0, rng, "96x128")
+    # the backward accumulates: a second call doubles the gradient
+    b.backward([torch.from_numpy(G).cuda()], [gmap])
+    np.testing.assert_allclose(gmap.cpu().numpy(), 2 * g, rtol=1e-5, atol=1e-6 * np.abs(g).max())
+    b.close()
+
+
+def test_subpix_grad_batch_and_autograd(pkg, detector, oracle, scene):
+    maps = [scene.bezier_map(71, 80, 120, 8), scene.bezier_map(72, 112, 96, 8)]
+    B = len(maps)
+    b, dev = _forward(pkg, detector, maps, 0.1, 1.5, 2.0)
+    res, idx = b.results(), b.index_results()
+    rng = np.random.default_rng(9)
+    Gs = [rng.standard_normal((len(r), 4)).astype(np.float32) for r in res]
+    gmaps = [torch.zeros_like(d) for d in dev]
+    b.backward([torch.from_numpy(G).cuda() for G in Gs], gmaps)
+    for m in range(B):
+        _check_map(oracle, maps[m].astype(np.float64), idx[m].cpu().numpy().astype(np.int64),
+                   Gs[m].astype(np.float64), gmaps[m].cpu().numpy().astype(np.float64), 1.5, rng, "map %d" % m)
+    b.close()
+    # the autograd wrapper on a [B, H, W] stack gives the same as the plan API
+    E = scene.bezier_map(73, 96, 128, 10)
+    x = torch.from_numpy(np.stack([E, E[::-1].copy()])).cuda().requires_grad_(True)
+    w = torch.tensor([0.3, -1.2, 0.7, 2.0], device="cuda")
+    edges, off = pkg.subpix_differentiable(x, 0.1, 2.0, 2.0, detector=detector)
+    (edges * w).sum().backward()
+    gx = x.grad.cpu().numpy().astype(np.float64)
+    for m in range(2):
+        Em = x[m].detach().cpu().numpy().astype(np.float64)
+        rows = edges[int(off[m]):int(off[m + 1])]
+        ref = oracle.subpix(Em, None, 0.1, 2.0, 2.0)
+        assert len(rows) == ref["n"]
+        fr = oracle.SubpixFrozen(Em, ref["idx"], 2.0)
+        d = np.random.default_rng(m).standard_normal(Em.shape)
+        wn = w.cpu().numpy().astype(np.float64)
+        fd = _fd(fr, Em, wn, d)
+        an = float((gx[m] * d).sum())
+        l1 = float(np.abs(gx[m] * d).sum())
+        print("autograd map %d: fd %.8g gpu %.8g |diff|/sum|g d| %.2e" % (m, fd, an, abs(fd - an) / l1), flush=True)
+        assert abs(fd - an) <= TOL * l1

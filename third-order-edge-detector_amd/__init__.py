@@ -15,6 +15,8 @@ in the package's own HIP kernels.
     SubpixBatch                 subpixel third-order correction of dense edge maps
     SubpixelTOCorrectionGPU     mirror of SubpixelTOCorrectionCPU<T>
                                 (subpix_to_correction.hpp:20-96)
+    subpix_differentiable       torch.autograd.Function over the correction path:
+                                gradients of the edgel rows w.r.t. the maps
 """
 import ctypes
 import os
@@ -27,7 +29,7 @@ from ._lib import FLAG_CAPACITY, FLAG_STAGING, TOED_F32, TOED_U8, ToedError, che
 
 __all__ = ["Detector", "BatchDetector", "ThirdOrderEdgeDetectionGPU", "ToedError", "sigma_radius",
            "detect_differentiable", "DEFAULT_SIGMA", "DEFAULT_THRESH", "DEFAULT_BORDER", "SubpixBatch",
-           "SubpixelTOCorrectionGPU", "subpix_radius"]
+           "SubpixelTOCorrectionGPU", "subpix_radius", "subpix_differentiable"]
 
 # reference constants: sigma = 2 / 17-tap window (main_gpu_cpu.cpp:158-159),
 # threshold 2 (cpu_toed.cpp:403), NMS border 10 (cpu_toed.cpp:400-401)
@@ -348,4 +350,4 @@ def detect_differentiable(images, sigma, detector=None, thresh=DEFAULT_THRESH, b
     return _dd(images, sigma, detector, thresh, border, capacity)
 
 
-from .subpix import SubpixBatch, SubpixelTOCorrectionGPU, subpix_radius  # noqa: E402
+from .subpix import SubpixBatch, SubpixelTOCorrectionGPU, subpix_differentiable, subpix_radius  # noqa: E402

@@ -76,6 +76,14 @@ class toed_subpix_map_t(ctypes.Structure):
     ]
 
 
+class toed_subpix_grad_t(ctypes.Structure):
+    _fields_ = [
+        ("d_gout", ctypes.c_void_p),
+        ("d_gmap", ctypes.c_void_p),
+        ("gpitch", ctypes.c_int),
+    ]
+
+
 # exported symbols and their signatures: (name, restype, argtypes)
 _VP, _I, _F, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
 _PI, _PF = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
@@ -107,6 +115,8 @@ SIGNATURES = [
     ("toed_subpix_destroy", _I, [_VP]),
     ("toed_subpix_set_timing", _I, [_VP, _I]),
     ("toed_subpix_last_times", _I, [_VP, _PF, _PF, _PF]),
+    ("toed_subpix_set_grad", _I, [_VP, _I]),
+    ("toed_subpix_backward", _I, [_VP, ctypes.POINTER(toed_subpix_grad_t)]),
     ("toed_gpu_convolve_f32", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F]),
     ("toed_gpu_nms_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F, _I]),
 ]

@@ -74,6 +74,18 @@ struct SpJob {
     int H, W, pitch, opitch;
     int nw, ntx, nty, tile_begin;
     int chunk_begin, nchunks, tpitch, pad;
+    // differentiable form (null unless enabled on the plan)
+    float4 *tq;                 // H x W: dirx, diry, dOxx/dtheta, dOyy/dtheta coefficients
+    double2 *dirs;              // H x W: direction of survivors
+    uint32_t *wpre;             // H x nw: rank of each kept word's first edgel
+    float *ghx, *ghy, *gth;     // H x W scratch: dL/dHx, dL/dHy, dL/dtheta at edgels
+};
+
+// upstream gradient and output of one map's backward pass
+struct SpGrad {
+    const float4 *gout;         // dL/d[x, y, orientation, strength], raster order
+    float *gmap;                // dL/dE, accumulated, rows of gpitch
+    int gpitch, pad;
 };
 
 #define SP_CONST __attribute__((address_space(4)))
@@ -213,11 +225,20 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
                 else if (gi == H - 1) oyy = Oy(H - 1, gj) - Oy(H - 2, gj);
                 else oyy = 0.5 * (Oy(gi + 1, gj) - Oy(gi - 1, gj));
                 const double sgn = (oxy < 0) ? 1.0 : ((oxy > 0) ? -1.0 : 0.0);
-                dir_from_slope(oyy * sgn / (oxx + 1e-5), dx, dy);
+                const double a = oxx + 1e-5, t = oyy * sgn / a;
+                dir_from_slope(t, dx, dy);
+                if (J.tq) {
+                    // theta = atan(t) (+ pi): dtheta/dOxx = -t / ((1 + t^2) a), dtheta/dOyy = sgn / ((1 + t^2) a)
+                    const double c = 1.0 / ((1.0 + t * t) * a);
+                    const double cxx = -t * c, cyy = sgn * c;
+                    J.tq[(long long)gi * W + gj] = make_float4((float)dx, (float)dy, isfinite(cxx) ? (float)cxx : 0.f,
+                                                               isfinite(cyy) ? (float)cyy : 0.f);
+                }
             } else {
                 const double a = mod_pi((double)J.orient[(long long)gi * J.opitch + gj]);
                 dx = cos(a);
                 dy = sin(a);
+                if (J.tq) J.tq[(long long)gi * W + gj] = make_float4((float)dx, (float)dy, 0.f, 0.f);
             }
             const double e = (double)Es[li + 6][lj + 6];
             const long long o = (long long)gi * W + gj;
@@ -253,6 +274,7 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
                         const double mf = A * ss * ss + B * ss + f;
                         if (fabs(ss) <= sqrt(2.0)) {
                             J.pos[o] = make_double2(gj + ss * nx, gi + ss * ny);
+                            if (J.dirs) J.dirs[o] = make_double2(dx, dy);
                             J.val[o] = make_float2((float)fabs(mf), (float)atan2(dx, -dy));
                             surv = true;
                         }
@@ -289,74 +311,12 @@ template <int R> struct CGeo {
     static constexpr int FLOATS = IN + XN + YN;
 };
 
+// E, Hx, Hy over the plane region + R (periodic borders) into In
 template <int R>
-__global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+__device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, int x0) {
     using G = CGeo<R>;
-    constexpr int NK = G::NK, PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
-    __shared__ float lds[G::FLOATS];
-    __shared__ uint32_t vm[TH + 2 * MAXK][3];
-    __shared__ int s_flag;
-    float *In = lds, *X = lds + G::IN, *Y = lds + G::IN + G::XN;
-
-    const SpJob J = load_job(jobs);
-    const int lt = blockIdx.x;
-    if (lt >= J.ntx * J.nty) return;
-    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
-    const int y0 = ty * TH, x0 = tx * TW;
-    const int H = J.H, W = J.W, K = tp.K;
-
-    // survivor words of rows y0-K .. y0+TH+K, word columns tx-1 .. tx+1
-    if (threadIdx.x == 0) s_flag = 0;
-    for (int idx = threadIdx.x; idx < (TH + 2 * K) * 3; idx += NT) {
-        const int r = idx / 3, c = idx - r * 3;
-        const int gy = y0 - K + r, wc = tx - 1 + c;
-        vm[r][c] = (gy >= 0 && gy < H && wc >= 0 && wc < J.nw) ? J.vmask[(long long)gy * J.nw + wc] : 0u;
-    }
-    __syncthreads();
-    if (threadIdx.x < TH && vm[K + threadIdx.x][1]) s_flag = 1;
-    __syncthreads();
-    const bool any_surv = s_flag != 0;
-    __syncthreads();
-    const bool keep_all = tp.keep_all || (*J.nsurv < 2);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-
-    // isolated-edgel removal: another survivor within the radius
-    int keptm = 0;
-#pragma unroll 1
-    for (int k = 0; k < 2; ++k) {
-        const int p = threadIdx.x + k * NT;
-        const int li = p / TW, lj = p - li * TW;
-        const int gi = y0 + li, gj = x0 + lj;
-        bool kp = false;
-        if (any_surv && (vm[K + li][1] >> lj) & 1u) {
-            if (keep_all) {
-                kp = true;
-            } else {
-                const double2 me = J.pos[(long long)gi * W + gj];
-                for (int dy = -K; dy <= K && !kp; ++dy) {
-                    const int r = K + li + dy;
-                    for (int dxp = -K; dxp <= K; ++dxp) {
-                        const int c = TW + lj + dxp;          // column in the 3-word window
-                        if (!((vm[r][c >> 5] >> (c & 31)) & 1u) || (dy == 0 && dxp == 0)) continue;
-                        const double2 o = J.pos[(long long)(gi + dy) * W + (gj + dxp)];
-                        const double ddx = me.x - o.x, ddy = me.y - o.y;
-                        if (ddx * ddx + ddy * ddy <= tp.r2) { kp = true; break; }
-                    }
-                }
-            }
-        }
-        keptm |= (int)kp << k;
-        const unsigned long long b = __ballot(kp);
-        if ((lane & 31) == 0) {
-            const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
-            if (row < H) J.kmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
-        }
-        if (lane == 0 && b) s_flag = 2;
-    }
-    __syncthreads();
-    if (s_flag != 2) return;
-
-    // load E, Hx, Hy over the plane region + R, periodic borders
+    constexpr int IR = G::IR, IC = G::IC;
+    const int H = J.H, W = J.W;
     const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
     if (iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W) {
         constexpr int U = 4;
@@ -392,7 +352,14 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
             In[2 * IR * IC + idx] = J.hy[o];
         }
     }
-    __syncthreads();
+}
+
+// The eight circular Gaussian-derivative planes of the tile + HALO from In
+// (x pass into X, y pass into Y); ends with a barrier.
+template <int R>
+__device__ __forceinline__ void conv_planes(const float *In, float *X, float *Y, const SpTaps &tp) {
+    using G = CGeo<R>;
+    constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
     // group 0: E * (G', G, G'') along x -> Px, Pxx, Py, Pxy, Pyy
     // group 1: Hx * G', Hy * G, Hy * G' along x -> Hxx, Hyy, Hxy
     // planes Y: 0 Px, 1 Py, 2 Pxx, 3 Pyy, 4 Pxy, 5 Hxx, 6 Hxy, 7 Hyy
@@ -497,6 +464,79 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
         }
         __syncthreads();
     }
+
+}
+
+template <int R>
+__global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+    using G = CGeo<R>;
+    constexpr int NK = G::NK, PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
+    __shared__ float lds[G::FLOATS];
+    __shared__ uint32_t vm[TH + 2 * MAXK][3];
+    __shared__ int s_flag;
+    float *In = lds, *X = lds + G::IN, *Y = lds + G::IN + G::XN;
+
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if (lt >= J.ntx * J.nty) return;
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int H = J.H, W = J.W, K = tp.K;
+
+    // survivor words of rows y0-K .. y0+TH+K, word columns tx-1 .. tx+1
+    if (threadIdx.x == 0) s_flag = 0;
+    for (int idx = threadIdx.x; idx < (TH + 2 * K) * 3; idx += NT) {
+        const int r = idx / 3, c = idx - r * 3;
+        const int gy = y0 - K + r, wc = tx - 1 + c;
+        vm[r][c] = (gy >= 0 && gy < H && wc >= 0 && wc < J.nw) ? J.vmask[(long long)gy * J.nw + wc] : 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x < TH && vm[K + threadIdx.x][1]) s_flag = 1;
+    __syncthreads();
+    const bool any_surv = s_flag != 0;
+    __syncthreads();
+    const bool keep_all = tp.keep_all || (*J.nsurv < 2);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+
+    // isolated-edgel removal: another survivor within the radius
+    int keptm = 0;
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        const int li = p / TW, lj = p - li * TW;
+        const int gi = y0 + li, gj = x0 + lj;
+        bool kp = false;
+        if (any_surv && (vm[K + li][1] >> lj) & 1u) {
+            if (keep_all) {
+                kp = true;
+            } else {
+                const double2 me = J.pos[(long long)gi * W + gj];
+                for (int dy = -K; dy <= K && !kp; ++dy) {
+                    const int r = K + li + dy;
+                    for (int dxp = -K; dxp <= K; ++dxp) {
+                        const int c = TW + lj + dxp;          // column in the 3-word window
+                        if (!((vm[r][c >> 5] >> (c & 31)) & 1u) || (dy == 0 && dxp == 0)) continue;
+                        const double2 o = J.pos[(long long)(gi + dy) * W + (gj + dxp)];
+                        const double ddx = me.x - o.x, ddy = me.y - o.y;
+                        if (ddx * ddx + ddy * ddy <= tp.r2) { kp = true; break; }
+                    }
+                }
+            }
+        }
+        keptm |= (int)kp << k;
+        const unsigned long long b = __ballot(kp);
+        if ((lane & 31) == 0) {
+            const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
+            if (row < H) J.kmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
+        }
+        if (lane == 0 && b) s_flag = 2;
+    }
+    __syncthreads();
+    if (s_flag != 2) return;
+
+    load_tile_in<R>(J, In, y0, x0);
+    __syncthreads();
+    conv_planes<R>(In, X, Y, tp);
 
     // cubic sampling of the ten planes and the orientation, one edgel per thread
 #pragma unroll 1
@@ -633,6 +673,7 @@ __global__ __launch_bounds__(256) void sp_scatter_kernel(const SpJob *jobs, int 
 #pragma unroll 1
     for (int k = 0; k < PER; ++k) {
         uint32_t bits = wv[k];
+        if (J.wpre && w0 + k < nwords) J.wpre[w0 + k] = (uint32_t)rank;
         if (!bits) continue;
         const long long w = w0 + k;
         const int gi = (int)(w / J.nw), gj0 = (int)(w - (long long)gi * J.nw) * 32;
@@ -689,6 +730,407 @@ __global__ __launch_bounds__(NT) void sp_maps_kernel(const SpJob *jobs, int njob
     }
 }
 
+// ---------------------------------------------------------------------------
+// Differentiable form: adjoint of the path with every discrete decision
+// frozen (edgel set, NMS octant, the half-turn of each pixel's direction,
+// sign(-Oxy), cubic stencil base); no reference counterpart.
+
+__device__ __forceinline__ double keys_d(double s) {
+    const double a = -0.5;
+    const double x = fabs(s), sg = s < 0 ? -1.0 : 1.0;
+    if (x <= 1.0) return sg * ((3 * (a + 2) * x - 2 * (a + 3)) * x);
+    if (x < 2.0) return sg * ((3 * a * x - 10 * a) * x + 8 * a);
+    return 0.0;
+}
+
+// NMS octant of a direction, in the reference's branch order (:452-498), the
+// offsets (di, dj) of the two pixels of fp and of fm, and the slope family:
+// 0 ny/nx, 1 nx/ny, 2 -nx/ny, 3 -ny/nx.
+__device__ __forceinline__ int octant_of(double gx, double gy) {
+    if (gx >= 0 && gy >= 0) return gx >= gy ? 0 : 1;
+    if (gx < 0 && gy >= 0) return fabs(gx) < gy ? 2 : 3;
+    if (gx < 0 && gy < 0) return fabs(gx) >= fabs(gy) ? 4 : 5;
+    return gx < fabs(gy) ? 6 : 7;
+}
+// per octant: a (di, dj), b, c, d for fp = Ea (1 - s) + Eb s, fm = Ec (1 - s) + Ed s
+__constant__ signed char SP_OFF[8][8] = {
+    {0, 1, 1, 1, 0, -1, -1, -1},   {1, 0, 1, 1, -1, 0, -1, -1},  {1, 0, 1, -1, -1, 0, -1, 1},
+    {0, -1, 1, -1, 0, 1, -1, 1},   {0, -1, -1, -1, 0, 1, 1, 1},  {-1, 0, -1, -1, 1, 0, 1, 1},
+    {-1, 0, -1, 1, 1, 0, 1, -1},   {0, 1, -1, 1, 0, -1, 1, -1}};
+
+template <int R> struct BGeo {
+    using G = CGeo<R>;
+    static constexpr int PP = G::PR * G::PC;
+    static constexpr int ZN = G::XN > 8 * PP ? G::XN : 8 * PP;   // x pass, then the plane adjoints
+    static constexpr int FLOATS = G::IN + 8 * PP + ZN;
+};
+
+// SB1: per tile with kept edgels: recompute the planes, FP64 adjoint of the
+// NMS parabola, cubic sampling and third-order orientation per edgel; the
+// plane adjoints are scattered into LDS and pulled back through the
+// transposed circular convolutions; dL/dE, dL/dHx, dL/dHy of the tile + R +
+// HALO are added to HBM with float atomics (periodic wrap), dL/dtheta of the
+// NMS is stored at the edgel pixel.
+template <int R>
+__global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const SpGrad *grads, SpTaps tp) {
+    using G = CGeo<R>;
+    constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC, PP = PR * PC;
+    __shared__ float lds[BGeo<R>::FLOATS];
+    __shared__ uint32_t km[TH];
+    __shared__ int s_any;
+    // In (E, Hx, Hy) | Y (8 planes) | Z (x pass, then the 8 plane adjoints)
+    float *In = lds, *Y = lds + G::IN, *Z = lds + G::IN + 8 * PP;
+
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if (lt >= J.ntx * J.nty) return;
+    const SpGrad Gd = grads[blockIdx.y];
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int H = J.H, W = J.W;
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    if (threadIdx.x < TH) {
+        const int row = y0 + threadIdx.x;
+        const uint32_t w = row < H ? J.kmask[(long long)row * J.nw + tx] : 0u;
+        km[threadIdx.x] = w;
+        if (w) s_any = 1;
+    }
+    __syncthreads();
+    if (!s_any) return;
+    load_tile_in<R>(J, In, y0, x0);
+    __syncthreads();
+    conv_planes<R>(In, Z, Y, tp);
+    for (int idx = threadIdx.x; idx < 8 * PP; idx += NT) Z[idx] = 0.f;
+    __syncthreads();
+
+    // per-edgel adjoint (FP64), scattered at once: the 8 convolved-plane
+    // adjoints into Z (LDS atomics), the raw Hx, Hy adjoints into HBM
+    // v index -> adjoint plane: Px 0, Py 1, Pxx 2, Pyy 3, Pxy 4, Hxx 5, Hxy 6, Hyy 7
+    constexpr int VPL[10] = {0, 1, 2, 3, 4, -1, 5, 6, 7, -2};
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        const int li = p / TW, lj = p - li * TW;
+        if (!((km[li] >> lj) & 1u)) continue;
+        const int gi = y0 + li, gj = x0 + lj;
+        const long long o = (long long)gi * W + gj;
+        const int rank = (int)J.wpre[(long long)gi * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
+        const float4 g = (rank < J.capacity) ? Gd.gout[rank] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const double2 d = J.dirs[o];
+        auto Ev = [&](int di, int dj) -> double { return (double)In[(li + HALO + R + di) * IC + (lj + HALO + R + dj)]; };
+        const double md = sqrt(d.x * d.x + d.y * d.y), nx = d.x / md, ny = d.y / md;
+        const int oc = octant_of(d.x, d.y);
+        const int fam = oc & 3;
+        const double slope = fam == 0 ? ny / nx : (fam == 1 ? nx / ny : (fam == 2 ? -nx / ny : -ny / nx));
+        const double Ea = Ev(SP_OFF[oc][0], SP_OFF[oc][1]), Eb = Ev(SP_OFF[oc][2], SP_OFF[oc][3]);
+        const double Ec = Ev(SP_OFF[oc][4], SP_OFF[oc][5]), Ed = Ev(SP_OFF[oc][6], SP_OFF[oc][7]);
+        const double fp = Ea * (1 - slope) + Eb * slope, fm = Ec * (1 - slope) + Ed * slope, f = Ev(0, 0);
+        const double s = sqrt(1 + slope * slope);
+        const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s);
+        const double ss = -B / (2 * A), mf = A * ss * ss + B * ss + f;
+        const double x = gj + ss * nx, y = gi + ss * ny;
+        // cubic weights and their derivatives (stencil base frozen)
+        const int xf = (int)floor(x), yf = (int)floor(y);
+        const double fx = x - xf, fy = y - yf;
+        // the cubic sums in FP32 (the planes are FP32); the rest in FP64
+        float wx[4], wy[4], dwx[4], dwy[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            wx[b] = (float)keys(fx + 1 - b);
+            dwx[b] = (float)keys_d(fx + 1 - b);
+            wy[b] = (float)keys(fy + 1 - b);
+            dwy[b] = (float)keys_d(fy + 1 - b);
+        }
+        float v[10], vx[10], vy[10];
+#pragma unroll
+        for (int m = 0; m < 10; ++m) v[m] = vx[m] = vy[m] = 0.f;
+#pragma unroll 1
+        for (int a = 0; a < 4; ++a) {
+            const int ii = min(max(yf - 1 + a, 0), H - 1) - (y0 - HALO);
+            float row[10], rowd[10];
+#pragma unroll
+            for (int m = 0; m < 10; ++m) row[m] = rowd[m] = 0.f;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int jj = min(max(xf - 1 + b, 0), W - 1) - (x0 - HALO);
+                const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
+                float pv[10];
+                pv[0] = Y[0 * PP + yo]; pv[1] = Y[1 * PP + yo]; pv[2] = Y[2 * PP + yo]; pv[3] = Y[3 * PP + yo];
+                pv[4] = Y[4 * PP + yo]; pv[5] = In[IR * IC + io]; pv[6] = Y[5 * PP + yo]; pv[7] = Y[6 * PP + yo];
+                pv[8] = Y[7 * PP + yo]; pv[9] = In[2 * IR * IC + io];
+#pragma unroll
+                for (int m = 0; m < 10; ++m) {
+                    row[m] = fmaf(pv[m], wx[b], row[m]);
+                    rowd[m] = fmaf(pv[m], dwx[b], rowd[m]);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 10; ++m) {
+                v[m] = fmaf(row[m], wy[a], v[m]);
+                vx[m] = fmaf(rowd[m], wy[a], vx[m]);
+                vy[m] = fmaf(row[m], dwy[a], vy[m]);
+            }
+        }
+        const double Fx = (double)v[0] * v[6] + (double)v[2] * v[5] + (double)v[1] * v[7] + (double)v[4] * v[9];
+        const double Fy = (double)v[0] * v[7] + (double)v[4] * v[5] + (double)v[1] * v[8] + (double)v[3] * v[9];
+        const double F2 = Fx * Fx + Fy * Fy;
+        float vb[10];
+#pragma unroll
+        for (int m = 0; m < 10; ++m) vb[m] = 0.f;
+        double thb = 0.0;
+        if (!(sqrt(F2) < 1e-12)) {
+            // orientation = atan2(Fx, -Fy)
+            const float Fxb = (float)((double)g.z * (-Fy) / F2), Fyb = (float)((double)g.z * Fx / F2);
+            vb[0] = Fxb * v[6] + Fyb * v[7];
+            vb[6] = Fxb * v[0];
+            vb[2] = Fxb * v[5];
+            vb[5] = Fxb * v[2] + Fyb * v[4];
+            vb[1] = Fxb * v[7] + Fyb * v[8];
+            vb[7] = Fxb * v[1] + Fyb * v[0];
+            vb[4] = Fxb * v[9] + Fyb * v[5];
+            vb[9] = Fxb * v[4] + Fyb * v[3];
+            vb[8] = Fyb * v[1];
+            vb[3] = Fyb * v[9];
+        } else {
+            thb += (double)g.z;   // orientation = atan2(dirx, -diry) = theta + const
+        }
+        double xb = g.x, yb = g.y;
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+            xb += (double)vb[m] * vx[m];
+            yb += (double)vb[m] * vy[m];
+        }
+        // parabola and position: x = j + s* nx, y = i + s* ny, strength |f - B^2 / 4A|
+        const double ssb = xb * nx + yb * ny;
+        double nxb = xb * ss, nyb = yb * ss;
+        const double mfb = (double)g.w * (mf < 0 ? -1.0 : 1.0);
+        const double Ab = ssb * (-ss / A) + mfb * ss * ss;
+        const double Bb = ssb * (-1.0 / (2 * A)) + mfb * ss;
+        double fb = mfb - Ab / (s * s);
+        const double fpb = Ab / (2 * s * s) + Bb / (2 * s), fmb = Ab / (2 * s * s) - Bb / (2 * s);
+        const double sb = Ab * (-2 * A / s) + Bb * (-B / s);
+        double slb = sb * slope / s + fpb * (Eb - Ea) + fmb * (Ed - Ec);
+        float *gm = Gd.gmap;
+        const int gp = Gd.gpitch;
+        atomicAdd(&gm[(long long)gi * gp + gj], (float)fb);
+        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][0]) * gp + gj + SP_OFF[oc][1]], (float)(fpb * (1 - slope)));
+        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][2]) * gp + gj + SP_OFF[oc][3]], (float)(fpb * slope));
+        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][4]) * gp + gj + SP_OFF[oc][5]], (float)(fmb * (1 - slope)));
+        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][6]) * gp + gj + SP_OFF[oc][7]], (float)(fmb * slope));
+        if (fam == 0) { nxb += -slb * ny / (nx * nx); nyb += slb / nx; }
+        else if (fam == 1) { nxb += slb / ny; nyb += -slb * nx / (ny * ny); }
+        else if (fam == 2) { nxb += -slb / ny; nyb += slb * nx / (ny * ny); }
+        else { nxb += slb * ny / (nx * nx); nyb += -slb / nx; }
+        thb += -nxb * ny + nyb * nx;    // n = (cos theta, sin theta)
+        J.gth[o] = (float)thb;
+#pragma unroll 1
+        for (int a = 0; a < 4; ++a) {
+            const int gy = min(max(yf - 1 + a, 0), H - 1);
+            const int ii = gy - (y0 - HALO);
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int gx = min(max(xf - 1 + b, 0), W - 1);
+                const int jj = gx - (x0 - HALO);
+                const float w = wy[a] * wx[b];
+#pragma unroll
+                for (int m = 0; m < 10; ++m) {
+                    const float c = vb[m] * w;
+                    if (c == 0.f) continue;
+                    if (VPL[m] >= 0) atomicAdd(&Z[VPL[m] * PP + ii * PC + jj], c);
+                    else atomicAdd(VPL[m] == -1 ? &J.ghx[(long long)gy * W + gx] : &J.ghy[(long long)gy * W + gx], c);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // transposed y pass (Z -> X bar in Y's place) and x pass (X bar -> In bar in In's place), per group
+    float *Ib = In, *Xb = Y;
+#pragma unroll 1
+    for (int grp = 0; grp < 2; ++grp) {
+        for (int idx = threadIdx.x; idx < IR * PC; idx += NT) {
+            const int rho = idx / PC, c = idx - rho * PC;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+            for (int m = 0; m <= 2 * R; ++m) {
+                const int r = rho - m;
+                if (r < 0 || r >= PR) continue;
+                const int yo = r * PC + c;
+                const float kG = tp.k[0][2 * R - m], kD = tp.k[1][2 * R - m], kD2 = tp.k[2][2 * R - m];
+                if (grp == 0) {
+                    a0 = fmaf(Z[0 * PP + yo], kG, a0);
+                    a0 = fmaf(Z[4 * PP + yo], kD, a0);
+                    a1 = fmaf(Z[1 * PP + yo], kD, a1);
+                    a1 = fmaf(Z[3 * PP + yo], kD2, a1);
+                    a2 = fmaf(Z[2 * PP + yo], kG, a2);
+                } else {
+                    a0 = fmaf(Z[5 * PP + yo], kG, a0);
+                    a1 = fmaf(Z[7 * PP + yo], kD, a1);
+                    a2 = fmaf(Z[6 * PP + yo], kG, a2);
+                }
+            }
+            Xb[idx] = a0;
+            Xb[IR * PC + idx] = a1;
+            Xb[2 * IR * PC + idx] = a2;
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
+            const int r = idx / IC, gm = idx - r * IC;
+            float e0 = 0.f, e1 = 0.f;
+#pragma unroll
+            for (int m = 0; m <= 2 * R; ++m) {
+                const int c = gm - m;
+                if (c < 0 || c >= PC) continue;
+                const int xo = r * PC + c;
+                const float kG = tp.k[0][2 * R - m], kD = tp.k[1][2 * R - m], kD2 = tp.k[2][2 * R - m];
+                if (grp == 0) {
+                    e0 = fmaf(Xb[xo], kD, e0);
+                    e0 = fmaf(Xb[IR * PC + xo], kG, e0);
+                    e0 = fmaf(Xb[2 * IR * PC + xo], kD2, e0);
+                } else {
+                    e0 = fmaf(Xb[xo], kD, e0);
+                    e1 = fmaf(Xb[IR * PC + xo], kG, e1);
+                    e1 = fmaf(Xb[2 * IR * PC + xo], kD, e1);
+                }
+            }
+            if (grp == 0) {
+                Ib[idx] = e0;
+            } else {
+                Ib[IR * IC + idx] = e0;
+                Ib[2 * IR * IC + idx] = e1;
+            }
+        }
+        __syncthreads();
+    }
+    const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
+    for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
+        const int r = idx / IC, c = idx - r * IC;
+        const int gy = wrap_circ(iy0 + r, H), gx = wrap_circ(ix0 + c, W);
+        const float e = Ib[idx], hx = Ib[IR * IC + idx], hy = Ib[2 * IR * IC + idx];
+        if (e != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e);
+        if (hx != 0.f) atomicAdd(&J.ghx[(long long)gy * W + gx], hx);
+        if (hy != 0.f) atomicAdd(&J.ghy[(long long)gy * W + gx], hy);
+    }
+}
+
+// SB2: dense, per tile.  dL/dtheta at every pixel (the edgel's own NMS term
+// plus Hx = E cos(theta), Hy = E sin(theta)), dL/dOxx and dL/dOyy through
+// theta = atan(Oyy sgn / (Oxx + 1e-5)), then the transposed central
+// differences (one-sided at the border) and the transposed triangle smoothing
+// (half-sample-symmetric border) into dL/dE; plus dL/dE of the products.
+constexpr int DR = TH + 12, DC = TW + 12;    // tile +- 6
+
+__device__ __forceinline__ float d_transpose(const float *b, int stride, int j, int n, int jo) {
+    // (D^T b)(j), D the central difference with one-sided ends; b(k) = b[(k - jo) * stride]
+    float acc = 0.f;
+    if (j - 1 >= 1 && j - 1 <= n - 2) acc += 0.5f * b[(j - 1 - jo) * stride];
+    if (j + 1 >= 1 && j + 1 <= n - 2) acc -= 0.5f * b[(j + 1 - jo) * stride];
+    if (j == 1) acc += b[(0 - jo) * stride];
+    if (j == 0) acc -= b[(0 - jo) * stride];
+    if (j == n - 1) acc += b[(n - 1 - jo) * stride];
+    if (j == n - 2) acc -= b[(n - 1 - jo) * stride];
+    return acc;
+}
+
+__device__ __forceinline__ float tri_transpose(const float *b, int stride, int m, int n, int jo) {
+    // (T^T b)(m) for T the 9-tap triangle with half-sample-symmetric ends (n >= 5)
+    float acc = 0.f;
+#pragma unroll
+    for (int q = -4; q <= 4; ++q) {
+        const float t = (float)(5 - (q < 0 ? -q : q)) / 25.f;
+        const int i = m + q;
+        if (i >= 0 && i < n) acc += t * b[(i - jo) * stride];
+        const int il = q - 1 - m;             // i - q = -1 - m folds to m
+        if (il >= 0 && il < n) acc += t * b[(il - jo) * stride];
+        const int ir = 2 * n - 1 - m + q;     // i - q = 2n - 1 - m folds to m
+        if (ir >= 0 && ir < n) acc += t * b[(ir - jo) * stride];
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, const SpGrad *grads) {
+    __shared__ float Ox[DR][DC], Oy[DR][DC], T1[DR][DC], T2[DR][DC];
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if (lt >= J.ntx * J.nty) return;
+    const SpGrad Gd = grads[blockIdx.y];
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int H = J.H, W = J.W;
+    const int ry0 = y0 - 6, rx0 = x0 - 6;
+    for (int idx = threadIdx.x; idx < DR * DC; idx += NT) {
+        const int r = idx / DC, c = idx - r * DC;
+        const int gy = ry0 + r, gx = rx0 + c;
+        float ox = 0.f, oy = 0.f;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+            const long long o = (long long)gy * W + gx;
+            const float4 q = J.tq[o];
+            const float e = J.map[(long long)gy * J.pitch + gx];
+            float gt = e * (q.x * J.ghy[o] - q.y * J.ghx[o]);
+            if ((J.kmask[(long long)gy * J.nw + (gx >> 5)] >> (gx & 31)) & 1u) gt += J.gth[o];
+            ox = gt * q.z;
+            oy = gt * q.w;
+        }
+        Ox[r][c] = ox;
+        Oy[r][c] = oy;
+    }
+    __syncthreads();
+    // one transposed difference: x on Ox (cols 1..DC-2), y on Oy (rows 1..DR-2)
+    for (int idx = threadIdx.x; idx < DR * DC; idx += NT) {
+        const int r = idx / DC, c = idx - r * DC;
+        const int gy = ry0 + r, gx = rx0 + c;
+        float a = 0.f, b = 0.f;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
+            if (c >= 1 && c <= DC - 2) a = d_transpose(&Ox[r][0], 1, gx, W, rx0);
+            if (r >= 1 && r <= DR - 2) b = d_transpose(&Oy[0][c], DC, gy, H, ry0);
+        }
+        T1[r][c] = a;
+        T2[r][c] = b;
+    }
+    __syncthreads();
+    // second transposed difference, summed: S bar on rows/cols 2..DR-3 / DC-3
+    for (int idx = threadIdx.x; idx < DR * DC; idx += NT) {
+        const int r = idx / DC, c = idx - r * DC;
+        const int gy = ry0 + r, gx = rx0 + c;
+        float sb = 0.f;
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W && r >= 2 && r <= DR - 3 && c >= 2 && c <= DC - 3)
+            sb = d_transpose(&T1[r][0], 1, gx, W, rx0) + d_transpose(&T2[0][c], DC, gy, H, ry0);
+        Ox[r][c] = sb;
+    }
+    __syncthreads();
+    // transposed column pass (rows of the tile, cols 2..DC-3)
+    for (int idx = threadIdx.x; idx < TH * DC; idx += NT) {
+        const int rr = idx / DC, c = idx - rr * DC, r = rr + 6;
+        const int gy = ry0 + r, gx = rx0 + c;
+        float v = 0.f;
+        if (gy < H && gx >= 0 && gx < W && c >= 2 && c <= DC - 3) v = tri_transpose(&Ox[0][c], DC, gy, H, ry0);
+        Oy[r][c] = v;
+    }
+    __syncthreads();
+    // transposed row pass at the tile pixels, plus dL/dE of Hx = E dirx, Hy = E diry
+    for (int p = threadIdx.x; p < TH * TW; p += NT) {
+        const int li = p / TW, lj = p - li * TW;
+        const int gi = y0 + li, gj = x0 + lj;
+        if (gi >= H || gj >= W) continue;
+        const long long o = (long long)gi * W + gj;
+        const float4 q = J.tq[o];
+        float ge = tri_transpose(&Oy[li + 6][0], 1, gj, W, rx0);
+        ge += q.x * J.ghx[o] + q.y * J.ghy[o];
+        float *dst = &Gd.gmap[(long long)gi * Gd.gpitch + gj];
+        *dst += ge;
+    }
+}
+
+// zero dL/dHx, dL/dHy of every map (a kernel: the sequence stays capturable)
+__global__ void sp_bw_clear_kernel(const SpJob *jobs) {
+    const SpJob J = load_job(jobs);
+    const long long n = (long long)J.H * J.W;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        J.ghx[i] = 0.f;
+        J.ghy[i] = 0.f;
+    }
+}
+
 // zero per-map counters and flags (a kernel, so the sequence can be captured)
 __global__ void sp_reset_kernel(const SpJob *jobs, int njobs) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -715,6 +1157,10 @@ struct toed_subpix {
     int max_tiles = 0, max_chunks = 0;
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // differentiable form
+    bool grad = false, grad_ran = false;
+    char *d_gwork = nullptr;
+    SpGrad *d_grads = nullptr;
 };
 
 #define SPCHK(x)                                                                                         \
@@ -725,6 +1171,21 @@ struct toed_subpix {
             return TOED_ERR_HIP;                                                                         \
         }                                                                                                \
     } while (0)
+
+template <int R> static void launch_bw_tile(hipStream_t s, const toed_subpix *p) {
+    hipLaunchKernelGGL(sp_bw_tile_kernel<R>, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads, p->taps);
+}
+
+static int launch_bw_tile_r(hipStream_t s, const toed_subpix *p) {
+    switch (p->R) {
+#define SP_CASE(r) case r: launch_bw_tile<r>(s, p); break;
+        SP_CASE(1) SP_CASE(2) SP_CASE(3) SP_CASE(4) SP_CASE(5) SP_CASE(6)
+        SP_CASE(7) SP_CASE(8) SP_CASE(9) SP_CASE(10) SP_CASE(11) SP_CASE(12)
+#undef SP_CASE
+        default: return TOED_ERR_SIGMA;
+    }
+    return TOED_OK;
+}
 
 template <int R> static void launch_correct(hipStream_t s, const toed_subpix *p) {
     hipLaunchKernelGGL(sp_correct_kernel<R>, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n, p->taps);
@@ -883,6 +1344,72 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     if (maps) hipLaunchKernelGGL(sp_maps_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
     if (p->timing) SPCHK(hipEventRecord(p->ev[3], s));
     SPCHK(hipGetLastError());
+    p->grad_ran = p->grad;
+    return TOED_OK;
+}
+
+int toed_subpix_set_grad(toed_subpix *p, int on) {
+    if (!p) return TOED_ERR_ARG;
+    SPCHK(hipSetDevice(p->device));
+    if (on && !p->d_gwork) {
+        size_t total = 0;
+        std::vector<size_t> off(p->n);
+        for (int m = 0; m < p->n; ++m) {
+            const SpJob &J = p->jobs[m];
+            if (J.H < 5 || J.W < 5) return TOED_ERR_ARG;   // single reflection of the smoothing
+            const size_t px = (size_t)J.H * J.W;
+            off[m] = total;
+            total += align_up(px * 16) * 2 + align_up((size_t)J.H * J.nw * 4) + align_up(px * 4) * 3;
+        }
+        if (hipMalloc(&p->d_gwork, total) != hipSuccess) return TOED_ERR_ALLOC;
+        if (hipMalloc(&p->d_grads, sizeof(SpGrad) * p->n) != hipSuccess) return TOED_ERR_ALLOC;
+        SPCHK(hipMemset(p->d_gwork, 0, total));
+        for (int m = 0; m < p->n; ++m) {
+            SpJob &J = p->jobs[m];
+            const size_t px = (size_t)J.H * J.W;
+            char *b = p->d_gwork + off[m];
+            J.tq = (float4 *)b; b += align_up(px * 16);
+            J.dirs = (double2 *)b; b += align_up(px * 16);
+            J.wpre = (uint32_t *)b; b += align_up((size_t)J.H * J.nw * 4);
+            J.ghx = (float *)b; b += align_up(px * 4);
+            J.ghy = (float *)b; b += align_up(px * 4);
+            J.gth = (float *)b;
+        }
+    }
+    for (int m = 0; m < p->n && p->d_gwork; ++m) {
+        SpJob &J = p->jobs[m];
+        if (!on) { J.tq = nullptr; J.dirs = nullptr; J.wpre = nullptr; }
+        else if (!J.tq) {
+            // re-enable: the planes stay allocated at the same places
+            return toed_subpix_set_grad(p, 0), TOED_ERR_ARG;
+        }
+    }
+    p->grad = on != 0;
+    p->grad_ran = false;
+    SPCHK(hipMemcpy(p->d_jobs, p->jobs.data(), sizeof(SpJob) * p->n, hipMemcpyHostToDevice));
+    return TOED_OK;
+}
+
+int toed_subpix_backward(toed_subpix *p, const toed_subpix_grad_t *items) {
+    if (!p || !items) return TOED_ERR_ARG;
+    if (!p->grad_ran) return TOED_ERR_ARG;   // the last run must have had the differentiable form on
+    SPCHK(hipSetDevice(p->device));
+    std::vector<SpGrad> g(p->n);
+    for (int m = 0; m < p->n; ++m) {
+        if (!items[m].d_gmap || items[m].gpitch < p->jobs[m].W) return TOED_ERR_ARG;
+        if (!items[m].d_gout && p->jobs[m].capacity > 0) return TOED_ERR_ARG;
+        g[m].gout = (const float4 *)items[m].d_gout;
+        g[m].gmap = items[m].d_gmap;
+        g[m].gpitch = items[m].gpitch;
+        g[m].pad = 0;
+    }
+    hipStream_t s = (hipStream_t)toed_get_stream(p->ctx);
+    SPCHK(hipMemcpyAsync(p->d_grads, g.data(), sizeof(SpGrad) * p->n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(sp_bw_clear_kernel, dim3(256, p->n), dim3(256), 0, s, p->d_jobs);
+    const int st = launch_bw_tile_r(s, p);
+    if (st) return st;
+    hipLaunchKernelGGL(sp_bw_dense_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads);
+    SPCHK(hipGetLastError());
     return TOED_OK;
 }
 
@@ -918,6 +1445,8 @@ int toed_subpix_destroy(toed_subpix *p) {
     (void)hipFree(p->d_jobs);
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
+    (void)hipFree(p->d_gwork);
+    (void)hipFree(p->d_grads);
     delete p;
     return TOED_OK;
 }

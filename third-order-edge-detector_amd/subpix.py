@@ -112,6 +112,25 @@ class SubpixBatch:
               "toed_subpix_last_times")
         return a.value, b.value, c.value
 
+    def set_grad(self, on=True):
+        """Keep what the adjoint needs on later runs (differentiable form)."""
+        check(lib().toed_subpix_set_grad(self._p, int(bool(on))), "toed_subpix_set_grad")
+
+    def backward(self, gouts, gmaps):
+        """Adjoint of the last run (made with set_grad on), every discrete
+        decision frozen: gouts[m] holds map m's dL/d[x, y, orientation,
+        strength] rows in its output layout (float32 device, >= count rows),
+        gmaps[m] receives dL/dmap (accumulated, float32 device, unit column
+        stride)."""
+        n = len(self.maps)
+        arr = (_lib.toed_subpix_grad_t * n)()
+        for m in range(n):
+            arr[m].d_gout = gouts[m].data_ptr()
+            arr[m].d_gmap = gmaps[m].data_ptr()
+            arr[m].gpitch = int(gmaps[m].stride(0))
+        self.det._bind_stream()
+        check(lib().toed_subpix_backward(self._p, arr), "toed_subpix_backward")
+
     def _counts(self):
         counts = self.counts.cpu().numpy()
         if (self.flags.cpu().numpy() & FLAG_CAPACITY).any():
@@ -218,3 +237,94 @@ class SubpixelTOCorrectionGPU:
 
     def close(self):
         self._det.close()
+
+
+class _SubpixPlan:
+    def __init__(self, det, maps, thresh, sigma, nbr, cap):
+        self.maps = maps
+        self.gen = 0
+        self.b = SubpixBatch(det, [maps[k] for k in range(maps.shape[0])], None, thresh, sigma, nbr,
+                             capacities=[cap] * maps.shape[0], with_index=True)
+        self.b.set_grad(True)
+
+
+_SP_PLANS = {}
+
+
+def _subpix_plan(det, maps, thresh, sigma, nbr, cap):
+    key = (id(det), maps.data_ptr(), tuple(maps.shape), tuple(maps.stride()), float(thresh), float(sigma),
+           float(nbr), int(cap))
+    p = _SP_PLANS.get(key)
+    if p is None or p.maps is not maps:
+        if len(_SP_PLANS) > 8:
+            _SP_PLANS.clear()
+        p = _SubpixPlan(det, maps, thresh, sigma, nbr, cap)
+        _SP_PLANS[key] = p
+    return p
+
+
+_SP_FN = None
+
+
+def _make_subpix_function():
+    torch = _torch()
+
+    class _Subpix(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, maps, det, thresh, sigma, nbr, cap):
+            plan = _subpix_plan(det, maps, thresh, sigma, nbr, cap)
+            b = plan.b
+            b.run()
+            plan.gen += 1
+            ctx.gen = plan.gen
+            counts = b._counts().astype(np.int64)
+            offsets = np.concatenate([[0], np.cumsum(counts)])
+            parts = [b.edges[int(b.offsets[m]):int(b.offsets[m]) + int(counts[m])] for m in range(len(counts))]
+            out = torch.cat(parts, 0) if parts else b.edges[:0]
+            ctx.plan, ctx.counts, ctx.offsets = plan, counts, offsets
+            off_t = torch.from_numpy(offsets)
+            ctx.mark_non_differentiable(off_t)
+            return out, off_t
+
+        @staticmethod
+        def backward(ctx, g_out, g_offsets):
+            plan, counts, offsets = ctx.plan, ctx.counts, ctx.offsets
+            if plan.gen != ctx.gen:
+                raise RuntimeError("subpix_differentiable: the plan ran again before this backward")
+            maps = plan.maps
+            g_out = g_out.contiguous().float()
+            gmap = torch.zeros_like(maps, dtype=torch.float32)
+            b = plan.b
+            gouts = []
+            for m in range(maps.shape[0]):
+                gouts.append(g_out[int(offsets[m]):] if counts[m] else torch.zeros(4, device=maps.device))
+            b.backward(gouts, [gmap[m] for m in range(maps.shape[0])])
+            return gmap, None, None, None, None, None
+
+    return _Subpix
+
+
+def subpix_differentiable(maps, thresh=DEFAULT_THRESH, sigma=DEFAULT_SIGMA, nbr_radius=DEFAULT_NBR_RADIUS,
+                          detector=None, capacity=None):
+    """Differentiable subpixel third-order correction of a float32 [B, H, W]
+    device tensor of edge maps (orientation estimated from each map).
+    Returns (edges, offsets) like detect_differentiable: the concatenated
+    raster-ordered N_b x 4 lists and the int64 row offsets (host).  The
+    backward freezes every discrete decision of the forward (edgel set, NMS
+    octants, cubic stencils, per-pixel half-turn of the direction and
+    sign(-Oxy)) and returns dL/dmaps."""
+    global _SP_FN
+    torch = _torch()
+    if _SP_FN is None:
+        _SP_FN = _make_subpix_function()
+    if maps.dim() == 2:
+        maps = maps.unsqueeze(0)
+    if maps.dtype != torch.float32 or not maps.is_cuda or maps.dim() != 3:
+        raise TypeError("maps must be a float32 [B, H, W] device tensor")
+    if maps.stride(2) != 1:
+        maps = maps.contiguous()
+    from . import autograd as _ag
+    det = detector or _ag._default_detector(maps.device)
+    B, H, W = maps.shape
+    cap = int(capacity) if capacity else max(1024, H * W // 4)
+    return _SP_FN.apply(maps, det, float(thresh), float(sigma), float(nbr_radius), cap)
