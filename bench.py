@@ -151,17 +151,34 @@ def bench_fwd_bwd(torch, pkg, scene, steps, warmup, B=64, H=512, W=512, sigma=1.
 def bench_subpix(torch, pkg, scene, steps, warmup, B=32, S=2048):
     """Subpixel third-order correction on B float32 Bezier edge maps of S x S
     (40 curves, thresh 0.1, sigma 2, orientation estimated, isolation radius
-    2) in one plan."""
-    maps = [torch.from_numpy(scene.bezier_map(5000 + b, S, S, 40)).cuda() for b in range(B)]
+    2) in one plan; then the differentiable form (autograd wrapper, loss
+    sum(a x + b y + c theta + d s) with (a, b, c, d) ~ N(0, 1) fixed from the
+    seed, gradient w.r.t. every map pixel): forward alone and forward+backward."""
+    x = torch.from_numpy(np.stack([scene.bezier_map(5000 + b, S, S, 40) for b in range(B)])).cuda()
     det = pkg.Detector(0)
-    sp = pkg.SubpixBatch(det, maps, None, 0.1, 2.0, 2.0, capacities=[S * S // 8] * B)
+    cap = S * S // 8
+    sp = pkg.SubpixBatch(det, [x[b] for b in range(B)], None, 0.1, 2.0, 2.0, capacities=[cap] * B)
     sec = time_loop(torch, sp.run, steps, warmup)
     counts = sp.counts.cpu().numpy()
     assert not sp.flags.cpu().numpy().any()
-    map0 = maps[0].cpu().numpy()
     sp.close()
+    xg = x.clone().requires_grad_(True)
+    w = torch.tensor(scene.loss_weights(12345), dtype=torch.float32, device="cuda")
+
+    def fwd():
+        with torch.no_grad():
+            pkg.subpix_differentiable(xg, 0.1, 2.0, 2.0, detector=det, capacity=cap)
+
+    def fwd_bwd():
+        xg.grad = None
+        e, off = pkg.subpix_differentiable(xg, 0.1, 2.0, 2.0, detector=det, capacity=cap)
+        (e * w).sum().backward()
+
+    t_f = time_loop(torch, fwd, steps, warmup)
+    t_fb = time_loop(torch, fwd_bwd, steps, warmup)
+    map0 = x[0].cpu().numpy()
     det.close()
-    return dict(px=float(B * S * S), sec=sec, edgels=int(counts.sum()), map0=map0, B=B, S=S)
+    return dict(px=float(B * S * S), sec=sec, edgels=int(counts.sum()), map0=map0, B=B, S=S, t_f=t_f, t_fb=t_fb)
 
 
 def cpu_baseline_subpix(map_np):
@@ -287,6 +304,12 @@ def main():
         sp = bench_subpix(torch, pkg, scene, max(3, args.steps // 4), 1)
         forms["subpix_32x2048x2048"] = {"mpx_s": round(sp["px"] / sp["sec"] / 1e6, 1), "ms": round(sp["sec"] * 1e3, 3),
                                         "edgels": sp["edgels"], "edgels_s": round(sp["edgels"] / sp["sec"], 1)}
+        forms["subpix_fwd_only_differentiable"] = {"mpx_s": round(sp["px"] / sp["t_f"] / 1e6, 1),
+                                                   "ms": round(sp["t_f"] * 1e3, 3)}
+        forms["subpix_fwd_bwd_32x2048x2048"] = {"mpx_s": round(sp["px"] / sp["t_fb"] / 1e6, 1),
+                                                "ms": round(sp["t_fb"] * 1e3, 3),
+                                                "ratio_to_fwd": round(sp["t_fb"] / sp["t_f"], 3),
+                                                "ratio_to_plain_fwd": round(sp["t_fb"] / sp["sec"], 3)}
         if not args.no_cpu:
             log("subpix cpu baseline ...")
             forms["subpix_32x2048x2048"]["cpu_baseline"] = cpu_baseline_subpix(sp["map0"])
