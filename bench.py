@@ -30,6 +30,10 @@ PKG_NAME = "third-order-edge-detector_amd"
 METRIC = "TOED Mpixel/s at 3840x2160: batched & fwd+bwd vs plain, same GPU; % HBM peak"
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the plain-frame tile kernel (2160x3840 u8, sigma 2), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, FETCH_SIZE scaled by
+# 2.0 (what the counter reads for a 256 MiB copy); the summary is kept under profiles/.
+TRAFFIC_TILES_PLAIN_BYTES = (2.0 * 24196.1 + 19272.1) * 1024
 BATCH_SIZES = [(480, 640), (480, 752), (720, 1280), (1080, 1920), (2160, 3840)]
 BATCH_SIGMAS = [1.0, 1.5, 2.0]
 
@@ -257,7 +261,8 @@ def main():
         flops = dense_flops_per_px(2.0, pkg) * px
         ach = flops / (plain["ms_tiles"] * 1e-3) / 1e12
         out["roofline"] = {"bound": "valu", "achieved": round(ach, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                           "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": round(TRAFFIC_TILES_PLAIN_BYTES),
+                           "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE)",
                            "kernel": "toed_tiles_kernel<uint8_t,16>",
                            "algorithmic": "2*(10*W1+2*W0) flop/px = %d flop/px at sigma 2, %d px/launch"
                                           % (dense_flops_per_px(2.0, pkg), px)}

@@ -1,0 +1,41 @@
+"""The C++ drop-in headers (include/toed_gpu.hpp, include/toed_subpix_gpu.hpp)
+compile as a host translation unit that instantiates the classes the way the
+reference's drivers do (CPU only: nothing is run)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SRC = r'''
+#include "toed_gpu.hpp"
+#include "toed_subpix_gpu.hpp"
+int use_classes(int H, int W, const double *E, double *out) {
+    // main_subpix_to_correction.cpp:279-289, class name changed
+    SubpixelTOCorrectionGPU<double> to_corr(H, W, 0.1, 2.0, 8, 0);
+    to_corr.set_output_dir("./out");
+    to_corr.set_neighbor_radius(2.0);
+    to_corr.set_edgemap(E);
+    to_corr.estimate_orientation();
+    int n = to_corr.run(out);
+    to_corr.write_array_to_file("TO_corrected_edges.txt", out, n, 4);
+    to_corr.write_array_to_file("TO_edgemap.txt", to_corr.TO_edgemap, H, W);
+    // main_gpu_cpu.cpp:158-190 with the GPU class
+    ThirdOrderEdgeDetectionGPU<float> det(0, H, W, 17, 2);
+    return n + det.edge_pt_list_idx;
+}
+'''
+
+
+def test_headers_compile(tmp_path):
+    gxx = shutil.which("g++")
+    if not gxx or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("no host compiler / ROCm headers")
+    src = tmp_path / "use.cpp"
+    src.write_text(SRC)
+    cmd = [gxx, "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + os.path.join(ROOT, "include"), str(src)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr[-3000:]
