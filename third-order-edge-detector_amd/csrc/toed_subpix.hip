@@ -64,7 +64,7 @@ struct SpJob {
     double2 *pos;               // H x W: FP64 subpixel position of survivors
     float2 *val;                // H x W: strength, orientation of survivors
     uint32_t *vmask, *kmask;    // H x nw survivor / kept words
-    int *nsurv;                 // survivors of the whole map
+    int *nsurv;                 // survivors of the whole map, counted up to 2 (own 256-byte slot)
     float *edges;
     int *index;
     float *to_e, *to_o;
@@ -287,7 +287,10 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
             const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
             if (row < H) J.vmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
         }
-        if (lane == 0 && b) atomicAdd(J.nsurv, (int)__popcll(b));
+        // only "fewer than two survivors" matters (:563-564): stop counting at 2, so the
+        // map's counter line sees a handful of atomics instead of one per wavefront
+        if (lane == 0 && b && __hip_atomic_load(J.nsurv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < 2)
+            atomicAdd(J.nsurv, (int)__popcll(b));
     }
 }
 
@@ -944,68 +947,99 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
         }
     }
     __syncthreads();
-    // transposed y pass (Z -> X bar in Y's place) and x pass (X bar -> In bar in In's place), per group
+    // transposed y pass (Z -> X bar in Y's place) and x pass (X bar -> In bar in In's place), per
+    // group.  Output u of a 4-output block at o0 sums window[u + m] * k[m], m = 0..2R, with
+    // window[t] = source(o0 - 2R + t) (zero outside the source): the transposed filters.
     float *Ib = In, *Xb = Y;
-#pragma unroll 1
-    for (int grp = 0; grp < 2; ++grp) {
-        for (int idx = threadIdx.x; idx < IR * PC; idx += NT) {
-            const int rho = idx / PC, c = idx - rho * PC;
-            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    constexpr int NB = 4, NWIN = 2 * R + NB;
+    constexpr int YB = (IR + NB - 1) / NB, XB = (IC + NB - 1) / NB;
 #pragma unroll
-            for (int m = 0; m <= 2 * R; ++m) {
-                const int r = rho - m;
-                if (r < 0 || r >= PR) continue;
-                const int yo = r * PC + c;
-                const float kG = tp.k[0][2 * R - m], kD = tp.k[1][2 * R - m], kD2 = tp.k[2][2 * R - m];
-                if (grp == 0) {
-                    a0 = fmaf(Z[0 * PP + yo], kG, a0);
-                    a0 = fmaf(Z[4 * PP + yo], kD, a0);
-                    a1 = fmaf(Z[1 * PP + yo], kD, a1);
-                    a1 = fmaf(Z[3 * PP + yo], kD2, a1);
-                    a2 = fmaf(Z[2 * PP + yo], kG, a2);
-                } else {
-                    a0 = fmaf(Z[5 * PP + yo], kG, a0);
-                    a1 = fmaf(Z[7 * PP + yo], kD, a1);
-                    a2 = fmaf(Z[6 * PP + yo], kG, a2);
+    for (int grp = 0; grp < 2; ++grp) {
+        for (int task = threadIdx.x; task < YB * PC; task += NT) {
+            const int blk = task / PC, c = task - blk * PC, r0 = blk * NB;
+            float acc[3][NB];
+#pragma unroll
+            for (int g = 0; g < 3; ++g)
+#pragma unroll
+                for (int u = 0; u < NB; ++u) acc[g][u] = 0.f;
+            // (adjoint plane, X bar plane, filter family) of the group
+            constexpr int NSRC = 5;
+            constexpr int src[2][NSRC] = {{0, 4, 1, 3, 2}, {5, 7, 6, -1, -1}};
+            constexpr int dst[2][NSRC] = {{0, 0, 1, 1, 2}, {0, 1, 2, -1, -1}};
+            constexpr int fam[2][NSRC] = {{0, 1, 1, 2, 0}, {0, 1, 0, -1, -1}};
+#pragma unroll
+            for (int q = 0; q < NSRC; ++q) {
+                if (src[grp][q] < 0) continue;
+                const float *zp = Z + src[grp][q] * PP + c;
+                float win[NWIN];
+#pragma unroll
+                for (int t = 0; t < NWIN; ++t) {
+                    const int r = r0 - 2 * R + t;
+                    win[t] = (r >= 0 && r < PR) ? zp[r * PC] : 0.f;
+                }
+                const int d = dst[grp][q], f = fam[grp][q];
+#pragma unroll
+                for (int m = 0; m <= 2 * R; ++m) {
+                    const float kk = tp.k[f][m];
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) acc[d][u] = fmaf(win[u + m], kk, acc[d][u]);
                 }
             }
-            Xb[idx] = a0;
-            Xb[IR * PC + idx] = a1;
-            Xb[2 * IR * PC + idx] = a2;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                if (r0 + u >= IR) continue;
+#pragma unroll
+                for (int g = 0; g < 3; ++g) Xb[g * IR * PC + (r0 + u) * PC + c] = acc[g][u];
+            }
         }
         __syncthreads();
-        for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
-            const int r = idx / IC, gm = idx - r * IC;
-            float e0 = 0.f, e1 = 0.f;
+        for (int task = threadIdx.x; task < IR * XB; task += NT) {
+            const int r = task / XB, blk = task - r * XB, c0 = blk * NB;
+            float e0[NB], e1[NB];
 #pragma unroll
-            for (int m = 0; m <= 2 * R; ++m) {
-                const int c = gm - m;
-                if (c < 0 || c >= PC) continue;
-                const int xo = r * PC + c;
-                const float kG = tp.k[0][2 * R - m], kD = tp.k[1][2 * R - m], kD2 = tp.k[2][2 * R - m];
-                if (grp == 0) {
-                    e0 = fmaf(Xb[xo], kD, e0);
-                    e0 = fmaf(Xb[IR * PC + xo], kG, e0);
-                    e0 = fmaf(Xb[2 * IR * PC + xo], kD2, e0);
-                } else {
-                    e0 = fmaf(Xb[xo], kD, e0);
-                    e1 = fmaf(Xb[IR * PC + xo], kG, e1);
-                    e1 = fmaf(Xb[2 * IR * PC + xo], kD, e1);
+            for (int u = 0; u < NB; ++u) e0[u] = e1[u] = 0.f;
+            // group 0: E bar = X0 * G' + X1 * G + X2 * G''; group 1: Hx bar = X0 * G',
+            // Hy bar = X1 * G + X2 * G'
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float *xp = Xb + q * IR * PC + r * PC;
+                float win[NWIN];
+#pragma unroll
+                for (int t = 0; t < NWIN; ++t) {
+                    const int c = c0 - 2 * R + t;
+                    win[t] = (c >= 0 && c < PC) ? xp[c] : 0.f;
+                }
+                const int f = grp == 0 ? (q == 0 ? 1 : (q == 1 ? 0 : 2)) : (q == 0 ? 1 : (q == 1 ? 0 : 1));
+                const bool to1 = grp == 1 && q > 0;
+#pragma unroll
+                for (int m = 0; m <= 2 * R; ++m) {
+                    const float kk = tp.k[f][m];
+#pragma unroll
+                    for (int u = 0; u < NB; ++u) {
+                        if (to1) e1[u] = fmaf(win[u + m], kk, e1[u]);
+                        else e0[u] = fmaf(win[u + m], kk, e0[u]);
+                    }
                 }
             }
-            if (grp == 0) {
-                Ib[idx] = e0;
-            } else {
-                Ib[IR * IC + idx] = e0;
-                Ib[2 * IR * IC + idx] = e1;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int gm = c0 + u;
+                if (gm >= IC) continue;
+                if (grp == 0) {
+                    Ib[r * IC + gm] = e0[u];
+                } else {
+                    Ib[IR * IC + r * IC + gm] = e0[u];
+                    Ib[2 * IR * IC + r * IC + gm] = e1[u];
+                }
             }
         }
         __syncthreads();
     }
     const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
+    const bool inner = iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W;
     for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
         const int r = idx / IC, c = idx - r * IC;
-        const int gy = wrap_circ(iy0 + r, H), gx = wrap_circ(ix0 + c, W);
+        const int gy = inner ? iy0 + r : wrap_circ(iy0 + r, H), gx = inner ? ix0 + c : wrap_circ(ix0 + c, W);
         const float e = Ib[idx], hx = Ib[IR * IC + idx], hy = Ib[2 * IR * IC + idx];
         if (e != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e);
         if (hx != 0.f) atomicAdd(&J.ghx[(long long)gy * W + gx], hx);
@@ -1261,7 +1295,7 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
     }
     if (hipSetDevice(p->device) != hipSuccess) { delete p; return TOED_ERR_HIP; }
     if (hipMalloc(&p->d_work, total) != hipSuccess) { delete p; return TOED_ERR_ALLOC; }
-    if (hipMalloc(&p->d_nsurv, sizeof(int) * n) != hipSuccess) { toed_subpix_destroy(p); return TOED_ERR_ALLOC; }
+    if (hipMalloc(&p->d_nsurv, 256 * (size_t)n) != hipSuccess) { toed_subpix_destroy(p); return TOED_ERR_ALLOC; }
     long long tiles = 0, chunks = 0;
     p->jobs.resize(n);
     for (int m = 0; m < n; ++m) {
@@ -1277,7 +1311,7 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
         J.nw = (M.W + 31) / 32;
         J.vmask = (uint32_t *)b; b += align_up((size_t)M.H * J.nw * 4);
         J.kmask = (uint32_t *)b;
-        J.nsurv = p->d_nsurv + m;
+        J.nsurv = p->d_nsurv + 64 * m;
         J.map = M.d_map;
         J.orient = M.d_orient;
         J.edges = M.d_edges;
