@@ -470,14 +470,14 @@ __device__ __forceinline__ void conv_planes(const float *In, float *X, float *Y,
 
 }
 
-template <int R>
-__global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
-    using G = CGeo<R>;
-    constexpr int NK = G::NK, PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
-    __shared__ float lds[G::FLOATS];
+// B1: isolated-edgel removal (:558-621) for every tile: an edgel is kept
+// when another survivor lies within the radius; survivors within
+// floor(r + 2 sqrt 2) pixels are candidates (an edgel lies within sqrt 2 of
+// its pixel).  Small LDS, so many tiles are in flight to hide the loads.
+__global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTaps tp) {
     __shared__ uint32_t vm[TH + 2 * MAXK][3];
+    __shared__ double2 posw[(TH + 2 * MAXK) * (TW + 2 * MAXK)];
     __shared__ int s_flag;
-    float *In = lds, *X = lds + G::IN, *Y = lds + G::IN + G::XN;
 
     const SpJob J = load_job(jobs);
     const int lt = blockIdx.x;
@@ -485,6 +485,7 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W, K = tp.K;
+    const int nsurv = *J.nsurv;
 
     // survivor words of rows y0-K .. y0+TH+K, word columns tx-1 .. tx+1
     if (threadIdx.x == 0) s_flag = 0;
@@ -498,97 +499,140 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     __syncthreads();
     const bool any_surv = s_flag != 0;
     __syncthreads();
-    const bool keep_all = tp.keep_all || (*J.nsurv < 2);
+    const bool keep_all = tp.keep_all || (nsurv < 2);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    // isolated-edgel removal: another survivor within the radius
-    int keptm = 0;
+    // the survivors' positions of the tile + K window are staged in LDS
+    const int WR = TH + 2 * K, WC = TW + 2 * K;
+    auto survivor = [&](int r, int c) -> bool {      // window (r, c): tile (r - K, c - K)
+        const int cc = TW - K + c;                   // column in the 3-word mask window
+        return (vm[r][cc >> 5] >> (cc & 31)) & 1u;
+    };
+    if (any_surv && !keep_all) {
+        // all of a thread's loads are issued before its LDS stores
+        constexpr int UW = ((TH + 2 * MAXK) * (TW + 2 * MAXK) + NT - 1) / NT;
+        double2 pv[UW];
+        bool sv[UW];
+#pragma unroll
+        for (int u = 0; u < UW; ++u) {
+            const int idx = threadIdx.x + u * NT;
+            const int r = idx / WC, c = idx - r * WC;
+            sv[u] = idx < WR * WC && survivor(r, c);
+            pv[u] = sv[u] ? J.pos[(long long)(y0 - K + r) * W + (x0 - K + c)] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < UW; ++u)
+            if (sv[u]) posw[threadIdx.x + u * NT] = pv[u];
+    }
+    __syncthreads();
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
         const int p = threadIdx.x + k * NT;
         const int li = p / TW, lj = p - li * TW;
-        const int gi = y0 + li, gj = x0 + lj;
         bool kp = false;
         if (any_surv && (vm[K + li][1] >> lj) & 1u) {
             if (keep_all) {
                 kp = true;
             } else {
-                const double2 me = J.pos[(long long)gi * W + gj];
+                const double2 me = posw[(li + K) * WC + (lj + K)];
                 for (int dy = -K; dy <= K && !kp; ++dy) {
                     const int r = K + li + dy;
                     for (int dxp = -K; dxp <= K; ++dxp) {
-                        const int c = TW + lj + dxp;          // column in the 3-word window
-                        if (!((vm[r][c >> 5] >> (c & 31)) & 1u) || (dy == 0 && dxp == 0)) continue;
-                        const double2 o = J.pos[(long long)(gi + dy) * W + (gj + dxp)];
+                        const int c = K + lj + dxp;
+                        if (!survivor(r, c) || (dy == 0 && dxp == 0)) continue;
+                        const double2 o = posw[r * WC + c];
                         const double ddx = me.x - o.x, ddy = me.y - o.y;
                         if (ddx * ddx + ddy * ddy <= tp.r2) { kp = true; break; }
                     }
                 }
             }
         }
-        keptm |= (int)kp << k;
         const unsigned long long b = __ballot(kp);
         if ((lane & 31) == 0) {
             const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
             if (row < H) J.kmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
         }
-        if (lane == 0 && b) s_flag = 2;
+    }
+}
+
+// B2: for tiles with kept edgels: the eight planes of the tile + HALO in LDS
+// and the cubic sampling / third-order orientation of every kept edgel.
+template <int R>
+__global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+    using G = CGeo<R>;
+    constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
+    __shared__ float lds[G::FLOATS];
+    __shared__ uint32_t km[TH];
+    __shared__ int s_nlist;
+    __shared__ short elist[TH * TW];
+    float *In = lds, *X = lds + G::IN, *Y = lds + G::IN + G::XN;
+
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if (lt >= J.ntx * J.nty) return;
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    const int y0 = ty * TH, x0 = tx * TW;
+    const int H = J.H, W = J.W;
+    if (threadIdx.x == 0) s_nlist = 0;
+    if (threadIdx.x < TH) {
+        const int row = y0 + threadIdx.x;
+        km[threadIdx.x] = row < H ? J.kmask[(long long)row * J.nw + tx] : 0u;
     }
     __syncthreads();
-    if (s_flag != 2) return;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        if ((km[p / TW] >> (p % TW)) & 1u) elist[atomicAdd(&s_nlist, 1)] = (short)p;
+    }
+    __syncthreads();
+    if (s_nlist == 0) return;
 
     load_tile_in<R>(J, In, y0, x0);
     __syncthreads();
     conv_planes<R>(In, X, Y, tp);
 
-    // cubic sampling of the ten planes and the orientation, one edgel per thread
+    // cubic sampling of the ten planes and the orientation: 16 lanes per edgel,
+    // lane (a, b) of the 4 x 4 stencil, FP64 sums over the group by shuffles
+    const int nlist = s_nlist;
+    const int sub = threadIdx.x & 15;
+    const int sa = sub >> 2, sb = sub & 3;
 #pragma unroll 1
-    for (int k = 0; k < 2; ++k) {
-        if (!((keptm >> k) & 1)) continue;
-        const int p = threadIdx.x + k * NT;
+    for (int e0 = 0; e0 < nlist; e0 += NT / 16) {
+        const int e = e0 + (threadIdx.x >> 4);
+        const bool act = e < nlist;
+        const int p = act ? elist[e] : 0;
         const int li = p / TW, lj = p - li * TW;
-        const int gi = y0 + li, gj = x0 + lj;
-        const long long o = (long long)gi * W + gj;
-        const double2 ps = J.pos[o];
+        const long long o = (long long)(y0 + li) * W + (x0 + lj);
+        const double2 ps = act ? J.pos[o] : make_double2(x0 + 4.0, y0 + 4.0);
         const int xf = (int)floor(ps.x), yf = (int)floor(ps.y);
-        const double fx = ps.x - xf, fy = ps.y - yf;
-        const double wx[4] = {keys(fx + 1), keys(fx), keys(fx - 1), keys(fx - 2)};
-        const double wy[4] = {keys(fy + 1), keys(fy), keys(fy - 1), keys(fy - 2)};
+        const double w = keys(ps.y - yf + 1 - sa) * keys(ps.x - xf + 1 - sb);
+        const int ii = min(max(yf - 1 + sa, 0), H - 1) - (y0 - HALO);
+        const int jj = min(max(xf - 1 + sb, 0), W - 1) - (x0 - HALO);
+        const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
         double v[10];
+        v[0] = (double)Y[0 * PR * PC + yo] * w;   // Px
+        v[1] = (double)Y[1 * PR * PC + yo] * w;   // Py
+        v[2] = (double)Y[2 * PR * PC + yo] * w;   // Pxx
+        v[3] = (double)Y[3 * PR * PC + yo] * w;   // Pyy
+        v[4] = (double)Y[4 * PR * PC + yo] * w;   // Pxy
+        v[5] = (double)In[IR * IC + io] * w;      // Hx
+        v[6] = (double)Y[5 * PR * PC + yo] * w;   // Hxx
+        v[7] = (double)Y[6 * PR * PC + yo] * w;   // Hxy
+        v[8] = (double)Y[7 * PR * PC + yo] * w;   // Hyy
+        v[9] = (double)In[2 * IR * IC + io] * w;  // Hy
 #pragma unroll
-        for (int m = 0; m < 10; ++m) v[m] = 0.0;
+        for (int m = 0; m < 10; ++m)
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int ii = min(max(yf - 1 + a, 0), H - 1) - (y0 - HALO);
-            double row[10];
-#pragma unroll
-            for (int m = 0; m < 10; ++m) row[m] = 0.0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int jj = min(max(xf - 1 + b, 0), W - 1) - (x0 - HALO);
-                const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
-                const double w = wx[b];
-                row[0] += (double)Y[0 * PR * PC + yo] * w;   // Px
-                row[1] += (double)Y[1 * PR * PC + yo] * w;   // Py
-                row[2] += (double)Y[2 * PR * PC + yo] * w;   // Pxx
-                row[3] += (double)Y[3 * PR * PC + yo] * w;   // Pyy
-                row[4] += (double)Y[4 * PR * PC + yo] * w;   // Pxy
-                row[5] += (double)In[IR * IC + io] * w;      // Hx
-                row[6] += (double)Y[5 * PR * PC + yo] * w;   // Hxx
-                row[7] += (double)Y[6 * PR * PC + yo] * w;   // Hxy
-                row[8] += (double)Y[7 * PR * PC + yo] * w;   // Hyy
-                row[9] += (double)In[2 * IR * IC + io] * w;  // Hy
+            for (int d = 8; d >= 1; d >>= 1) v[m] += __shfl_xor(v[m], d, 16);
+        if (act && sub == 0) {
+            double Fx = v[0] * v[6] + v[2] * v[5] + v[1] * v[7] + v[4] * v[9];
+            double Fy = v[0] * v[7] + v[4] * v[5] + v[1] * v[8] + v[3] * v[9];
+            const double Fm = sqrt(Fx * Fx + Fy * Fy);
+            if (!(Fm < 1e-12)) {
+                Fx /= Fm;
+                Fy /= Fm;
+                J.val[o].y = (float)atan2(Fx, -Fy);
             }
-#pragma unroll
-            for (int m = 0; m < 10; ++m) v[m] += row[m] * wy[a];
-        }
-        double Fx = v[0] * v[6] + v[2] * v[5] + v[1] * v[7] + v[4] * v[9];
-        double Fy = v[0] * v[7] + v[4] * v[5] + v[1] * v[8] + v[3] * v[9];
-        const double Fm = sqrt(Fx * Fx + Fy * Fy);
-        if (!(Fm < 1e-12)) {
-            Fx /= Fm;
-            Fy /= Fm;
-            J.val[o].y = (float)atan2(Fx, -Fy);
         }
     }
 }
@@ -780,7 +824,8 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC, PP = PR * PC;
     __shared__ float lds[BGeo<R>::FLOATS];
     __shared__ uint32_t km[TH];
-    __shared__ int s_any;
+    __shared__ int s_any, s_nlist;
+    __shared__ short elist[TH * TW];
     // In (E, Hx, Hy) | Y (8 planes) | Z (x pass, then the 8 plane adjoints)
     float *In = lds, *Y = lds + G::IN, *Z = lds + G::IN + 8 * PP;
 
@@ -791,7 +836,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
-    if (threadIdx.x == 0) s_any = 0;
+    if (threadIdx.x == 0) { s_any = 0; s_nlist = 0; }
     __syncthreads();
     if (threadIdx.x < TH) {
         const int row = y0 + threadIdx.x;
@@ -801,25 +846,36 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     }
     __syncthreads();
     if (!s_any) return;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        if ((km[p / TW] >> (p % TW)) & 1u) elist[atomicAdd(&s_nlist, 1)] = (short)p;
+    }
     load_tile_in<R>(J, In, y0, x0);
     __syncthreads();
     conv_planes<R>(In, Z, Y, tp);
     for (int idx = threadIdx.x; idx < 8 * PP; idx += NT) Z[idx] = 0.f;
     __syncthreads();
 
-    // per-edgel adjoint (FP64), scattered at once: the 8 convolved-plane
-    // adjoints into Z (LDS atomics), the raw Hx, Hy adjoints into HBM
+    // per-edgel adjoint, 16 lanes per edgel: lane (sa, sb) owns one pixel of the
+    // 4 x 4 cubic stencil; the stencil sums are reduced over the group (FP32
+    // planes), the parabola / position / orientation adjoint is evaluated in
+    // FP64 by every lane of the group, and each lane scatters its own pixel:
+    // the 8 convolved-plane adjoints into Z (LDS atomics), raw Hx, Hy into HBM.
     // v index -> adjoint plane: Px 0, Py 1, Pxx 2, Pyy 3, Pxy 4, Hxx 5, Hxy 6, Hyy 7
     constexpr int VPL[10] = {0, 1, 2, 3, 4, -1, 5, 6, 7, -2};
+    const int nlist = s_nlist;
+    const int sub = threadIdx.x & 15, sa = sub >> 2, sb = sub & 3;
 #pragma unroll 1
-    for (int k = 0; k < 2; ++k) {
-        const int p = threadIdx.x + k * NT;
+    for (int e0 = 0; e0 < nlist; e0 += NT / 16) {
+        const int e = e0 + (threadIdx.x >> 4);
+        const bool act = e < nlist;
+        const int p = act ? elist[e] : elist[0];
         const int li = p / TW, lj = p - li * TW;
-        if (!((km[li] >> lj) & 1u)) continue;
         const int gi = y0 + li, gj = x0 + lj;
         const long long o = (long long)gi * W + gj;
         const int rank = (int)J.wpre[(long long)gi * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
-        const float4 g = (rank < J.capacity) ? Gd.gout[rank] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 g = (act && rank < J.capacity) ? Gd.gout[rank] : make_float4(0.f, 0.f, 0.f, 0.f);
         const double2 d = J.dirs[o];
         auto Ev = [&](int di, int dj) -> double { return (double)In[(li + HALO + R + di) * IC + (lj + HALO + R + dj)]; };
         const double md = sqrt(d.x * d.x + d.y * d.y), nx = d.x / md, ny = d.y / md;
@@ -833,48 +889,34 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
         const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s);
         const double ss = -B / (2 * A), mf = A * ss * ss + B * ss + f;
         const double x = gj + ss * nx, y = gi + ss * ny;
-        // cubic weights and their derivatives (stencil base frozen)
+        // this lane's stencil pixel and its weights (stencil base frozen)
         const int xf = (int)floor(x), yf = (int)floor(y);
         const double fx = x - xf, fy = y - yf;
-        // the cubic sums in FP32 (the planes are FP32); the rest in FP64
-        float wx[4], wy[4], dwx[4], dwy[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            wx[b] = (float)keys(fx + 1 - b);
-            dwx[b] = (float)keys_d(fx + 1 - b);
-            wy[b] = (float)keys(fy + 1 - b);
-            dwy[b] = (float)keys_d(fy + 1 - b);
-        }
+        const float wxb = (float)keys(fx + 1 - sb), dwxb = (float)keys_d(fx + 1 - sb);
+        const float wya = (float)keys(fy + 1 - sa), dwya = (float)keys_d(fy + 1 - sa);
+        const int gy = min(max(yf - 1 + sa, 0), H - 1), gx = min(max(xf - 1 + sb, 0), W - 1);
+        const int ii = gy - (y0 - HALO), jj = gx - (x0 - HALO);
+        const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
+        float pv[10];
+        pv[0] = Y[0 * PP + yo]; pv[1] = Y[1 * PP + yo]; pv[2] = Y[2 * PP + yo]; pv[3] = Y[3 * PP + yo];
+        pv[4] = Y[4 * PP + yo]; pv[5] = In[IR * IC + io]; pv[6] = Y[5 * PP + yo]; pv[7] = Y[6 * PP + yo];
+        pv[8] = Y[7 * PP + yo]; pv[9] = In[2 * IR * IC + io];
         float v[10], vx[10], vy[10];
+        const float w = wya * wxb, wx_ = wya * dwxb, wy_ = dwya * wxb;
 #pragma unroll
-        for (int m = 0; m < 10; ++m) v[m] = vx[m] = vy[m] = 0.f;
-#pragma unroll 1
-        for (int a = 0; a < 4; ++a) {
-            const int ii = min(max(yf - 1 + a, 0), H - 1) - (y0 - HALO);
-            float row[10], rowd[10];
-#pragma unroll
-            for (int m = 0; m < 10; ++m) row[m] = rowd[m] = 0.f;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int jj = min(max(xf - 1 + b, 0), W - 1) - (x0 - HALO);
-                const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
-                float pv[10];
-                pv[0] = Y[0 * PP + yo]; pv[1] = Y[1 * PP + yo]; pv[2] = Y[2 * PP + yo]; pv[3] = Y[3 * PP + yo];
-                pv[4] = Y[4 * PP + yo]; pv[5] = In[IR * IC + io]; pv[6] = Y[5 * PP + yo]; pv[7] = Y[6 * PP + yo];
-                pv[8] = Y[7 * PP + yo]; pv[9] = In[2 * IR * IC + io];
-#pragma unroll
-                for (int m = 0; m < 10; ++m) {
-                    row[m] = fmaf(pv[m], wx[b], row[m]);
-                    rowd[m] = fmaf(pv[m], dwx[b], rowd[m]);
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < 10; ++m) {
-                v[m] = fmaf(row[m], wy[a], v[m]);
-                vx[m] = fmaf(rowd[m], wy[a], vx[m]);
-                vy[m] = fmaf(row[m], dwy[a], vy[m]);
-            }
+        for (int m = 0; m < 10; ++m) {
+            v[m] = pv[m] * w;
+            vx[m] = pv[m] * wx_;
+            vy[m] = pv[m] * wy_;
         }
+#pragma unroll
+        for (int m = 0; m < 10; ++m)
+#pragma unroll
+            for (int dd = 8; dd >= 1; dd >>= 1) {
+                v[m] += __shfl_xor(v[m], dd, 16);
+                vx[m] += __shfl_xor(vx[m], dd, 16);
+                vy[m] += __shfl_xor(vy[m], dd, 16);
+            }
         const double Fx = (double)v[0] * v[6] + (double)v[2] * v[5] + (double)v[1] * v[7] + (double)v[4] * v[9];
         const double Fy = (double)v[0] * v[7] + (double)v[4] * v[5] + (double)v[1] * v[8] + (double)v[3] * v[9];
         const double F2 = Fx * Fx + Fy * Fy;
@@ -910,39 +952,31 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
         const double mfb = (double)g.w * (mf < 0 ? -1.0 : 1.0);
         const double Ab = ssb * (-ss / A) + mfb * ss * ss;
         const double Bb = ssb * (-1.0 / (2 * A)) + mfb * ss;
-        double fb = mfb - Ab / (s * s);
+        const double fb = mfb - Ab / (s * s);
         const double fpb = Ab / (2 * s * s) + Bb / (2 * s), fmb = Ab / (2 * s * s) - Bb / (2 * s);
-        const double sb = Ab * (-2 * A / s) + Bb * (-B / s);
-        double slb = sb * slope / s + fpb * (Eb - Ea) + fmb * (Ed - Ec);
-        float *gm = Gd.gmap;
-        const int gp = Gd.gpitch;
-        atomicAdd(&gm[(long long)gi * gp + gj], (float)fb);
-        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][0]) * gp + gj + SP_OFF[oc][1]], (float)(fpb * (1 - slope)));
-        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][2]) * gp + gj + SP_OFF[oc][3]], (float)(fpb * slope));
-        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][4]) * gp + gj + SP_OFF[oc][5]], (float)(fmb * (1 - slope)));
-        atomicAdd(&gm[(long long)(gi + SP_OFF[oc][6]) * gp + gj + SP_OFF[oc][7]], (float)(fmb * slope));
+        const double sbar = Ab * (-2 * A / s) + Bb * (-B / s);
+        const double slb = sbar * slope / s + fpb * (Eb - Ea) + fmb * (Ed - Ec);
+        if (act && sub < 5) {
+            // lane 0: the edgel pixel; lanes 1..4: the two pixels of fp and of fm
+            const int c = sub - 1;
+            const int di = sub == 0 ? 0 : SP_OFF[oc][2 * c], dj = sub == 0 ? 0 : SP_OFF[oc][2 * c + 1];
+            const double val = sub == 0 ? fb : (sub == 1 ? fpb * (1 - slope) : (sub == 2 ? fpb * slope
+                                                : (sub == 3 ? fmb * (1 - slope) : fmb * slope)));
+            atomicAdd(&Gd.gmap[(long long)(gi + di) * Gd.gpitch + gj + dj], (float)val);
+        }
         if (fam == 0) { nxb += -slb * ny / (nx * nx); nyb += slb / nx; }
         else if (fam == 1) { nxb += slb / ny; nyb += -slb * nx / (ny * ny); }
         else if (fam == 2) { nxb += -slb / ny; nyb += slb * nx / (ny * ny); }
         else { nxb += slb * ny / (nx * nx); nyb += -slb / nx; }
         thb += -nxb * ny + nyb * nx;    // n = (cos theta, sin theta)
-        J.gth[o] = (float)thb;
-#pragma unroll 1
-        for (int a = 0; a < 4; ++a) {
-            const int gy = min(max(yf - 1 + a, 0), H - 1);
-            const int ii = gy - (y0 - HALO);
+        if (act && sub == 0) J.gth[o] = (float)thb;
+        if (act) {
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int gx = min(max(xf - 1 + b, 0), W - 1);
-                const int jj = gx - (x0 - HALO);
-                const float w = wy[a] * wx[b];
-#pragma unroll
-                for (int m = 0; m < 10; ++m) {
-                    const float c = vb[m] * w;
-                    if (c == 0.f) continue;
-                    if (VPL[m] >= 0) atomicAdd(&Z[VPL[m] * PP + ii * PC + jj], c);
-                    else atomicAdd(VPL[m] == -1 ? &J.ghx[(long long)gy * W + gx] : &J.ghy[(long long)gy * W + gx], c);
-                }
+            for (int m = 0; m < 10; ++m) {
+                const float c = vb[m] * w;
+                if (c == 0.f) continue;
+                if (VPL[m] >= 0) atomicAdd(&Z[VPL[m] * PP + yo], c);
+                else atomicAdd(VPL[m] == -1 ? &J.ghx[(long long)gy * W + gx] : &J.ghy[(long long)gy * W + gx], c);
             }
         }
     }
@@ -1367,6 +1401,7 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     if (p->timing) SPCHK(hipEventRecord(p->ev[0], s));
     hipLaunchKernelGGL(sp_orient_nms_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
     if (p->timing) SPCHK(hipEventRecord(p->ev[1], s));
+    hipLaunchKernelGGL(sp_isolate_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->taps);
     const int st = launch_correct_r(s, p);
     if (st) return st;
     if (p->timing) SPCHK(hipEventRecord(p->ev[2], s));
