@@ -586,6 +586,12 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     }
     __syncthreads();
     if (s_nlist == 0) return;
+    // the first round's positions are loaded now, overlapping the tile load
+    double2 ps_pre = make_double2(x0 + 4.0, y0 + 4.0);
+    if ((int)(threadIdx.x >> 4) < s_nlist) {
+        const int p = elist[threadIdx.x >> 4];
+        ps_pre = J.pos[(long long)(y0 + p / TW) * W + (x0 + p % TW)];
+    }
 
     load_tile_in<R>(J, In, y0, x0);
     __syncthreads();
@@ -603,7 +609,7 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
         const int p = act ? elist[e] : 0;
         const int li = p / TW, lj = p - li * TW;
         const long long o = (long long)(y0 + li) * W + (x0 + lj);
-        const double2 ps = act ? J.pos[o] : make_double2(x0 + 4.0, y0 + 4.0);
+        const double2 ps = e0 == 0 ? ps_pre : (act ? J.pos[o] : make_double2(x0 + 4.0, y0 + 4.0));
         const int xf = (int)floor(ps.x), yf = (int)floor(ps.y);
         const double w = keys(ps.y - yf + 1 - sa) * keys(ps.x - xf + 1 - sb);
         const int ii = min(max(yf - 1 + sa, 0), H - 1) - (y0 - HALO);
@@ -851,6 +857,20 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
         const int p = threadIdx.x + k * NT;
         if ((km[p / TW] >> (p % TW)) & 1u) elist[atomicAdd(&s_nlist, 1)] = (short)p;
     }
+    __syncthreads();
+    // the first round's per-edgel loads (rank -> upstream row, direction) are
+    // issued now, so they overlap the tile load and the convolutions
+    float4 g_pre = make_float4(0.f, 0.f, 0.f, 0.f);
+    double2 d_pre = make_double2(1.0, 0.0);
+    {
+        const int e = threadIdx.x >> 4;
+        if (e < s_nlist) {
+            const int p = elist[e], li = p / TW, lj = p - li * TW;
+            const int rank = (int)J.wpre[(long long)(y0 + li) * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
+            if (rank < J.capacity) g_pre = Gd.gout[rank];
+            d_pre = J.dirs[(long long)(y0 + li) * W + (x0 + lj)];
+        }
+    }
     load_tile_in<R>(J, In, y0, x0);
     __syncthreads();
     conv_planes<R>(In, Z, Y, tp);
@@ -874,9 +894,15 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
         const int li = p / TW, lj = p - li * TW;
         const int gi = y0 + li, gj = x0 + lj;
         const long long o = (long long)gi * W + gj;
-        const int rank = (int)J.wpre[(long long)gi * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
-        const float4 g = (act && rank < J.capacity) ? Gd.gout[rank] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const double2 d = J.dirs[o];
+        float4 g = g_pre;
+        double2 d = d_pre;
+        if (e0 > 0) {
+            const int rank = (int)J.wpre[(long long)gi * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
+            g = (act && rank < J.capacity) ? Gd.gout[rank] : make_float4(0.f, 0.f, 0.f, 0.f);
+            d = J.dirs[o];
+        } else if (!act) {
+            d = J.dirs[o];
+        }
         auto Ev = [&](int di, int dj) -> double { return (double)In[(li + HALO + R + di) * IC + (lj + HALO + R + dj)]; };
         const double md = sqrt(d.x * d.x + d.y * d.y), nx = d.x / md, ny = d.y / md;
         const int oc = octant_of(d.x, d.y);
