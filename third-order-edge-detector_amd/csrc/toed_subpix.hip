@@ -12,19 +12,20 @@
 //        Hy = E diry (FP32) and runs the octant NMS with the parabola in FP64
 //        (:413-519); survivors write their FP64 position and strength into
 //        dense per-pixel planes and set a bit in a per-row 32-bit mask word.
-//   B  sp_correct_kernel<R>   one workgroup per tile with survivors:
-//        isolated-edgel removal (:558-621) by looking up the survivor masks of
-//        the pixels within floor(r + 2 sqrt 2) (edgels lie within sqrt 2 of
-//        their pixel) -> kept mask words; then, for tiles with kept edgels,
-//        the eight circular separable Gaussian-derivative convolutions
+//   B1 sp_isolate_kernel      isolated-edgel removal (:558-621): a survivor is
+//        kept when another survivor lies within the radius; candidates are the
+//        survivors within floor(r + 2 sqrt 2) pixels (edgels lie within
+//        sqrt 2 of their pixel) -> kept mask words.
+//   C  sp_scan_chunks / sp_scan_maps / sp_prefix_kernel   exclusive scan of
+//        the kept-mask popcounts in raster order (chunks of 4096 words, then
+//        per map): the count, and the rank of each word's first kept edgel.
+//   B2 sp_correct_kernel<R>   one workgroup per tile with kept edgels: the
+//        eight circular separable Gaussian-derivative convolutions
 //        (:377-411, :641-648) of the tile plus a 3-pixel halo in LDS (FP32,
 //        x pass then y pass), Keys cubic sampling of the ten planes at every
-//        kept edgel (FP64, :521-556) and the third-order orientation
-//        (:663-700, FP64).
-//   C  sp_scan_chunks / sp_scan_maps   exclusive scan of the kept-mask
-//        popcounts in raster order (chunks of 4096 words, then per map).
-//   D  sp_scatter_kernel      edgel k of raster order -> row k of the N x 4
-//        list [x, y, orientation, strength] (:702-729).
+//        kept edgel (FP64, :521-556), the third-order orientation (:663-700,
+//        FP64); each edgel is written as row (rank) of the N x 4 list
+//        [x, y, orientation, strength] (:702-729).
 //   E  sp_maps_kernel         optional TO_edgemap / TO_orientation: each pixel
 //        gathers the kept edgels of its 3 x 3 neighbourhood whose rounded
 //        position is the pixel and takes the last in list order (:708-715).
@@ -74,10 +75,11 @@ struct SpJob {
     int H, W, pitch, opitch;
     int nw, ntx, nty, tile_begin;
     int chunk_begin, nchunks, tpitch, pad;
+    uint32_t *wpre;             // H x nw: rank of each kept word's first edgel
     // differentiable form (null unless enabled on the plan)
     float4 *tq;                 // H x W: dirx, diry, dOxx/dtheta, dOyy/dtheta coefficients
     double2 *dirs;              // H x W: direction of survivors
-    uint32_t *wpre;             // H x nw: rank of each kept word's first edgel
+    float4 *rec;                // 8 per edgel (by rank): the ten cubic sums, their x and y derivatives
     float *ghx, *ghy, *gth;     // H x W scratch: dL/dHx, dL/dHy, dL/dtheta at edgels
 };
 
@@ -301,6 +303,15 @@ __device__ __forceinline__ double keys(double s) {
     const double x = fabs(s);
     if (x <= 1.0) return ((a + 2) * x - (a + 3)) * x * x + 1;
     if (x < 2.0) return ((a * x - 5 * a) * x + 8 * a) * x - 4 * a;
+    return 0.0;
+}
+
+// derivative of the Keys kernel
+__device__ __forceinline__ double keys_d(double s) {
+    const double a = -0.5;
+    const double x = fabs(s), sg = s < 0 ? -1.0 : 1.0;
+    if (x <= 1.0) return sg * ((3 * (a + 2) * x - 2 * (a + 3)) * x);
+    if (x < 2.0) return sg * ((3 * a * x - 10 * a) * x + 8 * a);
     return 0.0;
 }
 
@@ -586,11 +597,17 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     }
     __syncthreads();
     if (s_nlist == 0) return;
-    // the first round's positions are loaded now, overlapping the tile load
+    // the first round's per-edgel loads (position, strength, rank) are issued
+    // now, overlapping the tile load
     double2 ps_pre = make_double2(x0 + 4.0, y0 + 4.0);
+    float2 vv_pre = make_float2(0.f, 0.f);
+    int rk_pre = 0;
     if ((int)(threadIdx.x >> 4) < s_nlist) {
-        const int p = elist[threadIdx.x >> 4];
-        ps_pre = J.pos[(long long)(y0 + p / TW) * W + (x0 + p % TW)];
+        const int p = elist[threadIdx.x >> 4], li = p / TW, lj = p - li * TW;
+        const long long o = (long long)(y0 + li) * W + (x0 + lj);
+        ps_pre = J.pos[o];
+        vv_pre = J.val[o];
+        rk_pre = (int)J.wpre[(long long)(y0 + li) * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
     }
 
     load_tile_in<R>(J, In, y0, x0);
@@ -598,7 +615,11 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     conv_planes<R>(In, X, Y, tp);
 
     // cubic sampling of the ten planes and the orientation: 16 lanes per edgel,
-    // lane (a, b) of the 4 x 4 stencil, FP64 sums over the group by shuffles
+    // lane (a, b) of the 4 x 4 stencil, FP64 sums over the group by shuffles;
+    // the edgel is written at its raster rank.  Differentiable form: the ten
+    // sums and their x and y derivatives (FP32) are kept per edgel for the
+    // adjoint, 32 floats at rec[8 rank].
+    float4 *out = reinterpret_cast<float4 *>(J.edges);
     const int nlist = s_nlist;
     const int sub = threadIdx.x & 15;
     const int sa = sub >> 2, sb = sub & 3;
@@ -609,23 +630,34 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
         const int p = act ? elist[e] : 0;
         const int li = p / TW, lj = p - li * TW;
         const long long o = (long long)(y0 + li) * W + (x0 + lj);
-        const double2 ps = e0 == 0 ? ps_pre : (act ? J.pos[o] : make_double2(x0 + 4.0, y0 + 4.0));
+        double2 ps = ps_pre;
+        float2 vv = vv_pre;
+        int rank = rk_pre;
+        if (e0 > 0) {
+            ps = act ? J.pos[o] : make_double2(x0 + 4.0, y0 + 4.0);
+            vv = act ? J.val[o] : make_float2(0.f, 0.f);
+            rank = act ? (int)J.wpre[(long long)(y0 + li) * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u)) : 0;
+        }
         const int xf = (int)floor(ps.x), yf = (int)floor(ps.y);
-        const double w = keys(ps.y - yf + 1 - sa) * keys(ps.x - xf + 1 - sb);
+        const double wya = keys(ps.y - yf + 1 - sa), wxb = keys(ps.x - xf + 1 - sb);
+        const double w = wya * wxb;
         const int ii = min(max(yf - 1 + sa, 0), H - 1) - (y0 - HALO);
         const int jj = min(max(xf - 1 + sb, 0), W - 1) - (x0 - HALO);
         const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
+        float pv[10];
+        pv[0] = Y[0 * PR * PC + yo];   // Px
+        pv[1] = Y[1 * PR * PC + yo];   // Py
+        pv[2] = Y[2 * PR * PC + yo];   // Pxx
+        pv[3] = Y[3 * PR * PC + yo];   // Pyy
+        pv[4] = Y[4 * PR * PC + yo];   // Pxy
+        pv[5] = In[IR * IC + io];      // Hx
+        pv[6] = Y[5 * PR * PC + yo];   // Hxx
+        pv[7] = Y[6 * PR * PC + yo];   // Hxy
+        pv[8] = Y[7 * PR * PC + yo];   // Hyy
+        pv[9] = In[2 * IR * IC + io];  // Hy
         double v[10];
-        v[0] = (double)Y[0 * PR * PC + yo] * w;   // Px
-        v[1] = (double)Y[1 * PR * PC + yo] * w;   // Py
-        v[2] = (double)Y[2 * PR * PC + yo] * w;   // Pxx
-        v[3] = (double)Y[3 * PR * PC + yo] * w;   // Pyy
-        v[4] = (double)Y[4 * PR * PC + yo] * w;   // Pxy
-        v[5] = (double)In[IR * IC + io] * w;      // Hx
-        v[6] = (double)Y[5 * PR * PC + yo] * w;   // Hxx
-        v[7] = (double)Y[6 * PR * PC + yo] * w;   // Hxy
-        v[8] = (double)Y[7 * PR * PC + yo] * w;   // Hyy
-        v[9] = (double)In[2 * IR * IC + io] * w;  // Hy
+#pragma unroll
+        for (int m = 0; m < 10; ++m) v[m] = (double)pv[m] * w;
 #pragma unroll
         for (int m = 0; m < 10; ++m)
 #pragma unroll
@@ -634,10 +666,37 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
             double Fx = v[0] * v[6] + v[2] * v[5] + v[1] * v[7] + v[4] * v[9];
             double Fy = v[0] * v[7] + v[4] * v[5] + v[1] * v[8] + v[3] * v[9];
             const double Fm = sqrt(Fx * Fx + Fy * Fy);
+            float ori = vv.y;               // atan2(dirx, -diry) from the orientation pass
             if (!(Fm < 1e-12)) {
                 Fx /= Fm;
                 Fy /= Fm;
-                J.val[o].y = (float)atan2(Fx, -Fy);
+                ori = (float)atan2(Fx, -Fy);
+                J.val[o].y = ori;
+            }
+            if (rank < J.capacity) {
+                out[rank] = make_float4((float)ps.x, (float)ps.y, ori, vv.x);
+                if (J.index) J.index[rank] = (int)o;
+            }
+        }
+        if (J.rec) {
+            const float wx = (float)(wya * keys_d(ps.x - xf + 1 - sb)), wy = (float)(keys_d(ps.y - yf + 1 - sa) * wxb);
+            float r[32];
+#pragma unroll
+            for (int m = 0; m < 10; ++m) {
+                r[m] = (float)v[m];
+                r[10 + m] = pv[m] * wx;
+                r[20 + m] = pv[m] * wy;
+            }
+            r[30] = r[31] = 0.f;
+#pragma unroll
+            for (int m = 10; m < 30; ++m)
+#pragma unroll
+                for (int d = 8; d >= 1; d >>= 1) r[m] += __shfl_xor(r[m], d, 16);
+            if (act && rank < J.capacity) {
+                float4 *dst = J.rec + (long long)rank * 8;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (sub == k) dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
             }
         }
     }
@@ -694,8 +753,9 @@ __global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chu
     }
 }
 
-// D: scatter in raster order
-__global__ __launch_bounds__(256) void sp_scatter_kernel(const SpJob *jobs, int njobs, const int *chunk_off) {
+// D: rank of the first kept edgel of every mask word (raster order), from
+// the chunk offsets: the correction kernel writes each edgel at its rank
+__global__ __launch_bounds__(256) void sp_prefix_kernel(const SpJob *jobs, int njobs, const int *chunk_off) {
     __shared__ int wsum[4];
     const SpJob J = load_job(jobs);
     if ((int)blockIdx.x >= J.nchunks) return;
@@ -722,26 +782,10 @@ __global__ __launch_bounds__(256) void sp_scatter_kernel(const SpJob *jobs, int 
     int rank = chunk_off[chunk];
     for (int w = 0; w < wave; ++w) rank += wsum[w];
     rank += inc - s;
-    float4 *out = reinterpret_cast<float4 *>(J.edges);
-#pragma unroll 1
+#pragma unroll
     for (int k = 0; k < PER; ++k) {
-        uint32_t bits = wv[k];
-        if (J.wpre && w0 + k < nwords) J.wpre[w0 + k] = (uint32_t)rank;
-        if (!bits) continue;
-        const long long w = w0 + k;
-        const int gi = (int)(w / J.nw), gj0 = (int)(w - (long long)gi * J.nw) * 32;
-        while (bits) {
-            const int b = __ffs(bits) - 1;
-            bits &= bits - 1;
-            if (rank < J.capacity) {
-                const long long o = (long long)gi * J.W + gj0 + b;
-                const double2 ps = J.pos[o];
-                const float2 vv = J.val[o];
-                out[rank] = make_float4((float)ps.x, (float)ps.y, vv.y, vv.x);
-                if (J.index) J.index[rank] = (int)o;
-            }
-            ++rank;
-        }
+        if (w0 + k < nwords) J.wpre[w0 + k] = (uint32_t)rank;
+        rank += __popc(wv[k]);
     }
 }
 
@@ -788,14 +832,6 @@ __global__ __launch_bounds__(NT) void sp_maps_kernel(const SpJob *jobs, int njob
 // frozen (edgel set, NMS octant, the half-turn of each pixel's direction,
 // sign(-Oxy), cubic stencil base); no reference counterpart.
 
-__device__ __forceinline__ double keys_d(double s) {
-    const double a = -0.5;
-    const double x = fabs(s), sg = s < 0 ? -1.0 : 1.0;
-    if (x <= 1.0) return sg * ((3 * (a + 2) * x - 2 * (a + 3)) * x);
-    if (x < 2.0) return sg * ((3 * a * x - 10 * a) * x + 8 * a);
-    return 0.0;
-}
-
 // NMS octant of a direction, in the reference's branch order (:452-498), the
 // offsets (di, dj) of the two pixels of fp and of fm, and the slope family:
 // 0 ny/nx, 1 nx/ny, 2 -nx/ny, 3 -ny/nx.
@@ -814,16 +850,29 @@ __constant__ signed char SP_OFF[8][8] = {
 template <int R> struct BGeo {
     using G = CGeo<R>;
     static constexpr int PP = G::PR * G::PC;
-    static constexpr int ZN = G::XN > 8 * PP ? G::XN : 8 * PP;   // x pass, then the plane adjoints
-    static constexpr int FLOATS = G::IN + 8 * PP + ZN;
+    static constexpr int XBN = 3 * G::IR * G::PC;    // transposed y pass, one group
+    static constexpr int SN = 14 * NT;               // per-edgel staging of one round
+    static constexpr int FLOATS = 8 * PP + (XBN > SN ? XBN : SN);
 };
 
-// SB1: per tile with kept edgels: recompute the planes, FP64 adjoint of the
-// NMS parabola, cubic sampling and third-order orientation per edgel; the
-// plane adjoints are scattered into LDS and pulled back through the
-// transposed circular convolutions; dL/dE, dL/dHx, dL/dHy of the tile + R +
-// HALO are added to HBM with float atomics (periodic wrap), dL/dtheta of the
-// NMS is stored at the edgel pixel.
+__device__ __forceinline__ float keys_f(float s) {
+    const float x = fabsf(s);
+    if (x <= 1.f) return (1.5f * x - 2.5f) * x * x + 1.f;
+    if (x < 2.f) return ((-0.5f * x + 2.5f) * x - 4.f) * x + 2.f;
+    return 0.f;
+}
+
+// SB1: per tile with kept edgels.  (a) one lane per edgel: from the edgel's
+// record (the ten cubic sums and their x, y derivatives, kept by the forward
+// pass), its direction and the map values of its NMS stencil, the FP64
+// adjoint of the third-order orientation, the position and the NMS parabola;
+// dL/dE of the parabola goes to HBM (float atomics), dL/dtheta of the NMS is
+// stored at the edgel pixel, and the ten sample adjoints with the cubic
+// stencil base are staged in LDS.  (b) 16 lanes per edgel, lane (a, b) one
+// pixel of the 4 x 4 stencil: the eight convolved-plane adjoints are
+// scattered into Z (LDS atomics), raw Hx, Hy into HBM.  Then the transposed
+// circular convolutions pull Z back to dL/dE, dL/dHx, dL/dHy over the
+// tile + HALO + R, added to HBM with float atomics (periodic wrap).
 template <int R>
 __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const SpGrad *grads, SpTaps tp) {
     using G = CGeo<R>;
@@ -832,8 +881,9 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     __shared__ uint32_t km[TH];
     __shared__ int s_any, s_nlist;
     __shared__ short elist[TH * TW];
-    // In (E, Hx, Hy) | Y (8 planes) | Z (x pass, then the 8 plane adjoints)
-    float *In = lds, *Y = lds + G::IN, *Z = lds + G::IN + 8 * PP;
+    // Z (the 8 plane adjoints) | Xb (transposed y pass), which holds the
+    // per-edgel staging St during the edgel phase
+    float *Z = lds, *Xb = lds + 8 * PP, *St = lds + 8 * PP;
 
     const SpJob J = load_job(jobs);
     const int lt = blockIdx.x;
@@ -857,162 +907,144 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
         const int p = threadIdx.x + k * NT;
         if ((km[p / TW] >> (p % TW)) & 1u) elist[atomicAdd(&s_nlist, 1)] = (short)p;
     }
-    __syncthreads();
-    // the first round's per-edgel loads (rank -> upstream row, direction) are
-    // issued now, so they overlap the tile load and the convolutions
-    float4 g_pre = make_float4(0.f, 0.f, 0.f, 0.f);
-    double2 d_pre = make_double2(1.0, 0.0);
-    {
-        const int e = threadIdx.x >> 4;
-        if (e < s_nlist) {
-            const int p = elist[e], li = p / TW, lj = p - li * TW;
-            const int rank = (int)J.wpre[(long long)(y0 + li) * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
-            if (rank < J.capacity) g_pre = Gd.gout[rank];
-            d_pre = J.dirs[(long long)(y0 + li) * W + (x0 + lj)];
-        }
-    }
-    load_tile_in<R>(J, In, y0, x0);
-    __syncthreads();
-    conv_planes<R>(In, Z, Y, tp);
     for (int idx = threadIdx.x; idx < 8 * PP; idx += NT) Z[idx] = 0.f;
     __syncthreads();
 
-    // per-edgel adjoint, 16 lanes per edgel: lane (sa, sb) owns one pixel of the
-    // 4 x 4 cubic stencil; the stencil sums are reduced over the group (FP32
-    // planes), the parabola / position / orientation adjoint is evaluated in
-    // FP64 by every lane of the group, and each lane scatters its own pixel:
-    // the 8 convolved-plane adjoints into Z (LDS atomics), raw Hx, Hy into HBM.
     // v index -> adjoint plane: Px 0, Py 1, Pxx 2, Pyy 3, Pxy 4, Hxx 5, Hxy 6, Hyy 7
     constexpr int VPL[10] = {0, 1, 2, 3, 4, -1, 5, 6, 7, -2};
     const int nlist = s_nlist;
     const int sub = threadIdx.x & 15, sa = sub >> 2, sb = sub & 3;
 #pragma unroll 1
-    for (int e0 = 0; e0 < nlist; e0 += NT / 16) {
-        const int e = e0 + (threadIdx.x >> 4);
-        const bool act = e < nlist;
-        const int p = act ? elist[e] : elist[0];
-        const int li = p / TW, lj = p - li * TW;
-        const int gi = y0 + li, gj = x0 + lj;
-        const long long o = (long long)gi * W + gj;
-        float4 g = g_pre;
-        double2 d = d_pre;
-        if (e0 > 0) {
-            const int rank = (int)J.wpre[(long long)gi * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
-            g = (act && rank < J.capacity) ? Gd.gout[rank] : make_float4(0.f, 0.f, 0.f, 0.f);
-            d = J.dirs[o];
-        } else if (!act) {
-            d = J.dirs[o];
-        }
-        auto Ev = [&](int di, int dj) -> double { return (double)In[(li + HALO + R + di) * IC + (lj + HALO + R + dj)]; };
-        const double md = sqrt(d.x * d.x + d.y * d.y), nx = d.x / md, ny = d.y / md;
-        const int oc = octant_of(d.x, d.y);
-        const int fam = oc & 3;
-        const double slope = fam == 0 ? ny / nx : (fam == 1 ? nx / ny : (fam == 2 ? -nx / ny : -ny / nx));
-        const double Ea = Ev(SP_OFF[oc][0], SP_OFF[oc][1]), Eb = Ev(SP_OFF[oc][2], SP_OFF[oc][3]);
-        const double Ec = Ev(SP_OFF[oc][4], SP_OFF[oc][5]), Ed = Ev(SP_OFF[oc][6], SP_OFF[oc][7]);
-        const double fp = Ea * (1 - slope) + Eb * slope, fm = Ec * (1 - slope) + Ed * slope, f = Ev(0, 0);
-        const double s = sqrt(1 + slope * slope);
-        const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s);
-        const double ss = -B / (2 * A), mf = A * ss * ss + B * ss + f;
-        const double x = gj + ss * nx, y = gi + ss * ny;
-        // this lane's stencil pixel and its weights (stencil base frozen)
-        const int xf = (int)floor(x), yf = (int)floor(y);
-        const double fx = x - xf, fy = y - yf;
-        const float wxb = (float)keys(fx + 1 - sb), dwxb = (float)keys_d(fx + 1 - sb);
-        const float wya = (float)keys(fy + 1 - sa), dwya = (float)keys_d(fy + 1 - sa);
-        const int gy = min(max(yf - 1 + sa, 0), H - 1), gx = min(max(xf - 1 + sb, 0), W - 1);
-        const int ii = gy - (y0 - HALO), jj = gx - (x0 - HALO);
-        const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
-        float pv[10];
-        pv[0] = Y[0 * PP + yo]; pv[1] = Y[1 * PP + yo]; pv[2] = Y[2 * PP + yo]; pv[3] = Y[3 * PP + yo];
-        pv[4] = Y[4 * PP + yo]; pv[5] = In[IR * IC + io]; pv[6] = Y[5 * PP + yo]; pv[7] = Y[6 * PP + yo];
-        pv[8] = Y[7 * PP + yo]; pv[9] = In[2 * IR * IC + io];
-        float v[10], vx[10], vy[10];
-        const float w = wya * wxb, wx_ = wya * dwxb, wy_ = dwya * wxb;
+    for (int e0 = 0; e0 < nlist; e0 += NT) {
+        // (a) one lane per edgel
+        {
+            const int e = e0 + threadIdx.x;
+            float vb[10];
 #pragma unroll
-        for (int m = 0; m < 10; ++m) {
-            v[m] = pv[m] * w;
-            vx[m] = pv[m] * wx_;
-            vy[m] = pv[m] * wy_;
-        }
+            for (int m = 0; m < 10; ++m) vb[m] = 0.f;
+            float fxs = 0.f, fys = 0.f;
+            int xfs = x0, yfs = y0;
+            if (e < nlist) {
+                const int p = elist[e], li = p / TW, lj = p - li * TW;
+                const int gi = y0 + li, gj = x0 + lj;
+                const long long o = (long long)gi * W + gj;
+                const int rank = (int)J.wpre[(long long)gi * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
+                double thb = 0.0;
+                if (rank < J.capacity) {
+                    const float4 g = Gd.gout[rank];
+                    const float4 *rp = J.rec + (long long)rank * 8;
+                    float r[32];
 #pragma unroll
-        for (int m = 0; m < 10; ++m)
+                    for (int k = 0; k < 8; ++k) {
+                        const float4 q = rp[k];
+                        r[4 * k] = q.x; r[4 * k + 1] = q.y; r[4 * k + 2] = q.z; r[4 * k + 3] = q.w;
+                    }
+                    const double2 d = J.dirs[o];
+                    const float *Em = J.map + (long long)gi * J.pitch + gj;
+                    auto Ev = [&](int di, int dj) -> double { return (double)Em[(long long)di * J.pitch + dj]; };
+                    const double md = sqrt(d.x * d.x + d.y * d.y), nx = d.x / md, ny = d.y / md;
+                    const int oc = octant_of(d.x, d.y);
+                    const int fam = oc & 3;
+                    const double slope = fam == 0 ? ny / nx : (fam == 1 ? nx / ny : (fam == 2 ? -nx / ny : -ny / nx));
+                    const double Ea = Ev(SP_OFF[oc][0], SP_OFF[oc][1]), Eb = Ev(SP_OFF[oc][2], SP_OFF[oc][3]);
+                    const double Ec = Ev(SP_OFF[oc][4], SP_OFF[oc][5]), Ed = Ev(SP_OFF[oc][6], SP_OFF[oc][7]);
+                    const double fp = Ea * (1 - slope) + Eb * slope, fm = Ec * (1 - slope) + Ed * slope, f = Ev(0, 0);
+                    const double s = sqrt(1 + slope * slope);
+                    const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s);
+                    const double ss = -B / (2 * A), mf = A * ss * ss + B * ss + f;
+                    const double x = gj + ss * nx, y = gi + ss * ny;
+                    const float *v = r, *vx = r + 10, *vy = r + 20;
+                    const double Fx = (double)v[0] * v[6] + (double)v[2] * v[5] + (double)v[1] * v[7] + (double)v[4] * v[9];
+                    const double Fy = (double)v[0] * v[7] + (double)v[4] * v[5] + (double)v[1] * v[8] + (double)v[3] * v[9];
+                    const double F2 = Fx * Fx + Fy * Fy;
+                    if (!(sqrt(F2) < 1e-12)) {
+                        // orientation = atan2(Fx, -Fy)
+                        const float Fxb = (float)((double)g.z * (-Fy) / F2), Fyb = (float)((double)g.z * Fx / F2);
+                        vb[0] = Fxb * v[6] + Fyb * v[7];
+                        vb[6] = Fxb * v[0];
+                        vb[2] = Fxb * v[5];
+                        vb[5] = Fxb * v[2] + Fyb * v[4];
+                        vb[1] = Fxb * v[7] + Fyb * v[8];
+                        vb[7] = Fxb * v[1] + Fyb * v[0];
+                        vb[4] = Fxb * v[9] + Fyb * v[5];
+                        vb[9] = Fxb * v[4] + Fyb * v[3];
+                        vb[8] = Fyb * v[1];
+                        vb[3] = Fyb * v[9];
+                    } else {
+                        thb += (double)g.z;   // orientation = atan2(dirx, -diry) = theta + const
+                    }
+                    double xb = g.x, yb = g.y;
 #pragma unroll
-            for (int dd = 8; dd >= 1; dd >>= 1) {
-                v[m] += __shfl_xor(v[m], dd, 16);
-                vx[m] += __shfl_xor(vx[m], dd, 16);
-                vy[m] += __shfl_xor(vy[m], dd, 16);
+                    for (int m = 0; m < 10; ++m) {
+                        xb += (double)vb[m] * vx[m];
+                        yb += (double)vb[m] * vy[m];
+                    }
+                    // parabola and position: x = j + s* nx, y = i + s* ny, strength |f - B^2 / 4A|
+                    const double ssb = xb * nx + yb * ny;
+                    double nxb = xb * ss, nyb = yb * ss;
+                    const double mfb = (double)g.w * (mf < 0 ? -1.0 : 1.0);
+                    const double Ab = ssb * (-ss / A) + mfb * ss * ss;
+                    const double Bb = ssb * (-1.0 / (2 * A)) + mfb * ss;
+                    const double fb = mfb - Ab / (s * s);
+                    const double fpb = Ab / (2 * s * s) + Bb / (2 * s), fmb = Ab / (2 * s * s) - Bb / (2 * s);
+                    const double sbar = Ab * (-2 * A / s) + Bb * (-B / s);
+                    const double slb = sbar * slope / s + fpb * (Eb - Ea) + fmb * (Ed - Ec);
+                    // the edgel pixel and the two pixels of fp and of fm
+                    float *gm = Gd.gmap + (long long)gi * Gd.gpitch + gj;
+                    atomicAdd(gm, (float)fb);
+                    atomicAdd(gm + (long long)SP_OFF[oc][0] * Gd.gpitch + SP_OFF[oc][1], (float)(fpb * (1 - slope)));
+                    atomicAdd(gm + (long long)SP_OFF[oc][2] * Gd.gpitch + SP_OFF[oc][3], (float)(fpb * slope));
+                    atomicAdd(gm + (long long)SP_OFF[oc][4] * Gd.gpitch + SP_OFF[oc][5], (float)(fmb * (1 - slope)));
+                    atomicAdd(gm + (long long)SP_OFF[oc][6] * Gd.gpitch + SP_OFF[oc][7], (float)(fmb * slope));
+                    if (fam == 0) { nxb += -slb * ny / (nx * nx); nyb += slb / nx; }
+                    else if (fam == 1) { nxb += slb / ny; nyb += -slb * nx / (ny * ny); }
+                    else if (fam == 2) { nxb += -slb / ny; nyb += slb * nx / (ny * ny); }
+                    else { nxb += slb * ny / (nx * nx); nyb += -slb / nx; }
+                    thb += -nxb * ny + nyb * nx;    // n = (cos theta, sin theta)
+                    // cubic stencil base (frozen) and fractional position
+                    xfs = (int)floor(x);
+                    yfs = (int)floor(y);
+                    fxs = (float)(x - xfs);
+                    fys = (float)(y - yfs);
+                }
+                J.gth[o] = (float)thb;
             }
-        const double Fx = (double)v[0] * v[6] + (double)v[2] * v[5] + (double)v[1] * v[7] + (double)v[4] * v[9];
-        const double Fy = (double)v[0] * v[7] + (double)v[4] * v[5] + (double)v[1] * v[8] + (double)v[3] * v[9];
-        const double F2 = Fx * Fx + Fy * Fy;
-        float vb[10];
 #pragma unroll
-        for (int m = 0; m < 10; ++m) vb[m] = 0.f;
-        double thb = 0.0;
-        if (!(sqrt(F2) < 1e-12)) {
-            // orientation = atan2(Fx, -Fy)
-            const float Fxb = (float)((double)g.z * (-Fy) / F2), Fyb = (float)((double)g.z * Fx / F2);
-            vb[0] = Fxb * v[6] + Fyb * v[7];
-            vb[6] = Fxb * v[0];
-            vb[2] = Fxb * v[5];
-            vb[5] = Fxb * v[2] + Fyb * v[4];
-            vb[1] = Fxb * v[7] + Fyb * v[8];
-            vb[7] = Fxb * v[1] + Fyb * v[0];
-            vb[4] = Fxb * v[9] + Fyb * v[5];
-            vb[9] = Fxb * v[4] + Fyb * v[3];
-            vb[8] = Fyb * v[1];
-            vb[3] = Fyb * v[9];
-        } else {
-            thb += (double)g.z;   // orientation = atan2(dirx, -diry) = theta + const
+            for (int m = 0; m < 10; ++m) St[m * NT + threadIdx.x] = vb[m];
+            St[10 * NT + threadIdx.x] = fxs;
+            St[11 * NT + threadIdx.x] = fys;
+            St[12 * NT + threadIdx.x] = __int_as_float(xfs);
+            St[13 * NT + threadIdx.x] = __int_as_float(yfs);
         }
-        double xb = g.x, yb = g.y;
-#pragma unroll
-        for (int m = 0; m < 10; ++m) {
-            xb += (double)vb[m] * vx[m];
-            yb += (double)vb[m] * vy[m];
-        }
-        // parabola and position: x = j + s* nx, y = i + s* ny, strength |f - B^2 / 4A|
-        const double ssb = xb * nx + yb * ny;
-        double nxb = xb * ss, nyb = yb * ss;
-        const double mfb = (double)g.w * (mf < 0 ? -1.0 : 1.0);
-        const double Ab = ssb * (-ss / A) + mfb * ss * ss;
-        const double Bb = ssb * (-1.0 / (2 * A)) + mfb * ss;
-        const double fb = mfb - Ab / (s * s);
-        const double fpb = Ab / (2 * s * s) + Bb / (2 * s), fmb = Ab / (2 * s * s) - Bb / (2 * s);
-        const double sbar = Ab * (-2 * A / s) + Bb * (-B / s);
-        const double slb = sbar * slope / s + fpb * (Eb - Ea) + fmb * (Ed - Ec);
-        if (act && sub < 5) {
-            // lane 0: the edgel pixel; lanes 1..4: the two pixels of fp and of fm
-            const int c = sub - 1;
-            const int di = sub == 0 ? 0 : SP_OFF[oc][2 * c], dj = sub == 0 ? 0 : SP_OFF[oc][2 * c + 1];
-            const double val = sub == 0 ? fb : (sub == 1 ? fpb * (1 - slope) : (sub == 2 ? fpb * slope
-                                                : (sub == 3 ? fmb * (1 - slope) : fmb * slope)));
-            atomicAdd(&Gd.gmap[(long long)(gi + di) * Gd.gpitch + gj + dj], (float)val);
-        }
-        if (fam == 0) { nxb += -slb * ny / (nx * nx); nyb += slb / nx; }
-        else if (fam == 1) { nxb += slb / ny; nyb += -slb * nx / (ny * ny); }
-        else if (fam == 2) { nxb += -slb / ny; nyb += slb * nx / (ny * ny); }
-        else { nxb += slb * ny / (nx * nx); nyb += -slb / nx; }
-        thb += -nxb * ny + nyb * nx;    // n = (cos theta, sin theta)
-        if (act && sub == 0) J.gth[o] = (float)thb;
-        if (act) {
+        __syncthreads();
+        // (b) 16 lanes per edgel: lane (sa, sb) scatters its stencil pixel
+        const int nr = min(NT, nlist - e0);
+#pragma unroll 1
+        for (int q = threadIdx.x >> 4; q < nr; q += NT / 16) {
+            const float fx = St[10 * NT + q], fy = St[11 * NT + q];
+            const int xf = __float_as_int(St[12 * NT + q]), yf = __float_as_int(St[13 * NT + q]);
+            const float w = keys_f(fy + 1 - sa) * keys_f(fx + 1 - sb);
+            const int gy = min(max(yf - 1 + sa, 0), H - 1), gx = min(max(xf - 1 + sb, 0), W - 1);
+            const int yo = (gy - (y0 - HALO)) * PC + (gx - (x0 - HALO));
 #pragma unroll
             for (int m = 0; m < 10; ++m) {
-                const float c = vb[m] * w;
+                const float c = St[m * NT + q] * w;
                 if (c == 0.f) continue;
                 if (VPL[m] >= 0) atomicAdd(&Z[VPL[m] * PP + yo], c);
                 else atomicAdd(VPL[m] == -1 ? &J.ghx[(long long)gy * W + gx] : &J.ghy[(long long)gy * W + gx], c);
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
-    // transposed y pass (Z -> X bar in Y's place) and x pass (X bar -> In bar in In's place), per
-    // group.  Output u of a 4-output block at o0 sums window[u + m] * k[m], m = 0..2R, with
-    // window[t] = source(o0 - 2R + t) (zero outside the source): the transposed filters.
-    float *Ib = In, *Xb = Y;
+
+    // transposed y pass (Z -> X bar), then x pass (X bar -> dL/dE, dL/dHx,
+    // dL/dHy added to HBM), per group.  Output u of a 4-output block at o0
+    // sums window[u + m] * k[m], m = 0..2R, with window[t] = source(o0 - 2R + t)
+    // (zero outside the source): the transposed filters.
     constexpr int NB = 4, NWIN = 2 * R + NB;
     constexpr int YB = (IR + NB - 1) / NB, XB = (IC + NB - 1) / NB;
+    const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
+    const bool inner = iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W;
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp) {
         for (int task = threadIdx.x; task < YB * PC; task += NT) {
@@ -1081,29 +1113,21 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
                     }
                 }
             }
+            const int gy = inner ? iy0 + r : wrap_circ(iy0 + r, H);
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
-                const int gm = c0 + u;
-                if (gm >= IC) continue;
+                const int cm = c0 + u;
+                if (cm >= IC) continue;
+                const int gx = inner ? ix0 + cm : wrap_circ(ix0 + cm, W);
                 if (grp == 0) {
-                    Ib[r * IC + gm] = e0[u];
+                    if (e0[u] != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e0[u]);
                 } else {
-                    Ib[IR * IC + r * IC + gm] = e0[u];
-                    Ib[2 * IR * IC + r * IC + gm] = e1[u];
+                    if (e0[u] != 0.f) atomicAdd(&J.ghx[(long long)gy * W + gx], e0[u]);
+                    if (e1[u] != 0.f) atomicAdd(&J.ghy[(long long)gy * W + gx], e1[u]);
                 }
             }
         }
-        __syncthreads();
-    }
-    const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
-    const bool inner = iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W;
-    for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
-        const int r = idx / IC, c = idx - r * IC;
-        const int gy = inner ? iy0 + r : wrap_circ(iy0 + r, H), gx = inner ? ix0 + c : wrap_circ(ix0 + c, W);
-        const float e = Ib[idx], hx = Ib[IR * IC + idx], hy = Ib[2 * IR * IC + idx];
-        if (e != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e);
-        if (hx != 0.f) atomicAdd(&J.ghx[(long long)gy * W + gx], hx);
-        if (hy != 0.f) atomicAdd(&J.ghy[(long long)gy * W + gx], hy);
+        if (grp == 0) __syncthreads();
     }
 }
 
@@ -1250,10 +1274,11 @@ struct toed_subpix {
     long long ntiles = 0, nchunks = 0;
     int max_tiles = 0, max_chunks = 0;
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // differentiable form
     bool grad = false, grad_ran = false;
     char *d_gwork = nullptr;
+    std::vector<size_t> goff;
     SpGrad *d_grads = nullptr;
 };
 
@@ -1351,7 +1376,7 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
     for (int m = 0; m < n; ++m) {
         const size_t px = (size_t)maps[m].H * maps[m].W, nw = (size_t)(maps[m].W + 31) / 32;
         off[m] = total;
-        total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(maps[m].H * nw * 4) * 2;
+        total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(maps[m].H * nw * 4) * 3;
     }
     if (hipSetDevice(p->device) != hipSuccess) { delete p; return TOED_ERR_HIP; }
     if (hipMalloc(&p->d_work, total) != hipSuccess) { delete p; return TOED_ERR_ALLOC; }
@@ -1370,7 +1395,8 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
         J.val = (float2 *)b; b += align_up(px * 8);
         J.nw = (M.W + 31) / 32;
         J.vmask = (uint32_t *)b; b += align_up((size_t)M.H * J.nw * 4);
-        J.kmask = (uint32_t *)b;
+        J.kmask = (uint32_t *)b; b += align_up((size_t)M.H * J.nw * 4);
+        J.wpre = (uint32_t *)b;
         J.nsurv = p->d_nsurv + 64 * m;
         J.map = M.d_map;
         J.orient = M.d_orient;
@@ -1428,16 +1454,18 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     hipLaunchKernelGGL(sp_orient_nms_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
     if (p->timing) SPCHK(hipEventRecord(p->ev[1], s));
     hipLaunchKernelGGL(sp_isolate_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->taps);
-    const int st = launch_correct_r(s, p);
-    if (st) return st;
     if (p->timing) SPCHK(hipEventRecord(p->ev[2], s));
     hipLaunchKernelGGL(sp_scan_chunks, dim3(p->max_chunks, p->n), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
     hipLaunchKernelGGL(sp_scan_maps, dim3(p->n), dim3(1024), 0, s, p->d_jobs, p->d_chunk);
-    hipLaunchKernelGGL(sp_scatter_kernel, dim3(p->max_chunks, p->n), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
+    hipLaunchKernelGGL(sp_prefix_kernel, dim3(p->max_chunks, p->n), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
+    if (p->timing) SPCHK(hipEventRecord(p->ev[3], s));
+    const int st = launch_correct_r(s, p);
+    if (st) return st;
+    if (p->timing) SPCHK(hipEventRecord(p->ev[4], s));
     bool maps = false;
     for (const auto &J : p->jobs) maps = maps || J.to_e || J.to_o;
     if (maps) hipLaunchKernelGGL(sp_maps_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
-    if (p->timing) SPCHK(hipEventRecord(p->ev[3], s));
+    if (p->timing) SPCHK(hipEventRecord(p->ev[5], s));
     SPCHK(hipGetLastError());
     p->grad_ran = p->grad;
     return TOED_OK;
@@ -1446,38 +1474,35 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
 int toed_subpix_set_grad(toed_subpix *p, int on) {
     if (!p) return TOED_ERR_ARG;
     SPCHK(hipSetDevice(p->device));
+    // per map: tq, dirs (H x W x 16 B), ghx, ghy, gth (H x W x 4 B) and the
+    // edgel records (min(capacity, H W) x 128 B: an edgel per pixel at most)
+    auto rec_rows = [](const SpJob &J) { return (size_t)std::min<long long>(J.capacity, (long long)J.H * J.W); };
     if (on && !p->d_gwork) {
         size_t total = 0;
-        std::vector<size_t> off(p->n);
+        p->goff.assign(p->n, 0);
         for (int m = 0; m < p->n; ++m) {
             const SpJob &J = p->jobs[m];
             if (J.H < 5 || J.W < 5) return TOED_ERR_ARG;   // single reflection of the smoothing
             const size_t px = (size_t)J.H * J.W;
-            off[m] = total;
-            total += align_up(px * 16) * 2 + align_up((size_t)J.H * J.nw * 4) + align_up(px * 4) * 3;
+            p->goff[m] = total;
+            total += align_up(px * 16) * 2 + align_up(px * 4) * 3 + align_up(rec_rows(J) * 128);
         }
         if (hipMalloc(&p->d_gwork, total) != hipSuccess) return TOED_ERR_ALLOC;
         if (hipMalloc(&p->d_grads, sizeof(SpGrad) * p->n) != hipSuccess) return TOED_ERR_ALLOC;
         SPCHK(hipMemset(p->d_gwork, 0, total));
-        for (int m = 0; m < p->n; ++m) {
-            SpJob &J = p->jobs[m];
-            const size_t px = (size_t)J.H * J.W;
-            char *b = p->d_gwork + off[m];
-            J.tq = (float4 *)b; b += align_up(px * 16);
-            J.dirs = (double2 *)b; b += align_up(px * 16);
-            J.wpre = (uint32_t *)b; b += align_up((size_t)J.H * J.nw * 4);
-            J.ghx = (float *)b; b += align_up(px * 4);
-            J.ghy = (float *)b; b += align_up(px * 4);
-            J.gth = (float *)b;
-        }
     }
-    for (int m = 0; m < p->n && p->d_gwork; ++m) {
+    for (int m = 0; m < p->n; ++m) {
         SpJob &J = p->jobs[m];
-        if (!on) { J.tq = nullptr; J.dirs = nullptr; J.wpre = nullptr; }
-        else if (!J.tq) {
-            // re-enable: the planes stay allocated at the same places
-            return toed_subpix_set_grad(p, 0), TOED_ERR_ARG;
-        }
+        J.tq = nullptr; J.dirs = nullptr; J.rec = nullptr; J.ghx = J.ghy = J.gth = nullptr;
+        if (!on) continue;
+        const size_t px = (size_t)J.H * J.W;
+        char *b = p->d_gwork + p->goff[m];
+        J.tq = (float4 *)b; b += align_up(px * 16);
+        J.dirs = (double2 *)b; b += align_up(px * 16);
+        J.ghx = (float *)b; b += align_up(px * 4);
+        J.ghy = (float *)b; b += align_up(px * 4);
+        J.gth = (float *)b; b += align_up(px * 4);
+        J.rec = (float4 *)b;
     }
     p->grad = on != 0;
     p->grad_ran = false;
@@ -1519,11 +1544,11 @@ int toed_subpix_set_timing(toed_subpix *p, int on) {
 
 int toed_subpix_last_times(toed_subpix *p, float *ms_orient_nms, float *ms_correct, float *ms_list) {
     if (!p || !p->ev[0]) return TOED_ERR_ARG;
-    SPCHK(hipEventSynchronize(p->ev[3]));
-    float a = 0.f, b = 0.f, c = 0.f;
-    SPCHK(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
-    SPCHK(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
-    SPCHK(hipEventElapsedTime(&c, p->ev[2], p->ev[3]));
+    SPCHK(hipEventSynchronize(p->ev[5]));
+    // orientation + NMS; isolation + correction; scans, ranks and maps
+    float t[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < 5; ++k) SPCHK(hipEventElapsedTime(&t[k], p->ev[k], p->ev[k + 1]));
+    const float a = t[0], b = t[1] + t[3], c = t[2] + t[4];
     if (ms_orient_nms) *ms_orient_nms = a;
     if (ms_correct) *ms_correct = b;
     if (ms_list) *ms_list = c;
