@@ -27,19 +27,30 @@ __host__ __device__ constexpr int XF(int k) { return k == 0 || k == 3 || k == 6 
 __host__ __device__ constexpr int YF(int k) { return k == 1 || k == 3 || k == 5 ? 1 : (k == 4 || k == 6 ? 2 : (k == 8 ? 3 : 0)); }
 
 // One edgel at interp pixel (R, C) of an H x W image, upstream row `up` =
-// dL/d[x, y, orientation, strength], group lane l.  st / sd: FP64 taps and
-// their sigma-derivatives, variants 0 unshifted, 1 unshifted with zero end
-// taps (window R0 of phase (0,0)), 2 shifted; stf the FP32 copy of st used by
-// the scatter (its contributions are rounded to FP32 for the atomics anyway).
-// img(i, j) returns the pixel
+// dL/d[x, y, orientation, strength], group lane l.  Tap tables, variants 0
+// unshifted, 1 unshifted with zero end taps (window R0 of phase (0,0)),
+// 2 shifted, each with one zero tap on either side so that index t + R1 + 1
+// is tap t = -R1-1 .. R1+1 (tap_tables_fill builds them): stp / sdp the FP64
+// taps and their sigma-derivatives, the four families (G to G''') of one tap
+// together; stf4 the FP32 copy of stp used by the scatter (its contributions
+// are rounded to FP32 for the atomics anyway).  img(i, j) returns the pixel
 // (0 outside the image); sink(i, j, v) adds v to dL/dimage.  Lane 0 writes
 // the edgel's dL/dsigma to *gs_out.
+//
+// The edgel and its four neighbours (+-1 interp px = 1/2 image px) have their
+// windows, and so their transposed filters, inside the P x P patch
+// (P = 2 R1 + 2) whose first row is (R >> 1) - U + (R & 1) and first column
+// (C >> 1) - U + (C & 1): over the patch every tap index is within
+// -R1-1 .. R1+1.  Lane l owns patch rows l + G rr.
 template <int R1, class Img, class Sink>
 __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, int H, int W,
-                                               const double (*st)[4][2 * R1 + 1], const double (*sd)[4][2 * R1 + 1],
-                                               const float (*stf)[4][2 * R1 + 1], const Img &img, const Sink &sink,
+                                               const double (*stp)[2 * R1 + 3][4], const double (*sdp)[2 * R1 + 3][4],
+                                               const float4 (*stf4)[2 * R1 + 3], const Img &img, const Sink &sink,
                                                double *gs_out) {
-    constexpr int U = R1 + 1;          // patch half-width around the edgel (neighbours are +-1 interp px)
+    constexpr int U = R1 + 1;
+    constexpr int P = 2 * U;
+    constexpr int NSR = (P + G - 1) / G;       // patch rows per lane
+    const int pr0 = (R >> 1) - U + (R & 1), pc0 = (C >> 1) - U + (C & 1);
 
     // Each lane owns window rows p = -R1 + l + G*rr (NRR of them); the column
     // loop is outermost so every tap read from LDS serves all of the lane's rows.
@@ -55,11 +66,11 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
         for (int rr = 0; rr < NRR; ++rr)
 #pragma unroll
             for (int f = 0; f < 4; ++f) { h[rr][f] = 0.0; hs[rr][f] = 0.0; }
-#pragma unroll 1
+#pragma unroll 3
         for (int q = -R1; q <= R1; ++q) {
             double xt[4], xs[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f) { xt[f] = st[vx][f][q + R1]; xs[f] = sd[vx][f][q + R1]; }
+            for (int f = 0; f < 4; ++f) { xt[f] = stp[vx][q + R1 + 1][f]; xs[f] = sdp[vx][q + R1 + 1][f]; }
 #pragma unroll
             for (int rr = 0; rr < NRR; ++rr) {
                 const int p = -R1 + l + G * rr;
@@ -80,7 +91,7 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
             if (p > R1) continue;
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                const double y = st[vy][YF(k)][p + R1], ys = sd[vy][YF(k)][p + R1];
+                const double y = stp[vy][p + R1 + 1][YF(k)], ys = sdp[vy][p + R1 + 1][YF(k)];
                 v9[k] = fma(h[rr][XF(k)], y, v9[k]);
                 s9[k] = fma(hs[rr][XF(k)], y, fma(h[rr][XF(k)], ys, s9[k]));
             }
@@ -88,100 +99,16 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
 #pragma unroll
         for (int k = 0; k < 9; ++k) { v9[k] = redG(v9[k]); s9[k] = redG(s9[k]); }
     }
-    // octant of the recomputed gradient (cpu_toed.cpp:415-462): x-major iff
-    // |gx| >= |gy|; neighbours one step along +-gradient, axis and diagonal
-    const double gx = v9[0], gy = v9[1];
-    const int sx = gx >= 0.0 ? 1 : -1, sy = gy >= 0.0 ? 1 : -1;
-    const bool xm = fabs(gx) >= fabs(gy);
-    const int nbr[4][2] = {{xm ? 0 : sy, xm ? sx : 0}, {sy, sx}, {xm ? 0 : -sy, xm ? -sx : 0}, {-sy, -sx}};
-
-    // ---- 1b. Ix, Iy at the neighbours (+ sigma-derivatives)
-    double nv[4][2], ns[4][2];
+    // ---- 2a. reverse pass of the third-order orientation theta = atan2(TOx, -TOy)
+    // (cpu_toed.cpp:227-232), which depends on the nine responses only:
+    // its adjoints of the responses (ao, FP32 beyond fx and fy, which the
+    // parabola also feeds) and their share of dL/dsigma
+    double ao0 = 0.0, ao1 = 0.0, gs = 0.0;
+    float ao[9];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const int Rn = R + nbr[m][0], Cn = C + nbr[m][1];
-        const int i = Rn >> 1, j = Cn >> 1, a = Rn & 1, b = Cn & 1, ph = 2 * a + b;
-        const int vx = b ? 2 : (ph == 0 ? 1 : 0), vy = a ? 2 : (ph == 0 ? 1 : 0);
-        double h0[NRR], h1[NRR], hs0[NRR], hs1[NRR];
-#pragma unroll
-        for (int rr = 0; rr < NRR; ++rr) { h0[rr] = 0; h1[rr] = 0; hs0[rr] = 0; hs1[rr] = 0; }
-#pragma unroll 1
-        for (int q = -R1; q <= R1; ++q) {
-            const double x0 = st[vx][0][q + R1], x1 = st[vx][1][q + R1];
-            const double xs0 = sd[vx][0][q + R1], xs1 = sd[vx][1][q + R1];
-#pragma unroll
-            for (int rr = 0; rr < NRR; ++rr) {
-                const int p = -R1 + l + G * rr;
-                if (p > R1) continue;
-                const double px = img(i - p, j - q);
-                h0[rr] = fma(px, x0, h0[rr]);
-                h1[rr] = fma(px, x1, h1[rr]);
-                hs0[rr] = fma(px, xs0, hs0[rr]);
-                hs1[rr] = fma(px, xs1, hs1[rr]);
-            }
-        }
-        double ix = 0, iy = 0, ixs = 0, iys = 0;
-#pragma unroll
-        for (int rr = 0; rr < NRR; ++rr) {
-            const int p = -R1 + l + G * rr;
-            if (p > R1) continue;
-            const double y0 = st[vy][0][p + R1], y1 = st[vy][1][p + R1];
-            const double ys0 = sd[vy][0][p + R1], ys1 = sd[vy][1][p + R1];
-            ix = fma(h1[rr], y0, ix);
-            iy = fma(h0[rr], y1, iy);
-            ixs = fma(hs1[rr], y0, fma(h1[rr], ys0, ixs));
-            iys = fma(hs0[rr], y1, fma(h0[rr], ys1, iys));
-        }
-        nv[m][0] = redG(ix); nv[m][1] = redG(iy);
-        ns[m][0] = redG(ixs); ns[m][1] = redG(iys);
-    }
-
-    // ---- 2. reverse pass (every lane holds the totals; all compute it)
-    const double ux = up.x, uy = up.y, uth = up.z, us = up.w;
-    const double f = sqrt(gx * gx + gy * gy);
-    const double nx = gx / f, ny = gy / f;
-    const double s = xm ? (sy * gy) / (sx * gx) : (sx * gx) / (sy * gy);
-    double M[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) M[m] = sqrt(nv[m][0] * nv[m][0] + nv[m][1] * nv[m][1]);
-    const double fp = M[0] + s * (M[1] - M[0]), fm = M[2] + s * (M[3] - M[2]);
-    const double q = 1.0 + s * s, sq = sqrt(q);
-    const double A = (fm + fp - 2.0 * f) / (2.0 * q);
-    const double B = (fp - fm) / (2.0 * sq);
-    const double t = -B / (2.0 * A);
-    const double maxf = f - B * B / (4.0 * A);
-
-    const double tbar = 0.5 * (ux * nx + uy * ny);
-    double nxbar = 0.5 * ux * t, nybar = 0.5 * uy * t;
-    const double mbar = us * (maxf >= 0.0 ? 1.0 : -1.0);
-    const double Abar = mbar * B * B / (4.0 * A * A) + tbar * B / (2.0 * A * A);
-    const double Bbar = -mbar * B / (2.0 * A) - tbar / (2.0 * A);
-    double fbar = mbar - Abar / q;
-    const double fpbar = Abar / (2.0 * q) + Bbar / (2.0 * sq);
-    const double fmbar = Abar / (2.0 * q) - Bbar / (2.0 * sq);
-    const double sqbar = -Bbar * B / sq;
-    const double qbar = -Abar * A / q + sqbar / (2.0 * sq);
-    double sbar = 2.0 * s * qbar + fpbar * (M[1] - M[0]) + fmbar * (M[3] - M[2]);
-    const double Mbar[4] = {fpbar * (1.0 - s), fpbar * s, fmbar * (1.0 - s), fmbar * s};
-    double gxbar = 0.0, gybar = 0.0;
-    if (xm) { gybar += sbar * sy / (sx * gx); gxbar -= sbar * s / gx; }
-    else    { gxbar += sbar * sx / (sy * gy); gybar -= sbar * s / gy; }
-    gxbar += nxbar / f;
-    gybar += nybar / f;
-    fbar -= (nxbar * gx + nybar * gy) / (f * f);
-    gxbar += fbar * gx / f;
-    gybar += fbar * gy / f;
-    double adj[17];
-    adj[0] = gxbar; adj[1] = gybar;
-#pragma unroll
-    for (int k = 2; k < 9; ++k) adj[k] = 0.0;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const double w = M[m] > 0.0 ? Mbar[m] / M[m] : 0.0;
-        adj[9 + 2 * m] = w * nv[m][0];
-        adj[10 + 2 * m] = w * nv[m][1];
-    }
-    {   // third-order orientation theta = atan2(TOx, -TOy) (cpu_toed.cpp:227-232)
+    for (int k = 0; k < 9; ++k) ao[k] = 0.f;
+    {
+        const double uth = up.z;
         const double fx = v9[0], fy = v9[1], fxx = v9[2], fxy = v9[3], fyy = v9[4];
         const double fxxy = v9[5], fxyy = v9[6], fxxx = v9[7], fyyy = v9[8];
         const double tox = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
@@ -191,87 +118,236 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
         const double den = tox * tox + toy * toy;
         if (den > 0.0 && uth != 0.0) {
             // d atan2(u, v) = (v du - u dv) / (u^2 + v^2) with u = TOx, v = -TOy
-            const double txb = uth * (-toy) / den, tyb = uth * tox / den;
-            adj[0] += txb * (2 * fxx * fxx + 2 * fxy * fxy + 2 * fy * fxxy + 2 * fx * fxxx)
-                    + tyb * (2 * fxx * fxy + 2 * fyy * fxy + 2 * fy * fxyy + 2 * fx * fxxy);
-            adj[1] += txb * (2 * fxx * fxy + 2 * fyy * fxy + 2 * fx * fxxy + 2 * fy * fxyy)
-                    + tyb * (2 * fyy * fyy + 2 * fxy * fxy + 2 * fx * fxyy + 2 * fy * fyyy);
-            adj[2] = txb * (4 * fx * fxx + 2 * fy * fxy) + tyb * (2 * fx * fxy);
-            adj[3] = txb * (4 * fx * fxy + 2 * fy * fxx + 2 * fy * fyy) + tyb * (2 * fx * fxx + 2 * fx * fyy + 4 * fy * fxy);
-            adj[4] = txb * (2 * fy * fxy) + tyb * (2 * fx * fxy + 4 * fy * fyy);
-            adj[5] = txb * (2 * fx * fy) + tyb * (fx * fx);
-            adj[6] = txb * (fy * fy) + tyb * (2 * fx * fy);
-            adj[7] = txb * (fx * fx);
-            adj[8] = tyb * (fy * fy);
+            const double rden = uth / den;
+            const double txb = -toy * rden, tyb = tox * rden;
+            double a[9];
+            a[0] = txb * (2 * fxx * fxx + 2 * fxy * fxy + 2 * fy * fxxy + 2 * fx * fxxx)
+                 + tyb * (2 * fxx * fxy + 2 * fyy * fxy + 2 * fy * fxyy + 2 * fx * fxxy);
+            a[1] = txb * (2 * fxx * fxy + 2 * fyy * fxy + 2 * fx * fxxy + 2 * fy * fxyy)
+                 + tyb * (2 * fyy * fyy + 2 * fxy * fxy + 2 * fx * fxyy + 2 * fy * fyyy);
+            a[2] = txb * (4 * fx * fxx + 2 * fy * fxy) + tyb * (2 * fx * fxy);
+            a[3] = txb * (4 * fx * fxy + 2 * fy * fxx + 2 * fy * fyy) + tyb * (2 * fx * fxx + 2 * fx * fyy + 4 * fy * fxy);
+            a[4] = txb * (2 * fy * fxy) + tyb * (2 * fx * fxy + 4 * fy * fyy);
+            a[5] = txb * (2 * fx * fy) + tyb * (fx * fx);
+            a[6] = txb * (fy * fy) + tyb * (2 * fx * fy);
+            a[7] = txb * (fx * fx);
+            a[8] = tyb * (fy * fy);
+            ao0 = a[0];
+            ao1 = a[1];
+#pragma unroll
+            for (int k = 2; k < 9; ++k) {
+                gs = fma(a[k], s9[k], gs);
+                ao[k] = (float)a[k];
+            }
         }
     }
-    if (!(isfinite(A) && A != 0.0 && f > 0.0)) return;
 
-    // ---- 3. dL/dsigma (summed per workgroup by the caller)
-    if (l == 0) {
-        double gs = 0.0;
-#pragma unroll
-        for (int k = 0; k < 9; ++k) gs = fma(adj[k], s9[k], gs);
-#pragma unroll
-        for (int m = 0; m < 4; ++m) gs = fma(adj[9 + 2 * m], ns[m][0], fma(adj[10 + 2 * m], ns[m][1], gs));
-        *gs_out = gs;
-    }
+    // octant of the recomputed gradient (cpu_toed.cpp:415-462): x-major iff
+    // |gx| >= |gy|; neighbours one step along +-gradient, axis and diagonal
+    const double gx = v9[0], gy = v9[1];
+    const int sx = gx >= 0.0 ? 1 : -1, sy = gy >= 0.0 ? 1 : -1;
+    const bool xm = fabs(gx) >= fabs(gy);
+    const int nbr[4][2] = {{xm ? 0 : sy, xm ? sx : 0}, {sy, sx}, {xm ? 0 : -sy, xm ? -sx : 0}, {-sy, -sx}};
 
-    // ---- 4. scatter the transposed filters into dL/dimage: lane rows of the
-    // (2U+1)^2 patch around the edgel; per point and patch column the four x
-    // taps are read once and applied to all of the lane's rows
-    constexpr int NSR = (2 * U + G) / G;
-    const int ic = R >> 1, jc = C >> 1;
-    float adjf[17];
+    // ---- 1b. Ix, Iy at the neighbours (+ sigma-derivatives), one pass over
+    // the patch columns: a lane row's image value serves all four neighbours
+    double nv[4][2], ns[4][2];
+    {
+        int qb[4], vxn[4];
 #pragma unroll
-    for (int k = 0; k < 17; ++k) adjf[k] = (float)adj[k];
-    float row[NSR][2 * U + 1];
+        for (int m = 0; m < 4; ++m) {
+            const int Cn = C + nbr[m][1], jn = Cn >> 1;
+            vxn[m] = (Cn & 1) ? 2 : (((R + nbr[m][0]) & 1) == 0 ? 1 : 0);
+            qb[m] = jn - pc0 + R1 + 1;           // - c: tap column index q = jn - image col
+        }
+        double h0[4][NSR], h1[4][NSR], hs0[4][NSR], hs1[4][NSR];
 #pragma unroll
-    for (int rr = 0; rr < NSR; ++rr)
+        for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int c = 0; c <= 2 * U; ++c) row[rr][c] = 0.f;
-    // point m = -1: the edgel (nine responses); m >= 0: neighbour m (Ix, Iy)
+            for (int rr = 0; rr < NSR; ++rr) { h0[m][rr] = 0; h1[m][rr] = 0; hs0[m][rr] = 0; hs1[m][rr] = 0; }
+#pragma unroll 2
+        for (int c = 0; c < P; ++c) {
+            double px[NSR];
 #pragma unroll
-    for (int m = -1; m < 4; ++m) {
-        const int Rp = m < 0 ? R : R + nbr[m][0], Cp = m < 0 ? C : C + nbr[m][1];
-        const int ip = Rp >> 1, jp = Cp >> 1, a = Rp & 1, b = Cp & 1, ph = 2 * a + b;
-        const int vx = b ? 2 : (ph == 0 ? 1 : 0), vy = a ? 2 : (ph == 0 ? 1 : 0);
-        float cf[NSR][4];                           // per lane row, per x-family coefficient
+            for (int rr = 0; rr < NSR; ++rr) {
+                const int r = l + G * rr;
+                px[rr] = r < P ? img(pr0 + r, pc0 + c) : 0.0;
+            }
 #pragma unroll
-        for (int rr = 0; rr < NSR; ++rr) {
-            const int p = ip - (ic - U + l + G * rr);   // tap row index: image row = ip - p
+            for (int m = 0; m < 4; ++m) {
+                const double *xt = stp[vxn[m]][qb[m] - c], *xs = sdp[vxn[m]][qb[m] - c];
+                const double x0 = xt[0], x1 = xt[1], xs0 = xs[0], xs1 = xs[1];
 #pragma unroll
-            for (int f = 0; f < 4; ++f) cf[rr][f] = 0.f;
-            if (p < -R1 || p > R1) continue;
-            if (m < 0) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) cf[rr][XF(k)] = fmaf(adjf[k], stf[vy][YF(k)][p + R1], cf[rr][XF(k)]);
-            } else {
-                cf[rr][1] = adjf[9 + 2 * m] * stf[vy][0][p + R1];     // Ix: G' in x, G in y
-                cf[rr][0] = adjf[10 + 2 * m] * stf[vy][1][p + R1];    // Iy: G in x, G' in y
+                for (int rr = 0; rr < NSR; ++rr) {
+                    h0[m][rr] = fma(px[rr], x0, h0[m][rr]);
+                    h1[m][rr] = fma(px[rr], x1, h1[m][rr]);
+                    hs0[m][rr] = fma(px[rr], xs0, hs0[m][rr]);
+                    hs1[m][rr] = fma(px[rr], xs1, hs1[m][rr]);
+                }
             }
         }
 #pragma unroll
-        for (int c = 0; c <= 2 * U; ++c) {
-            const int qq = jp - (jc - U + c);          // tap column index: image col = jp - q
-            if (qq < -R1 || qq > R1) continue;
-            const float x0 = stf[vx][0][qq + R1], x1 = stf[vx][1][qq + R1];
-            const float x2 = stf[vx][2][qq + R1], x3 = stf[vx][3][qq + R1];
+        for (int m = 0; m < 4; ++m) {
+            const int Rn = R + nbr[m][0], in = Rn >> 1;
+            const int vy = (Rn & 1) ? 2 : (((C + nbr[m][1]) & 1) == 0 ? 1 : 0);
+            double ix = 0, iy = 0, ixs = 0, iys = 0;
 #pragma unroll
-            for (int rr = 0; rr < NSR; ++rr)
-                row[rr][c] = fmaf(cf[rr][0], x0, fmaf(cf[rr][1], x1, fmaf(cf[rr][2], x2, fmaf(cf[rr][3], x3, row[rr][c]))));
+            for (int rr = 0; rr < NSR; ++rr) {
+                const int r = l + G * rr;
+                if (r >= P) continue;
+                const double *yt = stp[vy][in - (pr0 + r) + R1 + 1], *ys = sdp[vy][in - (pr0 + r) + R1 + 1];
+                const double y0 = yt[0], y1 = yt[1], ys0 = ys[0], ys1 = ys[1];
+                ix = fma(h1[m][rr], y0, ix);
+                iy = fma(h0[m][rr], y1, iy);
+                ixs = fma(hs1[m][rr], y0, fma(h1[m][rr], ys0, ixs));
+                iys = fma(hs0[m][rr], y1, fma(h0[m][rr], ys1, iys));
+            }
+            nv[m][0] = redG(ix); nv[m][1] = redG(iy);
+            ns[m][0] = redG(ixs); ns[m][1] = redG(iys);
         }
     }
+
+    // ---- 2b. reverse pass of the parabola fit, subpixel position and strength
+    // (every lane holds the totals; all compute it), reciprocals taken once
+    const double ux = up.x, uy = up.y, us = up.w;
+    const double f = sqrt(gx * gx + gy * gy), rf = 1.0 / f;
+    const double nx = gx * rf, ny = gy * rf;
+    const double s = xm ? (sy * gy) / (sx * gx) : (sx * gx) / (sy * gy);
+    double M[4];
 #pragma unroll
-    for (int rr = 0; rr < NSR; ++rr) {
-        const int o = -U + l + G * rr;
-        const int ii = ic + o;
-        if (o > U || ii < 0 || ii >= H) continue;
+    for (int m = 0; m < 4; ++m) M[m] = sqrt(nv[m][0] * nv[m][0] + nv[m][1] * nv[m][1]);
+    const double fp = M[0] + s * (M[1] - M[0]), fm = M[2] + s * (M[3] - M[2]);
+    const double q = 1.0 + s * s, sq = sqrt(q), rq = 1.0 / q, rsq = 1.0 / sq;
+    const double A = (fm + fp - 2.0 * f) * (0.5 * rq);
+    if (!(isfinite(A) && A != 0.0 && f > 0.0)) return;
+    const double rA = 1.0 / A;
+    const double B = (fp - fm) * (0.5 * rsq);
+    const double t = -0.5 * B * rA;
+    const double maxf = f - 0.25 * B * B * rA;
+
+    const double tbar = 0.5 * (ux * nx + uy * ny);
+    const double nxbar = 0.5 * ux * t, nybar = 0.5 * uy * t;
+    const double mbar = us * (maxf >= 0.0 ? 1.0 : -1.0);
+    const double Abar = (0.25 * mbar * B * B + 0.5 * tbar * B) * (rA * rA);
+    const double Bbar = -(0.5 * mbar * B + 0.5 * tbar) * rA;
+    double fbar = mbar - Abar * rq;
+    const double fpbar = 0.5 * Abar * rq + 0.5 * Bbar * rsq;
+    const double fmbar = 0.5 * Abar * rq - 0.5 * Bbar * rsq;
+    const double sqbar = -Bbar * B * rsq;
+    const double qbar = -Abar * A * rq + 0.5 * sqbar * rsq;
+    const double sbar = 2.0 * s * qbar + fpbar * (M[1] - M[0]) + fmbar * (M[3] - M[2]);
+    double gxbar = ao0, gybar = ao1;
+    if (xm) { const double rg = 1.0 / gx; gybar += sbar * sy * sx * rg; gxbar -= sbar * s * rg; }
+    else    { const double rg = 1.0 / gy; gxbar += sbar * sx * sy * rg; gybar -= sbar * s * rg; }
+    gxbar += nxbar * rf;
+    gybar += nybar * rf;
+    fbar -= (nxbar * gx + nybar * gy) * (rf * rf);
+    gxbar += fbar * gx * rf;
+    gybar += fbar * gy * rf;
+    // neighbour adjoints: M = |(Ix, Iy)|
+    double an[8];
 #pragma unroll
-        for (int c = 0; c <= 2 * U; ++c) {
-            const int jj = jc - U + c;
-            if (jj >= 0 && jj < W && row[rr][c] != 0.f) sink(ii, jj, row[rr][c]);
+    for (int m = 0; m < 4; ++m) {
+        const double Mb = (m == 0) ? fpbar * (1.0 - s) : (m == 1) ? fpbar * s : (m == 2) ? fmbar * (1.0 - s) : fmbar * s;
+        const double w = M[m] > 0.0 ? Mb / M[m] : 0.0;
+        an[2 * m] = w * nv[m][0];
+        an[2 * m + 1] = w * nv[m][1];
+    }
+
+    // ---- 3. dL/dsigma (summed per workgroup by the caller)
+    if (l == 0) {
+        gs = fma(gxbar, s9[0], fma(gybar, s9[1], gs));
+#pragma unroll
+        for (int m = 0; m < 4; ++m) gs = fma(an[2 * m], ns[m][0], fma(an[2 * m + 1], ns[m][1], gs));
+        *gs_out = gs;
+    }
+
+    // ---- 4. scatter the transposed filters into dL/dimage over the patch:
+    // per point and patch column one float4 of x taps serves all of the
+    // lane's rows.  The edgel uses all four x families, a neighbour two
+    // (Ix: G' in x, G in y; Iy: G in x, G' in y).
+    float adjf[17];
+    adjf[0] = (float)gxbar;
+    adjf[1] = (float)gybar;
+#pragma unroll
+    for (int k = 2; k < 9; ++k) adjf[k] = ao[k];
+#pragma unroll
+    for (int k = 9; k < 17; ++k) adjf[k] = (float)an[k - 9];
+    // per point m (0 the edgel, 1..4 the neighbours) and lane row: y taps
+    // times the adjoints, per x family; the point's x tap base and variant
+    float cf[5][NSR][4];
+    int qb[5], vxm[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const int Rp = m == 0 ? R : R + nbr[m - 1][0], Cp = m == 0 ? C : C + nbr[m - 1][1];
+        const int ip = Rp >> 1, jp = Cp >> 1, a = Rp & 1, b = Cp & 1, ph = 2 * a + b;
+        vxm[m] = b ? 2 : (ph == 0 ? 1 : 0);
+        const int vy = a ? 2 : (ph == 0 ? 1 : 0);
+        qb[m] = jp - pc0 + R1 + 1;                  // + (-c): tap column index q = jp - image col
+#pragma unroll
+        for (int rr = 0; rr < NSR; ++rr) {
+            const int r = l + G * rr;
+            // tap row index p = ip - image row, stored at p + R1 + 1
+            const float4 y4 = r < P ? stf4[vy][ip - (pr0 + r) + R1 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float yv[4] = {y4.x, y4.y, y4.z, y4.w};
+#pragma unroll
+            for (int f = 0; f < 4; ++f) cf[m][rr][f] = 0.f;
+            if (m == 0) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) cf[0][rr][XF(k)] = fmaf(adjf[k], yv[YF(k)], cf[0][rr][XF(k)]);
+            } else {
+                cf[m][rr][1] = adjf[7 + 2 * m] * yv[0];
+                cf[m][rr][0] = adjf[8 + 2 * m] * yv[1];
+            }
         }
+    }
+    // patch columns: per column five float4 tap reads serve all of the
+    // lane's rows, and the column's sums go straight to the sink
+#pragma unroll 2
+    for (int c = 0; c < P; ++c) {
+        float acc[NSR];
+#pragma unroll
+        for (int rr = 0; rr < NSR; ++rr) acc[rr] = 0.f;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const float4 x4 = stf4[vxm[m]][qb[m] - c];
+#pragma unroll
+            for (int rr = 0; rr < NSR; ++rr) {
+                if (m == 0)
+                    acc[rr] = fmaf(cf[0][rr][0], x4.x, fmaf(cf[0][rr][1], x4.y, fmaf(cf[0][rr][2], x4.z, fmaf(cf[0][rr][3], x4.w, acc[rr]))));
+                else
+                    acc[rr] = fmaf(cf[m][rr][0], x4.x, fmaf(cf[m][rr][1], x4.y, acc[rr]));
+            }
+        }
+        const int jj = pc0 + c;
+        if (jj < 0 || jj >= W) continue;
+#pragma unroll
+        for (int rr = 0; rr < NSR; ++rr) {
+            const int r = l + G * rr, ii = pr0 + r;
+            if (r < P && ii >= 0 && ii < H && acc[rr] != 0.f) sink(ii, jj, acc[rr]);
+        }
+    }
+}
+
+// The tap tables of edgel_backward from tap(v, f, t) and dtap(v, f, t)
+// (variant v, family f, t = 0 .. 2 R1), threads tid of nt: zero border at
+// index 0 and 2 R1 + 2.
+template <int R1, class TapAt, class DTapAt>
+__device__ __forceinline__ void tap_tables_fill(double (*stp)[2 * R1 + 3][4], double (*sdp)[2 * R1 + 3][4],
+                                                float4 (*stf4)[2 * R1 + 3], int tid, int nt, const TapAt &tap,
+                                                const DTapAt &dtap) {
+    constexpr int NP = 2 * R1 + 3;
+    for (int k = tid; k < 3 * NP; k += nt) {
+        const int v = k / NP, i = k - v * NP, t = i - 1;
+        const bool in = t >= 0 && t <= 2 * R1;
+        double a[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            a[f] = in ? tap(v, f, t) : 0.0;
+            stp[v][i][f] = a[f];
+            sdp[v][i][f] = in ? dtap(v, f, t) : 0.0;
+        }
+        stf4[v][i] = make_float4((float)a[0], (float)a[1], (float)a[2], (float)a[3]);
     }
 }
 

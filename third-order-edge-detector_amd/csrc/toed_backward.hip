@@ -95,8 +95,8 @@ struct GlobalSink {
 
 // One edgel of job J (lane group, lane l): see toed_adjoint.hpp.
 template <typename IN_T, int R1>
-__device__ __forceinline__ void edgel_adjoint(const BwJob &J, int e, int l, const double (*st)[4][2 * R1 + 1],
-                                              const double (*sd)[4][2 * R1 + 1], const float (*stf)[4][2 * R1 + 1],
+__device__ __forceinline__ void edgel_adjoint(const BwJob &J, int e, int l, const double (*stp)[2 * R1 + 3][4],
+                                              const double (*sdp)[2 * R1 + 3][4], const float4 (*stf4)[2 * R1 + 3],
                                               double *gs_out) {
     const int IW = 2 * J.W;
     const int idx = J.index[e];
@@ -104,30 +104,27 @@ __device__ __forceinline__ void edgel_adjoint(const BwJob &J, int e, int l, cons
     const float4 up = reinterpret_cast<const float4 *>(J.gout)[e];
     GlobalImg<IN_T> img{&J};
     GlobalSink sink{J.gimg, J.gpitch};
-    toedadj::edgel_backward<R1>(R, C, up, l, J.H, J.W, st, sd, stf, img, sink, gs_out);
+    toedadj::edgel_backward<R1>(R, C, up, l, J.H, J.W, stp, sdp, stf4, img, sink, gs_out);
 }
 
 template <typename IN_T, int R1>
 __global__ __launch_bounds__(NT) void toed_backward_kernel(const BwJob *jobs, int njobs) {
     constexpr int N = 2 * R1 + 1;
-    __shared__ double st[3][4][N];     // taps
-    __shared__ double sd[3][4][N];     // d taps / d sigma
-    __shared__ float stf[3][4][N];     // FP32 taps for the scatter
+    __shared__ double stp[3][N + 2][4];   // taps, zero border
+    __shared__ double sdp[3][N + 2][4];   // d taps / d sigma
+    __shared__ float4 stf4[3][N + 2];     // FP32 taps for the scatter
     __shared__ double sgs[NT / G];     // per-group dL/dsigma, reduced once per workgroup
     const int wg = blockIdx.x;
     const BwJob J = load_bwjob(jobs, njobs, wg);
-    for (int k = threadIdx.x; k < 3 * 4 * N; k += NT) {
-        const int v = k / (4 * N), f = (k / N) & 3, t = k % N;
-        st[v][f][t] = J.taps->t[v][f][t];
-        sd[v][f][t] = J.taps->d[v][f][t];
-        stf[v][f][t] = (float)J.taps->t[v][f][t];
-    }
+    toedadj::tap_tables_fill<R1>(stp, sdp, stf4, threadIdx.x, NT,
+                                 [&](int v, int f, int t) { return J.taps->t[v][f][t]; },
+                                 [&](int v, int f, int t) { return J.taps->d[v][f][t]; });
     __syncthreads();
     const int n = min(*J.count, J.max_edgels);
     const int g = threadIdx.x / G, l = threadIdx.x % G;
     const int e = (wg - J.wg_begin) * (NT / G) + g;
     if (l == 0) sgs[g] = 0.0;
-    if (e < n) edgel_adjoint<IN_T, R1>(J, e, l, st, sd, stf, &sgs[g]);
+    if (e < n) edgel_adjoint<IN_T, R1>(J, e, l, stp, sdp, stf4, &sgs[g]);
     __syncthreads();
     if (threadIdx.x == 0 && J.gsigma) {
         double t = 0.0;

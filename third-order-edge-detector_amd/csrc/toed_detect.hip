@@ -726,8 +726,10 @@ __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs,
 // its transposed filters are accumulated with LDS atomics into a
 // (TH + 2R1 + 2) x (32 + 2R1 + 2) gradient patch, and the patch is added to
 // dL/dimage once per tile.  dL/dsigma is summed per workgroup.
+// Occupancy 4 waves per SIMD (128 VGPRs, a few spills to scratch) measures
+// faster than the 2 that the unconstrained allocation (about 230 VGPRs) gives.
 template <typename IN_T, int TH, int R1>
-__global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
+__global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
                                                                 const GradOut *gouts, StageBuf stg,
                                                                 const int *tile_cnt, const int *offsets,
                                                                 const int *chunk_pref, int tile_begin) {
@@ -736,9 +738,9 @@ __global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *
     constexpr int PR = TH + 2 * U, PC = TW + 2 * U, LDG = PC | 1;
     __shared__ float simg[PR * LDG];
     __shared__ float sgrad[PR * LDG];
-    __shared__ double st[3][4][N];
-    __shared__ double sd[3][4][N];
-    __shared__ float stf[3][4][N];
+    __shared__ double stp[3][N + 2][4];
+    __shared__ double sdp[3][N + 2][4];
+    __shared__ float4 stf4[3][N + 2];
     __shared__ double sgs[NT / toedadj::G];
     __shared__ int s_rc[BW_CHUNK];           // interp (row << 16 | col) of a chunk's edgels, -1: no output row
     __shared__ float4 s_up[BW_CHUNK];        // their upstream gradient rows
@@ -759,15 +761,15 @@ __global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *
         simg[r * LDG + c] = v;
         sgrad[r * LDG + c] = 0.f;
     }
-    for (int k = threadIdx.x; k < 12 * N; k += NT) {
-        const int v = k / (4 * N), f = (k / N) & 3, t = k % N;
-        double a = (v == 2) ? tp->ds[f][t] : tp->du[f][t];
-        double b = (v == 2) ? tp->dss[f][t] : tp->dsu[f][t];
-        if (v == 1 && (t == 0 || t == N - 1)) { a = 0.0; b = 0.0; }
-        st[v][f][t] = a;
-        sd[v][f][t] = b;
-        stf[v][f][t] = (float)a;
-    }
+    // variant 1 (window R0) is variant 0 with zero end taps
+    toedadj::tap_tables_fill<R1>(
+        stp, sdp, stf4, threadIdx.x, NT,
+        [&](int v, int f, int t) {
+            return (v == 1 && (t == 0 || t == N - 1)) ? 0.0 : ((v == 2) ? tp->ds[f][t] : tp->du[f][t]);
+        },
+        [&](int v, int f, int t) {
+            return (v == 1 && (t == 0 || t == N - 1)) ? 0.0 : ((v == 2) ? tp->dss[f][t] : tp->dsu[f][t]);
+        });
     const int g = threadIdx.x / toedadj::G, l = threadIdx.x % toedadj::G;
     if (l == 0) sgs[g] = 0.0;
     __syncthreads();
@@ -810,7 +812,7 @@ __global__ __launch_bounds__(NT) void toed_tile_backward_kernel(const ImageJob *
             if (k >= cn || s_rc[k] < 0) continue;
             const int R = s_rc[k] >> 16, C = s_rc[k] & 0xffff;
             double gs = 0.0;
-            toedadj::edgel_backward<R1>(R, C, s_up[k], l, J.H, J.W, st, sd, stf, img, sink, &gs);
+            toedadj::edgel_backward<R1>(R, C, s_up[k], l, J.H, J.W, stp, sdp, stf4, img, sink, &gs);
             if (l == 0) sgs[g] += gs;
         }
         __syncthreads();
