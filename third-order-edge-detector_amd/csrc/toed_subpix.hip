@@ -545,13 +545,22 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
             if (keep_all) {
                 kp = true;
             } else {
+                // rows outward from the edgel's (dy = 0, -1, 1, -2, ...), and in a
+                // row only the survivors: the set bits of its 2K + 1 candidate
+                // columns, window mask columns 32 + lj - K .. 32 + lj + K
                 const double2 me = posw[(li + K) * WC + (lj + K)];
-                for (int dy = -K; dy <= K && !kp; ++dy) {
+                const int c0 = TW + lj - K, sh = c0 & 31, wsel = c0 >> 5;
+                const uint32_t fmask = (1u << (2 * K + 1)) - 1u;
+                for (int t = 0; t <= 2 * K && !kp; ++t) {
+                    const int dy = (t & 1) ? -((t + 1) >> 1) : (t >> 1);
                     const int r = K + li + dy;
-                    for (int dxp = -K; dxp <= K; ++dxp) {
-                        const int c = K + lj + dxp;
-                        if (!survivor(r, c) || (dy == 0 && dxp == 0)) continue;
-                        const double2 o = posw[r * WC + c];
+                    const uint64_t two = ((uint64_t)vm[r][wsel + 1] << 32) | vm[r][wsel];
+                    uint32_t bits = (uint32_t)(two >> sh) & fmask;
+                    if (dy == 0) bits &= ~(1u << K);          // the edgel itself
+                    while (bits) {
+                        const int b = __ffs(bits) - 1;
+                        bits &= bits - 1;
+                        const double2 o = posw[r * WC + lj + b];
                         const double ddx = me.x - o.x, ddy = me.y - o.y;
                         if (ddx * ddx + ddy * ddy <= tp.r2) { kp = true; break; }
                     }
