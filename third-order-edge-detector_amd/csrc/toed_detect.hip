@@ -414,18 +414,21 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
 }
 
 // Third-order orientation of the tile's survivors (cpu_toed.cpp:227-232) in
-// FP64 from the LDS image tile: 16 lanes per edgel, each lane one or two rows
-// p of the (2R1+1)^2 window, row sums for G..G''' in x then the nine products
-// with the row's y taps, reduced across the 16 lanes.  Phase (0,0) uses the
-// R0 window through taps whose end values are zero.
+// FP64 from the LDS image tile: 8 lanes per edgel, lane l the window rows
+// p = -R1 + l + 8k (the fewest idle lanes for 2R1+1 = 11, 15, 19 rows), the row
+// sums for the four x families over the columns (unrolled by 4; the four taps
+// of a column are adjacent: tapd[(fam * N + t) * 4 + f]), then the nine products
+// with the row's y taps, reduced across the 8 lanes.  Phase (0,0) uses the R0
+// window through taps whose end values are zero.
 template <int R1, int TH>
 __device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long long tile_local, StageBuf stg,
                                   const int *queue, int qn, const double *tapd, const unsigned long long *rowmask) {
     using Gm = Geo<R1, TH>;
     constexpr int N = 2 * R1 + 1;
+    constexpr int GT = 8, NK = (N + GT - 1) / GT;
     const float *img = lds + Gm::OFF_IMG;
-    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    for (int e0 = 0; e0 < qn; e0 += NT / 16) {
+    const int g = threadIdx.x / GT, l = threadIdx.x % GT;
+    for (int e0 = 0; e0 < qn; e0 += NT / GT) {
         const int e = e0 + g;
         const bool valid = e < qn;
         const int q = valid ? queue[e] : 0;
@@ -436,22 +439,23 @@ __device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long l
         const double *Y = tapd + (a ? 2 : (phase == 0 ? 1 : 0)) * 4 * N;
         const int ci = (r >> 1) + 1 + R1, cj = (c >> 1) + 1 + R1;
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 1
-        for (int k = 0; k < 2; ++k) {
-            const int p = -R1 + l + 16 * k;
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const int p = -R1 + l + GT * k;
             if (valid && p <= R1) {
                 const float *row = img + (ci - p) * Gm::LDI + cj;
                 double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
-#pragma unroll 1
+#pragma unroll 4
                 for (int qq = -R1; qq <= R1; ++qq) {
                     const double v = (double)row[-qq];
-                    h0 = fma(v, X[0 * N + qq + R1], h0);
-                    h1 = fma(v, X[1 * N + qq + R1], h1);
-                    h2 = fma(v, X[2 * N + qq + R1], h2);
-                    h3 = fma(v, X[3 * N + qq + R1], h3);
+                    const double *x = X + 4 * (qq + R1);
+                    h0 = fma(v, x[0], h0);
+                    h1 = fma(v, x[1], h1);
+                    h2 = fma(v, x[2], h2);
+                    h3 = fma(v, x[3], h3);
                 }
-                const double y0 = Y[0 * N + p + R1], y1 = Y[1 * N + p + R1];
-                const double y2 = Y[2 * N + p + R1], y3 = Y[3 * N + p + R1];
+                const double *y = Y + 4 * (p + R1);
+                const double y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3];
                 acc[0] = fma(h1, y0, acc[0]);   // fx
                 acc[1] = fma(h0, y1, acc[1]);   // fy
                 acc[2] = fma(h2, y0, acc[2]);   // fxx
@@ -466,7 +470,7 @@ __device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long l
 #pragma unroll
         for (int d = 0; d < 9; ++d) {
 #pragma unroll
-            for (int m = 8; m >= 1; m >>= 1) acc[d] += __shfl_xor(acc[d], m, 16);
+            for (int m = GT / 2; m >= 1; m >>= 1) acc[d] += __shfl_xor(acc[d], m, GT);
         }
         if (valid && l == 0) {
             const double fx = acc[0], fy = acc[1], fxx = acc[2], fxy = acc[3], fyy = acc[4];
@@ -564,7 +568,7 @@ __device__ __forceinline__ void tile_body(float *lds, int *sh_cnt, const ImageJo
             const int fam = k / (4 * N), f = (k / N) & 3, t = k % N;
             double v = (fam == 2) ? tp->ds[f][t] : tp->du[f][t];
             if (fam == 1 && (t == 0 || t == N - 1)) v = 0.0;
-            tapd[k] = v;
+            tapd[(fam * N + t) * 4 + f] = v;
         }
     }
     nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
