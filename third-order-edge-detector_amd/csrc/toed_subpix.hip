@@ -306,6 +306,24 @@ __device__ __forceinline__ double keys(double s) {
     return 0.0;
 }
 
+// Sum over the four lanes of a quad (lanes 4k .. 4k+3), every lane receiving
+// it: two DPP quad permutations (xor 1, then xor 2), no LDS traffic.
+__device__ __forceinline__ float quad_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));   // [1,0,3,2]
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));   // [2,3,0,1]
+    return v;
+}
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double quad_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    return v;
+}
+
 // derivative of the Keys kernel
 __device__ __forceinline__ double keys_d(double s) {
     const double a = -0.5;
@@ -608,11 +626,12 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     if (s_nlist == 0) return;
     // the first round's per-edgel loads (position, strength, rank) are issued
     // now, overlapping the tile load
+    constexpr int GQ = 4;                      // lanes per edgel: one cubic stencil row each
     double2 ps_pre = make_double2(x0 + 4.0, y0 + 4.0);
     float2 vv_pre = make_float2(0.f, 0.f);
     int rk_pre = 0;
-    if ((int)(threadIdx.x >> 4) < s_nlist) {
-        const int p = elist[threadIdx.x >> 4], li = p / TW, lj = p - li * TW;
+    if ((int)(threadIdx.x / GQ) < s_nlist) {
+        const int p = elist[threadIdx.x / GQ], li = p / TW, lj = p - li * TW;
         const long long o = (long long)(y0 + li) * W + (x0 + lj);
         ps_pre = J.pos[o];
         vv_pre = J.val[o];
@@ -623,18 +642,17 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
     __syncthreads();
     conv_planes<R>(In, X, Y, tp);
 
-    // cubic sampling of the ten planes and the orientation: 16 lanes per edgel,
-    // lane (a, b) of the 4 x 4 stencil, FP64 sums over the group by shuffles;
-    // the edgel is written at its raster rank.  Differentiable form: the ten
-    // sums and their x and y derivatives (FP32) are kept per edgel for the
-    // adjoint, 32 floats at rec[8 rank].
+    // cubic sampling of the ten planes and the orientation: 4 lanes per edgel,
+    // lane sa one row of the 4 x 4 stencil (its four pixels), FP64 sums over
+    // the quad by DPP; the edgel is written at its raster rank.  Differentiable
+    // form: the ten sums and their x and y derivatives (FP32) are kept per
+    // edgel for the adjoint, 32 floats at rec[8 rank].
     float4 *out = reinterpret_cast<float4 *>(J.edges);
     const int nlist = s_nlist;
-    const int sub = threadIdx.x & 15;
-    const int sa = sub >> 2, sb = sub & 3;
+    const int sa = threadIdx.x & (GQ - 1);
 #pragma unroll 1
-    for (int e0 = 0; e0 < nlist; e0 += NT / 16) {
-        const int e = e0 + (threadIdx.x >> 4);
+    for (int e0 = 0; e0 < nlist; e0 += NT / GQ) {
+        const int e = e0 + (int)(threadIdx.x / GQ);
         const bool act = e < nlist;
         const int p = act ? elist[e] : 0;
         const int li = p / TW, lj = p - li * TW;
@@ -648,30 +666,36 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
             rank = act ? (int)J.wpre[(long long)(y0 + li) * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u)) : 0;
         }
         const int xf = (int)floor(ps.x), yf = (int)floor(ps.y);
-        const double wya = keys(ps.y - yf + 1 - sa), wxb = keys(ps.x - xf + 1 - sb);
-        const double w = wya * wxb;
+        const double fx = ps.x - xf, fy = ps.y - yf;
+        const double wya = keys(fy + 1 - sa);
         const int ii = min(max(yf - 1 + sa, 0), H - 1) - (y0 - HALO);
-        const int jj = min(max(xf - 1 + sb, 0), W - 1) - (x0 - HALO);
-        const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
-        float pv[10];
-        pv[0] = Y[0 * PR * PC + yo];   // Px
-        pv[1] = Y[1 * PR * PC + yo];   // Py
-        pv[2] = Y[2 * PR * PC + yo];   // Pxx
-        pv[3] = Y[3 * PR * PC + yo];   // Pyy
-        pv[4] = Y[4 * PR * PC + yo];   // Pxy
-        pv[5] = In[IR * IC + io];      // Hx
-        pv[6] = Y[5 * PR * PC + yo];   // Hxx
-        pv[7] = Y[6 * PR * PC + yo];   // Hxy
-        pv[8] = Y[7 * PR * PC + yo];   // Hyy
-        pv[9] = In[2 * IR * IC + io];  // Hy
         double v[10];
 #pragma unroll
-        for (int m = 0; m < 10; ++m) v[m] = (double)pv[m] * w;
+        for (int m = 0; m < 10; ++m) v[m] = 0.0;
+        float pv[4][10];
+        double wxb[4];
 #pragma unroll
-        for (int m = 0; m < 10; ++m)
+        for (int sb = 0; sb < 4; ++sb) {
+            wxb[sb] = keys(fx + 1 - sb);
+            const int jj = min(max(xf - 1 + sb, 0), W - 1) - (x0 - HALO);
+            const int yo = ii * PC + jj, io = (ii + R) * IC + (jj + R);
+            pv[sb][0] = Y[0 * PR * PC + yo];   // Px
+            pv[sb][1] = Y[1 * PR * PC + yo];   // Py
+            pv[sb][2] = Y[2 * PR * PC + yo];   // Pxx
+            pv[sb][3] = Y[3 * PR * PC + yo];   // Pyy
+            pv[sb][4] = Y[4 * PR * PC + yo];   // Pxy
+            pv[sb][5] = In[IR * IC + io];      // Hx
+            pv[sb][6] = Y[5 * PR * PC + yo];   // Hxx
+            pv[sb][7] = Y[6 * PR * PC + yo];   // Hxy
+            pv[sb][8] = Y[7 * PR * PC + yo];   // Hyy
+            pv[sb][9] = In[2 * IR * IC + io];  // Hy
+            const double w = wya * wxb[sb];
 #pragma unroll
-            for (int d = 8; d >= 1; d >>= 1) v[m] += __shfl_xor(v[m], d, 16);
-        if (act && sub == 0) {
+            for (int m = 0; m < 10; ++m) v[m] = fma((double)pv[sb][m], w, v[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < 10; ++m) v[m] = quad_sum(v[m]);
+        if (act && sa == 0) {
             double Fx = v[0] * v[6] + v[2] * v[5] + v[1] * v[7] + v[4] * v[9];
             double Fy = v[0] * v[7] + v[4] * v[5] + v[1] * v[8] + v[3] * v[9];
             const double Fm = sqrt(Fx * Fx + Fy * Fy);
@@ -688,24 +712,31 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
             }
         }
         if (J.rec) {
-            const float wx = (float)(wya * keys_d(ps.x - xf + 1 - sb)), wy = (float)(keys_d(ps.y - yf + 1 - sa) * wxb);
+            const double dwya = keys_d(fy + 1 - sa);
             float r[32];
 #pragma unroll
             for (int m = 0; m < 10; ++m) {
                 r[m] = (float)v[m];
-                r[10 + m] = pv[m] * wx;
-                r[20 + m] = pv[m] * wy;
+                r[10 + m] = 0.f;
+                r[20 + m] = 0.f;
             }
             r[30] = r[31] = 0.f;
 #pragma unroll
-            for (int m = 10; m < 30; ++m)
+            for (int sb = 0; sb < 4; ++sb) {
+                const float wx = (float)(wya * keys_d(fx + 1 - sb)), wy = (float)(dwya * wxb[sb]);
 #pragma unroll
-                for (int d = 8; d >= 1; d >>= 1) r[m] += __shfl_xor(r[m], d, 16);
+                for (int m = 0; m < 10; ++m) {
+                    r[10 + m] = fmaf(pv[sb][m], wx, r[10 + m]);
+                    r[20 + m] = fmaf(pv[sb][m], wy, r[20 + m]);
+                }
+            }
+#pragma unroll
+            for (int m = 10; m < 30; ++m) r[m] = quad_sum(r[m]);
             if (act && rank < J.capacity) {
                 float4 *dst = J.rec + (long long)rank * 8;
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (sub == k) dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
+                    if ((k >> 1) == sa) dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
             }
         }
     }
