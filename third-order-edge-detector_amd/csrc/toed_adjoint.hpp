@@ -1,7 +1,7 @@
 // Per-edgel reverse pass of the TOED outputs with the edgel set frozen, shared
 // by the index-driven backward (toed_backward.hip) and the tile-driven
 // backward of a batch plan (toed_detect.hip).  FP64 throughout; one G-lane
-// group per edgel (lanes split the window rows, totals by __shfl_xor).
+// group per edgel (lanes split the window rows, totals by DPP group sums).
 //
 // Differentiated forward arithmetic: the nine responses at a pixel
 // (cpu_toed.cpp:194-221), octant choice and neighbour interpolation
@@ -11,15 +11,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include "toed_dpp.hpp"
+
 namespace toedadj {
 
 constexpr int G = 8;    // lanes per edgel (window rows split over the group)
 
-__device__ __forceinline__ double redG(double v) {
-#pragma unroll
-    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, G);
-    return v;
-}
+__device__ __forceinline__ double redG(double v) { return toeddpp::group_sum<G>(v); }
 
 // response k (fx, fy, fxx, fxy, fyy, fxxy, fxyy, fxxx, fyyy) -> x (column) and
 // y (row) filter family (0 G, 1 G', 2 G'', 3 G''')

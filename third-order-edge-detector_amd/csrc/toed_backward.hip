@@ -7,7 +7,7 @@
 //   1. recomputes in FP64 from the image the nine responses at the edgel and
 //      Ix, Iy at its four interpolation neighbours (octant from the recomputed
 //      gradient), each with its derivative with respect to sigma (window radii
-//      frozen), lanes splitting the window rows, reduced with __shfl_xor;
+//      frozen), lanes splitting the window rows, reduced with DPP group sums;
 //   2. runs the hand-written reverse pass of the parabola fit, the subpixel
 //      position, the strength and the third-order orientation for the upstream
 //      row (dL/dx, dL/dy, dL/dtheta, dL/dstrength), giving 17 adjoints

@@ -418,7 +418,7 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
 // p = -R1 + l + 8k (the fewest idle lanes for 2R1+1 = 11, 15, 19 rows), the row
 // sums for the four x families over the columns (unrolled by 4; the four taps
 // of a column are adjacent: tapd[(fam * N + t) * 4 + f]), then the nine products
-// with the row's y taps, reduced across the 8 lanes.  Phase (0,0) uses the R0
+// with the row's y taps, summed across the 8 lanes (DPP).  Phase (0,0) uses the R0
 // window through taps whose end values are zero.
 template <int R1, int TH>
 __device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long long tile_local, StageBuf stg,
@@ -468,10 +468,7 @@ __device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long l
             }
         }
 #pragma unroll
-        for (int d = 0; d < 9; ++d) {
-#pragma unroll
-            for (int m = GT / 2; m >= 1; m >>= 1) acc[d] += __shfl_xor(acc[d], m, GT);
-        }
+        for (int d = 0; d < 9; ++d) acc[d] = toeddpp::group_sum<GT>(acc[d]);
         if (valid && l == 0) {
             const double fx = acc[0], fy = acc[1], fxx = acc[2], fxy = acc[3], fyy = acc[4];
             const double fxxy = acc[5], fxyy = acc[6], fxxx = acc[7], fyyy = acc[8];

@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "toed_capi.h"
+#include "toed_dpp.hpp"
 
 namespace toedsp {
 
@@ -304,24 +305,6 @@ __device__ __forceinline__ double keys(double s) {
     if (x <= 1.0) return ((a + 2) * x - (a + 3)) * x * x + 1;
     if (x < 2.0) return ((a * x - 5 * a) * x + 8 * a) * x - 4 * a;
     return 0.0;
-}
-
-// Sum over the four lanes of a quad (lanes 4k .. 4k+3), every lane receiving
-// it: two DPP quad permutations (xor 1, then xor 2), no LDS traffic.
-__device__ __forceinline__ float quad_sum(float v) {
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));   // [1,0,3,2]
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));   // [2,3,0,1]
-    return v;
-}
-template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double quad_sum(double v) {
-    v += dpp_f64<0xB1>(v);
-    v += dpp_f64<0x4E>(v);
-    return v;
 }
 
 // derivative of the Keys kernel
@@ -694,7 +677,7 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
             for (int m = 0; m < 10; ++m) v[m] = fma((double)pv[sb][m], w, v[m]);
         }
 #pragma unroll
-        for (int m = 0; m < 10; ++m) v[m] = quad_sum(v[m]);
+        for (int m = 0; m < 10; ++m) v[m] = toeddpp::group_sum<GQ>(v[m]);
         if (act && sa == 0) {
             double Fx = v[0] * v[6] + v[2] * v[5] + v[1] * v[7] + v[4] * v[9];
             double Fy = v[0] * v[7] + v[4] * v[5] + v[1] * v[8] + v[3] * v[9];
@@ -731,7 +714,7 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
                 }
             }
 #pragma unroll
-            for (int m = 10; m < 30; ++m) r[m] = quad_sum(r[m]);
+            for (int m = 10; m < 30; ++m) r[m] = toeddpp::group_sum<GQ>(r[m]);
             if (act && rank < J.capacity) {
                 float4 *dst = J.rec + (long long)rank * 8;
 #pragma unroll
