@@ -177,22 +177,37 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
     }
     __syncthreads();
     if (est) {
-        // row pass (:329-336): tmp(i, j) = sum_q E(i, sym(j - q)) tri[q]
-        for (int idx = threadIdx.x; idx < AE_R * AT_C; idx += NT) {
-            const int r = idx / AT_C, c = idx - r * AT_C;
-            double acc = 0.0;
+        // row pass (:329-336): tmp(i, j) = sum_q E(i, sym(j - q)) tri[q], four
+        // outputs per thread from one 12-wide window (same order of terms per output)
+        constexpr int NO = 4, NCB = AT_C / NO, NRB = AS_R / NO;
+        static_assert(AT_C % NO == 0 && AS_R % NO == 0, "whole output blocks");
+        for (int idx = threadIdx.x; idx < AE_R * NCB; idx += NT) {
+            const int r = idx / NCB, c0 = (idx - r * NCB) * NO;
+            double w[NO + 8];
 #pragma unroll
-            for (int q = -4; q <= 4; ++q) acc += (double)Es[r][c + 4 - q] * tri_w(q);
-            Tr[r][c] = acc;
+            for (int t = 0; t < NO + 8; ++t) w[t] = (double)Es[r][c0 + t];
+#pragma unroll
+            for (int u = 0; u < NO; ++u) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = -4; q <= 4; ++q) acc += w[u + 4 - q] * tri_w(q);
+                Tr[r][c0 + u] = acc;
+            }
         }
         __syncthreads();
-        // column pass (:337-344)
-        for (int idx = threadIdx.x; idx < AS_R * AT_C; idx += NT) {
-            const int r = idx / AT_C, c = idx - r * AT_C;
-            double acc = 0.0;
+        // column pass (:337-344), four outputs per thread down a column
+        for (int idx = threadIdx.x; idx < NRB * AT_C; idx += NT) {
+            const int rb = idx / AT_C, c = idx - rb * AT_C, r0 = rb * NO;
+            double w[NO + 8];
 #pragma unroll
-            for (int p = -4; p <= 4; ++p) acc += Tr[r + 4 - p][c] * tri_w(p);
-            Ss[r][c] = acc;
+            for (int t = 0; t < NO + 8; ++t) w[t] = Tr[r0 + t][c];
+#pragma unroll
+            for (int u = 0; u < NO; ++u) {
+                double acc = 0.0;
+#pragma unroll
+                for (int p = -4; p <= 4; ++p) acc += w[u + 4 - p] * tri_w(p);
+                Ss[r0 + u][c] = acc;
+            }
         }
         __syncthreads();
     }
