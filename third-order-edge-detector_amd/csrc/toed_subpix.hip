@@ -918,6 +918,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     __shared__ float lds[BGeo<R>::FLOATS];
     __shared__ uint32_t km[TH];
     __shared__ int s_any, s_nlist;
+    __shared__ int s_box[4];           // rows, cols of Z that the edgels' stencils reach: min, max
     __shared__ short elist[TH * TW];
     // Z (the 8 plane adjoints) | Xb (transposed y pass), which holds the
     // per-edgel staging St during the edgel phase
@@ -930,7 +931,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
-    if (threadIdx.x == 0) { s_any = 0; s_nlist = 0; }
+    if (threadIdx.x == 0) { s_any = 0; s_nlist = 0; s_box[0] = PR; s_box[1] = -1; s_box[2] = PC; s_box[3] = -1; }
     __syncthreads();
     if (threadIdx.x < TH) {
         const int row = y0 + threadIdx.x;
@@ -1044,6 +1045,11 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
                     yfs = (int)floor(y);
                     fxs = (float)(x - xfs);
                     fys = (float)(y - yfs);
+                    // the Z rows and columns the stencil reaches (borders clamped)
+                    atomicMin(&s_box[0], max(yfs - 1, 0) - (y0 - HALO));
+                    atomicMax(&s_box[1], min(yfs + 2, H - 1) - (y0 - HALO));
+                    atomicMin(&s_box[2], max(xfs - 1, 0) - (x0 - HALO));
+                    atomicMax(&s_box[3], min(xfs + 2, W - 1) - (x0 - HALO));
                 }
                 J.gth[o] = (float)thb;
             }
@@ -1079,14 +1085,19 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     // dL/dHy added to HBM), per group.  Output u of a 4-output block at o0
     // sums window[u + m] * k[m], m = 0..2R, with window[t] = source(o0 - 2R + t)
     // (zero outside the source): the transposed filters.
+    // Only Z rows rlo .. rmax and columns clo .. chi can be nonzero: X bar rows
+    // rlo .. rlo + nrow - 1 (= rmax + 2R) and outputs up to column chi + 2R.
     constexpr int NB = 4, NWIN = 2 * R + NB;
-    constexpr int YB = (IR + NB - 1) / NB, XB = (IC + NB - 1) / NB;
+    const int rlo = s_box[0], clo = s_box[2], chi = s_box[3];
+    if (s_box[1] < rlo) return;                       // no stencil reached Z
+    const int nrow = s_box[1] + 2 * R - rlo + 1, ncol = chi - clo + 1;
+    const int nyb = (nrow + NB - 1) / NB, nxb = (ncol + 2 * R + NB - 1) / NB;
     const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
     const bool inner = iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W;
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp) {
-        for (int task = threadIdx.x; task < YB * PC; task += NT) {
-            const int blk = task / PC, c = task - blk * PC, r0 = blk * NB;
+        for (int task = threadIdx.x; task < nyb * ncol; task += NT) {
+            const int blk = task / ncol, c = clo + (task - blk * ncol), r0 = rlo + blk * NB;
             float acc[3][NB];
 #pragma unroll
             for (int g = 0; g < 3; ++g)
@@ -1117,14 +1128,14 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
             }
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
-                if (r0 + u >= IR) continue;
+                if (r0 + u >= rlo + nrow) continue;
 #pragma unroll
                 for (int g = 0; g < 3; ++g) Xb[g * IR * PC + (r0 + u) * PC + c] = acc[g][u];
             }
         }
         __syncthreads();
-        for (int task = threadIdx.x; task < IR * XB; task += NT) {
-            const int r = task / XB, blk = task - r * XB, c0 = blk * NB;
+        for (int task = threadIdx.x; task < nrow * nxb; task += NT) {
+            const int rr = task / nxb, blk = task - rr * nxb, r = rlo + rr, c0 = clo + blk * NB;
             float e0[NB], e1[NB];
 #pragma unroll
             for (int u = 0; u < NB; ++u) e0[u] = e1[u] = 0.f;
@@ -1137,7 +1148,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
 #pragma unroll
                 for (int t = 0; t < NWIN; ++t) {
                     const int c = c0 - 2 * R + t;
-                    win[t] = (c >= 0 && c < PC) ? xp[c] : 0.f;
+                    win[t] = (c >= clo && c <= chi) ? xp[c] : 0.f;
                 }
                 const int f = grp == 0 ? (q == 0 ? 1 : (q == 1 ? 0 : 2)) : (q == 0 ? 1 : (q == 1 ? 0 : 1));
                 const bool to1 = grp == 1 && q > 0;
@@ -1155,7 +1166,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
                 const int cm = c0 + u;
-                if (cm >= IC) continue;
+                if (cm > chi + 2 * R) continue;
                 const int gx = inner ? ix0 + cm : wrap_circ(ix0 + cm, W);
                 if (grp == 0) {
                     if (e0[u] != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e0[u]);
