@@ -1225,21 +1225,49 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
     const int ry0 = y0 - 6, rx0 = x0 - 6;
-    for (int idx = threadIdx.x; idx < DR * DC; idx += NT) {
-        const int r = idx / DC, c = idx - r * DC;
-        const int gy = ry0 + r, gx = rx0 + c;
-        float ox = 0.f, oy = 0.f;
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W) {
-            const long long o = (long long)gy * W + gx;
-            const float4 q = J.tq[o];
-            const float e = J.map[(long long)gy * J.pitch + gx];
-            float gt = e * (q.x * J.ghy[o] - q.y * J.ghx[o]);
-            if ((J.kmask[(long long)gy * J.nw + (gx >> 5)] >> (gx & 31)) & 1u) gt += J.gth[o];
-            ox = gt * q.z;
-            oy = gt * q.w;
+    {
+        // all of a thread's region loads are issued before any is used
+        constexpr int NU = (DR * DC + NT - 1) / NT;
+        float4 qv[NU];
+        float ev[NU], hxv[NU], hyv[NU], gtv[NU];
+        uint32_t kw[NU];
+        bool in[NU];
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int idx = threadIdx.x + u * NT;
+            const int r = idx / DC, c = idx - r * DC;
+            const int gy = ry0 + r, gx = rx0 + c;
+            in[u] = idx < DR * DC && gy >= 0 && gy < H && gx >= 0 && gx < W;
+            qv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            ev[u] = hxv[u] = hyv[u] = gtv[u] = 0.f;
+            kw[u] = 0u;
+            if (in[u]) {
+                const long long o = (long long)gy * W + gx;
+                qv[u] = J.tq[o];
+                ev[u] = J.map[(long long)gy * J.pitch + gx];
+                hxv[u] = J.ghx[o];
+                hyv[u] = J.ghy[o];
+                gtv[u] = J.gth[o];      // meaningful at kept edgels only (checked below)
+                kw[u] = J.kmask[(long long)gy * J.nw + (gx >> 5)];
+            }
         }
-        Ox[r][c] = ox;
-        Oy[r][c] = oy;
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int idx = threadIdx.x + u * NT;
+            if (idx >= DR * DC) continue;
+            const int r = idx / DC, c = idx - r * DC;
+            const int gx = rx0 + c;
+            float ox = 0.f, oy = 0.f;
+            if (in[u]) {
+                const float4 q = qv[u];
+                float gt = ev[u] * (q.x * hyv[u] - q.y * hxv[u]);
+                if ((kw[u] >> (gx & 31)) & 1u) gt += gtv[u];
+                ox = gt * q.z;
+                oy = gt * q.w;
+            }
+            Ox[r][c] = ox;
+            Oy[r][c] = oy;
+        }
     }
     __syncthreads();
     // one transposed difference: x on Ox (cols 1..DC-2), y on Oy (rows 1..DR-2)
@@ -1275,7 +1303,9 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
     }
     __syncthreads();
     // transposed row pass at the tile pixels, plus dL/dE of Hx = E dirx, Hy = E diry
-    for (int p = threadIdx.x; p < TH * TW; p += NT) {
+#pragma unroll
+    for (int k = 0; k < TH * TW / NT; ++k) {
+        const int p = threadIdx.x + k * NT;
         const int li = p / TW, lj = p - li * TW;
         const int gi = y0 + li, gj = x0 + lj;
         if (gi >= H || gj >= W) continue;
