@@ -178,8 +178,8 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags);
 int toed_subpix_destroy(toed_subpix *p);
 /* Per-stage timing of later runs with HIP events (the reference's time_nms,
  * time_conv + time_correct, subpix_to_correction.hpp:76-78), in ms:
- * orientation + NMS kernel, isolation + convolution + correction kernel, and
- * list scan + scatter + maps. */
+ * orientation + NMS kernel; isolation and convolution + correction kernels
+ * (the latter writes the list rows); rank scans + TO maps. */
 int toed_subpix_set_timing(toed_subpix *p, int on);
 int toed_subpix_last_times(toed_subpix *p, float *ms_orient_nms, float *ms_correct, float *ms_list);
 

@@ -17,8 +17,9 @@
 //   * cpu_nthreads is accepted and ignored; the GPU is the current HIP device.
 //   * n (interp_n, the bilinear work-grid upsampling of :226-232) must be 0.
 //   * time_nms / time_conv / time_correct are in ms (GPU events): the
-//     orientation + NMS kernel, the isolation + convolution + correction
-//     kernel, and the list build (the reference: seconds of CPU time).
+//     orientation + NMS kernel, the isolation and convolution + correction
+//     kernels (which write the list rows), and the rank scans + TO maps
+//     (the reference: seconds of CPU time).
 #ifndef TOED_SUBPIX_GPU_HPP
 #define TOED_SUBPIX_GPU_HPP
 

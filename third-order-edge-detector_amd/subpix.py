@@ -105,7 +105,8 @@ class SubpixBatch:
         check(lib().toed_subpix_set_timing(self._p, int(bool(on))), "toed_subpix_set_timing")
 
     def last_times(self):
-        """(orientation + NMS, isolation + convolutions + correction, list) in
+        """(orientation + NMS, isolation + convolutions + correction with the
+        list rows, rank scans + TO maps) in
         ms for the last run with timing on."""
         a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
         check(lib().toed_subpix_last_times(self._p, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
