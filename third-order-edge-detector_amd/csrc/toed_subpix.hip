@@ -213,7 +213,8 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
     }
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll 1
+    // the thread's two pixels unrolled: two independent FP64 chains in flight
+#pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int p = threadIdx.x + k * NT;
         const int li = p / TW, lj = p - li * TW;
