@@ -487,59 +487,124 @@ __device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long l
     }
 }
 
+// Image staging of a tile (the tile + 1 + R1 halo; zero outside the image,
+// cpu_toed.cpp:207-208): all of a thread's loads into registers first, then
+// the LDS stores.  u8 images with 4-byte aligned rows move aligned dwords of
+// 4 pixels (partial dwords at the image edges byte by byte).
+template <int R1, int TH> struct StageRegs {
+    using Gm = Geo<R1, TH>;
+    static constexpr int NDW = Gm::NRI * ((Gm::NCI + 3) / 4 + 1);       // dwords per tile, at most
+    static constexpr int NU = (NDW + NT - 1) / NT;
+    uint32_t v[NU];
+};
+
+// The register route is taken for u8 images with 4-byte aligned rows (two
+// dwords per thread); f32 tiles are staged pixel by pixel.
+__device__ __forceinline__ bool stage_aligned_u8(const ImageJob &J) {
+    return (J.pitch & 3) == 0 && ((uintptr_t)J.img & 3) == 0;
+}
+
+template <int R1, int TH>
+__device__ __forceinline__ void stage_fetch_u8(const ImageJob &J, int tile, StageRegs<R1, TH> &sr) {
+    using Gm = Geo<R1, TH>;
+    const long long tile_local = tile - J.tile_begin;
+    const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
+    const int gi0 = tr * TH - 1 - R1, gj0 = tc * TW - 1 - R1;
+    const uint8_t *base = (const uint8_t *)J.img;
+    const int gc0 = gj0 & ~3;
+    const int nd = (gj0 + Gm::NCI - gc0 + 3) >> 2;
+#pragma unroll
+    for (int u = 0; u < StageRegs<R1, TH>::NU; ++u) {
+        const int e = threadIdx.x + u * NT;
+        const int rr = e / nd, k = e - rr * nd;
+        const int gi = gi0 + rr, col0 = gc0 + 4 * k;
+        uint32_t w = 0;
+        if (e < Gm::NRI * nd && gi >= 0 && gi < J.H) {
+            const uint8_t *row = base + (long long)gi * J.pitch;
+            if (col0 >= 0 && col0 + 3 < J.W) {
+                w = *(const uint32_t *)(row + col0);
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if (col0 + b >= 0 && col0 + b < J.W) w |= (uint32_t)row[col0 + b] << (8 * b);
+            }
+        }
+        sr.v[u] = w;
+    }
+}
+
+template <int R1, int TH>
+__device__ __forceinline__ void stage_store_u8(float *lds, const ImageJob &J, int tile, const StageRegs<R1, TH> &sr) {
+    using Gm = Geo<R1, TH>;
+    float *img = lds + Gm::OFF_IMG;
+    const long long tile_local = tile - J.tile_begin;
+    const int tc = (int)(tile_local % J.ntc);
+    const int gj0 = tc * TW - 1 - R1;
+    const int gc0 = gj0 & ~3;
+    const int nd = (gj0 + Gm::NCI - gc0 + 3) >> 2;
+#pragma unroll
+    for (int u = 0; u < StageRegs<R1, TH>::NU; ++u) {
+        const int e = threadIdx.x + u * NT;
+        if (e >= Gm::NRI * nd) continue;
+        const int rr = e / nd, k = e - rr * nd, col0 = gc0 + 4 * k;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int lc = col0 + b - gj0;
+            if (lc >= 0 && lc < Gm::NCI) img[rr * Gm::LDI + lc] = (float)((sr.v[u] >> (8 * b)) & 255u);
+        }
+    }
+}
+
+// f32 images and u8 images whose rows are not 4-byte aligned: pixel by pixel,
+// straight to LDS
 template <typename IN_T, int R1, int TH>
-__device__ __forceinline__ void tile_body(float *lds, int *sh_cnt, const ImageJob &J, int tile,
+__device__ __forceinline__ void stage_direct(float *lds, const ImageJob &J, int tile) {
+    using Gm = Geo<R1, TH>;
+    float *img = lds + Gm::OFF_IMG;
+    const long long tile_local = tile - J.tile_begin;
+    const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
+    const int gi0 = tr * TH - 1 - R1, gj0 = tc * TW - 1 - R1;
+    for (int e = threadIdx.x; e < Gm::NRI * Gm::NCI; e += NT) {
+        const int rr = e / Gm::NCI, cc = e % Gm::NCI;
+        const int gi = gi0 + rr, gj = gj0 + cc;
+        float v = 0.f;
+        if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
+        img[rr * Gm::LDI + cc] = v;
+    }
+}
+
+template <typename IN_T, int R1, int TH>
+__device__ __forceinline__ void stage_tile(float *lds, const ImageJob &J, int tile) {
+    if constexpr (std::is_same<IN_T, uint8_t>::value) {
+        if (stage_aligned_u8(J)) {
+            StageRegs<R1, TH> sr;
+            stage_fetch_u8<R1, TH>(J, tile, sr);
+            stage_store_u8<R1, TH>(lds, J, tile, sr);
+            return;
+        }
+    }
+    stage_direct<IN_T, R1, TH>(lds, J, tile);
+}
+
+// per-tile LDS counters: 4 phase counters, orientation-queue length,
+// near-peak count; the survivor row masks
+template <int R1, int TH> __device__ __forceinline__ void tile_setup(float *lds, int *sh_cnt) {
+    using Gm = Geo<R1, TH>;
+    if (threadIdx.x < 6) sh_cnt[threadIdx.x] = 0;
+    unsigned long long *rowmask = (unsigned long long *)(lds + Gm::OFF_MASK);
+    if (threadIdx.x < 2 * TH) rowmask[threadIdx.x] = 0ull;
+}
+
+// Everything after staging: filters, NMS, orientation, counts.  The image and
+// the counters of the tile are in LDS (a barrier precedes the call).
+template <typename IN_T, int R1, int TH>
+__device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const ImageJob &J, int tile,
                           int *__restrict__ counts, StageBuf stg, int *__restrict__ tile_cnt) {
     using Gm = Geo<R1, TH>;
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
-    const int i0 = tr * TH, j0 = tc * TW;
     const TOED_CONST TapSet *tp = as_const(J.taps);
-
-    // 4 phase counters, orientation-queue length, near-peak count; row masks
-    if (threadIdx.x < 6) sh_cnt[threadIdx.x] = 0;
     unsigned long long *rowmask = (unsigned long long *)(lds + Gm::OFF_MASK);
-    if (threadIdx.x < 2 * TH) rowmask[threadIdx.x] = 0ull;
-    // stage the image tile with its halo; zero outside the image (cpu_toed.cpp:207-208)
-    {
-        float *img = lds + Gm::OFF_IMG;
-        const int gi0 = i0 - 1 - R1, gj0 = j0 - 1 - R1;
-        if (std::is_same<IN_T, uint8_t>::value && (J.pitch & 3) == 0 && ((uintptr_t)J.img & 3) == 0) {
-            // aligned dwords of 4 pixels; partial dwords at the image edges byte by byte
-            const uint8_t *base = (const uint8_t *)J.img;
-            const int gc0 = gj0 & ~3;
-            const int nd = (gj0 + Gm::NCI - gc0 + 3) >> 2;
-            for (int e = threadIdx.x; e < Gm::NRI * nd; e += NT) {
-                const int rr = e / nd, k = e - (e / nd) * nd;
-                const int gi = gi0 + rr, col0 = gc0 + 4 * k;
-                uint32_t w = 0;
-                if (gi >= 0 && gi < J.H) {
-                    const uint8_t *row = base + (long long)gi * J.pitch;
-                    if (col0 >= 0 && col0 + 3 < J.W) {
-                        w = *(const uint32_t *)(row + col0);
-                    } else {
-#pragma unroll
-                        for (int b = 0; b < 4; ++b)
-                            if (col0 + b >= 0 && col0 + b < J.W) w |= (uint32_t)row[col0 + b] << (8 * b);
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int lc = col0 + b - gj0;
-                    if (lc >= 0 && lc < Gm::NCI) img[rr * Gm::LDI + lc] = (float)((w >> (8 * b)) & 255u);
-                }
-            }
-        } else {
-            for (int e = threadIdx.x; e < Gm::NRI * Gm::NCI; e += NT) {
-                const int rr = e / Gm::NCI, cc = e % Gm::NCI;
-                const int gi = gi0 + rr, gj = gj0 + cc;
-                float v = 0.f;
-                if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
-                img[rr * Gm::LDI + cc] = v;
-            }
-        }
-    }
-    __syncthreads();
     // round A: R0 window, unshifted -> phase (0,0)
     row_pass<R1, TH, 0>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
@@ -599,7 +664,10 @@ __global__ __launch_bounds__(NT) void toed_tiles_kernel(const ImageJob *jobs, in
     if (ticket == nullptr) {
         const int tile = tile_begin + blockIdx.x;
         const ImageJob J = load_job(jobs, njobs, job0, tile);
-        tile_body<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        tile_setup<R1, TH>(lds, sh_cnt);
+        stage_tile<IN_T, R1, TH>(lds, J, tile);
+        __syncthreads();
+        tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
         return;
     }
     for (;;) {
@@ -609,7 +677,10 @@ __global__ __launch_bounds__(NT) void toed_tiles_kernel(const ImageJob *jobs, in
         __syncthreads();
         if (tile >= tile_end) break;
         const ImageJob J = load_job(jobs, njobs, job0, tile);
-        tile_body<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        tile_setup<R1, TH>(lds, sh_cnt);
+        stage_tile<IN_T, R1, TH>(lds, J, tile);
+        __syncthreads();
+        tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
         __syncthreads();
     }
 }
