@@ -231,6 +231,25 @@ class BatchDetector:
         self.det._bind_stream()
         check(lib().toed_batch_backward(self._b, arr), "toed_batch_backward")
 
+    def backward_ptrs(self, gout_ptrs, gimg_ptrs, gpitches, gsigma_ptr=None):
+        """backward() from raw device addresses (ints): per image the first
+        upstream row, the first dL/dimage element and its row pitch in
+        elements; gsigma_ptr the one-element float64 accumulator or None.  For
+        callers that already hold the batch in single tensors (no per-image
+        views)."""
+        n = len(self.images)
+        arr = getattr(self, "_grad_arr", None)
+        if arr is None:
+            arr = self._grad_arr = (_lib.toed_grad_out_t * n)()
+        for m in range(n):
+            it = arr[m]
+            it.d_gout = gout_ptrs[m]
+            it.d_gimg = gimg_ptrs[m]
+            it.gpitch = gpitches[m]
+            it.d_gsigma = gsigma_ptr
+        self.det._bind_stream()
+        check(lib().toed_batch_backward(self._b, arr), "toed_batch_backward")
+
     def index_results(self):
         counts = self.counts.cpu().numpy()
         return [self.index[int(self.offsets[m]):int(self.offsets[m]) + int(counts[m])] for m in range(len(counts))]

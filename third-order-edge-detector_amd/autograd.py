@@ -93,13 +93,18 @@ def _make_function():
             images = plan.images
             B, H, W = images.shape
             g_out = g_out.contiguous().float()
-            gimg = torch.zeros_like(images, dtype=torch.float32)
+            gimg = torch.zeros((B, H, W), dtype=torch.float32, device=images.device)
             gsig = torch.zeros(1, dtype=torch.float64, device=images.device)
             if plan.gen == ctx.gen:
-                # the plan's staging still holds this forward's edgels: tile-driven backward
-                rows = [g_out[int(offsets[b]):] if counts[b] else g_out for b in range(B)]
-                plan.bd.backward([r if r.numel() else torch.zeros(4, device=images.device) for r in rows],
-                                 [gimg[b] for b in range(B)], gsig)
+                # the plan's staging still holds this forward's edgels: tile-driven backward,
+                # per-image addresses computed from the two batch tensors (an image with
+                # no edgels reads no upstream row)
+                if g_out.numel() == 0:
+                    g_out = torch.zeros((1, 4), dtype=torch.float32, device=images.device)
+                g0, i0 = g_out.data_ptr(), gimg.data_ptr()
+                gout_ptrs = [g0 + 16 * int(offsets[b]) if counts[b] else g0 for b in range(B)]
+                gimg_ptrs = [i0 + 4 * H * W * b for b in range(B)]
+                plan.bd.backward_ptrs(gout_ptrs, gimg_ptrs, [W] * B, gsig.data_ptr())
                 gs = None
                 if ctx.sigma_shape is not None:
                     gs = gsig.reshape(()).to(ctx.sigma_dtype).reshape(ctx.sigma_shape)

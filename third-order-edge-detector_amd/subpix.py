@@ -132,6 +132,22 @@ class SubpixBatch:
         self.det._bind_stream()
         check(lib().toed_subpix_backward(self._p, arr), "toed_subpix_backward")
 
+    def backward_ptrs(self, gout_ptrs, gmap_ptrs, gpitches):
+        """backward() from raw device addresses (ints): per map its first
+        upstream row, its first dL/dmap element and that plane's row pitch in
+        elements.  For callers that hold the batch in single tensors."""
+        n = len(self.maps)
+        arr = getattr(self, "_grad_arr", None)
+        if arr is None:
+            arr = self._grad_arr = (_lib.toed_subpix_grad_t * n)()
+        for m in range(n):
+            it = arr[m]
+            it.d_gout = gout_ptrs[m]
+            it.d_gmap = gmap_ptrs[m]
+            it.gpitch = gpitches[m]
+        self.det._bind_stream()
+        check(lib().toed_subpix_backward(self._p, arr), "toed_subpix_backward")
+
     def _counts(self):
         counts = self.counts.cpu().numpy()
         if (self.flags.cpu().numpy() & FLAG_CAPACITY).any():
@@ -293,13 +309,15 @@ def _make_subpix_function():
             if plan.gen != ctx.gen:
                 raise RuntimeError("subpix_differentiable: the plan ran again before this backward")
             maps = plan.maps
+            B, H, W = maps.shape
             g_out = g_out.contiguous().float()
-            gmap = torch.zeros_like(maps, dtype=torch.float32)
-            b = plan.b
-            gouts = []
-            for m in range(maps.shape[0]):
-                gouts.append(g_out[int(offsets[m]):] if counts[m] else torch.zeros(4, device=maps.device))
-            b.backward(gouts, [gmap[m] for m in range(maps.shape[0])])
+            if g_out.numel() == 0:
+                g_out = torch.zeros((1, 4), dtype=torch.float32, device=maps.device)
+            gmap = torch.zeros((B, H, W), dtype=torch.float32, device=maps.device)
+            # per-map addresses from the two batch tensors (a map with no edgels reads no row)
+            g0, m0 = g_out.data_ptr(), gmap.data_ptr()
+            plan.b.backward_ptrs([g0 + 16 * int(offsets[m]) if counts[m] else g0 for m in range(B)],
+                                 [m0 + 4 * H * W * m for m in range(B)], [W] * B)
             return gmap, None, None, None, None, None
 
     return _Subpix
