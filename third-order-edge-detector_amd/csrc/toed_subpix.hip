@@ -891,7 +891,8 @@ template <int R> struct BGeo {
     static constexpr int PP = G::PR * G::PC;
     static constexpr int XBN = 3 * G::IR * G::PC;    // transposed y pass, one group
     static constexpr int SN = 14 * NT;               // per-edgel staging of one round
-    static constexpr int FLOATS = 8 * PP + (XBN > SN ? XBN : SN);
+    static constexpr int NZ = 10;                    // 8 plane adjoints + raw Hx, Hy adjoints
+    static constexpr int FLOATS = NZ * PP + (XBN > SN ? XBN : SN);
 };
 
 __device__ __forceinline__ float keys_f(float s) {
@@ -908,10 +909,11 @@ __device__ __forceinline__ float keys_f(float s) {
 // dL/dE of the parabola goes to HBM (float atomics), dL/dtheta of the NMS is
 // stored at the edgel pixel, and the ten sample adjoints with the cubic
 // stencil base are staged in LDS.  (b) 16 lanes per edgel, lane (a, b) one
-// pixel of the 4 x 4 stencil: the eight convolved-plane adjoints are
-// scattered into Z (LDS atomics), raw Hx, Hy into HBM.  Then the transposed
-// circular convolutions pull Z back to dL/dE, dL/dHx, dL/dHy over the
-// tile + HALO + R, added to HBM with float atomics (periodic wrap).
+// pixel of the 4 x 4 stencil: the eight convolved-plane adjoints and the raw
+// Hx, Hy sample adjoints are scattered into Z (LDS atomics).  Then the
+// transposed circular convolutions pull Z back to dL/dE, dL/dHx, dL/dHy over
+// the tile + HALO + R (the raw adjoints join dL/dHx, dL/dHy there), added to
+// HBM with float atomics (periodic wrap).
 template <int R>
 __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const SpGrad *grads, SpTaps tp) {
     using G = CGeo<R>;
@@ -921,9 +923,11 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     __shared__ int s_any, s_nlist;
     __shared__ int s_box[4];           // rows, cols of Z that the edgels' stencils reach: min, max
     __shared__ short elist[TH * TW];
-    // Z (the 8 plane adjoints) | Xb (transposed y pass), which holds the
-    // per-edgel staging St during the edgel phase
-    float *Z = lds, *Xb = lds + 8 * PP, *St = lds + 8 * PP;
+    // Z (the 8 plane adjoints, then the raw Hx, Hy sample adjoints) | Xb
+    // (transposed y pass), which holds the per-edgel staging St during the
+    // edgel phase
+    constexpr int NZ = BGeo<R>::NZ;
+    float *Z = lds, *Xb = lds + NZ * PP, *St = lds + NZ * PP;
 
     const SpJob J = load_job(jobs);
     const int lt = blockIdx.x;
@@ -947,11 +951,12 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
         const int p = threadIdx.x + k * NT;
         if ((km[p / TW] >> (p % TW)) & 1u) elist[atomicAdd(&s_nlist, 1)] = (short)p;
     }
-    for (int idx = threadIdx.x; idx < 8 * PP; idx += NT) Z[idx] = 0.f;
+    for (int idx = threadIdx.x; idx < NZ * PP; idx += NT) Z[idx] = 0.f;
     __syncthreads();
 
-    // v index -> adjoint plane: Px 0, Py 1, Pxx 2, Pyy 3, Pxy 4, Hxx 5, Hxy 6, Hyy 7
-    constexpr int VPL[10] = {0, 1, 2, 3, 4, -1, 5, 6, 7, -2};
+    // v index -> adjoint plane: Px 0, Py 1, Pxx 2, Pyy 3, Pxy 4, Hxx 5, Hxy 6, Hyy 7,
+    // raw Hx 8, raw Hy 9
+    constexpr int VPL[10] = {0, 1, 2, 3, 4, 8, 5, 6, 7, 9};
     const int nlist = s_nlist;
     const int sub = threadIdx.x & 15, sa = sub >> 2, sb = sub & 3;
 #pragma unroll 1
@@ -1074,9 +1079,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
 #pragma unroll
             for (int m = 0; m < 10; ++m) {
                 const float c = St[m * NT + q] * w;
-                if (c == 0.f) continue;
-                if (VPL[m] >= 0) atomicAdd(&Z[VPL[m] * PP + yo], c);
-                else atomicAdd(VPL[m] == -1 ? &J.ghx[(long long)gy * W + gx] : &J.ghy[(long long)gy * W + gx], c);
+                if (c != 0.f) atomicAdd(&Z[VPL[m] * PP + yo], c);
             }
         }
         __syncthreads();
@@ -1169,6 +1172,15 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
                 const int cm = c0 + u;
                 if (cm > chi + 2 * R) continue;
                 const int gx = inner ? ix0 + cm : wrap_circ(ix0 + cm, W);
+                if (grp == 1) {
+                    // the raw Hx, Hy sample adjoints of the cubic stencils (plane
+                    // coordinates = output coordinates - R)
+                    const int zr = r - R, zc = cm - R;
+                    if (zr >= 0 && zr < PR && zc >= 0 && zc < PC) {
+                        e0[u] += Z[8 * PP + zr * PC + zc];
+                        e1[u] += Z[9 * PP + zr * PC + zc];
+                    }
+                }
                 if (grp == 0) {
                     if (e0[u] != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e0[u]);
                 } else {
