@@ -76,3 +76,26 @@ def test_batch_float_images(pkg, detector, batch_inputs):
         assert torch.equal(res[m], res8[m])
     b.close()
     b8.close()
+
+
+def test_full_batch_200_mixed_equals_singles(pkg, detector, scene):
+    # the batched workload at its full size: 200 scenes, sizes drawn from
+    # {480x640, 480x752, 720x1280, 1080x1920, 2160x3840}, sigma from {1, 1.5, 2};
+    # the captured persistent plan must give each image's single-image list
+    sizes = [(480, 640), (480, 752), (720, 1280), (1080, 1920), (2160, 3840)]
+    rng = np.random.default_rng(2024)
+    shapes = [sizes[k] for k in rng.integers(0, len(sizes), 200)]
+    sigmas = [[1.0, 1.5, 2.0][k] for k in rng.integers(0, 3, 200)]
+    dev = [torch.from_numpy(scene.shapes_scene(2024 * 1000 + m, H, W)).cuda() for m, (H, W) in enumerate(shapes)]
+    b = pkg.BatchDetector(detector, dev, sigmas, persistent=True, with_index=True)
+    b.run(graph=True)
+    res = [r.cpu().numpy() for r in b.results()]
+    idx = [i.cpu().numpy() for i in b.index_results()]
+    b.close()
+    total = 0
+    for m, (t, s) in enumerate(zip(dev, sigmas)):
+        e, i = detector.detect(t, sigma=s, with_index=True)
+        assert np.array_equal(idx[m], i.cpu().numpy()), m
+        assert np.array_equal(res[m], e.cpu().numpy()), m
+        total += len(res[m])
+    assert total > 1000000

@@ -54,6 +54,17 @@ def test_seeded_scenes_sigma(detector, oracle, scene, sigma, shape):
     assert_parity(rep)
 
 
+def test_full_frame_2160x3840(detector, oracle, scene):
+    # the plain workload's own frame (96 shapes, 2160 x 3840 u8, sigma 2), edgel for edgel
+    img = scene.shapes_scene(2160 * 3840, 2160, 3840)
+    ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
+    e, i = _run(detector, img)
+    rep = compare(ref, e, i)
+    _report("scene 2160x3840 sigma 2", rep)
+    assert ref["n"] > 100000
+    assert_parity(rep)
+
+
 def test_float_input_matches_u8(detector, oracle, scene):
     img = scene.shapes_scene(21, 200, 260)
     e8, i8 = _run(detector, img)
