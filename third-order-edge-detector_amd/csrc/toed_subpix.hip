@@ -342,22 +342,33 @@ template <int R> struct CGeo {
     static constexpr int FLOATS = IN + XN + YN;
 };
 
-// E, Hx, Hy over the plane region + R (periodic borders) into In
+// The plane rows r0 .. r1 and columns c0 .. c1 (even .. odd) a tile's cubic
+// stencils can reach: a kept edgel at tile pixel (i, j) lies within sqrt 2
+// of it, so its 4 x 4 stencil stays in plane rows i .. i + 2 HALO and columns
+// j .. j + 2 HALO.  In is needed over rows r0 .. r1 + 2R and columns
+// c0 .. c1 + 2R, the x pass there, the y pass over the box itself.
+struct PlaneBox {
+    int r0, r1, c0, c1;
+};
+
+// E, Hx, Hy over the plane region + R (periodic borders) into In, rows and
+// columns of the box only
 template <int R>
-__device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, int x0) {
+__device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, int x0, const PlaneBox &bx) {
     using G = CGeo<R>;
     constexpr int IR = G::IR, IC = G::IC;
     const int H = J.H, W = J.W;
     const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
+    const int lr = bx.r1 + 2 * R - bx.r0 + 1, lc = bx.c1 + 2 * R - bx.c0 + 1, ln = lr * lc;
     if (iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W) {
         constexpr int U = 4;
-        for (int base = 0; base < IR * IC; base += U * NT) {
+        for (int base = 0; base < ln; base += U * NT) {
             float v[3][U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int idx = base + threadIdx.x + u * NT;
-                const int r = idx / IC, c = idx - r * IC;
-                const bool ok = idx < IR * IC;
+                const int k = base + threadIdx.x + u * NT;
+                const int r = bx.r0 + k / lc, c = bx.c0 + k % lc;
+                const bool ok = k < ln;
                 const long long o = (long long)(iy0 + r) * W + (ix0 + c);
                 v[0][u] = ok ? J.map[(long long)(iy0 + r) * J.pitch + (ix0 + c)] : 0.f;
                 v[1][u] = ok ? J.hx[o] : 0.f;
@@ -365,8 +376,9 @@ __device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, 
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int idx = base + threadIdx.x + u * NT;
-                if (idx < IR * IC) {
+                const int k = base + threadIdx.x + u * NT;
+                if (k < ln) {
+                    const int idx = (bx.r0 + k / lc) * IC + bx.c0 + k % lc;
                     In[idx] = v[0][u];
                     In[IR * IC + idx] = v[1][u];
                     In[2 * IR * IC + idx] = v[2][u];
@@ -374,8 +386,8 @@ __device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, 
             }
         }
     } else {
-        for (int idx = threadIdx.x; idx < IR * IC; idx += NT) {
-            const int r = idx / IC, c = idx - r * IC;
+        for (int k = threadIdx.x; k < ln; k += NT) {
+            const int r = bx.r0 + k / lc, c = bx.c0 + k % lc, idx = r * IC + c;
             const int gy = wrap_circ(iy0 + r, H), gx = wrap_circ(ix0 + c, W);
             const long long o = (long long)gy * W + gx;
             In[idx] = J.map[(long long)gy * J.pitch + gx];
@@ -388,7 +400,7 @@ __device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, 
 // The eight circular Gaussian-derivative planes of the tile + HALO from In
 // (x pass into X, y pass into Y); ends with a barrier.
 template <int R>
-__device__ __forceinline__ void conv_planes(const float *In, float *X, float *Y, const SpTaps &tp) {
+__device__ __forceinline__ void conv_planes(const float *In, float *X, float *Y, const SpTaps &tp, const PlaneBox &bx) {
     using G = CGeo<R>;
     constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
     // group 0: E * (G', G, G'') along x -> Px, Pxx, Py, Pxy, Pyy
@@ -398,10 +410,11 @@ __device__ __forceinline__ void conv_planes(const float *In, float *X, float *Y,
     // the y pass) share 2R + 2 loaded values.  Output u of a pass sums
     // window[m + u] * k[2R - m], m = 0..2R (true convolution).
     static_assert(PC % 2 == 0 && PR % 2 == 0, "two outputs per thread");
+    const int xr = bx.r1 + 2 * R - bx.r0 + 1, ncp = (bx.c1 - bx.c0 + 1) / 2, nrp = (bx.r1 - bx.r0 + 1) / 2;
 #pragma unroll 1
     for (int grp = 0; grp < 2; ++grp) {
-        for (int idx = threadIdx.x; idx < IR * (PC / 2); idx += NT) {
-            const int r = idx / (PC / 2), c = 2 * (idx - r * (PC / 2));
+        for (int idx = threadIdx.x; idx < xr * ncp; idx += NT) {
+            const int r = bx.r0 + idx / ncp, c = bx.c0 + 2 * (idx % ncp);
             float a[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
             if (grp == 0) {
                 const float *src = In + r * IC + c;
@@ -441,8 +454,8 @@ __device__ __forceinline__ void conv_planes(const float *In, float *X, float *Y,
             }
         }
         __syncthreads();
-        for (int idx = threadIdx.x; idx < (PR / 2) * PC; idx += NT) {
-            const int r = 2 * (idx / PC), c = idx - (idx / PC) * PC;
+        for (int idx = threadIdx.x; idx < nrp * (2 * ncp); idx += NT) {
+            const int r = bx.r0 + 2 * (idx / (2 * ncp)), c = bx.c0 + idx % (2 * ncp);
             const float *x0p = X + r * PC + c, *x1p = X + IR * PC + r * PC + c, *x2p = X + 2 * IR * PC + r * PC + c;
             if (grp == 0) {
                 float px[2] = {0.f, 0.f}, py[2] = {0.f, 0.f}, pxx[2] = {0.f, 0.f}, pxy[2] = {0.f, 0.f},
@@ -637,9 +650,21 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
         rk_pre = (int)J.wpre[(long long)(y0 + li) * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
     }
 
-    load_tile_in<R>(J, In, y0, x0);
+    PlaneBox bx;
+    {
+        int r0 = TH, r1 = -1;
+        uint32_t cols = 0u;
+#pragma unroll
+        for (int i = 0; i < TH; ++i)
+            if (km[i]) { r0 = min(r0, i); r1 = i; cols |= km[i]; }
+        bx.r0 = r0 & ~1;
+        bx.r1 = (r1 + 2 * HALO) | 1;
+        bx.c0 = (__ffs(cols) - 1) & ~1;
+        bx.c1 = (31 - __clz(cols) + 2 * HALO) | 1;
+    }
+    load_tile_in<R>(J, In, y0, x0, bx);
     __syncthreads();
-    conv_planes<R>(In, X, Y, tp);
+    conv_planes<R>(In, X, Y, tp, bx);
 
     // cubic sampling of the ten planes and the orientation: 4 lanes per edgel,
     // lane sa one row of the 4 x 4 stencil (its four pixels), FP64 sums over
