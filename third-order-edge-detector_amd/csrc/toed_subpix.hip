@@ -1228,6 +1228,12 @@ constexpr int DR = TH + 12, DC = TW + 12;    // tile +- 6
 __device__ __forceinline__ float d_transpose(const float *b, int stride, int j, int n, int jo) {
     // (D^T b)(j), D the central difference with one-sided ends; b(k) = b[(k - jo) * stride]
     float acc = 0.f;
+    if (j >= 2 && j <= n - 3) {
+        // interior: only the central terms reach j (same operations, same order)
+        acc += 0.5f * b[(j - 1 - jo) * stride];
+        acc -= 0.5f * b[(j + 1 - jo) * stride];
+        return acc;
+    }
     if (j - 1 >= 1 && j - 1 <= n - 2) acc += 0.5f * b[(j - 1 - jo) * stride];
     if (j + 1 >= 1 && j + 1 <= n - 2) acc -= 0.5f * b[(j + 1 - jo) * stride];
     if (j == 1) acc += b[(0 - jo) * stride];
@@ -1240,6 +1246,12 @@ __device__ __forceinline__ float d_transpose(const float *b, int stride, int j, 
 __device__ __forceinline__ float tri_transpose(const float *b, int stride, int m, int n, int jo) {
     // (T^T b)(m) for T the 9-tap triangle with half-sample-symmetric ends (n >= 5)
     float acc = 0.f;
+    if (m >= 4 && m <= n - 5) {
+        // interior: no folded term reaches m (same operations, same order)
+#pragma unroll
+        for (int q = -4; q <= 4; ++q) acc += (float)(5 - (q < 0 ? -q : q)) / 25.f * b[(m + q - jo) * stride];
+        return acc;
+    }
 #pragma unroll
     for (int q = -4; q <= 4; ++q) {
         const float t = (float)(5 - (q < 0 ? -q : q)) / 25.f;
