@@ -160,6 +160,8 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
     const bool est = (J.orient == nullptr);
+    // no pixel of the tile is within two pixels of the map border
+    const bool inner2 = x0 >= 2 && x0 + TW <= W - 2 && y0 >= 2 && y0 + TH <= H - 2;
 
     if (y0 >= 6 && y0 + TH + 6 <= H && x0 >= 6 && x0 + TW + 6 <= W) {
         // interior tile: direct indexing, no border folding
@@ -237,12 +239,20 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
                     return 0.5 * (S(i + 1, j) - S(i - 1, j));
                 };
                 double oxx, oxy, oyy;
+                if (inner2) {
+                    // central differences of central differences: the general branch's
+                    // operations for a pixel two or more pixels inside the map
+                    oxx = 0.5 * (0.5 * (S(gi, gj + 2) - S(gi, gj)) - 0.5 * (S(gi, gj) - S(gi, gj - 2)));
+                    oxy = 0.5 * (0.5 * (S(gi + 1, gj + 1) - S(gi - 1, gj + 1)) - 0.5 * (S(gi + 1, gj - 1) - S(gi - 1, gj - 1)));
+                    oyy = 0.5 * (0.5 * (S(gi + 2, gj) - S(gi, gj)) - 0.5 * (S(gi, gj) - S(gi - 2, gj)));
+                } else {
                 if (gj == 0) { oxx = Ox(gi, 1) - Ox(gi, 0); oxy = Oy(gi, 1) - Oy(gi, 0); }
                 else if (gj == W - 1) { oxx = Ox(gi, W - 1) - Ox(gi, W - 2); oxy = Oy(gi, W - 1) - Oy(gi, W - 2); }
                 else { oxx = 0.5 * (Ox(gi, gj + 1) - Ox(gi, gj - 1)); oxy = 0.5 * (Oy(gi, gj + 1) - Oy(gi, gj - 1)); }
                 if (gi == 0) oyy = Oy(1, gj) - Oy(0, gj);
                 else if (gi == H - 1) oyy = Oy(H - 1, gj) - Oy(H - 2, gj);
                 else oyy = 0.5 * (Oy(gi + 1, gj) - Oy(gi - 1, gj));
+                }
                 const double sgn = (oxy < 0) ? 1.0 : ((oxy > 0) ? -1.0 : 0.0);
                 const double a = oxx + 1e-5, t = oyy * sgn / a;
                 dir_from_slope(t, dx, dy);
