@@ -80,7 +80,11 @@ struct SpJob {
     // differentiable form (null unless enabled on the plan)
     float4 *tq;                 // H x W: dirx, diry, dOxx/dtheta, dOyy/dtheta coefficients
     double2 *dirs;              // H x W: direction of survivors
-    float4 *rec;                // 8 per edgel (by rank): the ten cubic sums, their x and y derivatives
+    float4 *rec;                // 8 per edgel (by rank): the ten cubic sums, their x and y derivatives,
+                                // the edgel's map pixel i W + j (int bits of float 30)
+    float4 *brec;               // 4 per edgel (by rank): the ten sample adjoints, the cubic
+                                // stencil's fraction (fx, fy) and base (int bits of xf, yf)
+    int *gcount;                // kept edgels of the map (device int)
     float *ghx, *ghy, *gth;     // H x W scratch: dL/dHx, dL/dHy, dL/dtheta at edgels
 };
 
@@ -100,6 +104,16 @@ __device__ __forceinline__ SpJob load_job(const SpJob *jobs) {
     return ((const SP_CONST SpJob *)jobs)[blockIdx.y];
 #else
     return jobs[0];
+#endif
+}
+
+// The map's gradient pointers, read like the job (scalar loads; accesses
+// through them compile to global, not flat, instructions).
+__device__ __forceinline__ SpGrad load_grad(const SpGrad *grads) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const SP_CONST SpGrad *)grads)[blockIdx.y];
+#else
+    return grads[0];
 #endif
 }
 
@@ -754,7 +768,8 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
                 r[10 + m] = 0.f;
                 r[20 + m] = 0.f;
             }
-            r[30] = r[31] = 0.f;
+            r[30] = __int_as_float((int)o);
+            r[31] = 0.f;
 #pragma unroll
             for (int sb = 0; sb < 4; ++sb) {
                 const float wx = (float)(wya * keys_d(fx + 1 - sb)), wy = (float)(dwya * wxb[sb]);
@@ -823,6 +838,7 @@ __global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chu
     }
     if (threadIdx.x == 0) {
         if (J.count_out) *J.count_out = carry;
+        if (J.gcount) *J.gcount = carry;
         if (carry > J.capacity && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_CAPACITY);
     }
 }
@@ -937,15 +953,111 @@ __device__ __forceinline__ float keys_f(float s) {
     return 0.f;
 }
 
-// SB1: per tile with kept edgels.  (a) one lane per edgel: from the edgel's
+// SB0: one lane per kept edgel, by rank (grid-stride).  From the edgel's
 // record (the ten cubic sums and their x, y derivatives, kept by the forward
 // pass), its direction and the map values of its NMS stencil, the FP64
-// adjoint of the third-order orientation, the position and the NMS parabola;
+// adjoint of the third-order orientation, the position and the NMS parabola:
 // dL/dE of the parabola goes to HBM (float atomics), dL/dtheta of the NMS is
 // stored at the edgel pixel, and the ten sample adjoints with the cubic
-// stencil base are staged in LDS.  (b) 16 lanes per edgel, lane (a, b) one
-// pixel of the 4 x 4 stencil: the eight convolved-plane adjoints and the raw
-// Hx, Hy sample adjoints are scattered into Z (LDS atomics).  Then the
+// stencil base and fraction go to brec[4 rank] for the tile kernel.
+__global__ __launch_bounds__(NT) void sp_bw_edgel_kernel(const SpJob *jobs, const SpGrad *grads) {
+    const SpJob J = load_job(jobs);
+    const SpGrad Gd = load_grad(grads);
+    const long long n = min((long long)*J.gcount, J.capacity);
+    const int W = J.W;
+    for (long long rank = (long long)blockIdx.x * NT + threadIdx.x; rank < n; rank += (long long)gridDim.x * NT) {
+        const float4 g = Gd.gout[rank];
+        const float4 *rp = J.rec + rank * 8;
+        float r[32];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float4 q = rp[k];
+            r[4 * k] = q.x; r[4 * k + 1] = q.y; r[4 * k + 2] = q.z; r[4 * k + 3] = q.w;
+        }
+        const int o = __float_as_int(r[30]);
+        const int gi = o / W, gj = o - gi * W;
+        float vb[10];
+#pragma unroll
+        for (int m = 0; m < 10; ++m) vb[m] = 0.f;
+        double thb = 0.0;
+        const double2 d = J.dirs[o];
+        const float *Em = J.map + (long long)gi * J.pitch + gj;
+        auto Ev = [&](int di, int dj) -> double { return (double)Em[(long long)di * J.pitch + dj]; };
+        const double md = sqrt(d.x * d.x + d.y * d.y), nx = d.x / md, ny = d.y / md;
+        const int oc = octant_of(d.x, d.y);
+        const int fam = oc & 3;
+        const double slope = fam == 0 ? ny / nx : (fam == 1 ? nx / ny : (fam == 2 ? -nx / ny : -ny / nx));
+        const double Ea = Ev(SP_OFF[oc][0], SP_OFF[oc][1]), Eb = Ev(SP_OFF[oc][2], SP_OFF[oc][3]);
+        const double Ec = Ev(SP_OFF[oc][4], SP_OFF[oc][5]), Ed = Ev(SP_OFF[oc][6], SP_OFF[oc][7]);
+        const double fp = Ea * (1 - slope) + Eb * slope, fm = Ec * (1 - slope) + Ed * slope, f = Ev(0, 0);
+        const double s = sqrt(1 + slope * slope);
+        const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s);
+        const double ss = -B / (2 * A), mf = A * ss * ss + B * ss + f;
+        const double x = gj + ss * nx, y = gi + ss * ny;
+        const float *v = r, *vx = r + 10, *vy = r + 20;
+        const double Fx = (double)v[0] * v[6] + (double)v[2] * v[5] + (double)v[1] * v[7] + (double)v[4] * v[9];
+        const double Fy = (double)v[0] * v[7] + (double)v[4] * v[5] + (double)v[1] * v[8] + (double)v[3] * v[9];
+        const double F2 = Fx * Fx + Fy * Fy;
+        if (!(sqrt(F2) < 1e-12)) {
+            // orientation = atan2(Fx, -Fy)
+            const float Fxb = (float)((double)g.z * (-Fy) / F2), Fyb = (float)((double)g.z * Fx / F2);
+            vb[0] = Fxb * v[6] + Fyb * v[7];
+            vb[6] = Fxb * v[0];
+            vb[2] = Fxb * v[5];
+            vb[5] = Fxb * v[2] + Fyb * v[4];
+            vb[1] = Fxb * v[7] + Fyb * v[8];
+            vb[7] = Fxb * v[1] + Fyb * v[0];
+            vb[4] = Fxb * v[9] + Fyb * v[5];
+            vb[9] = Fxb * v[4] + Fyb * v[3];
+            vb[8] = Fyb * v[1];
+            vb[3] = Fyb * v[9];
+        } else {
+            thb += (double)g.z;   // orientation = atan2(dirx, -diry) = theta + const
+        }
+        double xb = g.x, yb = g.y;
+#pragma unroll
+        for (int m = 0; m < 10; ++m) {
+            xb += (double)vb[m] * vx[m];
+            yb += (double)vb[m] * vy[m];
+        }
+        // parabola and position: x = j + s* nx, y = i + s* ny, strength |f - B^2 / 4A|
+        const double ssb = xb * nx + yb * ny;
+        double nxb = xb * ss, nyb = yb * ss;
+        const double mfb = (double)g.w * (mf < 0 ? -1.0 : 1.0);
+        const double Ab = ssb * (-ss / A) + mfb * ss * ss;
+        const double Bb = ssb * (-1.0 / (2 * A)) + mfb * ss;
+        const double fb = mfb - Ab / (s * s);
+        const double fpb = Ab / (2 * s * s) + Bb / (2 * s), fmb = Ab / (2 * s * s) - Bb / (2 * s);
+        const double sbar = Ab * (-2 * A / s) + Bb * (-B / s);
+        const double slb = sbar * slope / s + fpb * (Eb - Ea) + fmb * (Ed - Ec);
+        // the edgel pixel and the two pixels of fp and of fm
+        float *gm = Gd.gmap + (long long)gi * Gd.gpitch + gj;
+        atomicAdd(gm, (float)fb);
+        atomicAdd(gm + (long long)SP_OFF[oc][0] * Gd.gpitch + SP_OFF[oc][1], (float)(fpb * (1 - slope)));
+        atomicAdd(gm + (long long)SP_OFF[oc][2] * Gd.gpitch + SP_OFF[oc][3], (float)(fpb * slope));
+        atomicAdd(gm + (long long)SP_OFF[oc][4] * Gd.gpitch + SP_OFF[oc][5], (float)(fmb * (1 - slope)));
+        atomicAdd(gm + (long long)SP_OFF[oc][6] * Gd.gpitch + SP_OFF[oc][7], (float)(fmb * slope));
+        if (fam == 0) { nxb += -slb * ny / (nx * nx); nyb += slb / nx; }
+        else if (fam == 1) { nxb += slb / ny; nyb += -slb * nx / (ny * ny); }
+        else if (fam == 2) { nxb += -slb / ny; nyb += slb * nx / (ny * ny); }
+        else { nxb += slb * ny / (nx * nx); nyb += -slb / nx; }
+        thb += -nxb * ny + nyb * nx;    // n = (cos theta, sin theta)
+        J.gth[o] = (float)thb;
+        // cubic stencil base (frozen) and fractional position
+        const int xfs = (int)floor(x), yfs = (int)floor(y);
+        float4 *dst = J.brec + rank * 4;
+        dst[0] = make_float4(vb[0], vb[1], vb[2], vb[3]);
+        dst[1] = make_float4(vb[4], vb[5], vb[6], vb[7]);
+        dst[2] = make_float4(vb[8], vb[9], (float)(x - xfs), (float)(y - yfs));
+        dst[3] = make_float4(__int_as_float(xfs), __int_as_float(yfs), 0.f, 0.f);
+    }
+}
+
+// SB1: per tile with kept edgels.  (a) one lane per edgel: its record from
+// the edgel kernel is staged in LDS, and the Z rows and columns its cubic
+// stencil reaches widen the tile's box.  (b) 16 lanes per edgel, lane (a, b)
+// one pixel of the 4 x 4 stencil: the eight convolved-plane adjoints and the
+// raw Hx, Hy sample adjoints are scattered into Z (LDS atomics).  Then the
 // transposed circular convolutions pull Z back to dL/dE, dL/dHx, dL/dHy over
 // the tile + HALO + R (the raw adjoints join dL/dHx, dL/dHy there), added to
 // HBM with float atomics (periodic wrap).
@@ -967,7 +1079,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     const SpJob J = load_job(jobs);
     const int lt = blockIdx.x;
     if (lt >= J.ntx * J.nty) return;
-    const SpGrad Gd = grads[blockIdx.y];
+    const SpGrad Gd = load_grad(grads);
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
@@ -996,108 +1108,37 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     const int sub = threadIdx.x & 15, sa = sub >> 2, sb = sub & 3;
 #pragma unroll 1
     for (int e0 = 0; e0 < nlist; e0 += NT) {
-        // (a) one lane per edgel
+        // (a) one lane per edgel: its staged record, or nothing past the capacity
         {
             const int e = e0 + threadIdx.x;
-            float vb[10];
-#pragma unroll
-            for (int m = 0; m < 10; ++m) vb[m] = 0.f;
-            float fxs = 0.f, fys = 0.f;
+            float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0;
             int xfs = x0, yfs = y0;
             if (e < nlist) {
                 const int p = elist[e], li = p / TW, lj = p - li * TW;
                 const int gi = y0 + li, gj = x0 + lj;
-                const long long o = (long long)gi * W + gj;
                 const int rank = (int)J.wpre[(long long)gi * J.nw + tx] + __popc(km[li] & ((1u << lj) - 1u));
-                double thb = 0.0;
                 if (rank < J.capacity) {
-                    const float4 g = Gd.gout[rank];
-                    const float4 *rp = J.rec + (long long)rank * 8;
-                    float r[32];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        const float4 q = rp[k];
-                        r[4 * k] = q.x; r[4 * k + 1] = q.y; r[4 * k + 2] = q.z; r[4 * k + 3] = q.w;
-                    }
-                    const double2 d = J.dirs[o];
-                    const float *Em = J.map + (long long)gi * J.pitch + gj;
-                    auto Ev = [&](int di, int dj) -> double { return (double)Em[(long long)di * J.pitch + dj]; };
-                    const double md = sqrt(d.x * d.x + d.y * d.y), nx = d.x / md, ny = d.y / md;
-                    const int oc = octant_of(d.x, d.y);
-                    const int fam = oc & 3;
-                    const double slope = fam == 0 ? ny / nx : (fam == 1 ? nx / ny : (fam == 2 ? -nx / ny : -ny / nx));
-                    const double Ea = Ev(SP_OFF[oc][0], SP_OFF[oc][1]), Eb = Ev(SP_OFF[oc][2], SP_OFF[oc][3]);
-                    const double Ec = Ev(SP_OFF[oc][4], SP_OFF[oc][5]), Ed = Ev(SP_OFF[oc][6], SP_OFF[oc][7]);
-                    const double fp = Ea * (1 - slope) + Eb * slope, fm = Ec * (1 - slope) + Ed * slope, f = Ev(0, 0);
-                    const double s = sqrt(1 + slope * slope);
-                    const double A = (fm + fp - 2 * f) / (2 * s * s), B = (fp - fm) / (2 * s);
-                    const double ss = -B / (2 * A), mf = A * ss * ss + B * ss + f;
-                    const double x = gj + ss * nx, y = gi + ss * ny;
-                    const float *v = r, *vx = r + 10, *vy = r + 20;
-                    const double Fx = (double)v[0] * v[6] + (double)v[2] * v[5] + (double)v[1] * v[7] + (double)v[4] * v[9];
-                    const double Fy = (double)v[0] * v[7] + (double)v[4] * v[5] + (double)v[1] * v[8] + (double)v[3] * v[9];
-                    const double F2 = Fx * Fx + Fy * Fy;
-                    if (!(sqrt(F2) < 1e-12)) {
-                        // orientation = atan2(Fx, -Fy)
-                        const float Fxb = (float)((double)g.z * (-Fy) / F2), Fyb = (float)((double)g.z * Fx / F2);
-                        vb[0] = Fxb * v[6] + Fyb * v[7];
-                        vb[6] = Fxb * v[0];
-                        vb[2] = Fxb * v[5];
-                        vb[5] = Fxb * v[2] + Fyb * v[4];
-                        vb[1] = Fxb * v[7] + Fyb * v[8];
-                        vb[7] = Fxb * v[1] + Fyb * v[0];
-                        vb[4] = Fxb * v[9] + Fyb * v[5];
-                        vb[9] = Fxb * v[4] + Fyb * v[3];
-                        vb[8] = Fyb * v[1];
-                        vb[3] = Fyb * v[9];
-                    } else {
-                        thb += (double)g.z;   // orientation = atan2(dirx, -diry) = theta + const
-                    }
-                    double xb = g.x, yb = g.y;
-#pragma unroll
-                    for (int m = 0; m < 10; ++m) {
-                        xb += (double)vb[m] * vx[m];
-                        yb += (double)vb[m] * vy[m];
-                    }
-                    // parabola and position: x = j + s* nx, y = i + s* ny, strength |f - B^2 / 4A|
-                    const double ssb = xb * nx + yb * ny;
-                    double nxb = xb * ss, nyb = yb * ss;
-                    const double mfb = (double)g.w * (mf < 0 ? -1.0 : 1.0);
-                    const double Ab = ssb * (-ss / A) + mfb * ss * ss;
-                    const double Bb = ssb * (-1.0 / (2 * A)) + mfb * ss;
-                    const double fb = mfb - Ab / (s * s);
-                    const double fpb = Ab / (2 * s * s) + Bb / (2 * s), fmb = Ab / (2 * s * s) - Bb / (2 * s);
-                    const double sbar = Ab * (-2 * A / s) + Bb * (-B / s);
-                    const double slb = sbar * slope / s + fpb * (Eb - Ea) + fmb * (Ed - Ec);
-                    // the edgel pixel and the two pixels of fp and of fm
-                    float *gm = Gd.gmap + (long long)gi * Gd.gpitch + gj;
-                    atomicAdd(gm, (float)fb);
-                    atomicAdd(gm + (long long)SP_OFF[oc][0] * Gd.gpitch + SP_OFF[oc][1], (float)(fpb * (1 - slope)));
-                    atomicAdd(gm + (long long)SP_OFF[oc][2] * Gd.gpitch + SP_OFF[oc][3], (float)(fpb * slope));
-                    atomicAdd(gm + (long long)SP_OFF[oc][4] * Gd.gpitch + SP_OFF[oc][5], (float)(fmb * (1 - slope)));
-                    atomicAdd(gm + (long long)SP_OFF[oc][6] * Gd.gpitch + SP_OFF[oc][7], (float)(fmb * slope));
-                    if (fam == 0) { nxb += -slb * ny / (nx * nx); nyb += slb / nx; }
-                    else if (fam == 1) { nxb += slb / ny; nyb += -slb * nx / (ny * ny); }
-                    else if (fam == 2) { nxb += -slb / ny; nyb += slb * nx / (ny * ny); }
-                    else { nxb += slb * ny / (nx * nx); nyb += -slb / nx; }
-                    thb += -nxb * ny + nyb * nx;    // n = (cos theta, sin theta)
-                    // cubic stencil base (frozen) and fractional position
-                    xfs = (int)floor(x);
-                    yfs = (int)floor(y);
-                    fxs = (float)(x - xfs);
-                    fys = (float)(y - yfs);
+                    const float4 *bp = J.brec + (long long)rank * 4;
+                    q0 = bp[0];
+                    q1 = bp[1];
+                    q2 = bp[2];
+                    const float4 q3 = bp[3];
+                    xfs = __float_as_int(q3.x);
+                    yfs = __float_as_int(q3.y);
                     // the Z rows and columns the stencil reaches (borders clamped)
                     atomicMin(&s_box[0], max(yfs - 1, 0) - (y0 - HALO));
                     atomicMax(&s_box[1], min(yfs + 2, H - 1) - (y0 - HALO));
                     atomicMin(&s_box[2], max(xfs - 1, 0) - (x0 - HALO));
                     atomicMax(&s_box[3], min(xfs + 2, W - 1) - (x0 - HALO));
+                } else {
+                    J.gth[(long long)gi * W + gj] = 0.f;   // not in the output: no gradient
                 }
-                J.gth[o] = (float)thb;
             }
+            const float vb[10] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y};
 #pragma unroll
             for (int m = 0; m < 10; ++m) St[m * NT + threadIdx.x] = vb[m];
-            St[10 * NT + threadIdx.x] = fxs;
-            St[11 * NT + threadIdx.x] = fys;
+            St[10 * NT + threadIdx.x] = q2.z;
+            St[11 * NT + threadIdx.x] = q2.w;
             St[12 * NT + threadIdx.x] = __int_as_float(xfs);
             St[13 * NT + threadIdx.x] = __int_as_float(yfs);
         }
@@ -1280,7 +1321,7 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
     const SpJob J = load_job(jobs);
     const int lt = blockIdx.x;
     if (lt >= J.ntx * J.nty) return;
-    const SpGrad Gd = grads[blockIdx.y];
+    const SpGrad Gd = load_grad(grads);
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
@@ -1418,6 +1459,7 @@ struct toed_subpix {
     bool grad = false, grad_ran = false;
     char *d_gwork = nullptr;
     std::vector<size_t> goff;
+    int eblocks = 0;            // edgel-kernel workgroups per map
     SpGrad *d_grads = nullptr;
 };
 
@@ -1613,8 +1655,9 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
 int toed_subpix_set_grad(toed_subpix *p, int on) {
     if (!p) return TOED_ERR_ARG;
     SPCHK(hipSetDevice(p->device));
-    // per map: tq, dirs (H x W x 16 B), ghx, ghy, gth (H x W x 4 B) and the
-    // edgel records (min(capacity, H W) x 128 B: an edgel per pixel at most)
+    // per map: tq, dirs (H x W x 16 B), ghx, ghy, gth (H x W x 4 B), the edgel
+    // count and the edgel records (min(capacity, H W) x (128 + 64) B: an edgel
+    // per pixel at most)
     auto rec_rows = [](const SpJob &J) { return (size_t)std::min<long long>(J.capacity, (long long)J.H * J.W); };
     if (on && !p->d_gwork) {
         size_t total = 0;
@@ -1624,7 +1667,8 @@ int toed_subpix_set_grad(toed_subpix *p, int on) {
             if (J.H < 5 || J.W < 5) return TOED_ERR_ARG;   // single reflection of the smoothing
             const size_t px = (size_t)J.H * J.W;
             p->goff[m] = total;
-            total += align_up(px * 16) * 2 + align_up(px * 4) * 3 + align_up(rec_rows(J) * 128);
+            total += align_up(px * 16) * 2 + align_up(px * 4) * 3 + align_up(4) + align_up(rec_rows(J) * 128) +
+                     align_up(rec_rows(J) * 64);
         }
         if (hipMalloc(&p->d_gwork, total) != hipSuccess) return TOED_ERR_ALLOC;
         if (hipMalloc(&p->d_grads, sizeof(SpGrad) * p->n) != hipSuccess) return TOED_ERR_ALLOC;
@@ -1632,7 +1676,8 @@ int toed_subpix_set_grad(toed_subpix *p, int on) {
     }
     for (int m = 0; m < p->n; ++m) {
         SpJob &J = p->jobs[m];
-        J.tq = nullptr; J.dirs = nullptr; J.rec = nullptr; J.ghx = J.ghy = J.gth = nullptr;
+        J.tq = nullptr; J.dirs = nullptr; J.rec = nullptr; J.brec = nullptr; J.gcount = nullptr;
+        J.ghx = J.ghy = J.gth = nullptr;
         if (!on) continue;
         const size_t px = (size_t)J.H * J.W;
         char *b = p->d_gwork + p->goff[m];
@@ -1641,7 +1686,13 @@ int toed_subpix_set_grad(toed_subpix *p, int on) {
         J.ghx = (float *)b; b += align_up(px * 4);
         J.ghy = (float *)b; b += align_up(px * 4);
         J.gth = (float *)b; b += align_up(px * 4);
-        J.rec = (float4 *)b;
+        J.gcount = (int *)b; b += align_up(4);
+        J.rec = (float4 *)b; b += align_up(rec_rows(J) * 128);
+        J.brec = (float4 *)b;
+        // the edgel kernel's grid: enough workgroups to fill the device over
+        // all maps, none beyond the largest map's records
+        const long long need = ((long long)rec_rows(J) + NT - 1) / NT;
+        p->eblocks = (int)std::max<long long>(p->eblocks, std::min<long long>(need, std::max(64, 4096 / p->n)));
     }
     p->grad = on != 0;
     p->grad_ran = false;
@@ -1665,6 +1716,8 @@ int toed_subpix_backward(toed_subpix *p, const toed_subpix_grad_t *items) {
     hipStream_t s = (hipStream_t)toed_get_stream(p->ctx);
     SPCHK(hipMemcpyAsync(p->d_grads, g.data(), sizeof(SpGrad) * p->n, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(sp_bw_clear_kernel, dim3(256, p->n), dim3(256), 0, s, p->d_jobs);
+    if (p->eblocks > 0)
+        hipLaunchKernelGGL(sp_bw_edgel_kernel, dim3(p->eblocks, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads);
     const int st = launch_bw_tile_r(s, p);
     if (st) return st;
     hipLaunchKernelGGL(sp_bw_dense_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads);
