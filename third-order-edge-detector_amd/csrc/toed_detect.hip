@@ -228,9 +228,10 @@ __device__ __forceinline__ void row_pass(float *lds, const TOED_CONST float *t0,
             for (int t = 0; t < Gm::KH + 2 * R1; ++t) w[t] = row[t];
 #pragma unroll
             for (int d = 0; d < Gm::KH; ++d) {
-                float a0 = 0.f, a1 = 0.f;
+                // the first term as a product (an fma onto +0 would cost one more op)
+                float a0 = w[d + R1 + R] * t0[R1 - R], a1 = w[d + R1 + R] * t1[R1 - R];
 #pragma unroll
-                for (int q = -R; q <= R; ++q) {
+                for (int q = -R + 1; q <= R; ++q) {
                     const float v = w[d + R1 - q];
                     a0 = fmaf(v, t0[q + R1], a0);
                     a1 = fmaf(v, t1[q + R1], a1);
@@ -275,9 +276,9 @@ __device__ __forceinline__ void col_pass(float *lds, int B,
 #pragma unroll
             for (int d = 0; d < KV; ++d) {
                 const int v = v0 + d;
-                float ix = 0.f, iy = 0.f;
+                float ix = w1[d + R1 + R] * y0[R1 - R], iy = w0[d + R1 + R] * y1[R1 - R];
 #pragma unroll
-                for (int p = -R; p <= R; ++p) {
+                for (int p = -R + 1; p <= R; ++p) {
                     ix = fmaf(w1[d + R1 - p], y0[p + R1], ix);
                     iy = fmaf(w0[d + R1 - p], y1[p + R1], iy);
                 }
