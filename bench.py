@@ -100,7 +100,11 @@ def bench_batch(torch, pkg, scene, steps, warmup, n_images):
     px = float(sum(im.size for im in imgs))
     det = pkg.Detector(0, staging_fraction=0.25)
     caps = [max(1024, (4 * im.size) // 8) for im in imgs]
-    b = pkg.BatchDetector(det, dev, sigmas, capacities=caps, persistent=True)
+    # the project's autotuner picks tile rows and the launch grid on these images
+    tune = det.autotune_batch(dev, sigmas, capacities=caps, reps=3)
+    best = tune["best"]
+    b = pkg.BatchDetector(det, dev, sigmas, capacities=caps, persistent=best["persistent"],
+                          wg_per_cu=best["wg_per_cu"])
     sec_graph = time_loop(torch, lambda: b.run(graph=True), steps, warmup)
     counts = b.counts.cpu().numpy()
     assert not b.flags.cpu().numpy().any()
@@ -121,7 +125,7 @@ def bench_batch(torch, pkg, scene, steps, warmup, n_images):
     b_plain.close()
     det.close()
     return dict(px=px, n_images=len(imgs), sec_graph=sec_graph, sec_batch_plain=sec_batch_plain, sec_loop=sec_loop,
-                edgels=int(counts.sum()), sigmas=sigmas, shapes=[im.shape for im in imgs])
+                edgels=int(counts.sum()), sigmas=sigmas, shapes=[im.shape for im in imgs], tune=tune)
 
 
 def bench_fwd_bwd(torch, pkg, scene, steps, warmup, B=64, H=512, W=512, sigma=1.5):
@@ -297,12 +301,18 @@ def main():
         forms["batch_loop_of_plain"] = {"mpx_s": round(bt["px"] / bt["sec_loop"] / 1e6, 1),
                                         "ms": round(bt["sec_loop"] * 1e3, 3)}
         forms["batch_config"] = {"images": bt["n_images"], "mpx": round(bt["px"] / 1e6, 2), "edgels": bt["edgels"]}
+        tb = bt["tune"]["best"]
+        forms["batch_autotune"] = {"tile_rows": tb["tile_rows"], "persistent": tb["persistent"],
+                                   "wg_per_cu": tb["wg_per_cu"], "waves_per_cu": 4 * tb["wg_per_cu"],
+                                   "ms": round(tb["ms"], 3), "configs_timed": len(bt["tune"]["results"])}
         if "value" not in out:
             out["value"] = forms["batched_graph"]["mpx_s"]
             out["ms_per_step"] = round(bt["sec_graph"] * 1e3, 3)
             out["config"] = {"workload": "batched TOED: 200 mixed-size scenes, mixed sigma", "global_batch": 200}
-        log("batch: graph %.1f Mpx/s, loop-of-plain %.1f Mpx/s" % (forms["batched_graph"]["mpx_s"],
-                                                                    forms["batch_loop_of_plain"]["mpx_s"]))
+        log("batch: graph %.1f Mpx/s, loop-of-plain %.1f Mpx/s; autotuned: %d tile rows, %s (%d configurations)"
+            % (forms["batched_graph"]["mpx_s"], forms["batch_loop_of_plain"]["mpx_s"], tb["tile_rows"],
+               "persistent, %d workgroups per CU" % tb["wg_per_cu"] if tb["persistent"] else "one workgroup per tile",
+               len(bt["tune"]["results"])))
     if args.workload in ("all", "fwdbwd"):
         log("forward+backward 64 x 512x512 ...")
         fb = bench_fwd_bwd(torch, pkg, scene, max(3, args.steps // 4), 1)

@@ -107,6 +107,28 @@ int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags);
 /* Capture the launch sequence once in a hipGraph and replay it. */
 int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags);
 int toed_batch_destroy(toed_batch *b);
+/* Persistent grid size: workgroups per CU for the persistent tile kernel
+ * (0 = as many as fit on a CU, the default).  No reference counterpart. */
+int toed_batch_set_grid(toed_batch *b, int wg_per_cu);
+
+/* Autotuner of the batched form (no reference counterpart: the reference runs
+ * a process per image, run_gpu_batch.sh:152-167).  Plans the batch with 8 and
+ * 16 tile rows (the halo follows from each image's sigma) and times every
+ * launch configuration -- one workgroup per tile, and a persistent grid of 1 ..
+ * (as many as fit) workgroups of 4 waves per CU -- as reps hipGraph replays
+ * on the context's stream; a configuration whose staging overflows is
+ * skipped.  The fastest goes to *best and the context keeps its tile rows for
+ * later plans; every configuration timed goes to results[0 .. *n_results)
+ * (at most max_results; results and n_results may be NULL).  The images'
+ * outputs are overwritten. */
+typedef struct {
+    int tile_rows;           /* 8 or 16 */
+    int persistent;          /* 0: one workgroup per tile; 1: persistent grid */
+    int wg_per_cu;           /* persistent workgroups per CU (0 when not persistent) */
+    float ms;                /* mean time of one replay */
+} toed_tune_t;
+int toed_batch_autotune(toed_ctx *ctx, int n, const toed_image_t *imgs, int reps, toed_tune_t *best,
+                        toed_tune_t *results, int max_results, int *n_results);
 
 /* Differentiable form -------------------------------------------------------
  * No reference counterpart (the reference has no differentiation).  Adjoint

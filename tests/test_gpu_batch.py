@@ -63,6 +63,37 @@ def test_batch_graph_replays(pkg, detector, batch_inputs, oracle):
     b.close()
 
 
+def test_autotune_times_every_config_and_keeps_results(pkg, batch_inputs):
+    """The autotuner times tile rows 8 and 16, each with one workgroup per
+    tile and with persistent grids of 1 .. (what fits) workgroups per CU,
+    keeps the fastest, and a plan built with its choice (and with every
+    persistent grid size) gives the default plan's edgels."""
+    imgs, dev = batch_inputs
+    det = pkg.Detector(0)
+    ref = pkg.BatchDetector(det, dev, SIGMAS)
+    ref.run()
+    base = [r.cpu().numpy().copy() for r in ref.results()]
+    ref.close()
+    tune = det.autotune_batch(dev, SIGMAS, reps=2)
+    res, best = tune["results"], tune["best"]
+    print("autotune:", best, "of", len(res))
+    assert {r["tile_rows"] for r in res} == {8, 16}
+    assert {(r["tile_rows"], r["persistent"]) for r in res} == {(8, False), (8, True), (16, False), (16, True)}
+    assert all(r["ms"] > 0 for r in res) and all((r["wg_per_cu"] > 0) == r["persistent"] for r in res)
+    assert best["ms"] == min(r["ms"] for r in res)
+    grids = sorted({r["wg_per_cu"] for r in res if r["persistent"]})
+    assert grids[0] == 1 and grids == list(range(1, grids[-1] + 1))
+    for w in [best["wg_per_cu"]] + grids:
+        b = pkg.BatchDetector(det, dev, SIGMAS, persistent=best["persistent"] if w == best["wg_per_cu"] else True,
+                              wg_per_cu=w)
+        b.run(graph=True)
+        out = [r.cpu().numpy() for r in b.results()]
+        for m in range(len(dev)):
+            assert np.array_equal(out[m], base[m]), (w, m)
+        b.close()
+    det.close()
+
+
 def test_batch_float_images(pkg, detector, batch_inputs):
     imgs, dev = batch_inputs
     devf = [t.float() for t in dev[:3]]

@@ -113,6 +113,38 @@ class Detector:
     def set_tile_rows(self, tile_rows):
         check(lib().toed_set_tile_rows(self._ctx, int(tile_rows)), "toed_set_tile_rows")
 
+    def autotune_batch(self, images, sigmas, thresh=DEFAULT_THRESH, border=DEFAULT_BORDER, capacities=None,
+                       reps=5, max_results=32):
+        """Time the batched form's launch configurations on these images
+        (``toed_batch_autotune``: tile rows 8 / 16, one workgroup per tile or
+        a persistent grid of 1 .. what fits workgroups per CU) and keep the
+        fastest tile rows on this context.  Returns ``{"best": cfg, "results":
+        [cfg, ...]}``, cfg = dict(tile_rows, persistent, wg_per_cu, ms); pass
+        ``persistent`` and ``wg_per_cu`` of the best to :class:`BatchDetector`."""
+        torch = _torch()
+        images = list(images)
+        n = len(images)
+        if n == 0:
+            raise ValueError("empty batch")
+        sig = list(sigmas) if np.ndim(sigmas) else [float(sigmas)] * n
+        caps = [int(c) for c in (capacities or [4 * int(im.shape[0]) * int(im.shape[1]) for im in images])]
+        offs = np.concatenate([[0], np.cumsum(caps)]).astype(np.int64)
+        edges = torch.empty((int(offs[-1]), 4), dtype=torch.float32, device=images[0].device)
+        arr = (_lib.toed_image_t * n)()
+        for m, im in enumerate(images):
+            arr[m] = _image_struct(im, sig[m], thresh, border, edges[int(offs[m]):], caps[m])
+        best = _lib.toed_tune_t()
+        res = (_lib.toed_tune_t * max_results)()
+        nres = ctypes.c_int(0)
+        self._bind_stream()
+        check(lib().toed_batch_autotune(self._ctx, n, arr, int(reps), ctypes.byref(best), res, int(max_results),
+                                        ctypes.byref(nres)), "toed_batch_autotune")
+
+        def as_dict(t):
+            return dict(tile_rows=t.tile_rows, persistent=bool(t.persistent), wg_per_cu=t.wg_per_cu, ms=float(t.ms))
+
+        return {"best": as_dict(best), "results": [as_dict(res[k]) for k in range(nres.value)]}
+
     def last_times(self):
         a, b = ctypes.c_float(), ctypes.c_float()
         check(lib().toed_last_times(self._ctx, ctypes.byref(a), ctypes.byref(b)), "toed_last_times")
@@ -168,7 +200,7 @@ class BatchDetector:
     start at row ``offsets[m]``; ``counts[m]`` is its edgel count."""
 
     def __init__(self, detector, images, sigmas, thresh=DEFAULT_THRESH, border=DEFAULT_BORDER, capacities=None,
-                 persistent=False, with_index=False):
+                 persistent=False, with_index=False, wg_per_cu=0):
         torch = _torch()
         self.det = detector
         self.images = list(images)
@@ -196,6 +228,8 @@ class BatchDetector:
         detector._batches.add(self)
         if persistent:
             check(lib().toed_batch_set_persistent(self._b, 1), "toed_batch_set_persistent")
+            if wg_per_cu:
+                check(lib().toed_batch_set_grid(self._b, int(wg_per_cu)), "toed_batch_set_grid")
 
     def run(self, graph=False):
         self.det._bind_stream()

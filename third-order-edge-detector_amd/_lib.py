@@ -84,6 +84,15 @@ class toed_subpix_grad_t(ctypes.Structure):
     ]
 
 
+class toed_tune_t(ctypes.Structure):
+    _fields_ = [
+        ("tile_rows", ctypes.c_int),
+        ("persistent", ctypes.c_int),
+        ("wg_per_cu", ctypes.c_int),
+        ("ms", ctypes.c_float),
+    ]
+
+
 # exported symbols and their signatures: (name, restype, argtypes)
 _VP, _I, _F, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
 _PI, _PF = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
@@ -108,6 +117,9 @@ SIGNATURES = [
     ("toed_batch_run", _I, [_VP, _VP, _VP]),
     ("toed_batch_graph_run", _I, [_VP, _VP, _VP]),
     ("toed_batch_destroy", _I, [_VP]),
+    ("toed_batch_set_grid", _I, [_VP, _I]),
+    ("toed_batch_autotune", _I, [_VP, _I, ctypes.POINTER(toed_image_t), _I, ctypes.POINTER(toed_tune_t),
+                                 ctypes.POINTER(toed_tune_t), _I, _PI]),
     ("toed_subpix_radius", _I, [ctypes.c_double, _PI]),
     ("toed_subpix_plan", _I, [_VP, _I, ctypes.POINTER(toed_subpix_map_t), ctypes.c_double, ctypes.c_double,
                               ctypes.POINTER(_VP)]),

@@ -1009,6 +1009,7 @@ using namespace toed;
 // tiles of the images with one window radius (contiguous in the tile order)
 struct TileGroup {
     int R1 = 0, tile_begin = 0, tile_end = 0, grid_persistent = 0;
+    int occ = 1;                // tile-kernel workgroups that fit on one CU
 };
 
 struct toed_ctx {
@@ -1040,6 +1041,7 @@ struct toed_batch {
     uint32_t *d_skey = nullptr;
     long long ntiles = 0, ncounts = 0, nchunks = 0, nstaging = 0;
     int in_type = 0, tile_rows = 16, persistent = 0;
+    int dev_cu = 1, wg_per_cu = 0;     // persistent grid: workgroups per CU (0: as many as fit)
     std::vector<int> order;
     std::vector<TileGroup> groups;
     GradOut *d_gouts = nullptr;
@@ -1326,6 +1328,15 @@ int toed_detect_sync(toed_ctx *c, const toed_image_t *im, long long *n_out) {
     return TOED_OK;
 }
 
+// persistent grid of each radius group: wg_per_cu workgroups on every CU
+// (at most what fits), never more than the group's tiles
+static void set_grids(toed_batch *b) {
+    for (TileGroup &g : b->groups) {
+        const int per_cu = b->wg_per_cu > 0 ? std::min(b->wg_per_cu, g.occ) : g.occ;
+        g.grid_persistent = std::max(1, std::min(b->dev_cu * per_cu, g.tile_end - g.tile_begin));
+    }
+}
+
 int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **out) {
     if (!c || n <= 0 || !imgs || !out) return TOED_ERR_ARG;
     *out = nullptr;
@@ -1404,8 +1415,10 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
                     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, toed_tiles_kernel<float, T, R>, NT, 0);
             });
         });
-        g.grid_persistent = std::max(1, std::min(dev_cu * std::max(occ, 1), g.tile_end - g.tile_begin));
+        g.occ = std::max(occ, 1);
     }
+    b->dev_cu = std::max(dev_cu, 1);
+    set_grids(b);
     *out = b;
     return TOED_OK;
 }
@@ -1413,6 +1426,15 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
 int toed_batch_set_persistent(toed_batch *b, int p) {
     if (!b) return TOED_ERR_ARG;
     b->persistent = p ? 1 : 0;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    return TOED_OK;
+}
+
+int toed_batch_set_grid(toed_batch *b, int wg_per_cu) {
+    if (!b || wg_per_cu < 0) return TOED_ERR_ARG;
+    b->wg_per_cu = wg_per_cu;
+    set_grids(b);
     if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     return TOED_OK;
@@ -1473,6 +1495,78 @@ int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags) {
         b->g_counts = d_counts; b->g_flags = d_flags;
     }
     HIPCHK(hipGraphLaunch(b->gexec, s));
+    return TOED_OK;
+}
+
+// Times one configuration of a plan: a graph replay to capture and warm up,
+// then reps replays between two events on the replay stream.  A run whose
+// staging overflows (flags) is not a candidate: *ms = -1.
+static int time_config(toed_batch *b, int reps, int *d_cf, hipEvent_t e0, hipEvent_t e1, float *ms) {
+    *ms = -1.f;
+    hipStream_t s = b->ctx->stream ? b->ctx->stream : b->ctx->own_stream;
+    int st = toed_batch_graph_run(b, d_cf, d_cf + b->n);
+    if (st) return st;
+    std::vector<int> fl(b->n);
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(fl.data(), d_cf + b->n, sizeof(int) * b->n, hipMemcpyDeviceToHost));
+    for (int f : fl)
+        if (f & TOED_FLAG_STAGING) return TOED_OK;
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+        st = toed_batch_graph_run(b, d_cf, d_cf + b->n);
+        if (st) return st;
+    }
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    return TOED_OK;
+}
+
+int toed_batch_autotune(toed_ctx *c, int n, const toed_image_t *imgs, int reps, toed_tune_t *best,
+                        toed_tune_t *results, int max_results, int *n_results) {
+    if (!c || n <= 0 || !imgs || !best || reps <= 0 || max_results < 0) return TOED_ERR_ARG;
+    if (n_results) *n_results = 0;
+    HIPCHK(hipSetDevice(c->device));
+    int *d_cf = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipMalloc(&d_cf, sizeof(int) * 2 * n) != hipSuccess) return TOED_ERR_ALLOC;
+    const int saved_rows = c->tile_rows;
+    int st = (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) ? TOED_OK : TOED_ERR_HIP;
+    toed_tune_t top{0, 0, 0, -1.f};
+    int nres = 0;
+    const int rows[2] = {8, 16};
+    for (int k = 0; k < 2 && !st; ++k) {
+        c->tile_rows = rows[k];
+        toed_batch *b = nullptr;
+        st = toed_batch_plan(c, n, imgs, &b);
+        if (st) break;
+        int occ = 1;
+        for (const TileGroup &g : b->groups) occ = std::max(occ, g.occ);
+        // one workgroup per tile, then persistent grids of 1 .. occ workgroups per CU
+        for (int w = -1; w <= occ && !st; ++w) {
+            if (w == 0) continue;
+            st = toed_batch_set_persistent(b, w > 0);
+            if (!st) st = toed_batch_set_grid(b, w > 0 ? w : 0);
+            float ms = -1.f;
+            if (!st) st = time_config(b, reps, d_cf, e0, e1, &ms);
+            if (st || ms < 0.f) continue;
+            const toed_tune_t r{rows[k], w > 0 ? 1 : 0, w > 0 ? w : 0, ms};
+            if (results && nres < max_results) results[nres] = r;
+            ++nres;
+            if (top.ms < 0.f || ms < top.ms) top = r;
+        }
+        (void)toed_batch_destroy(b);
+    }
+    c->tile_rows = (!st && top.ms >= 0.f) ? top.tile_rows : saved_rows;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_cf);
+    if (st) return st;
+    if (top.ms < 0.f) return TOED_ERR_STAGING;   // every configuration overflowed its staging
+    *best = top;
+    if (n_results) *n_results = std::min(nres, max_results);
     return TOED_OK;
 }
 
