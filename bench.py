@@ -72,6 +72,7 @@ def bench_plain(torch, pkg, scene, steps, warmup):
     sec = time_loop(torch, lambda: det.detect_into(img, edges, counts), steps, warmup)
     n, flags = (int(v) for v in counts.cpu())
     assert flags == 0, flags
+    edges_np = edges[:n].cpu().numpy()
     # per-kernel durations (HIP events on the library stream), synchronous form
     tiles, fin = [], []
     t0 = time.perf_counter()
@@ -83,7 +84,7 @@ def bench_plain(torch, pkg, scene, steps, warmup):
     sync_sec = (time.perf_counter() - t0) / steps
     det.close()
     return dict(H=H, W=W, sec=sec, n=n, ms_tiles=float(np.mean(tiles)), ms_final=float(np.mean(fin)),
-                sync_sec=sync_sec, img_np=img_np)
+                sync_sec=sync_sec, img_np=img_np, edges_np=edges_np)
 
 
 def batch_images(scene, n=200, seed=2024):
@@ -169,6 +170,7 @@ def bench_subpix(torch, pkg, scene, steps, warmup, B=32, S=2048):
     sec = time_loop(torch, sp.run, steps, warmup)
     counts = sp.counts.cpu().numpy()
     assert not sp.flags.cpu().numpy().any()
+    edges0 = sp.results()[0].cpu().numpy()
     sp.close()
     xg = x.clone().requires_grad_(True)
     w = torch.tensor(scene.loss_weights(12345), dtype=torch.float32, device="cuda")
@@ -186,43 +188,73 @@ def bench_subpix(torch, pkg, scene, steps, warmup, B=32, S=2048):
     t_fb = time_loop(torch, fwd_bwd, steps, warmup)
     map0 = x[0].cpu().numpy()
     det.close()
-    return dict(px=float(B * S * S), sec=sec, edgels=int(counts.sum()), map0=map0, B=B, S=S, t_f=t_f, t_fb=t_fb)
+    return dict(px=float(B * S * S), sec=sec, edgels=int(counts.sum()), map0=map0, edges0=edges0, B=B, S=S,
+                t_f=t_f, t_fb=t_fb)
 
 
 CPU_SAMPLE_SEC = 10.0   # CPU baseline: run the sample until this much measured time
 CPU_MAX_RUNS = 25
 
 
-def cpu_baseline_subpix(map_np):
+def cpu_baseline_subpix(map_np, gpu_edges=None):
     """The reference's subpix_to_correction.cpp (FP64, OpenMP) on one map of the
-    subpix workload; the oracle port when the reference harness is not built."""
+    subpix workload; the oracle port when the reference harness is not built.
+    Its first run's list is also the checker of the GPU list of that map."""
     import oracle
     cores = max(1, min(16, os.cpu_count() or 1))
     H, W = map_np.shape
     sec, runs = 0.0, 0
+    first = None
     while runs < CPU_MAX_RUNS and sec < CPU_SAMPLE_SEC:   # a bounded sample: about 10 s of CPU work
         if oracle.subpix_ref_available():
             r = oracle.run_subpix_ref(map_np.astype(np.float64), None, 0.1, 2.0, 2.0, nthreads=cores)
-            sec, kind = sec + r["times"][3], "reference"
+            sec, kind, edges = sec + r["times"][3], "reference", r["edges"]
         else:
             t0 = time.perf_counter()
-            oracle.subpix(map_np.astype(np.float64), None, 0.1, 2.0, 2.0, nthreads=cores)
+            edges = oracle.subpix(map_np.astype(np.float64), None, 0.1, 2.0, 2.0, nthreads=cores)["edges"]
             sec, kind = sec + time.perf_counter() - t0, "port"
+        if first is None:
+            first = edges
         runs += 1
-    return {"value": round(runs * H * W / sec / 1e6, 4), "unit": "Mpixel/s", "cores": cores, "kind": kind,
-            "sample": "%d runs on one %dx%d Bezier map, FP64, orientation estimate through the edgel list "
-                      "(%.2f s in total)" % (runs, H, W, sec)}
+    res = {"value": round(runs * H * W / sec / 1e6, 4), "unit": "Mpixel/s", "cores": cores, "kind": kind,
+           "sample": "%d runs on one %dx%d Bezier map, FP64, orientation estimate through the edgel list "
+                     "(%.2f s in total)" % (runs, H, W, sec)}
+    return res, (accuracy(gpu_edges, first) if gpu_edges is not None else None)
 
 
-def cpu_baseline(img_np):
+def accuracy(gpu, ref):
+    """Errors of the GPU edgel list against the FP64 CPU list of the same frame:
+    edgels matched by position (nearest within 1e-2 px), then the largest
+    position, orientation and relative strength differences over the matches."""
+    from scipy.spatial import cKDTree
+    ref = np.asarray(ref, np.float64)
+    gpu = np.asarray(gpu, np.float64)
+    out = {"edgels_gpu": int(len(gpu)), "edgels_cpu": int(len(ref))}
+    if len(gpu) == 0 or len(ref) == 0:
+        return out
+    d, j = cKDTree(ref[:, :2]).query(gpu[:, :2], distance_upper_bound=1e-2)
+    ok = np.isfinite(d)
+    g, r = gpu[ok], ref[j[ok]]
+    dth = np.abs((g[:, 2] - r[:, 2] + np.pi) % (2 * np.pi) - np.pi)
+    out.update({"matched": int(ok.sum()),
+                "max_pos_err_px": float(np.abs(g[:, :2] - r[:, :2]).max()),
+                "max_orient_err_rad": float(dth.max()),
+                "max_strength_rel_err": float((np.abs(g[:, 3] - r[:, 3]) / np.maximum(np.abs(r[:, 3]), 1e-12)).max()),
+                "tolerances": "1e-3 px, 1e-3 rad, 1e-4 relative"})
+    return out
+
+
+def cpu_baseline(img_np, gpu_edges=None):
     """The reference CPU path (cpu_toed.cpp, FP64, OpenMP) on one frame of the
     same workload, on this host's cores; the oracle port when the reference
-    harness is not built."""
+    harness is not built.  Its first run's list is also the checker of the GPU
+    list of the same frame (``accuracy``)."""
     import oracle
     cores = max(1, min(16, os.cpu_count() or 1))
     H, W = img_np.shape
     sec = wall = 0.0
     runs = 0
+    first = None
     while runs < CPU_MAX_RUNS and sec < CPU_SAMPLE_SEC:   # a bounded sample: about 10 s of CPU work
         t0 = time.perf_counter()
         if oracle.ref_available():
@@ -230,15 +262,19 @@ def cpu_baseline(img_np):
             kind = "reference"
             sec += tc + tn
         else:
-            oracle.detect(img_np.astype(np.float64), 2.0, nthreads=cores)
+            edges = oracle.detect(img_np.astype(np.float64), 2.0, nthreads=cores)["edges"]
             kind = "port"
             sec += time.perf_counter() - t0
         wall += time.perf_counter() - t0
+        if first is None:
+            first = edges
         runs += 1
-    return {"value": round(runs * H * W / sec / 1e6, 4), "unit": "Mpixel/s", "cores": cores, "kind": kind,
-            "sample": "%d runs on one %dx%d frame of the same workload, FP64, convolution + NMS timed at the "
-                      "reference's omp_get_wtime boundaries (%.2f s in total; %.2f s wall incl. list build and I/O)"
-                      % (runs, H, W, sec, wall)}
+    res = {"value": round(runs * H * W / sec / 1e6, 4), "unit": "Mpixel/s", "cores": cores, "kind": kind,
+           "sample": "%d runs on one %dx%d frame of the same workload, FP64, convolution + NMS timed at the "
+                     "reference's omp_get_wtime boundaries (%.2f s in total; %.2f s wall incl. list build and I/O)"
+                     % (runs, H, W, sec, wall)}
+    acc = accuracy(gpu_edges, first) if gpu_edges is not None else None
+    return res, acc
 
 
 def main():
@@ -338,7 +374,9 @@ def main():
                                                 "ratio_to_plain_fwd": round(sp["t_fb"] / sp["sec"], 3)}
         if not args.no_cpu:
             log("subpix cpu baseline ...")
-            forms["subpix_32x2048x2048"]["cpu_baseline"] = cpu_baseline_subpix(sp["map0"])
+            forms["subpix_32x2048x2048"]["cpu_baseline"], acc = cpu_baseline_subpix(sp["map0"], sp["edges0"])
+            forms["subpix_32x2048x2048"]["accuracy_vs_cpu_fp64"] = acc
+            log("subpix accuracy vs the FP64 CPU path (map 0): %s" % json.dumps(acc))
         if "value" not in out:
             out["value"] = forms["subpix_32x2048x2048"]["mpx_s"]
             out["ms_per_step"] = round(sp["sec"] * 1e3, 3)
@@ -348,7 +386,9 @@ def main():
     out["forms"] = forms
     if plain is not None and not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(plain["img_np"])
+        out["cpu_baseline"], acc = cpu_baseline(plain["img_np"], plain["edges_np"])
+        out["accuracy_vs_cpu_fp64"] = acc
+        log("accuracy vs the FP64 CPU path: %s" % json.dumps(acc))
     print(json.dumps(out))
 
 
