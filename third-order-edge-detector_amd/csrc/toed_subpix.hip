@@ -1316,7 +1316,7 @@ __device__ __forceinline__ float tri_transpose(const float *b, int stride, int m
     return acc;
 }
 
-__global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, const SpGrad *grads) {
+__global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, const SpGrad *grads, int R) {
     __shared__ float Ox[DR][DC], Oy[DR][DC], T1[DR][DC], T2[DR][DC];
     const SpJob J = load_job(jobs);
     const int lt = blockIdx.x;
@@ -1326,6 +1326,28 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
     const int ry0 = y0 - 6, rx0 = x0 - 6;
+    {
+        // Every input of this tile is zero unless a kept edgel lies within
+        // 6 + HALO + R pixels of it: dL/dHx, dL/dHy reach HALO + R from an
+        // edgel's pixel (cubic stencil, transposed filters), dL/dtheta of the
+        // NMS sits at the edgel, and the transposed differences and smoothing
+        // read 6 more.  Such interior tiles leave dL/dE unchanged; tiles near
+        // the border always run (the filters wrap around the map).
+        const int reach = 6 + HALO + R;
+        if (y0 - reach >= 0 && y0 + TH + reach <= H && x0 - reach >= 0 && x0 + TW + reach <= W) {
+            const int r0 = y0 - reach, nr = TH + 2 * reach;
+            const int c0 = x0 - reach, c1 = x0 + TW - 1 + reach;
+            const int w0 = c0 >> 5, nwd = (c1 >> 5) - w0 + 1;
+            int hit = 0;
+            for (int k = threadIdx.x; k < nr * nwd; k += NT) {
+                const int r = r0 + k / nwd, wi = w0 + k % nwd;
+                const int lo = max(c0 - 32 * wi, 0), hi = min(c1 - 32 * wi, 31);
+                const uint32_t keep = (hi == 31 ? 0xffffffffu : ((1u << (hi + 1)) - 1u)) & ~((1u << lo) - 1u);
+                hit |= (J.kmask[(long long)r * J.nw + wi] & keep) != 0u;
+            }
+            if (!__syncthreads_or(hit)) return;
+        }
+    }
     {
         // all of a thread's region loads are issued before any is used
         constexpr int NU = (DR * DC + NT - 1) / NT;
@@ -1720,7 +1742,8 @@ int toed_subpix_backward(toed_subpix *p, const toed_subpix_grad_t *items) {
         hipLaunchKernelGGL(sp_bw_edgel_kernel, dim3(p->eblocks, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads);
     const int st = launch_bw_tile_r(s, p);
     if (st) return st;
-    hipLaunchKernelGGL(sp_bw_dense_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads);
+    hipLaunchKernelGGL(sp_bw_dense_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads,
+                       p->taps.R);
     SPCHK(hipGetLastError());
     return TOED_OK;
 }
