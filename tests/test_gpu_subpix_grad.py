@@ -119,3 +119,32 @@ def test_subpix_grad_batch_and_autograd(pkg, detector, oracle, scene):
         l1 = float(np.abs(gx[m] * d).sum())
         print("autograd map %d: fd %.8g gpu %.8g |diff|/sum|g d| %.2e" % (m, fd, an, abs(fd - an) / l1), flush=True)
         assert abs(fd - an) <= TOL * l1
+
+
+def test_subpix_plan_rerun_on_sparser_map(pkg, detector, oracle, scene):
+    """A plan's work planes and masks carry nothing from one run to the next:
+    the same plan run on a dense map and then on a sparse one (most of whose
+    tiles have no edgel, and whose dense backward skips them) gives the
+    sparse map's own edgels and gradient (against the oracle), and the
+    forward agrees with a fresh plan's."""
+    dense = scene.bezier_map(81, 320, 384, 40)
+    sparse = scene.bezier_map(82, 320, 384, 2)
+    b, dev = _forward(pkg, detector, [dense])
+    b.results()
+    dev[0].copy_(torch.from_numpy(sparse))
+    b.run()
+    e = b.results()[0].cpu().numpy()
+    idx = b.index_results()[0].cpu().numpy().astype(np.int64)
+    ref = oracle.subpix(sparse.astype(np.float64), None, 0.1, 2.0, 2.0)
+    assert np.array_equal(idx, ref["idx"]) and len(idx) > 20
+    fresh, fdev = _forward(pkg, detector, [sparse])
+    assert np.array_equal(fresh.results()[0].cpu().numpy(), e)
+    fresh.close()
+    rng = np.random.default_rng(11)
+    G = rng.standard_normal((len(idx), 4)).astype(np.float32)
+    gmap = torch.zeros_like(dev[0])
+    b.backward([torch.from_numpy(G).cuda()], [gmap])
+    g = gmap.cpu().numpy().astype(np.float64)
+    assert np.isfinite(g).all()
+    _check_map(oracle, sparse.astype(np.float64), idx, G.astype(np.float64), g, 2.0, rng, "rerun 320x384")
+    b.close()
