@@ -24,7 +24,7 @@
 //        a per-tile, per-phase staging area.
 //   K2 toed_scan_chunks / toed_scan_images   exclusive scan of the counts
 //        (row-major over (interp row, tile column)), segmented per image.
-//   K3 toed_finalize_kernel   one workgroup per tile, phases processed in turn
+//   K3 toed_finalize_kernel   one wavefront per tile, phases processed in turn
 //        so the filter taps are wavefront-uniform: FP64 recomputation of the
 //        nine responses at each edgel, third-order orientation, write of the
 //        edgel at offset[row, tile] + rank (raster order).
@@ -764,32 +764,36 @@ __global__ __launch_bounds__(1024) void toed_scan_images(const ImageJob *jobs, i
 }
 
 // K3: scatter every staged edgel to offset[row, tile column] + rank, the
-// reference's raster order of the interp grid (cpu_toed.cpp:510-538).
+// reference's raster order of the interp grid (cpu_toed.cpp:510-538).  One
+// wavefront per tile (a tile stages a handful of edgels per phase).
 __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0, int TH,
                                                            const int *offsets, const int *chunk_pref,
                                                            StageBuf stg, const int *tile_cnt, int ntiles,
                                                            int *ticket) {
-    const int tile = blockIdx.x;
-    const ImageJob J = load_job(jobs, njobs, job0, tile);
-    const long long tile_local = tile - J.tile_begin;
-    const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
-    const int IW = 2 * J.W;
-    for (int phase = 0; phase < 4; ++phase) {
-        const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
-        const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
-        for (int e = threadIdx.x; e < n; e += NT) {
-            const uint32_t key = stg.key[base + e];
-            const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
-            const int R = 2 * tr * TH + r, C = 2 * tc * TW + c;
-            const long long ce = J.counts_base + (long long)R * J.ntc + tc;
-            const long long idx = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
-            if (idx >= J.capacity) continue;
-            reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[base + e];
-            if (J.index_out) J.index_out[idx] = R * IW + C;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (tile < ntiles) {
+        const ImageJob J = load_job(jobs, njobs, job0, tile);
+        const long long tile_local = tile - J.tile_begin;
+        const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
+        const int IW = 2 * J.W;
+        for (int phase = 0; phase < 4; ++phase) {
+            const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
+            const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
+            for (int e = lane; e < n; e += 64) {
+                const uint32_t key = stg.key[base + e];
+                const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
+                const int R = 2 * tr * TH + r, C = 2 * tc * TW + c;
+                const long long ce = J.counts_base + (long long)R * J.ntc + tc;
+                const long long idx = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
+                if (idx >= J.capacity) continue;
+                reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[base + e];
+                if (J.index_out) J.index_out[idx] = R * IW + C;
+            }
         }
     }
     // the persistent tile kernels' tickets are re-armed once every tile is final
-    if (ticket && tile == ntiles - 1 && threadIdx.x < NGROUPS) ticket[threadIdx.x] = 0;
+    if (ticket && blockIdx.x == gridDim.x - 1 && threadIdx.x < NGROUPS) ticket[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1182,8 +1186,9 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
     hipLaunchKernelGGL(toed_scan_chunks, dim3((unsigned)nchunks), dim3(256), 0, s, d_jobs, njobs, job0, counts,
                        offsets, chunks);
     hipLaunchKernelGGL(toed_scan_images, dim3(njobs), dim3(1024), 0, s, d_jobs, njobs, job0, chunks);
-    hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs, njobs, job0, TH,
-                       offsets, chunks, staging, tilecnt, (int)ntiles, tickets);
+    constexpr int TPB = NT / 64;    // finalize: one wavefront per tile
+    hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)((ntiles + TPB - 1) / TPB)), dim3(NT), 0, s, d_jobs,
+                       njobs, job0, TH, offsets, chunks, staging, tilecnt, (int)ntiles, tickets);
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     HIPCHK(hipGetLastError());
     return TOED_OK;
