@@ -64,6 +64,18 @@ typedef struct {
                                 each edgel, capacity ints (no reference counterpart) */
 } toed_image_t;
 
+/* Image files ---------------------------------------------------------------
+ * The reference reads its input in main_gpu_cpu.cpp:114-132 (cv::imread, or
+ * `>>` on the PGM header and one get() per pixel, gpu_toed.hpp:189-198).
+ * toed_image_load decodes an 8-bit PGM (P5, maxval <= 255) or a PNG (8/16-bit
+ * grey, grey + alpha, RGB, RGBA, not interlaced; alpha dropped, 16-bit samples
+ * keep their high byte, colour to grey with BT.601 weights) into an H x W
+ * buffer the caller releases with toed_image_free.  pgm_raw_offset = 1 reads a
+ * PGM as the reference's no-OpenCV path does: its `>>` leaves the whitespace
+ * byte after the header in the stream, so pixel k is the file's byte k - 1. */
+int toed_image_load(const char *path, int pgm_raw_offset, unsigned char **pixels, int *H, int *W);
+void toed_image_free(unsigned char *pixels);
+
 /* Library / device queries -------------------------------------------------- */
 int toed_version(void);
 /* sigma -> window radii as used by the kernels; TOED_ERR_SIGMA if unsupported */

@@ -1,0 +1,73 @@
+"""The command-line drop-in (toed_detect_cli) on the GPU: the reference's argv
+and output file (main_gpu_cpu.cpp:103-153, gpu_toed.hpp:452-478), and the
+batch mode that replaces run_gpu_batch.sh's process per image; both against
+the library's own single-image detection."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_image_io import _pgm, _png
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "third-order-edge-detector_amd",
+                   "toed_detect_cli")
+
+
+def _read_list(path):
+    rows = [l.rstrip("\n").split("\t") for l in open(path) if l.strip()]
+    for r in rows:
+        assert r[-1] == "" and len(r) == 5        # a tab after every value
+    return np.array([[float(v) for v in r[:4]] for r in rows], np.float64).reshape(-1, 4)
+
+
+def _run(args, timeout=120):
+    print("cli:", " ".join(args), flush=True)
+    return subprocess.run([CLI] + args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                          timeout=timeout)
+
+
+def test_single_image_reference_argv(pkg, detector, scene, tmp_path):
+    assert os.path.exists(CLI), "toed_detect_cli is not built"
+    img = scene.shapes_scene(321, 240, 320)
+    p = str(tmp_path / "scene.pgm")
+    _pgm(p, img)
+    out_dir = str(tmp_path / "out" / "nested")
+    r = _run([p, "4", "0", out_dir])
+    print(r.stdout)
+    assert r.returncode == 0
+    got = _read_list(os.path.join(out_dir, "data_final_output_gpu.txt"))
+    want = detector.detect(torch.from_numpy(img).cuda()).cpu().numpy().astype(np.float64)
+    assert "Number of GPU edges = %d" % len(want) in r.stdout
+    assert got.shape == want.shape and len(want) > 100
+    # precision 10 text of FP32 values
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
+
+
+def test_missing_file_and_usage(tmp_path):
+    r = _run([str(tmp_path / "nope.pgm")])
+    assert r.returncode == 0 and "not found" in r.stdout     # main_gpu_cpu.cpp:114-120
+    r = _run([])
+    assert r.returncode == 0 and "Usage" in r.stdout
+
+
+def test_batch_mode_matches_single_images(pkg, detector, scene, tmp_path):
+    imgs = {"a_scene": scene.shapes_scene(401, 200, 300), "b_scene": scene.shapes_scene(402, 480, 640),
+            "c_scene": scene.shapes_scene(403, 150, 100)}
+    paths = []
+    for k, (name, im) in enumerate(imgs.items()):
+        p = str(tmp_path / (name + (".png" if k == 1 else ".pgm")))
+        _png(p, im, 0) if k == 1 else _pgm(p, im)
+        paths.append(p)
+    out_dir = str(tmp_path / "batch_out")
+    r = _run(["--batch", "--out", out_dir, "--autotune"] + paths)
+    print(r.stdout)
+    assert r.returncode == 0
+    for name, im in imgs.items():
+        got = _read_list(os.path.join(out_dir, name + "_edges_gpu.txt"))
+        want = detector.detect(torch.from_numpy(im).cuda()).cpu().numpy().astype(np.float64)
+        assert got.shape == want.shape and len(want) > 10, name
+        np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)

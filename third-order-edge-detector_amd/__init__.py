@@ -28,7 +28,7 @@ from . import _lib
 from ._lib import FLAG_CAPACITY, FLAG_STAGING, TOED_F32, TOED_U8, ToedError, check, lib
 
 __all__ = ["Detector", "BatchDetector", "ThirdOrderEdgeDetectionGPU", "ToedError", "sigma_radius",
-           "detect_differentiable", "DEFAULT_SIGMA", "DEFAULT_THRESH", "DEFAULT_BORDER", "SubpixBatch",
+           "detect_differentiable", "load_image", "DEFAULT_SIGMA", "DEFAULT_THRESH", "DEFAULT_BORDER", "SubpixBatch",
            "SubpixelTOCorrectionGPU", "subpix_radius", "subpix_differentiable"]
 
 # reference constants: sigma = 2 / 17-tap window (main_gpu_cpu.cpp:158-159),
@@ -76,6 +76,22 @@ def _image_struct(img, sigma, thresh, border, edges, capacity, index=None):
     s.capacity = int(capacity)
     s.d_index = index.data_ptr() if index is not None else None
     return s
+
+
+def load_image(path, pgm_raw_offset=False):
+    """Read an 8-bit PGM (P5) or PNG into a uint8 H x W numpy array with the
+    library's decoder (``toed_image_load``; host code, no GPU needed).  With
+    ``pgm_raw_offset`` a PGM is read as the reference's no-OpenCV path reads
+    it (main_gpu_cpu.cpp:131 leaves the byte after the header unread, so
+    pixel k is the file's byte k - 1)."""
+    px = ctypes.POINTER(ctypes.c_ubyte)()
+    H, W = ctypes.c_int(), ctypes.c_int()
+    check(lib().toed_image_load(os.fsencode(path), 1 if pgm_raw_offset else 0, ctypes.byref(px), ctypes.byref(H),
+                                ctypes.byref(W)), "toed_image_load(%s)" % path)
+    try:
+        return np.ctypeslib.as_array(px, shape=(H.value, W.value)).copy()
+    finally:
+        lib().toed_image_free(px)
 
 
 class Detector:

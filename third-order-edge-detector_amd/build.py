@@ -2,6 +2,8 @@
 
     libtoed_hip.so    HIP kernels + C ABI (include/toed_capi.h), gfx950 only
     libtoed_scene.so  deterministic synthetic-input generators (host C)
+    toed_detect_cli   command-line drop-in for the reference's binary and batch
+                      script (host C++ over the C ABI)
 
 Run:  python third-order-edge-detector_amd/build.py
 """
@@ -23,6 +25,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "
 HIP_LIBS = {
     "libtoed_hip.so": ["toed_detect.hip", "toed_backward.hip", "toed_subpix.hip"],
 }
+HOST_SRCS = {"libtoed_hip.so": ["toed_image.cpp"]}   # host-only parts of the same library
+HOST_LIBS = ["-lz"]
 
 
 def _run(cmd):
@@ -37,11 +41,27 @@ def build_hip(verbose=False):
     for lib, srcs in HIP_LIBS.items():
         out = os.path.join(PKG, lib)
         tmp = out + ".tmp"
-        cmd = [HIPCC] + HIP_FLAGS + ["-I" + INC] + [os.path.join(CSRC, s) for s in srcs] + ["-o", tmp]
+        objs = []
+        for hs in HOST_SRCS.get(lib, []):
+            obj = os.path.join(PKG, os.path.splitext(hs)[0] + ".host.o")
+            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-I" + INC, "-c", os.path.join(CSRC, hs), "-o", obj])
+            objs.append(obj)
+        # "-x none": the host objects are linker inputs, not HIP sources
+        cmd = ([HIPCC] + HIP_FLAGS + ["-I" + INC] + [os.path.join(CSRC, s) for s in srcs] + ["-x", "none"] + objs +
+               HOST_LIBS + ["-o", tmp])
         log = _run(cmd)
         os.replace(tmp, out)
+        for obj in objs:
+            os.remove(obj)
         if verbose and log:
             print(log)
+
+
+def build_cli():
+    out = os.path.join(PKG, "toed_detect_cli")
+    _run([HIPCC, "-O2", "-std=c++17", "-I" + INC, os.path.join(PKG, "cli", "toed_detect_cli.cpp"),
+          "-L" + PKG, "-ltoed_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
 
 
 def build_scene():
@@ -54,6 +74,7 @@ def build_scene():
 def build_all(verbose=False):
     build_scene()
     build_hip(verbose)
+    build_cli()
 
 
 if __name__ == "__main__":
