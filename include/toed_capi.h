@@ -95,6 +95,14 @@ void *toed_get_stream(toed_ctx *ctx);
 int toed_get_device(toed_ctx *ctx);
 /* Tile rows per workgroup (8 or 16); 0 = autotuned default. */
 int toed_set_tile_rows(toed_ctx *ctx, int tile_rows);
+/* Where the third-order orientation is evaluated, for later detections and
+ * plans: 0 (default) at the edgel's interp pixel, as cpu_toed.cpp:227-232 and
+ * the CUDA path do; 1 at its subpixel location, the nine responses
+ * interpolated by cubic convolution (Keys, a = -1/2) from the 4 x 4 interp
+ * pixels around it, as the MATLAB detector's QaD path does
+ * (third_order_edge_detector.m:91-121).  Positions and strengths are the same
+ * in both modes. */
+int toed_set_orientation_mode(toed_ctx *ctx, int mode);
 /* Staging records per tile and phase as a fraction of the phase's pixels
  * (0 < frac <= 1; 1 can never overflow).  Applies to later plans. */
 int toed_set_staging_fraction(toed_ctx *ctx, float frac);

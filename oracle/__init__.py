@@ -153,6 +153,44 @@ def to_theta(*d):
     return lib().toed_oracle_to_theta(*[float(v) for v in d])
 
 
+def keys(s):
+    """Cubic convolution kernel of Keys with a = -1/2 (MATLAB interp2 'cubic')."""
+    x = np.abs(np.asarray(s, np.float64))
+    return np.where(x <= 1, (1.5 * x - 2.5) * x * x + 1, np.where(x < 2, ((-0.5 * x + 2.5) * x - 4) * x + 2, 0.0))
+
+
+def qad_responses(resp9, Xs, Ys):
+    """The nine interp-grid response maps (9, IH, IW) interpolated at interp
+    coordinates (Xs, Ys) by cubic convolution over the 4 x 4 interp pixels
+    around each point (third_order_edge_detector.m:103-112)."""
+    Xs, Ys = np.atleast_1d(np.asarray(Xs, np.float64)), np.atleast_1d(np.asarray(Ys, np.float64))
+    rx, ry = np.floor(Xs).astype(np.int64), np.floor(Ys).astype(np.int64)
+    fx, fy = Xs - rx, Ys - ry
+    acc = np.zeros((9, len(Xs)))
+    for i in range(4):
+        for j in range(4):
+            acc += keys(fy + 1 - i) * keys(fx + 1 - j) * resp9[:, ry - 1 + i, rx - 1 + j]
+    return acc
+
+
+def detect_qad(img, sigma=2.0, thresh=2.0, border=10, nthreads=0, margins=False):
+    """FP64 detection with the third-order orientation evaluated at each
+    edgel's subpixel location (the MATLAB detector's QaD path,
+    third_order_edge_detector.m:91-121 and 125-133): the nine response maps of
+    cpu_toed.cpp:128-369 interpolated there (qad_responses), then the formula
+    of cpu_toed.cpp:227-232.  Positions and strengths are those of detect()."""
+    pl = convolve(img, sigma, nthreads, resp9=True)
+    res = nms(pl, thresh, border, nthreads, margins)
+    e = res["edges"].copy()
+    if len(e):
+        acc = qad_responses(pl["resp9"], 2 * e[:, 0] + 1, 2 * e[:, 1] + 1)
+        e[:, 2] = [to_theta(*acc[:, k]) for k in range(len(e))]
+    res["edges"] = e
+    res["resp9"] = pl["resp9"]
+    res["orient"] = pl["orient"]
+    return res
+
+
 REF_BIN = os.path.join(HERE, "_ref", "toed_ref")
 
 

@@ -54,6 +54,34 @@ def test_seeded_scenes_sigma(detector, oracle, scene, sigma, shape):
     assert_parity(rep)
 
 
+@pytest.mark.parametrize("sigma", [1.0, 2.0])
+def test_orientation_at_subpixel_location(pkg, oracle, scene, sigma):
+    """Orientation mode 1 (the MATLAB detector's QaD path) against the
+    oracle's detect_qad: the edgels, positions and strengths of mode 0, the
+    orientation from the responses interpolated at the subpixel location
+    within the parity tolerances; the batch plan gives the same."""
+    img = scene.shapes_scene(700 + int(10 * sigma), 160, 224)
+    det = pkg.Detector(0)
+    det.set_orientation_mode(1)
+    e1, i1 = _run(det, img, sigma)
+    b = pkg.BatchDetector(det, [torch.from_numpy(img).cuda()], [sigma])
+    b.run(graph=True)
+    eb = b.results()[0].cpu().numpy()
+    b.close()
+    det.set_orientation_mode(0)
+    e0, i0 = _run(det, img, sigma)
+    det.close()
+    assert np.array_equal(eb, e1)
+    assert np.array_equal(i1, i0) and np.array_equal(e1[:, [0, 1, 3]], e0[:, [0, 1, 3]])
+    ref = oracle.detect_qad(img.astype(np.float64), sigma, margins=True)
+    rep = compare(ref, e1, i1)
+    _report("QaD orientation, scene 160x224 sigma %g" % sigma, rep)
+    assert_parity(rep)
+    moved = np.abs(np.angle(np.exp(1j * (e1[:, 2] - e0[:, 2]))))
+    print("orientation change from mode 0: max %.3g rad, median %.3g rad" % (moved.max(), np.median(moved)))
+    assert moved.max() > 1e-4
+
+
 def test_full_frame_2160x3840(detector, oracle, scene):
     # the plain workload's own frame (96 shapes, 2160 x 3840 u8, sigma 2), edgel for edgel
     img = scene.shapes_scene(2160 * 3840, 2160, 3840)

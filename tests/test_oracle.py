@@ -201,3 +201,46 @@ def test_tiny_and_empty_images(oracle):
     for (H, W) in [(1, 1), (5, 7), (10, 40)]:
         img = np.random.default_rng(0).uniform(0, 255, (H, W))
         assert oracle.detect(img, 2.0)["n"] == 0
+
+
+def test_qad_cubic_convolution_reproduces_quadratics(oracle):
+    """Keys' kernel with a = -1/2 reproduces polynomials up to degree 2 (and not
+    3): the interpolation of the QaD orientation mode."""
+    yy, xx = np.mgrid[0:12, 0:12].astype(np.float64)
+    quad = 1 + 2 * xx - 0.5 * yy + 0.25 * xx * xx + 0.75 * xx * yy - 0.3 * yy * yy
+    maps = np.stack([quad] * 9)
+    rng = np.random.default_rng(3)
+    X, Y = rng.uniform(3, 8, 50), rng.uniform(3, 8, 50)
+    want = 1 + 2 * X - 0.5 * Y + 0.25 * X * X + 0.75 * X * Y - 0.3 * Y * Y
+    np.testing.assert_allclose(oracle.qad_responses(maps, X, Y)[4], want, rtol=1e-12, atol=1e-12)
+    cub = np.stack([xx ** 3] * 9)
+    assert np.abs(oracle.qad_responses(cub, X, Y)[0] - X ** 3).max() > 1e-3
+
+
+def test_qad_at_the_pixel_is_the_pixel_value(oracle, scene):
+    """At integer interp coordinates the interpolation returns the samples, so
+    the QaD orientation there is the plain orientation map's value."""
+    img = scene.shapes_scene(31, 64, 96).astype(np.float64)
+    pl = oracle.convolve(img, 2.0, resp9=True)
+    rng = np.random.default_rng(4)
+    R, C = rng.integers(20, 108, 40), rng.integers(20, 172, 40)
+    acc = oracle.qad_responses(pl["resp9"], C.astype(np.float64), R.astype(np.float64))
+    th = np.array([oracle.to_theta(*acc[:, k]) for k in range(len(R))])
+    np.testing.assert_allclose(th, pl["orient"][R, C], rtol=0, atol=1e-12)
+
+
+def test_qad_orientation_of_a_straight_edge(oracle):
+    """For a straight edge I = g(n . p) the third-order vector is parallel to n
+    everywhere, so the orientation evaluated at the subpixel location is the
+    line's direction, as at the pixel, up to the sampled filters' error
+    (about 6e-4 rad at sigma 2 in both modes)."""
+    for phi in (0.37, 1.1, 2.5):
+        yy, xx = np.mgrid[0:96, 0:96].astype(np.float64)
+        d = (xx - 48) * np.cos(phi) + (yy - 48) * np.sin(phi)
+        img = 40 + 120 / (1 + np.exp(-d / 0.8))
+        e = oracle.detect_qad(img, 2.0)["edges"]
+        inner = (np.abs(e[:, 0] - 48) < 20) & (np.abs(e[:, 1] - 48) < 20)
+        assert inner.sum() > 20
+        # orientation is atan2 of the tangent direction: phi + pi/2 modulo pi
+        diff = np.angle(np.exp(2j * (e[inner, 2] - (phi + np.pi / 2)))) / 2
+        assert np.abs(diff).max() < 1e-3, phi

@@ -129,6 +129,14 @@ class Detector:
     def set_tile_rows(self, tile_rows):
         check(lib().toed_set_tile_rows(self._ctx, int(tile_rows)), "toed_set_tile_rows")
 
+    def set_orientation_mode(self, mode):
+        """0: third-order orientation at the edgel's interp pixel (cpu_toed.cpp:
+        227-232, the default); 1: at its subpixel location, the responses
+        interpolated by cubic convolution from the 4 x 4 interp pixels around
+        it (the MATLAB detector's QaD path).  Applies to later detections and
+        plans."""
+        check(lib().toed_set_orientation_mode(self._ctx, int(mode)), "toed_set_orientation_mode")
+
     def autotune_batch(self, images, sigmas, thresh=DEFAULT_THRESH, border=DEFAULT_BORDER, capacities=None,
                        reps=5, max_results=32):
         """Time the batched form's launch configurations on these images

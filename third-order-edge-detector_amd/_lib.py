@@ -109,6 +109,7 @@ SIGNATURES = [
     ("toed_batch_backward", _I, [_VP, ctypes.POINTER(toed_grad_out_t)]),
     ("toed_set_tile_rows", _I, [_VP, _I]),
     ("toed_set_staging_fraction", _I, [_VP, _F]),
+    ("toed_set_orientation_mode", _I, [_VP, _I]),
     ("toed_last_times", _I, [_VP, _PF, _PF]),
     ("toed_detect", _I, [_VP, ctypes.POINTER(toed_image_t), _VP, _VP]),
     ("toed_detect_sync", _I, [_VP, ctypes.POINTER(toed_image_t), ctypes.POINTER(_LL)]),

@@ -90,7 +90,7 @@ struct ImageJob {
     int R0, R1, ntc, ntr;
     int tile_begin, nchunks, border, stage_cap;   // stage_cap = records per tile and phase
     float thresh;
-    int pad0;
+    int qad;                   // 1: orientation at the subpixel location (toed_set_orientation_mode)
     int *index_out;            // optional interp index per edgel
 };
 
@@ -763,6 +763,88 @@ __global__ __launch_bounds__(1024) void toed_scan_images(const ImageJob *jobs, i
     }
 }
 
+// Orientation mode 1: the nine responses are evaluated at the edgel's
+// subpixel location instead of at its interp pixel, by cubic convolution
+// (Keys, a = -1/2) of the interp-grid responses of the 4 x 4 interp pixels
+// around it -- the MATLAB detector's QaD path (third_order_edge_detector.m:
+// 91-121, interp2 'cubic'), then the same third-order formula
+// (cpu_toed.cpp:227-232).  Each of the 16 responses is the FP64 separable
+// sum of the phase's taps over the image window (zero outside the image), as
+// the orientation pass computes it at the pixel.  One lane per edgel.
+__device__ __forceinline__ double keys_w(double s) {
+    const double x = fabs(s);
+    if (x <= 1.0) return (1.5 * x - 2.5) * x * x + 1.0;
+    if (x < 2.0) return ((-0.5 * x + 2.5) * x - 4.0) * x + 2.0;
+    return 0.0;
+}
+
+__device__ float qad_theta(const ImageJob &J, float x, float y) {
+    const TOED_CONST TapSet *tp = as_const(J.taps);
+    const int R1 = J.R1;
+    // interp-grid coordinates of the subpixel position: x = (X - 1) / 2
+    const double Xs = 2.0 * (double)x + 1.0, Ys = 2.0 * (double)y + 1.0;
+    const int rx = (int)floor(Xs), ry = (int)floor(Ys);
+    const double fxr = Xs - rx, fyr = Ys - ry;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; ++i) {
+        const int Rp = ry - 1 + i, a = Rp & 1, ci = Rp >> 1;
+        const double wy = keys_w(fyr + 1 - i);
+        for (int j = 0; j < 4; ++j) {
+            const int Cp = rx - 1 + j, b = Cp & 1, cj = Cp >> 1;
+            const double w = wy * keys_w(fxr + 1 - j);
+            if (w == 0.0) continue;
+            // tap families as in the orientation pass: shifted for an odd
+            // coordinate, the R0 window (zero end taps) for phase (0, 0)
+            const bool p00 = (a == 0 && b == 0);
+            const TOED_CONST double(*X)[MAXTAPS] = b ? tp->ds : tp->du;
+            const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
+            double r9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int p = -R1; p <= R1; ++p) {
+                if (p00 && (p == -R1 || p == R1)) continue;
+                const int gi = ci - p;
+                if (gi < 0 || gi >= J.H) continue;
+                double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+                for (int q = -R1; q <= R1; ++q) {
+                    if (p00 && (q == -R1 || q == R1)) continue;
+                    const int gj = cj - q;
+                    if (gj < 0 || gj >= J.W) continue;
+                    const long long o = (long long)gi * J.pitch + gj;
+                    const double v = J.in_type == TOED_U8 ? (double)((const uint8_t *)J.img)[o]
+                                                          : (double)((const float *)J.img)[o];
+                    const int t = q + R1;
+                    h0 = fma(v, X[0][t], h0);
+                    h1 = fma(v, X[1][t], h1);
+                    h2 = fma(v, X[2][t], h2);
+                    h3 = fma(v, X[3][t], h3);
+                }
+                const int t = p + R1;
+                const double y0 = Y[0][t], y1 = Y[1][t], y2 = Y[2][t], y3 = Y[3][t];
+                r9[0] = fma(h1, y0, r9[0]);   // fx
+                r9[1] = fma(h0, y1, r9[1]);   // fy
+                r9[2] = fma(h2, y0, r9[2]);   // fxx
+                r9[3] = fma(h1, y1, r9[3]);   // fxy
+                r9[4] = fma(h0, y2, r9[4]);   // fyy
+                r9[5] = fma(h2, y1, r9[5]);   // fxxy
+                r9[6] = fma(h1, y2, r9[6]);   // fxyy
+                r9[7] = fma(h3, y0, r9[7]);   // fxxx
+                r9[8] = fma(h0, y3, r9[8]);   // fyyy
+            }
+#pragma unroll
+            for (int d = 0; d < 9; ++d) acc[d] = fma(w, r9[d], acc[d]);
+        }
+    }
+    const double fx = acc[0], fy = acc[1], fxx = acc[2], fxy = acc[3], fyy = acc[4];
+    const double fxxy = acc[5], fxyy = acc[6], fxxx = acc[7], fyyy = acc[8];
+    double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+              + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+    double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+              + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+    const double m = sqrt(tx * tx + ty * ty);
+    tx /= m;
+    ty /= m;
+    return (float)atan2(tx, -ty);
+}
+
 // K3: scatter every staged edgel to offset[row, tile column] + rank, the
 // reference's raster order of the interp grid (cpu_toed.cpp:510-538).  One
 // wavefront per tile (a tile stages a handful of edgels per phase).
@@ -787,7 +869,9 @@ __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs,
                 const long long ce = J.counts_base + (long long)R * J.ntc + tc;
                 const long long idx = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
                 if (idx >= J.capacity) continue;
-                reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[base + e];
+                float4 rec = stg.rec[base + e];
+                if (J.qad) rec.z = qad_theta(J, rec.x, rec.y);
+                reinterpret_cast<float4 *>(J.out)[idx] = rec;
                 if (J.index_out) J.index_out[idx] = R * IW + C;
             }
         }
@@ -1022,6 +1106,7 @@ struct toed_ctx {
     hipStream_t stream = nullptr;
     int tile_rows = 16;
     float stage_frac = 1.0f;
+    int orient_mode = 0;       // 1: orientation at the subpixel location
     // tap sets by sigma
     std::vector<float> tap_sigmas;
     std::vector<TapSet *> tap_dev;
@@ -1269,6 +1354,12 @@ int toed_set_tile_rows(toed_ctx *c, int tr) {
     return TOED_OK;
 }
 
+int toed_set_orientation_mode(toed_ctx *c, int mode) {
+    if (!c || (mode != 0 && mode != 1)) return TOED_ERR_ARG;
+    c->orient_mode = mode;
+    return TOED_OK;
+}
+
 int toed_set_staging_fraction(toed_ctx *c, float f) {
     if (!c || !(f > 0.f) || f > 1.f) return TOED_ERR_ARG;
     c->stage_frac = f;
@@ -1297,6 +1388,7 @@ static int detect_impl(toed_ctx *c, const toed_image_t *im, int *d_count, int *d
     J.capacity = im->capacity; J.in_type = im->in_type; J.H = im->H; J.W = im->W; J.pitch = im->pitch;
     radius_for(im->sigma, &J.R0, &J.R1);
     J.border = im->border; J.thresh = im->thresh;
+    J.qad = c->orient_mode;
     job_geometry(J, TH, stage_cap_for(c->stage_frac, TH));
     const long long ntiles = (long long)J.ntr * J.ntc;
     const long long ncounts = (long long)J.nchunks * CHUNK;
@@ -1379,6 +1471,7 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
         J.in_type = in_type; J.H = im->H; J.W = im->W; J.pitch = im->pitch;
         radius_for(im->sigma, &J.R0, &J.R1);
         J.border = im->border; J.thresh = im->thresh;
+        J.qad = c->orient_mode;
         job_geometry(J, TH, scap);
         J.tile_begin = (int)tiles;
         J.chunk_base = chunks;
