@@ -82,9 +82,13 @@ def bench_plain(torch, pkg, scene, steps, warmup):
         tiles.append(a)
         fin.append(b)
     sync_sec = (time.perf_counter() - t0) / steps
+    # the optional orientation at the subpixel location (MATLAB QaD mode)
+    det.set_orientation_mode(1)
+    qad_sec = time_loop(torch, lambda: det.detect_into(img, edges, counts), max(3, steps // 4), 1)
+    det.set_orientation_mode(0)
     det.close()
     return dict(H=H, W=W, sec=sec, n=n, ms_tiles=float(np.mean(tiles)), ms_final=float(np.mean(fin)),
-                sync_sec=sync_sec, img_np=img_np, edges_np=edges_np)
+                sync_sec=sync_sec, img_np=img_np, edges_np=edges_np, qad_sec=qad_sec)
 
 
 def batch_images(scene, n=200, seed=2024):
@@ -309,6 +313,8 @@ def main():
         forms["plain_sync"] = {"mpx_s": round(px / plain["sync_sec"] / 1e6, 1),
                                "ms": round(plain["sync_sec"] * 1e3, 4)}
         forms["kernels_ms"] = {"tiles": round(plain["ms_tiles"], 4), "scan_finalize": round(plain["ms_final"], 4)}
+        forms["plain_orientation_at_subpixel"] = {"mpx_s": round(px / plain["qad_sec"] / 1e6, 1),
+                                                  "ms": round(plain["qad_sec"] * 1e3, 4)}
         flops = dense_flops_per_px(2.0, pkg) * px
         ach = flops / (plain["ms_tiles"] * 1e-3) / 1e12
         out["roofline"] = {"bound": "valu", "achieved": round(ach, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

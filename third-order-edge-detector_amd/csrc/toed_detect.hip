@@ -770,7 +770,7 @@ __global__ __launch_bounds__(1024) void toed_scan_images(const ImageJob *jobs, i
 // 91-121, interp2 'cubic'), then the same third-order formula
 // (cpu_toed.cpp:227-232).  Each of the 16 responses is the FP64 separable
 // sum of the phase's taps over the image window (zero outside the image), as
-// the orientation pass computes it at the pixel.  One lane per edgel.
+// the orientation pass computes it at the pixel.
 __device__ __forceinline__ double keys_w(double s) {
     const double x = fabs(s);
     if (x <= 1.0) return (1.5 * x - 2.5) * x * x + 1.0;
@@ -778,71 +778,92 @@ __device__ __forceinline__ double keys_w(double s) {
     return 0.0;
 }
 
-__device__ float qad_theta(const ImageJob &J, float x, float y) {
+// The Keys-weighted nine responses at one of the 16 interp pixels around the
+// subpixel position (interp coordinates Xs = 2x + 1, Ys = 2y + 1), point
+// pt = 4 i + j of the 4 x 4 stencil.
+__device__ void qad_point(const ImageJob &J, float x, float y, int pt, double r9[9]) {
     const TOED_CONST TapSet *tp = as_const(J.taps);
     const int R1 = J.R1;
-    // interp-grid coordinates of the subpixel position: x = (X - 1) / 2
     const double Xs = 2.0 * (double)x + 1.0, Ys = 2.0 * (double)y + 1.0;
     const int rx = (int)floor(Xs), ry = (int)floor(Ys);
-    const double fxr = Xs - rx, fyr = Ys - ry;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = 0; i < 4; ++i) {
-        const int Rp = ry - 1 + i, a = Rp & 1, ci = Rp >> 1;
-        const double wy = keys_w(fyr + 1 - i);
-        for (int j = 0; j < 4; ++j) {
-            const int Cp = rx - 1 + j, b = Cp & 1, cj = Cp >> 1;
-            const double w = wy * keys_w(fxr + 1 - j);
-            if (w == 0.0) continue;
-            // tap families as in the orientation pass: shifted for an odd
-            // coordinate, the R0 window (zero end taps) for phase (0, 0)
-            const bool p00 = (a == 0 && b == 0);
-            const TOED_CONST double(*X)[MAXTAPS] = b ? tp->ds : tp->du;
-            const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
-            double r9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int p = -R1; p <= R1; ++p) {
-                if (p00 && (p == -R1 || p == R1)) continue;
-                const int gi = ci - p;
-                if (gi < 0 || gi >= J.H) continue;
-                double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
-                for (int q = -R1; q <= R1; ++q) {
-                    if (p00 && (q == -R1 || q == R1)) continue;
-                    const int gj = cj - q;
-                    if (gj < 0 || gj >= J.W) continue;
-                    const long long o = (long long)gi * J.pitch + gj;
-                    const double v = J.in_type == TOED_U8 ? (double)((const uint8_t *)J.img)[o]
-                                                          : (double)((const float *)J.img)[o];
-                    const int t = q + R1;
-                    h0 = fma(v, X[0][t], h0);
-                    h1 = fma(v, X[1][t], h1);
-                    h2 = fma(v, X[2][t], h2);
-                    h3 = fma(v, X[3][t], h3);
-                }
-                const int t = p + R1;
-                const double y0 = Y[0][t], y1 = Y[1][t], y2 = Y[2][t], y3 = Y[3][t];
-                r9[0] = fma(h1, y0, r9[0]);   // fx
-                r9[1] = fma(h0, y1, r9[1]);   // fy
-                r9[2] = fma(h2, y0, r9[2]);   // fxx
-                r9[3] = fma(h1, y1, r9[3]);   // fxy
-                r9[4] = fma(h0, y2, r9[4]);   // fyy
-                r9[5] = fma(h2, y1, r9[5]);   // fxxy
-                r9[6] = fma(h1, y2, r9[6]);   // fxyy
-                r9[7] = fma(h3, y0, r9[7]);   // fxxx
-                r9[8] = fma(h0, y3, r9[8]);   // fyyy
-            }
+    const int i = pt >> 2, j = pt & 3;
+    const double w = keys_w(Ys - ry + 1 - i) * keys_w(Xs - rx + 1 - j);
+    const int Rp = ry - 1 + i, Cp = rx - 1 + j, a = Rp & 1, b = Cp & 1, ci = Rp >> 1, cj = Cp >> 1;
+    // tap families as in the orientation pass: shifted for an odd coordinate,
+    // the R0 window (zero end taps) for phase (0, 0)
+    const int lo = (a == 0 && b == 0) ? 1 : 0;
+    const TOED_CONST double(*X)[MAXTAPS] = b ? tp->ds : tp->du;
+    const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
 #pragma unroll
-            for (int d = 0; d < 9; ++d) acc[d] = fma(w, r9[d], acc[d]);
+    for (int d = 0; d < 9; ++d) r9[d] = 0.0;
+    if (w == 0.0) return;
+    for (int p = -R1 + lo; p <= R1 - lo; ++p) {
+        const int gi = ci - p;
+        if (gi < 0 || gi >= J.H) continue;
+        double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+        const long long row = (long long)gi * J.pitch;
+        for (int q = -R1 + lo; q <= R1 - lo; ++q) {
+            const int gj = cj - q;
+            if (gj < 0 || gj >= J.W) continue;
+            const double v = J.in_type == TOED_U8 ? (double)((const uint8_t *)J.img)[row + gj]
+                                                  : (double)((const float *)J.img)[row + gj];
+            const int t = q + R1;
+            h0 = fma(v, X[0][t], h0);
+            h1 = fma(v, X[1][t], h1);
+            h2 = fma(v, X[2][t], h2);
+            h3 = fma(v, X[3][t], h3);
+        }
+        const int t = p + R1;
+        const double y0 = Y[0][t], y1 = Y[1][t], y2 = Y[2][t], y3 = Y[3][t];
+        r9[0] = fma(h1, y0, r9[0]);   // fx
+        r9[1] = fma(h0, y1, r9[1]);   // fy
+        r9[2] = fma(h2, y0, r9[2]);   // fxx
+        r9[3] = fma(h1, y1, r9[3]);   // fxy
+        r9[4] = fma(h0, y2, r9[4]);   // fyy
+        r9[5] = fma(h2, y1, r9[5]);   // fxxy
+        r9[6] = fma(h1, y2, r9[6]);   // fxyy
+        r9[7] = fma(h3, y0, r9[7]);   // fxxx
+        r9[8] = fma(h0, y3, r9[8]);   // fyyy
+    }
+#pragma unroll
+    for (int d = 0; d < 9; ++d) r9[d] *= w;
+}
+
+// K2b (orientation mode 1 only): per tile, after the tile kernel and before
+// the scatter, each staged edgel's orientation is recomputed at its subpixel
+// location; 16 lanes per edgel, lane pt one stencil point, the weighted
+// responses summed over the 16 lanes (DPP).
+__global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int njobs, ImageJob job0, StageBuf stg,
+                                                      const int *tile_cnt) {
+    const int tile = blockIdx.x;
+    const ImageJob J = load_job(jobs, njobs, job0, tile);
+    if (!J.qad) return;
+    const long long tile_local = tile - J.tile_begin;
+    const int g = threadIdx.x >> 4, pt = threadIdx.x & 15;
+    for (int phase = 0; phase < 4; ++phase) {
+        const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
+        const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
+        for (int e = g; e < n; e += NT / 16) {
+            // every lane of a 16-lane group has the same e: the group is a DPP row
+            const float4 rec = stg.rec[base + e];
+            double r9[9];
+            qad_point(J, rec.x, rec.y, pt, r9);
+#pragma unroll
+            for (int d = 0; d < 9; ++d) r9[d] = toeddpp::group_sum<16>(r9[d]);
+            if (pt == 0) {
+                const double fx = r9[0], fy = r9[1], fxx = r9[2], fxy = r9[3], fyy = r9[4];
+                const double fxxy = r9[5], fxyy = r9[6], fxxx = r9[7], fyyy = r9[8];
+                double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+                          + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+                double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+                          + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+                const double m = sqrt(tx * tx + ty * ty);
+                tx /= m;
+                ty /= m;
+                stg.rec[base + e].z = (float)atan2(tx, -ty);
+            }
         }
     }
-    const double fx = acc[0], fy = acc[1], fxx = acc[2], fxy = acc[3], fyy = acc[4];
-    const double fxxy = acc[5], fxyy = acc[6], fxxx = acc[7], fyyy = acc[8];
-    double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
-              + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
-    double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
-              + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
-    const double m = sqrt(tx * tx + ty * ty);
-    tx /= m;
-    ty /= m;
-    return (float)atan2(tx, -ty);
 }
 
 // K3: scatter every staged edgel to offset[row, tile column] + rank, the
@@ -869,9 +890,7 @@ __global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs,
                 const long long ce = J.counts_base + (long long)R * J.ntc + tc;
                 const long long idx = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
                 if (idx >= J.capacity) continue;
-                float4 rec = stg.rec[base + e];
-                if (J.qad) rec.z = qad_theta(J, rec.x, rec.y);
-                reinterpret_cast<float4 *>(J.out)[idx] = rec;
+                reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[base + e];
                 if (J.index_out) J.index_out[idx] = R * IW + C;
             }
         }
@@ -1131,6 +1150,7 @@ struct toed_batch {
     long long ntiles = 0, ncounts = 0, nchunks = 0, nstaging = 0;
     int in_type = 0, tile_rows = 16, persistent = 0;
     int dev_cu = 1, wg_per_cu = 0;     // persistent grid: workgroups per CU (0: as many as fit)
+    bool qad = false;                  // orientation at the subpixel location (all jobs)
     std::vector<int> order;
     std::vector<TileGroup> groups;
     GradOut *d_gouts = nullptr;
@@ -1246,7 +1266,8 @@ static const void *tiles_kernel_ptr() { return (const void *)toed_tiles_kernel<I
 // enqueue the kernels for jobs (device table) or job0 (single image)
 static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, int njobs, const ImageJob &job0,
                    const std::vector<TileGroup> &groups, long long ntiles, long long nchunks, int *counts,
-                   int *offsets, int *chunks, StageBuf staging, int *tilecnt, int *tickets, hipEvent_t *ev) {
+                   int *offsets, int *chunks, StageBuf staging, int *tilecnt, int *tickets, hipEvent_t *ev,
+                   bool qad) {
     if (ntiles <= 0) return TOED_OK;
     hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
@@ -1267,6 +1288,9 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
             });
         });
     }
+    if (qad)
+        hipLaunchKernelGGL(toed_qad_kernel, dim3((unsigned)ntiles), dim3(NT), 0, s, d_jobs, njobs, job0, staging,
+                           tilecnt);
     if (ev) HIPCHK(hipEventRecord(ev[1], s));
     hipLaunchKernelGGL(toed_scan_chunks, dim3((unsigned)nchunks), dim3(256), 0, s, d_jobs, njobs, job0, counts,
                        offsets, chunks);
@@ -1403,7 +1427,7 @@ static int detect_impl(toed_ctx *c, const toed_image_t *im, int *d_count, int *d
     g.R1 = J.R1; g.tile_begin = 0; g.tile_end = (int)ntiles;
     return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, std::vector<TileGroup>{g}, ntiles, J.nchunks,
                    c->d_counts, c->d_offsets, c->d_chunks, StageBuf{c->d_srec, c->d_skey}, c->d_tilecnt, nullptr,
-                   timed ? c->ev : nullptr);
+                   timed ? c->ev : nullptr, J.qad != 0);
 }
 
 int toed_detect(toed_ctx *c, const toed_image_t *im, int *d_count, int *d_flags) {
@@ -1442,7 +1466,7 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
     const int TH = c->tile_rows;
     const int scap = stage_cap_for(c->stage_frac, TH);
     toed_batch *b = new toed_batch();
-    b->ctx = c; b->n = n; b->in_type = in_type; b->tile_rows = TH;
+    b->ctx = c; b->n = n; b->in_type = in_type; b->tile_rows = TH; b->qad = c->orient_mode != 0;
     b->jobs.resize(n);
     // jobs ordered by window radius so each radius' tiles are contiguous and
     // run in one specialised kernel; order[k] is the caller's image index
@@ -1553,7 +1577,7 @@ static int batch_enqueue(toed_batch *b, hipStream_t s) {
     std::memset(&dummy, 0, sizeof dummy);
     return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->groups, b->ntiles, b->nchunks,
                    b->d_counts, b->d_offsets, b->d_chunks, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
-                   b->persistent ? b->d_ticket : nullptr, nullptr);
+                   b->persistent ? b->d_ticket : nullptr, nullptr, b->qad);
 }
 
 int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags) {
