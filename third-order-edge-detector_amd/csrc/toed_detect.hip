@@ -1,5 +1,5 @@
 // TOED detection path for MI355X (gfx950): fused tile kernel, segmented scan,
-// FP64 per-edgel finalize, host orchestration and the C ABI (toed_capi.h).
+// raster-order scatter, host orchestration and the C ABI (toed_capi.h).
 //
 // Algorithm (restated from the reference's CPU detector, cpu_toed.cpp:128-545):
 //   * four interp samples per original pixel (i, j): (2i+a, 2j+b), a, b in {0,1},
@@ -19,15 +19,17 @@
 //        (R0-window G, G') -> phase (0,0); + end taps -> phase (1,0);
 //        (shifted G, G') -> phases (0,1), (1,1).  Ix, Iy, |grad I| of the tile
 //        plus a one-pixel interp halo stay in LDS; each wavefront runs NMS on one
-//        64-wide interp row at a time, computes survivors' parabola in FP64 and
-//        compacts them with a ballot: per-(row, tile) counts go to HBM, records to
-//        a per-tile, per-phase staging area.
+//        64-wide interp row at a time, decides near-peaks and the parabola in
+//        FP64, and marks survivors in per-row masks: per-(row, tile) counts go to
+//        HBM, records to a per-tile, per-phase staging area; then the nine
+//        responses at each survivor in FP64 from the staged image (8 lanes per
+//        edgel) and its third-order orientation.
 //   K2 toed_scan_chunks / toed_scan_images   exclusive scan of the counts
 //        (row-major over (interp row, tile column)), segmented per image.
-//   K3 toed_finalize_kernel   one wavefront per tile, phases processed in turn
-//        so the filter taps are wavefront-uniform: FP64 recomputation of the
-//        nine responses at each edgel, third-order orientation, write of the
-//        edgel at offset[row, tile] + rank (raster order).
+//   K2b toed_qad_kernel   orientation mode 1 only: the orientation evaluated at
+//        each edgel's subpixel location instead (16 lanes per edgel).
+//   K3 toed_finalize_kernel   one wavefront per tile: each staged edgel to
+//        offset[row, tile] + rank (raster order).
 // No MFMA: the filters are short separable stencils, not a contraction.
 
 #include <hip/hip_runtime.h>
