@@ -71,3 +71,23 @@ def test_batch_mode_matches_single_images(pkg, detector, scene, tmp_path):
         want = detector.detect(torch.from_numpy(im).cuda()).cpu().numpy().astype(np.float64)
         assert got.shape == want.shape and len(want) > 10, name
         np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
+
+
+def test_reference_pgm_byte_shift(pkg, detector, scene, tmp_path):
+    """TOED_PGM_RAW_OFFSET=1 reads the PGM as the reference's no-OpenCV build
+    does (the header's last whitespace byte becomes pixel 0, every pixel one
+    position late), and the edgels are those of that shifted image."""
+    img = scene.shapes_scene(404, 120, 160)
+    p = str(tmp_path / "shift.pgm")
+    _pgm(p, img, header=b"P5\n%d %d\n255\n")
+    out_dir = str(tmp_path / "out_shift")
+    env = dict(os.environ, TOED_PGM_RAW_OFFSET="1")
+    r = subprocess.run([CLI, p, "1", "0", out_dir], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120, env=env)
+    print(r.stdout)
+    assert r.returncode == 0
+    shifted = np.concatenate([[ord("\n")], img.reshape(-1)[:-1]]).astype(np.uint8).reshape(img.shape)
+    want = detector.detect(torch.from_numpy(shifted).cuda()).cpu().numpy().astype(np.float64)
+    got = _read_list(os.path.join(out_dir, "data_final_output_gpu.txt"))
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
