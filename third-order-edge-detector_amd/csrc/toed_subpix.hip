@@ -940,8 +940,11 @@ __constant__ signed char SP_OFF[8][8] = {
 template <int R> struct BGeo {
     using G = CGeo<R>;
     static constexpr int PP = G::PR * G::PC;
-    static constexpr int XBN = 3 * G::IR * G::PC;    // transposed y pass, one group
-    static constexpr int SN = 14 * NT;               // per-edgel staging of one round
+    static constexpr int XBN = G::IR * G::PC;        // transposed y pass, one X bar plane
+    static constexpr int RS = 96;                    // edgels staged per round
+    static constexpr int SN = 14 * RS;               // per-edgel staging of one round
+    // x-pass tasks of a thread: rows (PR + 2R) x blocks of 4 outputs over PC + 2R columns
+    static constexpr int SLOTS = ((G::PR + 2 * R) * ((G::PC + 2 * R + 3) / 4) + NT - 1) / NT;
     static constexpr int NZ = 10;                    // 8 plane adjoints + raw Hx, Hy adjoints
     static constexpr int FLOATS = NZ * PP + (XBN > SN ? XBN : SN);
 };
@@ -1106,10 +1109,11 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     constexpr int VPL[10] = {0, 1, 2, 3, 4, 8, 5, 6, 7, 9};
     const int nlist = s_nlist;
     const int sub = threadIdx.x & 15, sa = sub >> 2, sb = sub & 3;
+    constexpr int RS = BGeo<R>::RS;
 #pragma unroll 1
-    for (int e0 = 0; e0 < nlist; e0 += NT) {
+    for (int e0 = 0; e0 < nlist; e0 += RS) {
         // (a) one lane per edgel: its staged record, or nothing past the capacity
-        {
+        if (threadIdx.x < RS) {
             const int e = e0 + threadIdx.x;
             float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0;
             int xfs = x0, yfs = y0;
@@ -1136,25 +1140,25 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
             }
             const float vb[10] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y};
 #pragma unroll
-            for (int m = 0; m < 10; ++m) St[m * NT + threadIdx.x] = vb[m];
-            St[10 * NT + threadIdx.x] = q2.z;
-            St[11 * NT + threadIdx.x] = q2.w;
-            St[12 * NT + threadIdx.x] = __int_as_float(xfs);
-            St[13 * NT + threadIdx.x] = __int_as_float(yfs);
+            for (int m = 0; m < 10; ++m) St[m * RS + threadIdx.x] = vb[m];
+            St[10 * RS + threadIdx.x] = q2.z;
+            St[11 * RS + threadIdx.x] = q2.w;
+            St[12 * RS + threadIdx.x] = __int_as_float(xfs);
+            St[13 * RS + threadIdx.x] = __int_as_float(yfs);
         }
         __syncthreads();
         // (b) 16 lanes per edgel: lane (sa, sb) scatters its stencil pixel
-        const int nr = min(NT, nlist - e0);
+        const int nr = min(RS, nlist - e0);
 #pragma unroll 1
         for (int q = threadIdx.x >> 4; q < nr; q += NT / 16) {
-            const float fx = St[10 * NT + q], fy = St[11 * NT + q];
-            const int xf = __float_as_int(St[12 * NT + q]), yf = __float_as_int(St[13 * NT + q]);
+            const float fx = St[10 * RS + q], fy = St[11 * RS + q];
+            const int xf = __float_as_int(St[12 * RS + q]), yf = __float_as_int(St[13 * RS + q]);
             const float w = keys_f(fy + 1 - sa) * keys_f(fx + 1 - sb);
             const int gy = min(max(yf - 1 + sa, 0), H - 1), gx = min(max(xf - 1 + sb, 0), W - 1);
             const int yo = (gy - (y0 - HALO)) * PC + (gx - (x0 - HALO));
 #pragma unroll
             for (int m = 0; m < 10; ++m) {
-                const float c = St[m * NT + q] * w;
+                const float c = St[m * RS + q] * w;
                 if (c != 0.f) atomicAdd(&Z[VPL[m] * PP + yo], c);
             }
         }
@@ -1162,110 +1166,118 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     }
 
     // transposed y pass (Z -> X bar), then x pass (X bar -> dL/dE, dL/dHx,
-    // dL/dHy added to HBM), per group.  Output u of a 4-output block at o0
-    // sums window[u + m] * k[m], m = 0..2R, with window[t] = source(o0 - 2R + t)
-    // (zero outside the source): the transposed filters.
+    // dL/dHy added to HBM), per group and one X bar plane at a time: the x
+    // pass adds each plane's share into per-thread registers (the tasks of a
+    // thread are the same for the three planes), so only one X bar plane is
+    // held in LDS.  Output u of a 4-output block at o0 sums window[u + m] *
+    // k[m], m = 0..2R, with window[t] = source(o0 - 2R + t) (zero outside the
+    // source): the transposed filters.
     // Only Z rows rlo .. rmax and columns clo .. chi can be nonzero: X bar rows
     // rlo .. rlo + nrow - 1 (= rmax + 2R) and outputs up to column chi + 2R.
     constexpr int NB = 4, NWIN = 2 * R + NB;
+    constexpr int SLOTS = BGeo<R>::SLOTS;
     const int rlo = s_box[0], clo = s_box[2], chi = s_box[3];
     if (s_box[1] < rlo) return;                       // no stencil reached Z
     const int nrow = s_box[1] + 2 * R - rlo + 1, ncol = chi - clo + 1;
     const int nyb = (nrow + NB - 1) / NB, nxb = (ncol + 2 * R + NB - 1) / NB;
     const int iy0 = y0 - HALO - R, ix0 = x0 - HALO - R;
     const bool inner = iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W;
+    // (adjoint plane, X bar plane, filter family) of each group
+    constexpr int NSRC = 5;
+    constexpr int src[2][NSRC] = {{0, 4, 1, 3, 2}, {5, 7, 6, -1, -1}};
+    constexpr int dst[2][NSRC] = {{0, 0, 1, 1, 2}, {0, 1, 2, -1, -1}};
+    constexpr int fam[2][NSRC] = {{0, 1, 1, 2, 0}, {0, 1, 0, -1, -1}};
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp) {
-        for (int task = threadIdx.x; task < nyb * ncol; task += NT) {
-            const int blk = task / ncol, c = clo + (task - blk * ncol), r0 = rlo + blk * NB;
-            float acc[3][NB];
+        float ex[SLOTS][2][NB];
 #pragma unroll
-            for (int g = 0; g < 3; ++g)
+        for (int sl = 0; sl < SLOTS; ++sl)
 #pragma unroll
-                for (int u = 0; u < NB; ++u) acc[g][u] = 0.f;
-            // (adjoint plane, X bar plane, filter family) of the group
-            constexpr int NSRC = 5;
-            constexpr int src[2][NSRC] = {{0, 4, 1, 3, 2}, {5, 7, 6, -1, -1}};
-            constexpr int dst[2][NSRC] = {{0, 0, 1, 1, 2}, {0, 1, 2, -1, -1}};
-            constexpr int fam[2][NSRC] = {{0, 1, 1, 2, 0}, {0, 1, 0, -1, -1}};
+            for (int u = 0; u < NB; ++u) ex[sl][0][u] = ex[sl][1][u] = 0.f;
 #pragma unroll
-            for (int q = 0; q < NSRC; ++q) {
-                if (src[grp][q] < 0) continue;
-                const float *zp = Z + src[grp][q] * PP + c;
-                float win[NWIN];
+        for (int q = 0; q < 3; ++q) {
+            for (int task = threadIdx.x; task < nyb * ncol; task += NT) {
+                const int blk = task / ncol, c = clo + (task - blk * ncol), r0 = rlo + blk * NB;
+                float acc[NB];
 #pragma unroll
-                for (int t = 0; t < NWIN; ++t) {
-                    const int r = r0 - 2 * R + t;
-                    win[t] = (r >= 0 && r < PR) ? zp[r * PC] : 0.f;
+                for (int u = 0; u < NB; ++u) acc[u] = 0.f;
+#pragma unroll
+                for (int s = 0; s < NSRC; ++s) {
+                    if (src[grp][s] < 0 || dst[grp][s] != q) continue;
+                    const float *zp = Z + src[grp][s] * PP + c;
+                    float win[NWIN];
+#pragma unroll
+                    for (int t = 0; t < NWIN; ++t) {
+                        const int r = r0 - 2 * R + t;
+                        win[t] = (r >= 0 && r < PR) ? zp[r * PC] : 0.f;
+                    }
+                    const int f = fam[grp][s];
+#pragma unroll
+                    for (int m = 0; m <= 2 * R; ++m) {
+                        const float kk = tp.k[f][m];
+#pragma unroll
+                        for (int u = 0; u < NB; ++u) acc[u] = fmaf(win[u + m], kk, acc[u]);
+                    }
                 }
-                const int d = dst[grp][q], f = fam[grp][q];
 #pragma unroll
-                for (int m = 0; m <= 2 * R; ++m) {
-                    const float kk = tp.k[f][m];
-#pragma unroll
-                    for (int u = 0; u < NB; ++u) acc[d][u] = fmaf(win[u + m], kk, acc[d][u]);
-                }
+                for (int u = 0; u < NB; ++u)
+                    if (r0 + u < rlo + nrow) Xb[(r0 + u) * PC + c] = acc[u];
             }
-#pragma unroll
-            for (int u = 0; u < NB; ++u) {
-                if (r0 + u >= rlo + nrow) continue;
-#pragma unroll
-                for (int g = 0; g < 3; ++g) Xb[g * IR * PC + (r0 + u) * PC + c] = acc[g][u];
-            }
-        }
-        __syncthreads();
-        for (int task = threadIdx.x; task < nrow * nxb; task += NT) {
-            const int rr = task / nxb, blk = task - rr * nxb, r = rlo + rr, c0 = clo + blk * NB;
-            float e0[NB], e1[NB];
-#pragma unroll
-            for (int u = 0; u < NB; ++u) e0[u] = e1[u] = 0.f;
+            __syncthreads();
             // group 0: E bar = X0 * G' + X1 * G + X2 * G''; group 1: Hx bar = X0 * G',
             // Hy bar = X1 * G + X2 * G'
+            const int f = grp == 0 ? (q == 0 ? 1 : (q == 1 ? 0 : 2)) : (q == 0 ? 1 : (q == 1 ? 0 : 1));
+            const int to1 = (grp == 1 && q > 0) ? 1 : 0;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const float *xp = Xb + q * IR * PC + r * PC;
+            for (int sl = 0; sl < SLOTS; ++sl) {
+                const int task = threadIdx.x + sl * NT;
+                if (task >= nrow * nxb) continue;
+                const int rr = task / nxb, blk = task - rr * nxb, r = rlo + rr, c0 = clo + blk * NB;
+                const float *xp = Xb + r * PC;
                 float win[NWIN];
 #pragma unroll
                 for (int t = 0; t < NWIN; ++t) {
                     const int c = c0 - 2 * R + t;
                     win[t] = (c >= clo && c <= chi) ? xp[c] : 0.f;
                 }
-                const int f = grp == 0 ? (q == 0 ? 1 : (q == 1 ? 0 : 2)) : (q == 0 ? 1 : (q == 1 ? 0 : 1));
-                const bool to1 = grp == 1 && q > 0;
 #pragma unroll
                 for (int m = 0; m <= 2 * R; ++m) {
                     const float kk = tp.k[f][m];
 #pragma unroll
-                    for (int u = 0; u < NB; ++u) {
-                        if (to1) e1[u] = fmaf(win[u + m], kk, e1[u]);
-                        else e0[u] = fmaf(win[u + m], kk, e0[u]);
-                    }
+                    for (int u = 0; u < NB; ++u) ex[sl][to1][u] = fmaf(win[u + m], kk, ex[sl][to1][u]);
                 }
             }
+            __syncthreads();      // the next plane's y pass overwrites X bar
+        }
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; ++sl) {
+            const int task = threadIdx.x + sl * NT;
+            if (task >= nrow * nxb) continue;
+            const int rr = task / nxb, blk = task - rr * nxb, r = rlo + rr, c0 = clo + blk * NB;
             const int gy = inner ? iy0 + r : wrap_circ(iy0 + r, H);
 #pragma unroll
             for (int u = 0; u < NB; ++u) {
                 const int cm = c0 + u;
                 if (cm > chi + 2 * R) continue;
                 const int gx = inner ? ix0 + cm : wrap_circ(ix0 + cm, W);
+                float e0 = ex[sl][0][u], e1 = ex[sl][1][u];
                 if (grp == 1) {
                     // the raw Hx, Hy sample adjoints of the cubic stencils (plane
                     // coordinates = output coordinates - R)
                     const int zr = r - R, zc = cm - R;
                     if (zr >= 0 && zr < PR && zc >= 0 && zc < PC) {
-                        e0[u] += Z[8 * PP + zr * PC + zc];
-                        e1[u] += Z[9 * PP + zr * PC + zc];
+                        e0 += Z[8 * PP + zr * PC + zc];
+                        e1 += Z[9 * PP + zr * PC + zc];
                     }
                 }
                 if (grp == 0) {
-                    if (e0[u] != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e0[u]);
+                    if (e0 != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e0);
                 } else {
-                    if (e0[u] != 0.f) atomicAdd(&J.ghx[(long long)gy * W + gx], e0[u]);
-                    if (e1[u] != 0.f) atomicAdd(&J.ghy[(long long)gy * W + gx], e1[u]);
+                    if (e0 != 0.f) atomicAdd(&J.ghx[(long long)gy * W + gx], e0);
+                    if (e1 != 0.f) atomicAdd(&J.ghy[(long long)gy * W + gx], e1);
                 }
             }
         }
-        if (grp == 0) __syncthreads();
     }
 }
 
