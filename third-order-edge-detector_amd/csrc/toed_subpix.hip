@@ -360,8 +360,8 @@ template <int R> struct CGeo {
     static constexpr int NK = 2 * R + 1;
     static constexpr int PR = TH + 2 * HALO, PC = TW + 2 * HALO;   // plane region
     static constexpr int IR = PR + 2 * R, IC = PC + 2 * R;         // input region
-    static constexpr int IN = 3 * IR * IC;                         // E, Hx, Hy
-    static constexpr int XN = 3 * IR * PC;                         // x-pass, one group
+    static constexpr int IN = 2 * IR * IC;                         // E, or Hx and Hy
+    static constexpr int XN = IR * PC;                             // x-pass, one plane
     static constexpr int YN = 8 * PR * PC;                         // 8 planes
     static constexpr int FLOATS = IN + XN + YN;
 };
@@ -375,9 +375,10 @@ struct PlaneBox {
     int r0, r1, c0, c1;
 };
 
-// E, Hx, Hy over the plane region + R (periodic borders) into In, rows and
-// columns of the box only
-template <int R>
+// E (WHICH = 0, into slot 0) or Hx, Hy (WHICH = 1, into slots 0 and 1) over
+// the plane region + R (periodic borders) into In, rows and columns of the
+// box only
+template <int R, int WHICH>
 __device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, int x0, const PlaneBox &bx) {
     using G = CGeo<R>;
     constexpr int IR = G::IR, IC = G::IC;
@@ -387,16 +388,19 @@ __device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, 
     if (iy0 >= 0 && iy0 + IR <= H && ix0 >= 0 && ix0 + IC <= W) {
         constexpr int U = 4;
         for (int base = 0; base < ln; base += U * NT) {
-            float v[3][U];
+            float v[2][U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int k = base + threadIdx.x + u * NT;
                 const int r = bx.r0 + k / lc, c = bx.c0 + k % lc;
                 const bool ok = k < ln;
                 const long long o = (long long)(iy0 + r) * W + (ix0 + c);
-                v[0][u] = ok ? J.map[(long long)(iy0 + r) * J.pitch + (ix0 + c)] : 0.f;
-                v[1][u] = ok ? J.hx[o] : 0.f;
-                v[2][u] = ok ? J.hy[o] : 0.f;
+                if (WHICH == 0) {
+                    v[0][u] = ok ? J.map[(long long)(iy0 + r) * J.pitch + (ix0 + c)] : 0.f;
+                } else {
+                    v[0][u] = ok ? J.hx[o] : 0.f;
+                    v[1][u] = ok ? J.hy[o] : 0.f;
+                }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -404,8 +408,7 @@ __device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, 
                 if (k < ln) {
                     const int idx = (bx.r0 + k / lc) * IC + bx.c0 + k % lc;
                     In[idx] = v[0][u];
-                    In[IR * IC + idx] = v[1][u];
-                    In[2 * IR * IC + idx] = v[2][u];
+                    if (WHICH == 1) In[IR * IC + idx] = v[1][u];
                 }
             }
         }
@@ -414,125 +417,84 @@ __device__ __forceinline__ void load_tile_in(const SpJob &J, float *In, int y0, 
             const int r = bx.r0 + k / lc, c = bx.c0 + k % lc, idx = r * IC + c;
             const int gy = wrap_circ(iy0 + r, H), gx = wrap_circ(ix0 + c, W);
             const long long o = (long long)gy * W + gx;
-            In[idx] = J.map[(long long)gy * J.pitch + gx];
-            In[IR * IC + idx] = J.hx[o];
-            In[2 * IR * IC + idx] = J.hy[o];
+            if (WHICH == 0) {
+                In[idx] = J.map[(long long)gy * J.pitch + gx];
+            } else {
+                In[idx] = J.hx[o];
+                In[IR * IC + idx] = J.hy[o];
+            }
         }
     }
 }
 
-// The eight circular Gaussian-derivative planes of the tile + HALO from In
-// (x pass into X, y pass into Y); ends with a barrier.
-template <int R>
-__device__ __forceinline__ void conv_planes(const float *In, float *X, float *Y, const SpTaps &tp, const PlaneBox &bx) {
+// The circular Gaussian-derivative planes of one group for the tile + HALO,
+// one x-pass plane at a time (x pass into X, y pass into Y); ends with a
+// barrier.  Group 0 from E (In slot 0): E * G' -> Px (G), Pxy (G'); E * G ->
+// Py (G'), Pyy (G''); E * G'' -> Pxx (G).  Group 1 from Hx, Hy (slots 0, 1):
+// Hx * G' -> Hxx (G); Hy * G -> Hyy (G'); Hy * G' -> Hxy (G).  Planes Y: 0 Px,
+// 1 Py, 2 Pxx, 3 Pyy, 4 Pxy, 5 Hxx, 6 Hxy, 7 Hyy.
+// Two outputs per thread (adjacent columns in the x pass, adjacent rows in
+// the y pass) share 2R + 2 loaded values.  Output u of a pass sums
+// window[m + u] * k[2R - m], m = 0..2R (true convolution).
+template <int R, int GRP>
+__device__ __forceinline__ void conv_group(const float *In, float *X, float *Y, const SpTaps &tp, const PlaneBox &bx) {
     using G = CGeo<R>;
     constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
-    // group 0: E * (G', G, G'') along x -> Px, Pxx, Py, Pxy, Pyy
-    // group 1: Hx * G', Hy * G, Hy * G' along x -> Hxx, Hyy, Hxy
-    // planes Y: 0 Px, 1 Py, 2 Pxx, 3 Pyy, 4 Pxy, 5 Hxx, 6 Hxy, 7 Hyy
-// Two outputs per thread (adjacent columns in the x pass, adjacent rows in
-    // the y pass) share 2R + 2 loaded values.  Output u of a pass sums
-    // window[m + u] * k[2R - m], m = 0..2R (true convolution).
     static_assert(PC % 2 == 0 && PR % 2 == 0, "two outputs per thread");
     const int xr = bx.r1 + 2 * R - bx.r0 + 1, ncp = (bx.c1 - bx.c0 + 1) / 2, nrp = (bx.r1 - bx.r0 + 1) / 2;
-#pragma unroll 1
-    for (int grp = 0; grp < 2; ++grp) {
+    // per x-pass plane: source slot, x filter, then up to two (Y plane, y filter)
+    constexpr int SRC[2][3] = {{0, 0, 0}, {0, 1, 1}};
+    constexpr int FX[2][3] = {{1, 0, 2}, {1, 0, 1}};
+    constexpr int YA[2][3] = {{0, 1, 2}, {5, 7, 6}};
+    constexpr int FA[2][3] = {{0, 1, 0}, {0, 1, 0}};
+    constexpr int YB[2][3] = {{4, 3, -1}, {-1, -1, -1}};
+    constexpr int FB[2][3] = {{1, 2, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+        const int fx = FX[GRP][pl];
         for (int idx = threadIdx.x; idx < xr * ncp; idx += NT) {
             const int r = bx.r0 + idx / ncp, c = bx.c0 + 2 * (idx % ncp);
-            float a[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
-            if (grp == 0) {
-                const float *src = In + r * IC + c;
+            const float *src = In + SRC[GRP][pl] * IR * IC + r * IC + c;
+            float a[2] = {0.f, 0.f};
 #pragma unroll
-                for (int m = 0; m <= 2 * R + 1; ++m) {
-                    const float v = src[m];
+            for (int m = 0; m <= 2 * R + 1; ++m) {
+                const float v = src[m];
 #pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const int t = 2 * R - (m - u);
-                        if (t < 0 || t > 2 * R) continue;
-                        a[0][u] = fmaf(v, tp.k[1][t], a[0][u]);
-                        a[1][u] = fmaf(v, tp.k[0][t], a[1][u]);
-                        a[2][u] = fmaf(v, tp.k[2][t], a[2][u]);
-                    }
-                }
-            } else {
-                const float *sx = In + IR * IC + r * IC + c;
-                const float *sy = In + 2 * IR * IC + r * IC + c;
-#pragma unroll
-                for (int m = 0; m <= 2 * R + 1; ++m) {
-                    const float vx = sx[m], vy = sy[m];
-#pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const int t = 2 * R - (m - u);
-                        if (t < 0 || t > 2 * R) continue;
-                        a[0][u] = fmaf(vx, tp.k[1][t], a[0][u]);
-                        a[1][u] = fmaf(vy, tp.k[0][t], a[1][u]);
-                        a[2][u] = fmaf(vy, tp.k[1][t], a[2][u]);
-                    }
+                for (int u = 0; u < 2; ++u) {
+                    const int t = 2 * R - (m - u);
+                    if (t < 0 || t > 2 * R) continue;
+                    a[u] = fmaf(v, tp.k[fx][t], a[u]);
                 }
             }
-            const int o = r * PC + c;
-#pragma unroll
-            for (int g = 0; g < 3; ++g) {
-                X[g * IR * PC + o] = a[g][0];
-                X[g * IR * PC + o + 1] = a[g][1];
-            }
+            X[r * PC + c] = a[0];
+            X[r * PC + c + 1] = a[1];
         }
         __syncthreads();
+        const int ya = YA[GRP][pl], fa = FA[GRP][pl], yb = YB[GRP][pl], fb = FB[GRP][pl];
         for (int idx = threadIdx.x; idx < nrp * (2 * ncp); idx += NT) {
             const int r = bx.r0 + 2 * (idx / (2 * ncp)), c = bx.c0 + idx % (2 * ncp);
-            const float *x0p = X + r * PC + c, *x1p = X + IR * PC + r * PC + c, *x2p = X + 2 * IR * PC + r * PC + c;
-            if (grp == 0) {
-                float px[2] = {0.f, 0.f}, py[2] = {0.f, 0.f}, pxx[2] = {0.f, 0.f}, pxy[2] = {0.f, 0.f},
-                      pyy[2] = {0.f, 0.f};
+            const float *xp = X + r * PC + c;
+            float pa[2] = {0.f, 0.f}, pb[2] = {0.f, 0.f};
 #pragma unroll
-                for (int m = 0; m <= 2 * R + 1; ++m) {
-                    const float dgx = x0p[m * PC], gx = x1p[m * PC], d2gx = x2p[m * PC];
-#pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const int t = 2 * R - (m - u);
-                        if (t < 0 || t > 2 * R) continue;
-                        px[u] = fmaf(dgx, tp.k[0][t], px[u]);
-                        pxy[u] = fmaf(dgx, tp.k[1][t], pxy[u]);
-                        py[u] = fmaf(gx, tp.k[1][t], py[u]);
-                        pyy[u] = fmaf(gx, tp.k[2][t], pyy[u]);
-                        pxx[u] = fmaf(d2gx, tp.k[0][t], pxx[u]);
-                    }
-                }
+            for (int m = 0; m <= 2 * R + 1; ++m) {
+                const float v = xp[m * PC];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    const int o = (r + u) * PC + c;
-                    Y[0 * PR * PC + o] = px[u];
-                    Y[1 * PR * PC + o] = py[u];
-                    Y[2 * PR * PC + o] = pxx[u];
-                    Y[3 * PR * PC + o] = pyy[u];
-                    Y[4 * PR * PC + o] = pxy[u];
+                    const int t = 2 * R - (m - u);
+                    if (t < 0 || t > 2 * R) continue;
+                    pa[u] = fmaf(v, tp.k[fa][t], pa[u]);
+                    if (yb >= 0) pb[u] = fmaf(v, tp.k[fb][t], pb[u]);
                 }
-            } else {
-                float hxx[2] = {0.f, 0.f}, hyy[2] = {0.f, 0.f}, hxy[2] = {0.f, 0.f};
+            }
 #pragma unroll
-                for (int m = 0; m <= 2 * R + 1; ++m) {
-                    const float v0 = x0p[m * PC], v1 = x1p[m * PC], v2 = x2p[m * PC];
-#pragma unroll
-                    for (int u = 0; u < 2; ++u) {
-                        const int t = 2 * R - (m - u);
-                        if (t < 0 || t > 2 * R) continue;
-                        hxx[u] = fmaf(v0, tp.k[0][t], hxx[u]);
-                        hyy[u] = fmaf(v1, tp.k[1][t], hyy[u]);
-                        hxy[u] = fmaf(v2, tp.k[0][t], hxy[u]);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int o = (r + u) * PC + c;
-                    Y[5 * PR * PC + o] = hxx[u];
-                    Y[6 * PR * PC + o] = hxy[u];
-                    Y[7 * PR * PC + o] = hyy[u];
-                }
+            for (int u = 0; u < 2; ++u) {
+                const int o = (r + u) * PC + c;
+                Y[ya * PR * PC + o] = pa[u];
+                if (yb >= 0) Y[yb * PR * PC + o] = pb[u];
             }
         }
         __syncthreads();
     }
-
 }
 
 // B1: isolated-edgel removal (:558-621) for every tile: an edgel is kept
@@ -631,8 +593,10 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
 
 // B2: for tiles with kept edgels: the eight planes of the tile + HALO in LDS
 // and the cubic sampling / third-order orientation of every kept edgel.
+// Three workgroups per CU: E, then Hx and Hy, share the input planes (LDS
+// 51.8 KB) and the registers are capped to fit (a couple of spills).
 template <int R>
-__global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+__global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
     using G = CGeo<R>;
     constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
     __shared__ float lds[G::FLOATS];
@@ -686,9 +650,12 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
         bx.c0 = (__ffs(cols) - 1) & ~1;
         bx.c1 = (31 - __clz(cols) + 2 * HALO) | 1;
     }
-    load_tile_in<R>(J, In, y0, x0, bx);
+    load_tile_in<R, 0>(J, In, y0, x0, bx);
     __syncthreads();
-    conv_planes<R>(In, X, Y, tp, bx);
+    conv_group<R, 0>(In, X, Y, tp, bx);
+    load_tile_in<R, 1>(J, In, y0, x0, bx);
+    __syncthreads();
+    conv_group<R, 1>(In, X, Y, tp, bx);
 
     // cubic sampling of the ten planes and the orientation: 4 lanes per edgel,
     // lane sa one row of the 4 x 4 stencil (its four pixels), FP64 sums over
@@ -732,11 +699,11 @@ __global__ __launch_bounds__(NT) void sp_correct_kernel(const SpJob *jobs, int n
             pv[sb][2] = Y[2 * PR * PC + yo];   // Pxx
             pv[sb][3] = Y[3 * PR * PC + yo];   // Pyy
             pv[sb][4] = Y[4 * PR * PC + yo];   // Pxy
-            pv[sb][5] = In[IR * IC + io];      // Hx
+            pv[sb][5] = In[io];                // Hx
             pv[sb][6] = Y[5 * PR * PC + yo];   // Hxx
             pv[sb][7] = Y[6 * PR * PC + yo];   // Hxy
             pv[sb][8] = Y[7 * PR * PC + yo];   // Hyy
-            pv[sb][9] = In[2 * IR * IC + io];  // Hy
+            pv[sb][9] = In[IR * IC + io];      // Hy
             const double w = wya * wxb[sb];
 #pragma unroll
             for (int m = 0; m < 10; ++m) v[m] = fma((double)pv[sb][m], w, v[m]);
