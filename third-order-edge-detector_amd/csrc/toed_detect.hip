@@ -181,19 +181,21 @@ template <int R1, int TH> struct Geo {
     static constexpr int NVR = TH + 2;                     // column-pass output rows
     static constexpr int RH = 2 * TH + 2, RW = 2 * TW + 2; // interp results incl. 1-px halo
     static constexpr int LDR = RW;
-    // LDS layout (floats): staged image | Ix, Iy, |grad| with 1-px halo |
-    // two row-pass planes, later reused for the survivor queue + FP64 taps
+    // LDS layout (floats): staged image | Ix, Iy with 1-px halo | two row-pass
+    // planes, later reused for the near-peak queue (which becomes the
+    // survivor queue) + FP64 taps | row masks.  |grad I| is recomputed from
+    // Ix, Iy where needed: without its plane the u8 16-row tile fits in 35.9 KB
+    // and four workgroups share a CU
     static constexpr int OFF_IMG = 0;
     static constexpr int OFF_IX = (NRI * LDI + 1) & ~1;
     static constexpr int OFF_IY = OFF_IX + RH * LDR;
-    static constexpr int OFF_MG = OFF_IY + RH * LDR;
-    static constexpr int OFF_P0 = (OFF_MG + RH * LDR + 1) & ~1;      // 8-byte aligned
+    static constexpr int OFF_P0 = (OFF_IY + RH * LDR + 1) & ~1;      // 8-byte aligned
     static constexpr int OFF_P1 = OFF_P0 + NRI * LDP;
     static constexpr int QMAX = 2 * TH * 2 * TW;                     // survivor queue (ints)
     static constexpr int P_END = OFF_P1 + NRI * LDP;
     static constexpr int Q_END = OFF_P0 + QMAX + 2 * 12 * (2 * R1 + 1);
-    static constexpr int OFF_CAND = ((P_END > Q_END ? P_END : Q_END) + 1) & ~1;   // near-peak queue (ints)
-    static constexpr int OFF_MASK = OFF_CAND + QMAX;                              // 2TH u64 row masks
+    static constexpr int OFF_CAND = OFF_P0;      // near-peak queue (ints), then the survivor queue
+    static constexpr int OFF_MASK = ((P_END > Q_END ? P_END : Q_END) + 1) & ~1;   // 2TH u64 row masks
     static constexpr int TOTAL = OFF_MASK + 2 * (2 * TH);
     // row pass: items = (row, block of KH columns)
     static constexpr int KH = 2;
@@ -257,7 +259,6 @@ __device__ __forceinline__ void col_pass(float *lds, int B,
     const float *P1 = lds + Gm::OFF_P1;
     float *RX = lds + Gm::OFF_IX;
     float *RY = lds + Gm::OFF_IY;
-    float *RM = lds + Gm::OFF_MG;
     constexpr int KV = Gm::KV;
     constexpr int NW = KV + 2 * R1;
     for (int it = threadIdx.x; it < Gm::NBV * Gm::NCP; it += NT) {
@@ -288,9 +289,6 @@ __device__ __forceinline__ void col_pass(float *lds, int B,
                 if (v < Gm::NVR && rr >= 0 && rr < Gm::RH && cc >= 0 && cc < Gm::RW) {
                     RX[rr * Gm::LDR + cc] = ix;
                     RY[rr * Gm::LDR + cc] = iy;
-                    // hardware square root (1 ulp): the correctly rounded library
-                    // sequence costs about 18 VALU instructions per pixel here
-                    RM[rr * Gm::LDR + cc] = __builtin_amdgcn_sqrtf(ix * ix + iy * iy);
                 }
             }
         }
@@ -339,7 +337,11 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
     using Gm = Geo<R1, TH>;
     const float *RX = lds + Gm::OFF_IX;
     const float *RY = lds + Gm::OFF_IY;
-    const float *RM = lds + Gm::OFF_MG;
+    // |grad I| from Ix, Iy (hardware square root, 1 ulp) wherever it is needed
+    auto MG = [&](int o) -> float {
+        const float x = RX[o], y = RY[o];
+        return __builtin_amdgcn_sqrtf(x * x + y * y);
+    };
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int IH = 2 * J.H, IW = 2 * J.W;
@@ -351,9 +353,9 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
         const int rr = r + 1, cc = lane + 1;
         bool cnd = false;
         const bool inside = R >= J.border && R < IH - J.border && C >= J.border && C < IW - J.border;
-        const float f = RM[rr * Gm::LDR + cc];
+        const float gx = RX[rr * Gm::LDR + cc], gy = RY[rr * Gm::LDR + cc];
+        const float f = __builtin_amdgcn_sqrtf(gx * gx + gy * gy);
         if (inside && f > J.thresh) {
-            const float gx = RX[rr * Gm::LDR + cc], gy = RY[rr * Gm::LDR + cc];
             const float ax = fabsf(gx), ay = fabsf(gy);
             if (!(ax < 10e-6f && ay < 10e-6f)) {
                 // x-major iff |gx| >= |gy|; step signs from the gradient signs
@@ -361,8 +363,8 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
                 const bool xm = ax >= ay;
                 const float sl = (xm ? ay : ax) * __builtin_amdgcn_rcpf(xm ? ax : ay);
                 const int pr = xm ? 0 : sy, pc = xm ? sx : 0;        // primary neighbour
-                const float mp = RM[(rr + pr) * Gm::LDR + cc + pc], md = RM[(rr + sy) * Gm::LDR + cc + sx];
-                const float np = RM[(rr - pr) * Gm::LDR + cc - pc], nd = RM[(rr - sy) * Gm::LDR + cc - sx];
+                const float mp = MG((rr + pr) * Gm::LDR + cc + pc), md = MG((rr + sy) * Gm::LDR + cc + sx);
+                const float np = MG((rr - pr) * Gm::LDR + cc - pc), nd = MG((rr - sy) * Gm::LDR + cc - sx);
                 const float fp = mp + sl * (md - mp), fm = np + sl * (nd - np);
                 cnd = f >= fmaxf(fp, fm) * (1.0f - 1e-5f);
             }
@@ -379,14 +381,26 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
     // step 2: exact FP64 decision and parabola on the queued pixels
     const int ncand = sh_cnt[5];
     const double SQ2 = 1.4142135623730951;
-    for (int k = threadIdx.x; k < ncand; k += NT) {
-        const int q = cand[k];
+    // the queue of survivors is written over the candidates: each thread reads
+    // its candidates first
+    constexpr int CPT = Gm::QMAX / NT;
+    int qv[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int k = threadIdx.x + j * NT;
+        qv[j] = k < ncand ? cand[k] : -1;
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int j = 0; j < CPT; ++j) {
+        const int q = qv[j];
+        if (q < 0) break;
         const int r = q & 63, c = q >> 6;
         const int rr = r + 1, cc = c + 1;
-        const double fd = RM[rr * Gm::LDR + cc];
+        const double fd = MG(rr * Gm::LDR + cc);
         const double gxd = RX[rr * Gm::LDR + cc], gyd = RY[rr * Gm::LDR + cc];
         const double nxd = gxd / fd, nyd = gyd / fd;
-        auto Md = [&](int dr, int dc) { return (double)RM[(rr + dr) * Gm::LDR + cc + dc]; };
+        auto Md = [&](int dr, int dc) { return (double)MG((rr + dr) * Gm::LDR + cc + dc); };
         double sld, fpd, fmd;
         if (!octant<double>(gxd, gyd, nxd, nyd, Md, sld, fpd, fmd)) continue;
         if (!((fd > fmd && fd > fpd) || (fd > fmd && fd >= fpd) || (fd >= fmd && fd > fpd))) continue;
@@ -659,8 +673,10 @@ __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0
 // from a device counter (re-armed by K3's last workgroup).  One
 // instantiation per window radius R1; a launch covers the tiles
 // [tile_begin, tile_end) of the images with that radius.
+// Four workgroups per CU (16 waves): registers capped at 128 (a few spills),
+// 0.310 -> 0.265 ms per 2160x3840 frame against three workgroups.
 template <typename IN_T, int TH, int R1>
-__global__ __launch_bounds__(NT) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
+__global__ __launch_bounds__(NT, 4) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
                                                         int *counts, StageBuf staging, int *tile_cnt,
                                                         int tile_begin, int tile_end, int *ticket) {
     __shared__ float lds[Geo<R1, TH>::TOTAL];
