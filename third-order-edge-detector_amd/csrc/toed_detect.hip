@@ -673,10 +673,11 @@ __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0
 // from a device counter (re-armed by K3's last workgroup).  One
 // instantiation per window radius R1; a launch covers the tiles
 // [tile_begin, tile_end) of the images with that radius.
-// Four workgroups per CU (16 waves): registers capped at 128 (a few spills),
-// 0.310 -> 0.265 ms per 2160x3840 frame against three workgroups.
+// Four workgroups per CU (16 waves) for 16-row tiles: registers capped at 128
+// (a few spills), 0.310 -> 0.265 ms per 2160x3840 frame against three
+// workgroups.  Five for 8-row tiles (96 registers; 0.372 -> 0.334 ms there).
 template <typename IN_T, int TH, int R1>
-__global__ __launch_bounds__(NT, 4) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
+__global__ __launch_bounds__(NT, TH == 8 ? 5 : 4) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
                                                         int *counts, StageBuf staging, int *tile_cnt,
                                                         int tile_begin, int tile_end, int *ticket) {
     __shared__ float lds[Geo<R1, TH>::TOTAL];
