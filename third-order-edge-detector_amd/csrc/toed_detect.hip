@@ -27,7 +27,7 @@
 //   K2 toed_scan_chunks / toed_scan_images   exclusive scan of the counts
 //        (row-major over (interp row, tile column)), segmented per image.
 //   K2b toed_qad_kernel   orientation mode 1 only: the orientation evaluated at
-//        each edgel's subpixel location instead (16 lanes per edgel).
+//        each edgel's subpixel location instead (one wavefront per edgel).
 //   K3 toed_finalize_kernel   one wavefront per tile: each staged edgel to
 //        offset[row, tile] + rank (raster order).
 // No MFMA: the filters are short separable stencils, not a contraction.
@@ -797,79 +797,125 @@ __device__ __forceinline__ double keys_w(double s) {
     return 0.0;
 }
 
-// The Keys-weighted nine responses at one of the 16 interp pixels around the
-// subpixel position (interp coordinates Xs = 2x + 1, Ys = 2y + 1), point
-// pt = 4 i + j of the 4 x 4 stencil.
-__device__ void qad_point(const ImageJob &J, float x, float y, int pt, double r9[9]) {
-    const TOED_CONST TapSet *tp = as_const(J.taps);
-    const int R1 = J.R1;
-    const double Xs = 2.0 * (double)x + 1.0, Ys = 2.0 * (double)y + 1.0;
-    const int rx = (int)floor(Xs), ry = (int)floor(Ys);
-    const int i = pt >> 2, j = pt & 3;
-    const double w = keys_w(Ys - ry + 1 - i) * keys_w(Xs - rx + 1 - j);
-    const int Rp = ry - 1 + i, Cp = rx - 1 + j, a = Rp & 1, b = Cp & 1, ci = Rp >> 1, cj = Cp >> 1;
-    // tap families as in the orientation pass: shifted for an odd coordinate,
-    // the R0 window (zero end taps) for phase (0, 0)
-    const int lo = (a == 0 && b == 0) ? 1 : 0;
-    const TOED_CONST double(*X)[MAXTAPS] = b ? tp->ds : tp->du;
-    const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
-#pragma unroll
-    for (int d = 0; d < 9; ++d) r9[d] = 0.0;
-    if (w == 0.0) return;
-    for (int p = -R1 + lo; p <= R1 - lo; ++p) {
-        const int gi = ci - p;
-        if (gi < 0 || gi >= J.H) continue;
-        double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
-        const long long row = (long long)gi * J.pitch;
-        for (int q = -R1 + lo; q <= R1 - lo; ++q) {
-            const int gj = cj - q;
-            if (gj < 0 || gj >= J.W) continue;
-            const double v = J.in_type == TOED_U8 ? (double)((const uint8_t *)J.img)[row + gj]
-                                                  : (double)((const float *)J.img)[row + gj];
-            const int t = q + R1;
-            h0 = fma(v, X[0][t], h0);
-            h1 = fma(v, X[1][t], h1);
-            h2 = fma(v, X[2][t], h2);
-            h3 = fma(v, X[3][t], h3);
-        }
-        const int t = p + R1;
-        const double y0 = Y[0][t], y1 = Y[1][t], y2 = Y[2][t], y3 = Y[3][t];
-        r9[0] = fma(h1, y0, r9[0]);   // fx
-        r9[1] = fma(h0, y1, r9[1]);   // fy
-        r9[2] = fma(h2, y0, r9[2]);   // fxx
-        r9[3] = fma(h1, y1, r9[3]);   // fxy
-        r9[4] = fma(h0, y2, r9[4]);   // fyy
-        r9[5] = fma(h2, y1, r9[5]);   // fxxy
-        r9[6] = fma(h1, y2, r9[6]);   // fxyy
-        r9[7] = fma(h3, y0, r9[7]);   // fxxx
-        r9[8] = fma(h0, y3, r9[8]);   // fyyy
-    }
-#pragma unroll
-    for (int d = 0; d < 9; ++d) r9[d] *= w;
-}
-
 // K2b (orientation mode 1 only): per tile, after the tile kernel and before
 // the scatter, each staged edgel's orientation is recomputed at its subpixel
-// location; 16 lanes per edgel, lane pt one stencil point, the weighted
-// responses summed over the 16 lanes (DPP).
+// location, one wavefront per edgel.  With Xs = 2x + 1, Ys = 2y + 1 the
+// stencil's interp pixels are (Rp, Cp) = (floor(Ys) - 1 + i, floor(Xs) - 1 + j),
+// i, j = 0..3, on image pixel (Rp >> 1, Cp >> 1) with phase (Rp & 1, Cp & 1).
+// The 16 responses share one union window of at most 2 R1 + 3 image rows, so
+// the column sums are formed once per (window row, stencil column):
+//   (1) lane (h, r) takes window row r and stencil columns 2h and 2h + 1 (one
+//       even and one odd interp column; for the even one also the sum without
+//       the two end taps, phase (0, 0)'s R0 window) and leaves them in LDS;
+//   (2) lanes 0..15, one per stencil point, sum their column's sums against
+//       the row taps and apply the point's Keys weight;
+//   (3) the 16 weighted responses are summed with DPP.
+// Each sum runs over the taps from -R1 to R1 in the same order as a direct
+// per-point evaluation would.
+constexpr int QROWS = 2 * MAXR1 + 3;
+constexpr int QWAVES = NT / 64;
+
 __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int njobs, ImageJob job0, StageBuf stg,
                                                       const int *tile_cnt) {
+    // per wavefront: [stencil column][full sums 0..3, R0-window sums 4..7][window row]
+    __shared__ double hs[QWAVES][4][8][QROWS];
     const int tile = blockIdx.x;
     const ImageJob J = load_job(jobs, njobs, job0, tile);
     if (!J.qad) return;
+    const TOED_CONST TapSet *tp = as_const(J.taps);
+    const int R1 = J.R1;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    double(*Hs)[8][QROWS] = hs[wave];
     const long long tile_local = tile - J.tile_begin;
-    const int g = threadIdx.x >> 4, pt = threadIdx.x & 15;
     for (int phase = 0; phase < 4; ++phase) {
         const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
         const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
-        for (int e = g; e < n; e += NT / 16) {
-            // every lane of a 16-lane group has the same e: the group is a DPP row
+        for (int e = wave; e < n; e += QWAVES) {
             const float4 rec = stg.rec[base + e];
-            double r9[9];
-            qad_point(J, rec.x, rec.y, pt, r9);
+            const double Xs = 2.0 * (double)rec.x + 1.0, Ys = 2.0 * (double)rec.y + 1.0;
+            const int rx = __builtin_amdgcn_readfirstlane((int)floor(Xs));
+            const int ry = __builtin_amdgcn_readfirstlane((int)floor(Ys));
+            const int ci0 = (ry - 1) >> 1;      // image row of stencil row 0
+            {
+                const int r = lane & 31, hh = lane >> 5;
+                // d = 1 when stencil column 0 is odd: the even column of each
+                // pair is then column 2h + 1, one image pixel right of the odd one
+                const int d = (rx - 1) & 1;
+                const int je = 2 * hh + d, jo = 2 * hh + 1 - d;
+                const int co = (rx - 1 + jo) >> 1;
+                const int gi = ci0 - R1 + r;
+                double fe[4] = {0, 0, 0, 0}, ie[4] = {0, 0, 0, 0}, fo[4] = {0, 0, 0, 0};
+                if (r < QROWS && gi >= 0 && gi < J.H) {
+                    const long long row = (long long)gi * J.pitch;
+                    // image column co + R1 + d - m: tap m - R1 of the even
+                    // column (m <= 2 R1), tap m - d - R1 of the odd one (m >= d)
+                    for (int m = 0; m <= 2 * R1 + d; ++m) {
+                        const int gj = co + R1 + d - m;
+                        double v = 0.0;
+                        if (gj >= 0 && gj < J.W)
+                            v = J.in_type == TOED_U8 ? (double)((const uint8_t *)J.img)[row + gj]
+                                                     : (double)((const float *)J.img)[row + gj];
+                        if (m <= 2 * R1) {
 #pragma unroll
-            for (int d = 0; d < 9; ++d) r9[d] = toeddpp::group_sum<16>(r9[d]);
-            if (pt == 0) {
+                            for (int f = 0; f < 4; ++f) fe[f] = fma(v, tp->du[f][m], fe[f]);
+                            if (m != 0 && m != 2 * R1) {
+#pragma unroll
+                                for (int f = 0; f < 4; ++f) ie[f] = fma(v, tp->du[f][m], ie[f]);
+                            }
+                        }
+                        if (m >= d) {
+#pragma unroll
+                            for (int f = 0; f < 4; ++f) fo[f] = fma(v, tp->ds[f][m - d], fo[f]);
+                        }
+                    }
+                }
+                if (r < QROWS) {
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) {
+                        Hs[je][f][r] = fe[f];
+                        Hs[je][4 + f][r] = ie[f];
+                        Hs[jo][f][r] = fo[f];
+                    }
+                }
+            }
+            // the sums written above are read by other lanes of this wavefront
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            double r9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if (lane < 16) {
+                const int i = lane >> 2, j = lane & 3;
+                const double w = keys_w(Ys - ry + 1 - i) * keys_w(Xs - rx + 1 - j);
+                const int Rp = ry - 1 + i, Cp = rx - 1 + j, a = Rp & 1, b = Cp & 1, ci = Rp >> 1;
+                const int lo = (a == 0 && b == 0) ? 1 : 0;
+                const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
+                const double(*Hj)[QROWS] = Hs[j] + 4 * lo;
+                if (w != 0.0) {
+                    for (int p = -R1 + lo; p <= R1 - lo; ++p) {
+                        const int gi = ci - p;
+                        if (gi < 0 || gi >= J.H) continue;
+                        const int rr = gi - (ci0 - R1);
+                        const double h0 = Hj[0][rr], h1 = Hj[1][rr], h2 = Hj[2][rr], h3 = Hj[3][rr];
+                        const int t = p + R1;
+                        const double y0 = Y[0][t], y1 = Y[1][t], y2 = Y[2][t], y3 = Y[3][t];
+                        r9[0] = fma(h1, y0, r9[0]);   // fx
+                        r9[1] = fma(h0, y1, r9[1]);   // fy
+                        r9[2] = fma(h2, y0, r9[2]);   // fxx
+                        r9[3] = fma(h1, y1, r9[3]);   // fxy
+                        r9[4] = fma(h0, y2, r9[4]);   // fyy
+                        r9[5] = fma(h2, y1, r9[5]);   // fxxy
+                        r9[6] = fma(h1, y2, r9[6]);   // fxyy
+                        r9[7] = fma(h3, y0, r9[7]);   // fxxx
+                        r9[8] = fma(h0, y3, r9[8]);   // fyyy
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) r9[k] *= w;
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) r9[k] = toeddpp::group_sum<16>(r9[k]);
+            if (lane == 0) {
                 const double fx = r9[0], fy = r9[1], fxx = r9[2], fxy = r9[3], fyy = r9[4];
                 const double fxxy = r9[5], fxyy = r9[6], fxxx = r9[7], fyyy = r9[8];
                 double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
@@ -881,6 +927,10 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
                 ty /= m;
                 stg.rec[base + e].z = (float)atan2(tx, -ty);
             }
+            // the next edgel's sums overwrite this one's only after they were read
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 }
