@@ -828,110 +828,126 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double(*Hs)[8][QROWS] = hs[wave];
     const long long tile_local = tile - J.tile_begin;
-    for (int phase = 0; phase < 4; ++phase) {
-        const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
-        const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
-        for (int e = wave; e < n; e += QWAVES) {
-            const float4 rec = stg.rec[base + e];
-            const double Xs = 2.0 * (double)rec.x + 1.0, Ys = 2.0 * (double)rec.y + 1.0;
-            const int rx = __builtin_amdgcn_readfirstlane((int)floor(Xs));
-            const int ry = __builtin_amdgcn_readfirstlane((int)floor(Ys));
-            const int ci0 = (ry - 1) >> 1;      // image row of stencil row 0
-            {
-                const int r = lane & 31, hh = lane >> 5;
-                // d = 1 when stencil column 0 is odd: the even column of each
-                // pair is then column 2h + 1, one image pixel right of the odd one
-                const int d = (rx - 1) & 1;
-                const int je = 2 * hh + d, jo = 2 * hh + 1 - d;
-                const int co = (rx - 1 + jo) >> 1;
-                const int gi = ci0 - R1 + r;
-                double fe[4] = {0, 0, 0, 0}, ie[4] = {0, 0, 0, 0}, fo[4] = {0, 0, 0, 0};
-                if (r < QROWS && gi >= 0 && gi < J.H) {
-                    const long long row = (long long)gi * J.pitch;
-                    // image column co + R1 + d - m: tap m - R1 of the even
-                    // column (m <= 2 R1), tap m - d - R1 of the odd one (m >= d)
-                    for (int m = 0; m <= 2 * R1 + d; ++m) {
-                        const int gj = co + R1 + d - m;
-                        double v = 0.0;
-                        if (gj >= 0 && gj < J.W)
-                            v = J.in_type == TOED_U8 ? (double)((const uint8_t *)J.img)[row + gj]
-                                                     : (double)((const float *)J.img)[row + gj];
-                        if (m <= 2 * R1) {
-#pragma unroll
-                            for (int f = 0; f < 4; ++f) fe[f] = fma(v, tp->du[f][m], fe[f]);
-                            if (m != 0 && m != 2 * R1) {
-#pragma unroll
-                                for (int f = 0; f < 4; ++f) ie[f] = fma(v, tp->du[f][m], ie[f]);
-                            }
-                        }
-                        if (m >= d) {
-#pragma unroll
-                            for (int f = 0; f < 4; ++f) fo[f] = fma(v, tp->ds[f][m - d], fo[f]);
-                        }
-                    }
-                }
-                if (r < QROWS) {
-#pragma unroll
-                    for (int f = 0; f < 4; ++f) {
-                        Hs[je][f][r] = fe[f];
-                        Hs[je][4 + f][r] = ie[f];
-                        Hs[jo][f][r] = fo[f];
-                    }
+    // the tile's edgels of all four phases are dealt to the wavefronts in turn
+    const int n0 = min(tile_cnt[(long long)tile * 4 + 0], J.stage_cap);
+    const int n1 = min(tile_cnt[(long long)tile * 4 + 1], J.stage_cap);
+    const int n2 = min(tile_cnt[(long long)tile * 4 + 2], J.stage_cap);
+    const int n3 = min(tile_cnt[(long long)tile * 4 + 3], J.stage_cap);
+    const int total = n0 + n1 + n2 + n3;
+    for (int k = wave; k < total; k += QWAVES) {
+        int phase = 0, e = k;
+        if (e >= n0) {
+            e -= n0;
+            phase = 1;
+            if (e >= n1) {
+                e -= n1;
+                phase = 2;
+                if (e >= n2) {
+                    e -= n2;
+                    phase = 3;
                 }
             }
-            // the sums written above are read by other lanes of this wavefront
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            double r9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            if (lane < 16) {
-                const int i = lane >> 2, j = lane & 3;
-                const double w = keys_w(Ys - ry + 1 - i) * keys_w(Xs - rx + 1 - j);
-                const int Rp = ry - 1 + i, Cp = rx - 1 + j, a = Rp & 1, b = Cp & 1, ci = Rp >> 1;
-                const int lo = (a == 0 && b == 0) ? 1 : 0;
-                const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
-                const double(*Hj)[QROWS] = Hs[j] + 4 * lo;
-                if (w != 0.0) {
-                    for (int p = -R1 + lo; p <= R1 - lo; ++p) {
-                        const int gi = ci - p;
-                        if (gi < 0 || gi >= J.H) continue;
-                        const int rr = gi - (ci0 - R1);
-                        const double h0 = Hj[0][rr], h1 = Hj[1][rr], h2 = Hj[2][rr], h3 = Hj[3][rr];
-                        const int t = p + R1;
-                        const double y0 = Y[0][t], y1 = Y[1][t], y2 = Y[2][t], y3 = Y[3][t];
-                        r9[0] = fma(h1, y0, r9[0]);   // fx
-                        r9[1] = fma(h0, y1, r9[1]);   // fy
-                        r9[2] = fma(h2, y0, r9[2]);   // fxx
-                        r9[3] = fma(h1, y1, r9[3]);   // fxy
-                        r9[4] = fma(h0, y2, r9[4]);   // fyy
-                        r9[5] = fma(h2, y1, r9[5]);   // fxxy
-                        r9[6] = fma(h1, y2, r9[6]);   // fxyy
-                        r9[7] = fma(h3, y0, r9[7]);   // fxxx
-                        r9[8] = fma(h0, y3, r9[8]);   // fyyy
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 9; ++k) r9[k] *= w;
-            }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) r9[k] = toeddpp::group_sum<16>(r9[k]);
-            if (lane == 0) {
-                const double fx = r9[0], fy = r9[1], fxx = r9[2], fxy = r9[3], fyy = r9[4];
-                const double fxxy = r9[5], fxyy = r9[6], fxxx = r9[7], fyyy = r9[8];
-                double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
-                          + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
-                double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
-                          + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
-                const double m = sqrt(tx * tx + ty * ty);
-                tx /= m;
-                ty /= m;
-                stg.rec[base + e].z = (float)atan2(tx, -ty);
-            }
-            // the next edgel's sums overwrite this one's only after they were read
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+        const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
+        const float4 rec = stg.rec[base + e];
+        const double Xs = 2.0 * (double)rec.x + 1.0, Ys = 2.0 * (double)rec.y + 1.0;
+        const int rx = __builtin_amdgcn_readfirstlane((int)floor(Xs));
+        const int ry = __builtin_amdgcn_readfirstlane((int)floor(Ys));
+        const int ci0 = (ry - 1) >> 1;      // image row of stencil row 0
+        {
+            const int r = lane & 31, hh = lane >> 5;
+            // d = 1 when stencil column 0 is odd: the even column of each
+            // pair is then column 2h + 1, one image pixel right of the odd one
+            const int d = (rx - 1) & 1;
+            const int je = 2 * hh + d, jo = 2 * hh + 1 - d;
+            const int co = (rx - 1 + jo) >> 1;
+            const int gi = ci0 - R1 + r;
+            double fe[4] = {0, 0, 0, 0}, ie[4] = {0, 0, 0, 0}, fo[4] = {0, 0, 0, 0};
+            if (r < QROWS && gi >= 0 && gi < J.H) {
+                const long long row = (long long)gi * J.pitch;
+                // image column co + R1 + d - m: tap m - R1 of the even
+                // column (m <= 2 R1), tap m - d - R1 of the odd one (m >= d)
+                for (int m = 0; m <= 2 * R1 + d; ++m) {
+                    const int gj = co + R1 + d - m;
+                    double v = 0.0;
+                    if (gj >= 0 && gj < J.W)
+                        v = J.in_type == TOED_U8 ? (double)((const uint8_t *)J.img)[row + gj]
+                                                 : (double)((const float *)J.img)[row + gj];
+                    if (m <= 2 * R1) {
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) fe[f] = fma(v, tp->du[f][m], fe[f]);
+                        if (m != 0 && m != 2 * R1) {
+#pragma unroll
+                            for (int f = 0; f < 4; ++f) ie[f] = fma(v, tp->du[f][m], ie[f]);
+                        }
+                    }
+                    if (m >= d) {
+#pragma unroll
+                        for (int f = 0; f < 4; ++f) fo[f] = fma(v, tp->ds[f][m - d], fo[f]);
+                    }
+                }
+            }
+            if (r < QROWS) {
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    Hs[je][f][r] = fe[f];
+                    Hs[je][4 + f][r] = ie[f];
+                    Hs[jo][f][r] = fo[f];
+                }
+            }
+        }
+        // the sums written above are read by other lanes of this wavefront
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double r9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (lane < 16) {
+            const int i = lane >> 2, j = lane & 3;
+            const double w = keys_w(Ys - ry + 1 - i) * keys_w(Xs - rx + 1 - j);
+            const int Rp = ry - 1 + i, Cp = rx - 1 + j, a = Rp & 1, b = Cp & 1, ci = Rp >> 1;
+            const int lo = (a == 0 && b == 0) ? 1 : 0;
+            const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
+            const double(*Hj)[QROWS] = Hs[j] + 4 * lo;
+            if (w != 0.0) {
+                for (int p = -R1 + lo; p <= R1 - lo; ++p) {
+                    const int gi = ci - p;
+                    if (gi < 0 || gi >= J.H) continue;
+                    const int rr = gi - (ci0 - R1);
+                    const double h0 = Hj[0][rr], h1 = Hj[1][rr], h2 = Hj[2][rr], h3 = Hj[3][rr];
+                    const int t = p + R1;
+                    const double y0 = Y[0][t], y1 = Y[1][t], y2 = Y[2][t], y3 = Y[3][t];
+                    r9[0] = fma(h1, y0, r9[0]);   // fx
+                    r9[1] = fma(h0, y1, r9[1]);   // fy
+                    r9[2] = fma(h2, y0, r9[2]);   // fxx
+                    r9[3] = fma(h1, y1, r9[3]);   // fxy
+                    r9[4] = fma(h0, y2, r9[4]);   // fyy
+                    r9[5] = fma(h2, y1, r9[5]);   // fxxy
+                    r9[6] = fma(h1, y2, r9[6]);   // fxyy
+                    r9[7] = fma(h3, y0, r9[7]);   // fxxx
+                    r9[8] = fma(h0, y3, r9[8]);   // fyyy
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 9; ++k) r9[k] *= w;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) r9[k] = toeddpp::group_sum<16>(r9[k]);
+        if (lane == 0) {
+            const double fx = r9[0], fy = r9[1], fxx = r9[2], fxy = r9[3], fyy = r9[4];
+            const double fxxy = r9[5], fxyy = r9[6], fxxx = r9[7], fyyy = r9[8];
+            double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+                      + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+            double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+                      + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+            const double m = sqrt(tx * tx + ty * ty);
+            tx /= m;
+            ty /= m;
+            stg.rec[base + e].z = (float)atan2(tx, -ty);
+        }
+        // the next edgel's sums overwrite this one's only after they were read
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
