@@ -54,7 +54,7 @@ def test_seeded_scenes_sigma(detector, oracle, scene, sigma, shape):
     assert_parity(rep)
 
 
-@pytest.mark.parametrize("sigma", [1.0, 2.0])
+@pytest.mark.parametrize("sigma", [1.0, 1.5, 2.0])
 def test_orientation_at_subpixel_location(pkg, oracle, scene, sigma):
     """Orientation mode 1 (the MATLAB detector's QaD path) against the
     oracle's detect_qad: the edgels, positions and strengths of mode 0, the
@@ -97,6 +97,23 @@ def test_float_input_matches_u8(detector, oracle, scene):
     img = scene.shapes_scene(21, 200, 260)
     e8, i8 = _run(detector, img)
     ef, i_f = _run(detector, img, dtype=torch.float32)
+    assert np.array_equal(i8, i_f)
+    assert np.array_equal(e8, ef)
+
+
+def test_orientation_at_subpixel_location_float_input(pkg, scene):
+    """Orientation mode 1 reads float images through its own load path; on
+    integer-valued pixels it gives the u8 result bit for bit (NMS border at its
+    minimum of one pixel, so that windows near the image edge are clipped)."""
+    img = scene.shapes_scene(23, 150, 190)
+    det = pkg.Detector(0)
+    det.set_orientation_mode(1)
+    try:
+        e8, i8 = _run(det, img, 1.5, border=1)
+        ef, i_f = _run(det, img, 1.5, dtype=torch.float32, border=1)
+    finally:
+        det.close()
+    assert len(e8) > 100
     assert np.array_equal(i8, i_f)
     assert np.array_equal(e8, ef)
 
