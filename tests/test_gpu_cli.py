@@ -1,7 +1,8 @@
 """The command-line drop-in (toed_detect_cli) on the GPU: the reference's argv
 and output file (main_gpu_cpu.cpp:103-153, gpu_toed.hpp:452-478), and the
-batch mode that replaces run_gpu_batch.sh's process per image; both against
-the library's own single-image detection."""
+batch mode that replaces run_gpu_batch.sh's process per image; and the
+header-only C++ class driven as main_gpu_cpu.cpp drives the reference's; all
+against the library's own single-image detection."""
 import os
 import subprocess
 
@@ -15,6 +16,7 @@ torch = pytest.importorskip("torch")
 
 CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "third-order-edge-detector_amd",
                    "toed_detect_cli")
+DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "toed_class_driver")
 
 
 def _read_list(path):
@@ -91,3 +93,31 @@ def test_reference_pgm_byte_shift(pkg, detector, scene, tmp_path):
     got = _read_list(os.path.join(out_dir, "data_final_output_gpu.txt"))
     assert got.shape == want.shape
     np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-9)
+
+
+def test_header_class_float_and_double(pkg, detector, scene, tmp_path):
+    """ThirdOrderEdgeDetectionGPU<T> from include/toed_gpu.hpp, run as
+    main_gpu_cpu.cpp:195-209 runs the reference's class: T = float returns the
+    library's own list, T = double the same rows widened, and each writes its
+    text list."""
+    assert os.path.exists(DRIVER), "tests/cpp/toed_class_driver is not built"
+    img = scene.shapes_scene(405, 180, 240)
+    raw = tmp_path / "img.raw"
+    raw.write_bytes(img.tobytes())
+    for t in ("f32", "f64"):
+        (tmp_path / t).mkdir()
+    print("driver:", DRIVER, flush=True)
+    r = subprocess.run([DRIVER, str(raw), "180", "240", "2", str(tmp_path)], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0
+    want = detector.detect(torch.from_numpy(img).cuda()).cpu().numpy()
+    assert len(want) > 100
+    assert "f32 edges = %d" % len(want) in r.stdout and "f64 edges = %d" % len(want) in r.stdout
+    f32 = np.fromfile(str(tmp_path / "edges_f32.bin"), np.float32).reshape(-1, 4)
+    f64 = np.fromfile(str(tmp_path / "edges_f64.bin"), np.float64).reshape(-1, 4)
+    assert np.array_equal(f32, want)
+    assert np.array_equal(f64, want.astype(np.float64))
+    for t in ("f32", "f64"):
+        got = _read_list(str(tmp_path / t / "data_final_output_gpu.txt"))
+        np.testing.assert_allclose(got, want.astype(np.float64), rtol=1e-9, atol=1e-9)

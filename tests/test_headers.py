@@ -24,7 +24,10 @@ int use_classes(int H, int W, const double *E, double *out) {
     to_corr.write_array_to_file("TO_edgemap.txt", to_corr.TO_edgemap, H, W);
     // main_gpu_cpu.cpp:158-190 with the GPU class
     ThirdOrderEdgeDetectionGPU<float> det(0, H, W, 17, 2);
-    return n + det.edge_pt_list_idx;
+    // main_gpu_cpu.cpp:200-209: the double instantiation
+    ThirdOrderEdgeDetectionGPU<double> det64(0, H, W, 17, 2);
+    det64.convolve_img();
+    return n + det.edge_pt_list_idx + det64.non_maximum_suppresion(out);
 }
 '''
 

@@ -4,6 +4,8 @@
     libtoed_scene.so  deterministic synthetic-input generators (host C)
     toed_detect_cli   command-line drop-in for the reference's binary and batch
                       script (host C++ over the C ABI)
+    tests/cpp/toed_class_driver   the header-only C++ class run for T = float
+                      and double (test program)
 
 Run:  python third-order-edge-detector_amd/build.py
 """
@@ -64,6 +66,14 @@ def build_cli():
     os.replace(out + ".tmp", out)
 
 
+def build_class_driver():
+    d = os.path.join(ROOT, "tests", "cpp")
+    out = os.path.join(d, "toed_class_driver")
+    _run([HIPCC, "-O2", "-std=c++17", "-I" + INC, os.path.join(d, "toed_class_driver.cpp"),
+          "-L" + PKG, "-ltoed_hip", "-Wl,-rpath,$ORIGIN/../../third-order-edge-detector_amd", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+
+
 def build_scene():
     out = os.path.join(PKG, "libtoed_scene.so")
     _run(["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-o", out + ".tmp",
@@ -75,6 +85,7 @@ def build_all(verbose=False):
     build_scene()
     build_hip(verbose)
     build_cli()
+    build_class_driver()
 
 
 if __name__ == "__main__":
