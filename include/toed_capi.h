@@ -4,8 +4,8 @@
  * device allocations owned by the caller unless stated otherwise.
  *
  * Reference interfaces replaced (paths relative to the reference tree):
- *   gpu_kernels.hpp:24-45   gpu_convolve(...) f32/f64 overloads  -> toed_gpu_convolve_f32
- *   gpu_kernels.hpp:48-63   gpu_nms(...) f32/f64 overloads       -> toed_gpu_nms_f32
+ *   gpu_kernels.hpp:24-45   gpu_convolve(...) f32/f64 overloads  -> toed_gpu_convolve_f32/_f64
+ *   gpu_kernels.hpp:48-63   gpu_nms(...) f32/f64 overloads       -> toed_gpu_nms_f32/_f64
  *   gpu_toed.hpp:79-135     ThirdOrderEdgeDetectionGPU ctor: device buffers,
  *                           events                               -> toed_create / toed_plan
  *   gpu_toed.hpp:340-441    convolve_img + non_maximum_suppresion + host
@@ -256,6 +256,16 @@ int toed_gpu_nms_f32(int device_id, int interp_h, int interp_w,
                      const float *dev_Ix, const float *dev_Iy, const float *dev_I_grad_mag,
                      float *dev_subpix_pos_x_map, float *dev_subpix_pos_y_map,
                      float thresh, int border);
+/* toed_gpu_convolve_f64 / toed_gpu_nms_f64: the T = double overloads,
+ * gpu_kernels.hpp:35-45 and :56-63.  The same planes and maps in FP64: FP64
+ * image, taps, sums and orientation; the NMS decision wholly in FP64. */
+int toed_gpu_convolve_f64(int device_id, int h, int w, int interp_h, int interp_w,
+                          const double *dev_img, double *dev_Ix, double *dev_Iy,
+                          double *dev_I_grad_mag, double *dev_I_orient, double sigma);
+int toed_gpu_nms_f64(int device_id, int interp_h, int interp_w,
+                     const double *dev_Ix, const double *dev_Iy, const double *dev_I_grad_mag,
+                     double *dev_subpix_pos_x_map, double *dev_subpix_pos_y_map,
+                     double thresh, int border);
 
 #ifdef __cplusplus
 }

@@ -23,7 +23,8 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for must in ["toed_create", "toed_destroy", "toed_detect", "toed_detect_sync", "toed_batch_plan",
-                 "toed_batch_run", "toed_batch_graph_run", "toed_gpu_convolve_f32", "toed_gpu_nms_f32"]:
+                 "toed_batch_run", "toed_batch_graph_run", "toed_gpu_convolve_f32", "toed_gpu_nms_f32",
+                 "toed_gpu_convolve_f64", "toed_gpu_nms_f64"]:
         assert must in names
 
 

@@ -234,6 +234,46 @@ def test_reference_class_mirror(pkg, oracle, scene, tmp_path):
         pkg.ThirdOrderEdgeDetectionGPU(0, 10, 10, 19, 2)
 
 
+def test_unfused_launch_layer_f64(pkg, oracle, scene):
+    """toed_gpu_convolve_f64 / toed_gpu_nms_f64 (the T = double overloads of
+    the reference's launch layer) against the oracle's FP64 planes and list:
+    planes to 1e-9 of their range, the same edgel set apart from decisions
+    within 1e-9 of a tie, positions to 1e-8 interp px."""
+    import ctypes
+    L = pkg._lib.lib()
+    H, W = 90, 120
+    img = scene.shapes_scene(41, H, W)
+    t = torch.from_numpy(img.astype(np.float64)).cuda()
+    planes = [torch.zeros((2 * H, 2 * W), dtype=torch.float64, device="cuda") for _ in range(4)]
+    ptr = lambda x: ctypes.c_void_p(x.data_ptr())
+    assert L.toed_gpu_convolve_f64(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in planes], 2.0) == 0
+    torch.cuda.synchronize()
+    c = oracle.convolve(img.astype(np.float64), 2.0)
+    Ix, Iy, mag, ori = (p.cpu().numpy() for p in planes)
+    for got, key in ((Ix, "Ix"), (Iy, "Iy"), (mag, "mag")):
+        assert np.abs(got - c[key]).max() < 1e-9 * np.abs(c[key]).max(), key
+    strong = c["mag"] > 2
+    d = np.abs(ori - c["orient"])[strong]
+    d = np.minimum(d, np.abs(d - 2 * np.pi))
+    assert np.nanmax(d) < 1e-6
+    px = torch.zeros((2 * H, 2 * W), dtype=torch.float64, device="cuda")
+    py = torch.zeros_like(px)
+    assert L.toed_gpu_nms_f64(0, 2 * H, 2 * W, ptr(planes[0]), ptr(planes[1]), ptr(planes[2]), ptr(px), ptr(py),
+                              2.0, 10) == 0
+    torch.cuda.synchronize()
+    pxm, pym = px.cpu().numpy().reshape(-1), py.cpu().numpy().reshape(-1)
+    sel = np.flatnonzero(pxm)
+    ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
+    only = np.setxor1d(sel, ref["eidx"])
+    mm, ms = ref["mmag"].reshape(-1), ref["msub"].reshape(-1)
+    assert np.all((mm[only] < 1e-9) | (ms[only] < 1e-9))
+    assert len(sel) > 100 and len(only) <= 2
+    common, ia, _ = np.intersect1d(ref["eidx"], sel, return_indices=True)
+    e = ref["edges"][ia]
+    assert np.abs(pxm[common] - (2 * e[:, 0] + 1)).max() < 1e-8
+    assert np.abs(pym[common] - (2 * e[:, 1] + 1)).max() < 1e-8
+
+
 def test_unfused_launch_layer(pkg, oracle, scene):
     """toed_gpu_convolve_f32 / toed_gpu_nms_f32 (the reference's launch layer)
     against the oracle's planes and its subpixel maps."""

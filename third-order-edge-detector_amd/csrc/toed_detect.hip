@@ -1191,6 +1191,94 @@ __global__ __launch_bounds__(256) void toed_nms_maps_kernel(int IH, int IW, cons
     py[o] = oy;
 }
 
+
+// FP64 form of the unfused launch layer (the reference's T = double overloads
+// of gpu_convolve / gpu_nms, gpu_kernels.hpp:35-45,56-63): the same separable
+// sums on the FP64 taps, one thread per interp pixel, and the NMS decided
+// wholly in FP64 (cpu_toed.cpp:400-494).
+__global__ __launch_bounds__(256) void toed_planes_f64_kernel(int H, int W, const double *img, double *Ix,
+                                                              double *Iy, double *mag, double *orient,
+                                                              const TapSet *tp) {
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long IW = 2LL * W;
+    if (o >= 4LL * H * W) return;
+    const int R = (int)(o / IW), C = (int)(o % IW);
+    const int i = R >> 1, j = C >> 1, a = R & 1, b = C & 1;
+    const int R1 = tp->R1, rad = (a | b) ? tp->R1 : tp->R0;
+    const double(*X)[MAXTAPS] = b ? tp->ds : tp->du;
+    const double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
+    double g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = -rad; p <= rad; ++p) {
+        const int ii = i - p;
+        if (ii < 0 || ii >= H) continue;
+        double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
+        for (int q = -rad; q <= rad; ++q) {
+            const int jj = j - q;
+            if (jj < 0 || jj >= W) continue;
+            const double v = img[(long long)ii * W + jj];
+            const int t = q + R1;
+            h0 = fma(v, X[0][t], h0);
+            h1 = fma(v, X[1][t], h1);
+            h2 = fma(v, X[2][t], h2);
+            h3 = fma(v, X[3][t], h3);
+        }
+        const int t = p + R1;
+        g[0] = fma(h1, Y[0][t], g[0]);   // fx
+        g[1] = fma(h0, Y[1][t], g[1]);   // fy
+        g[2] = fma(h2, Y[0][t], g[2]);   // fxx
+        g[3] = fma(h1, Y[1][t], g[3]);   // fxy
+        g[4] = fma(h0, Y[2][t], g[4]);   // fyy
+        g[5] = fma(h2, Y[1][t], g[5]);   // fxxy
+        g[6] = fma(h1, Y[2][t], g[6]);   // fxyy
+        g[7] = fma(h3, Y[0][t], g[7]);   // fxxx
+        g[8] = fma(h0, Y[3][t], g[8]);   // fyyy
+    }
+    const double fx = g[0], fy = g[1], fxx = g[2], fxy = g[3], fyy = g[4];
+    const double fxxy = g[5], fxyy = g[6], fxxx = g[7], fyyy = g[8];
+    double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+              + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+    double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+              + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+    const double m = sqrt(tx * tx + ty * ty);
+    tx /= m;
+    ty /= m;
+    Ix[o] = fx;
+    Iy[o] = fy;
+    mag[o] = sqrt(fx * fx + fy * fy);
+    orient[o] = atan2(tx, -ty);
+}
+
+__global__ __launch_bounds__(256) void toed_nms_maps_f64_kernel(int IH, int IW, const double *Ix, const double *Iy,
+                                                                const double *mag, double *px, double *py,
+                                                                double thresh, int border) {
+    const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (long long)IH * IW) return;
+    const int i = (int)(o / IW), j = (int)(o % IW);
+    double ox = 0.0, oy = 0.0;
+    const double f = mag[o];
+    if (i >= border && i < IH - border && j >= border && j < IW - border && f > thresh) {
+        const double gx = Ix[o], gy = Iy[o];
+        if (!(fabs(gx) < 10e-6 && fabs(gy) < 10e-6)) {
+            auto M = [&](int dr, int dc) { return mag[o + (long long)dr * IW + dc]; };
+            const double nx = gx / f, ny = gy / f;
+            double sl, fp, fm;
+            if (octant<double>(gx, gy, nx, ny, M, sl, fp, fm) &&
+                ((f > fm && f > fp) || (f > fm && f >= fp) || (f >= fm && f > fp))) {
+                const double s = sqrt(1.0 + sl * sl);
+                const double A = (fm + fp - 2.0 * f) / (2.0 * s * s);
+                const double Bc = (fp - fm) / (2.0 * s);
+                const double ss = -Bc / (2.0 * A);
+                if (fabs(ss) <= 1.4142135623730951) {
+                    ox = j + ss * nx;
+                    oy = i + ss * ny;
+                }
+            }
+        }
+    }
+    px[o] = ox;
+    py[o] = oy;
+}
+
 }  // namespace toed
 
 // ===========================================================================
@@ -1871,6 +1959,33 @@ int toed_gpu_nms_f32(int device_id, int ih, int iw, const float *Ix, const float
     const long long n = (long long)ih * iw;
     hipLaunchKernelGGL(toed_nms_maps_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, ih, iw, Ix,
                        Iy, mag, px, py, thresh, border);
+    HIPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+int toed_gpu_convolve_f64(int device_id, int h, int w, int ih, int iw, const double *img, double *Ix, double *Iy,
+                          double *mag, double *orient, double sigma) {
+    if (h <= 0 || w <= 0 || ih != 2 * h || iw != 2 * w || !img || !Ix || !Iy || !mag || !orient) return TOED_ERR_ARG;
+    if ((long long)h * w > (1LL << 28)) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(device_id));
+    int st = 0;
+    TapSet *tp = compat_taps((float)sigma, &st);
+    if (!tp) return st;
+    const long long n = 4LL * h * w;
+    hipLaunchKernelGGL(toed_planes_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, h, w, img,
+                       Ix, Iy, mag, orient, tp);
+    HIPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+int toed_gpu_nms_f64(int device_id, int ih, int iw, const double *Ix, const double *Iy, const double *mag,
+                     double *px, double *py, double thresh, int border) {
+    if (ih <= 0 || iw <= 0 || !Ix || !Iy || !mag || !px || !py || border < 1) return TOED_ERR_ARG;
+    if ((long long)ih * iw > (1LL << 30)) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(device_id));
+    const long long n = (long long)ih * iw;
+    hipLaunchKernelGGL(toed_nms_maps_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, ih, iw,
+                       Ix, Iy, mag, px, py, thresh, border);
     HIPCHK(hipGetLastError());
     return TOED_OK;
 }
