@@ -209,6 +209,26 @@ def test_bad_arguments(detector, pkg):
         detector.detect(img, border=0)
 
 
+def test_reference_class_mirror_double(pkg, scene, tmp_path):
+    """The T = double instantiation (main_gpu_cpu.cpp:200-209): the same
+    detection, its FP32 rows widened into the double buffer."""
+    img = scene.shapes_scene(31, 120, 150)
+    out = {}
+    for dt in (np.float32, np.float64):
+        cls = pkg.ThirdOrderEdgeDetectionGPU(0, 120, 150, 17, 2, dtype=dt)
+        cls.set_output_dir(str(tmp_path))
+        cls.preprocessing(img)
+        cls.convolve_img()
+        buf = np.zeros((4 * 120 * 150, 4), dt)
+        n = cls.non_maximum_suppresion(buf)
+        assert cls.subpix_edge_pts_final.dtype == dt
+        out[dt] = buf[:n]
+    assert len(out[np.float32]) > 100
+    assert np.array_equal(out[np.float64], out[np.float32].astype(np.float64))
+    with pytest.raises(NotImplementedError):
+        pkg.ThirdOrderEdgeDetectionGPU(0, 120, 150, 17, 2, dtype=np.float16)
+
+
 def test_reference_class_mirror(pkg, oracle, scene, tmp_path):
     img = scene.shapes_scene(31, 120, 150)
     cls = pkg.ThirdOrderEdgeDetectionGPU(0, 120, 150, 17, 2)

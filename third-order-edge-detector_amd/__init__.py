@@ -339,8 +339,11 @@ class ThirdOrderEdgeDetectionGPU:
     num_of_edge_data = 4
 
     def __init__(self, device, H, W, kernel_size=17, sigma=2, dtype="float32"):
-        if dtype not in ("float32", np.float32):
-            raise NotImplementedError("the GPU path computes in FP32 (T = float)")
+        # T = float or double: one computation (FP32 dense filters, FP64 per
+        # edgel); for double the FP32 rows are widened (main_gpu_cpu.cpp:200-209)
+        if np.dtype(dtype) not in (np.dtype(np.float32), np.dtype(np.float64)):
+            raise NotImplementedError("T is float32 or float64")
+        self.dtype = np.dtype(dtype)
         sig = float(sigma)
         r0, _ = sigma_radius(sig)
         if int(kernel_size) != 2 * r0 + 1:
@@ -359,7 +362,7 @@ class ThirdOrderEdgeDetectionGPU:
         self._img = torch.zeros((self.img_height, self.img_width), dtype=torch.uint8, device=dev)
         cap = self.interp_img_height * self.interp_img_width     # main_gpu_cpu.cpp:56-66
         self._edges = torch.empty((cap, 4), dtype=torch.float32, device=dev)
-        self.subpix_edge_pts_final = np.zeros((0, 4), np.float32)
+        self.subpix_edge_pts_final = np.zeros((0, 4), self.dtype)
         self._n = 0
 
     def preprocessing(self, image):
@@ -386,7 +389,7 @@ class ThirdOrderEdgeDetectionGPU:
 
     def non_maximum_suppresion(self, TOED_edges=None):
         n = self._n
-        self.subpix_edge_pts_final = self._edges[:n].cpu().numpy()
+        self.subpix_edge_pts_final = self._edges[:n].cpu().numpy().astype(self.dtype, copy=False)
         if TOED_edges is not None:
             TOED_edges[:n, :4] = self.subpix_edge_pts_final
         self.edge_pt_list_idx = n
