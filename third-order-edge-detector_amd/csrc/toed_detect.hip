@@ -817,8 +817,10 @@ constexpr int QWAVES = NT / 64;
 
 __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int njobs, ImageJob job0, StageBuf stg,
                                                       const int *tile_cnt) {
-    // per wavefront: [stencil column][full sums 0..3, R0-window sums 4..7][window row]
-    __shared__ double hs[QWAVES][4][8][QROWS];
+    // per wavefront: full sums [stencil column][filter][window row], and the
+    // R0-window sums of the even column of each pair [pair][filter][window row]
+    __shared__ double hs[QWAVES][4][4][QROWS];
+    __shared__ double hr[QWAVES][2][4][QROWS];
     const int tile = blockIdx.x;
     const ImageJob J = load_job(jobs, njobs, job0, tile);
     if (!J.qad) return;
@@ -826,7 +828,8 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
     const int R1 = J.R1;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double(*Hs)[8][QROWS] = hs[wave];
+    double(*Hs)[4][QROWS] = hs[wave];
+    double(*Hr)[4][QROWS] = hr[wave];
     const long long tile_local = tile - J.tile_begin;
     // the tile's edgels of all four phases are dealt to the wavefronts in turn
     const int n0 = min(tile_cnt[(long long)tile * 4 + 0], J.stage_cap);
@@ -891,7 +894,7 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
 #pragma unroll
                 for (int f = 0; f < 4; ++f) {
                     Hs[je][f][r] = fe[f];
-                    Hs[je][4 + f][r] = ie[f];
+                    Hr[hh][f][r] = ie[f];
                     Hs[jo][f][r] = fo[f];
                 }
             }
@@ -907,7 +910,7 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
             const int Rp = ry - 1 + i, Cp = rx - 1 + j, a = Rp & 1, b = Cp & 1, ci = Rp >> 1;
             const int lo = (a == 0 && b == 0) ? 1 : 0;
             const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
-            const double(*Hj)[QROWS] = Hs[j] + 4 * lo;
+            const double(*Hj)[QROWS] = lo ? Hr[j >> 1] : Hs[j];
             if (w != 0.0) {
                 for (int p = -R1 + lo; p <= R1 - lo; ++p) {
                     const int gi = ci - p;
