@@ -821,10 +821,16 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
     // R0-window sums of the even column of each pair [pair][filter][window row]
     __shared__ double hs[QWAVES][4][4][QROWS];
     __shared__ double hr[QWAVES][2][4][QROWS];
+    __shared__ double ytaps[2][4][MAXTAPS];     // du, ds: the row taps of the vertical pass
     const int tile = blockIdx.x;
     const ImageJob J = load_job(jobs, njobs, job0, tile);
     if (!J.qad) return;
     const TOED_CONST TapSet *tp = as_const(J.taps);
+    for (int k = threadIdx.x; k < 8 * MAXTAPS; k += NT) {
+        const int set = k / (4 * MAXTAPS), f = (k / MAXTAPS) & 3, t = k % MAXTAPS;
+        ytaps[set][f][t] = set ? tp->ds[f][t] : tp->du[f][t];
+    }
+    __syncthreads();
     const int R1 = J.R1;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -909,7 +915,7 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
             const double w = keys_w(Ys - ry + 1 - i) * keys_w(Xs - rx + 1 - j);
             const int Rp = ry - 1 + i, Cp = rx - 1 + j, a = Rp & 1, b = Cp & 1, ci = Rp >> 1;
             const int lo = (a == 0 && b == 0) ? 1 : 0;
-            const TOED_CONST double(*Y)[MAXTAPS] = a ? tp->ds : tp->du;
+            const double(*Y)[MAXTAPS] = ytaps[a];
             const double(*Hj)[QROWS] = lo ? Hr[j >> 1] : Hs[j];
             if (w != 0.0) {
                 for (int p = -R1 + lo; p <= R1 - lo; ++p) {
