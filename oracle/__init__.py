@@ -63,10 +63,10 @@ def lib():
                                          ctypes.c_double, ctypes.c_int, _dp, ctypes.c_long, _i64p, _dp, _dp]
         L.toed_subpix_oracle.restype = ctypes.c_long
         _i8p = ctypes.POINTER(ctypes.c_byte)
-        L.toed_subpix_frozen_state.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_long, _i8p, _i8p,
-                                               _ip, _ip, _ip, ctypes.c_double]
-        L.toed_subpix_frozen_loss.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _i64p, ctypes.c_long,
-                                              _dp, ctypes.c_int, _i8p, _i8p, _ip, _ip, _ip, _dp]
+        L.toed_subpix_frozen_state.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, _i64p, ctypes.c_long, _i8p,
+                                               _i8p, _ip, _ip, _ip, ctypes.c_double]
+        L.toed_subpix_frozen_loss.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, _i64p,
+                                              ctypes.c_long, _dp, ctypes.c_int, _i8p, _i8p, _ip, _ip, _ip, _dp]
         L.toed_subpix_frozen_loss.restype = ctypes.c_double
         _LIB = L
     return _LIB
@@ -365,12 +365,14 @@ class SubpixFrozen:
     """The subpixel correction path as a function of the map E with every
     discrete decision frozen at E0 (edgel set idx, octants, the half-turn of
     each pixel's direction, sign(-Oxy), cubic stencil bases): FP64, for
-    finite-difference checks of the GPU adjoint."""
+    finite-difference checks of the GPU adjoint.  theta: a given orientation
+    map (radians), a constant input; None estimates it from E."""
 
-    def __init__(self, E0, idx, sigma=2.0):
+    def __init__(self, E0, idx, sigma=2.0, theta=None):
         E0 = np.ascontiguousarray(E0, np.float64)
         self.h, self.w = E0.shape
         self.sigma = float(sigma)
+        self.theta = None if theta is None else np.ascontiguousarray(theta, np.float64)
         self.idx = np.ascontiguousarray(idx, np.int64)
         n, npx = len(self.idx), E0.size
         self.tneg = np.zeros(npx, np.int8)
@@ -380,9 +382,10 @@ class SubpixFrozen:
         self.fb = np.zeros(n, np.int32)
         i8 = ctypes.POINTER(ctypes.c_byte)
         ip = ctypes.POINTER(ctypes.c_int)
-        lib().toed_subpix_frozen_state(_p(E0), self.h, self.w, _p(self.idx, _i64p), n, self.tneg.ctypes.data_as(i8),
-                                       self.sxy.ctypes.data_as(i8), self.oct.ctypes.data_as(ip),
-                                       self.base.ctypes.data_as(ip), self.fb.ctypes.data_as(ip), self.sigma)
+        lib().toed_subpix_frozen_state(_p(E0), self.h, self.w, _p(self.theta), _p(self.idx, _i64p), n,
+                                       self.tneg.ctypes.data_as(i8), self.sxy.ctypes.data_as(i8),
+                                       self.oct.ctypes.data_as(ip), self.base.ctypes.data_as(ip),
+                                       self.fb.ctypes.data_as(ip), self.sigma)
 
     def loss(self, E, w, outputs=False):
         E = np.ascontiguousarray(E, np.float64)
@@ -392,8 +395,8 @@ class SubpixFrozen:
         outs = np.zeros((n, 4), np.float64)
         i8 = ctypes.POINTER(ctypes.c_byte)
         ip = ctypes.POINTER(ctypes.c_int)
-        L = lib().toed_subpix_frozen_loss(_p(E), self.h, self.w, self.sigma, _p(self.idx, _i64p), n, _p(w), per,
-                                          self.tneg.ctypes.data_as(i8), self.sxy.ctypes.data_as(i8),
+        L = lib().toed_subpix_frozen_loss(_p(E), self.h, self.w, _p(self.theta), self.sigma, _p(self.idx, _i64p), n,
+                                          _p(w), per, self.tneg.ctypes.data_as(i8), self.sxy.ctypes.data_as(i8),
                                           self.oct.ctypes.data_as(ip), self.base.ctypes.data_as(ip),
                                           self.fb.ctypes.data_as(ip), _p(outs))
         return (L, outs) if outputs else L

@@ -349,6 +349,8 @@ long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, d
  *     the |F| < 1e-12 fallback.
  * toed_subpix_frozen_state() extracts the state from E; toed_subpix_frozen_loss()
  * evaluates L = sum_e w_e . [x, y, orientation, strength] with it (FP64).
+ * With a given orientation map theta (set_orientation, :226-232) the direction
+ * (cos, sin)(theta mod pi) is a constant input and tneg / sxy are unused.
  * Central differences of the frozen loss pin the GPU adjoint.
  */
 static void sp_smooth(const double *E, int h, int w, double *S) {
@@ -432,21 +434,28 @@ static void sp_nms_interp(const double *E, int w, int i, int j, double nx, doubl
 
 /* frozen state at E: tneg[p] = 1 if t < 0 at pixel p, sxy[p] = sgn(-Oxy) (-1, 0, 1),
  * per edgel: octant, stencil base (floor x, floor y), fallback flag. */
-int toed_subpix_frozen_state(const double *E, int h, int w, const int64_t *idx, long n, signed char *tneg,
-                             signed char *sxy, int *oct, int *base, int *fallback, double sigma) {
+int toed_subpix_frozen_state(const double *E, int h, int w, const double *theta_in, const int64_t *idx, long n,
+                             signed char *tneg, signed char *sxy, int *oct, int *base, int *fallback, double sigma) {
     const size_t np = (size_t)h * w;
-    double *Axx = (double *)malloc(sizeof(double) * np), *Ayy = (double *)malloc(sizeof(double) * np),
-           *Axy = (double *)malloc(sizeof(double) * np);
-    sp_second(E, h, w, Axx, Ayy, Axy);
-    for (size_t p = 0; p < np; ++p) {
-        const double sg = Axy[p] < 0 ? 1.0 : (Axy[p] > 0 ? -1.0 : 0.0);
-        sxy[p] = (signed char)sg;
-        tneg[p] = (Ayy[p] * sg / Axx[p]) < 0.0 ? 1 : 0;
+    (void)sigma;
+    if (!theta_in) {
+        double *Axx = (double *)malloc(sizeof(double) * np), *Ayy = (double *)malloc(sizeof(double) * np),
+               *Axy = (double *)malloc(sizeof(double) * np);
+        sp_second(E, h, w, Axx, Ayy, Axy);
+        for (size_t p = 0; p < np; ++p) {
+            const double sg = Axy[p] < 0 ? 1.0 : (Axy[p] > 0 ? -1.0 : 0.0);
+            sxy[p] = (signed char)sg;
+            tneg[p] = (Ayy[p] * sg / Axx[p]) < 0.0 ? 1 : 0;
+        }
+        free(Axx); free(Ayy); free(Axy);
+    } else {
+        memset(sxy, 0, np);
+        memset(tneg, 0, np);
     }
-    free(Axx); free(Ayy); free(Axy);
     /* octants and cubic stencil bases at the forward point (:446-516, :521-530) */
     double *th = (double *)malloc(sizeof(double) * np);
-    toed_subpix_estimate_orientation(E, h, w, th);
+    if (theta_in) memcpy(th, theta_in, sizeof(double) * np);
+    else toed_subpix_estimate_orientation(E, h, w, th);
     for (long e = 0; e < n; ++e) {
         const size_t p = (size_t)idx[e];
         const int i = (int)(p / w), j = (int)(p % w);
@@ -466,23 +475,33 @@ int toed_subpix_frozen_state(const double *E, int h, int w, const int64_t *idx, 
 }
 
 /* The frozen loss.  w4: n x 4 weights (or one row of 4 when per_edgel == 0). */
-double toed_subpix_frozen_loss(const double *E, int h, int w, double sigma, const int64_t *idx, long n,
-                               const double *w4, int per_edgel, const signed char *tneg, const signed char *sxy,
-                               const int *oct, const int *base, const int *fallback, double *outs) {
+double toed_subpix_frozen_loss(const double *E, int h, int w, const double *theta_in, double sigma,
+                               const int64_t *idx, long n, const double *w4, int per_edgel, const signed char *tneg,
+                               const signed char *sxy, const int *oct, const int *base, const int *fallback,
+                               double *outs) {
     const size_t np = (size_t)h * w;
     int rad;
     double G[129], dG[129], d2G[129];
     if (toed_subpix_taps(sigma, &rad, G, dG, d2G)) return NAN;
-    double *Axx = (double *)malloc(sizeof(double) * np), *Ayy = (double *)malloc(sizeof(double) * np);
-    sp_second(E, h, w, Axx, Ayy, NULL);
     double *dxp = (double *)malloc(sizeof(double) * np), *dyp = (double *)malloc(sizeof(double) * np);
-    for (size_t p = 0; p < np; ++p) {
-        const double t = Ayy[p] * sxy[p] / Axx[p];
-        const double inv = 1.0 / sqrt(1.0 + t * t), sg = tneg[p] ? -1.0 : 1.0;
-        dxp[p] = sg * inv;
-        dyp[p] = sg * t * inv;
+    if (theta_in) {
+        /* given orientation: a constant direction (:296-305) */
+        for (size_t p = 0; p < np; ++p) {
+            const double a = wrap_mod_pi(theta_in[p]);
+            dxp[p] = cos(a);
+            dyp[p] = sin(a);
+        }
+    } else {
+        double *Axx = (double *)malloc(sizeof(double) * np), *Ayy = (double *)malloc(sizeof(double) * np);
+        sp_second(E, h, w, Axx, Ayy, NULL);
+        for (size_t p = 0; p < np; ++p) {
+            const double t = Ayy[p] * sxy[p] / Axx[p];
+            const double inv = 1.0 / sqrt(1.0 + t * t), sg = tneg[p] ? -1.0 : 1.0;
+            dxp[p] = sg * inv;
+            dyp[p] = sg * t * inv;
+        }
+        free(Axx); free(Ayy);
     }
-    free(Axx); free(Ayy);
     double *P[10], *tmp = (double *)malloc(sizeof(double) * np);
     for (int k = 0; k < 10; ++k) P[k] = (double *)malloc(sizeof(double) * np);
     for (size_t p = 0; p < np; ++p) { P[5][p] = E[p] * dxp[p]; P[9][p] = E[p] * dyp[p]; }

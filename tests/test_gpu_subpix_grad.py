@@ -35,9 +35,25 @@ def _forward(pkg, detector, maps, thresh=0.1, sigma=2.0, nbr=2.0):
     return b, dev
 
 
-def _check_map(oracle, E, idx, G, g, sigma, rng, label):
-    fr = oracle.SubpixFrozen(E, idx, sigma)
+def _check_rows(outs, rows, label):
+    """The frozen loss's rows at the forward point against the GPU forward's
+    rows: the FP32 list's tolerances (position 1e-4 px, orientation 1e-4 rad
+    modulo 2 pi, strength 1e-5 relative)."""
+    rows = np.asarray(rows, np.float64)
+    assert outs.shape == rows.shape, (label, outs.shape, rows.shape)
+    dxy = np.abs(outs[:, :2] - rows[:, :2]).max()
+    dth = np.abs(np.angle(np.exp(1j * (outs[:, 2] - rows[:, 2])))).max()
+    dst = (np.abs(outs[:, 3] - rows[:, 3]) / np.maximum(np.abs(outs[:, 3]), 1e-12)).max()
+    print("%s: frozen rows vs GPU rows: |dxy| %.2e px, |dtheta| %.2e rad, strength %.2e rel" % (label, dxy, dth, dst),
+          flush=True)
+    assert dxy <= 1e-4 and dth <= 1e-4 and dst <= 1e-5
+
+
+def _check_map(oracle, E, idx, G, g, sigma, rng, label, rows=None, theta=None):
+    fr = oracle.SubpixFrozen(E, idx, sigma, theta)
     L0, outs = fr.loss(E, G, outputs=True)
+    if rows is not None:
+        _check_rows(outs, rows, label)
     scale = np.abs(g).max()
     assert scale > 0
     # random directional derivative over every pixel
@@ -79,10 +95,36 @@ def test_subpix_grad_matches_fd(pkg, detector, oracle):
     gmap = torch.zeros_like(dev[0])
     b.backward([torch.from_numpy(G).cuda()], [gmap])
     g = gmap.cpu().numpy().astype(np.float64)
-    _check_map(oracle, E, idx, G.astype(np.float64), g, 2.0, rng, "96x128")
+    _check_map(oracle, E, idx, G.astype(np.float64), g, 2.0, rng, "96x128", rows=e.cpu().numpy())
     # the backward accumulates: a second call doubles the gradient
     b.backward([torch.from_numpy(G).cuda()], [gmap])
     np.testing.assert_allclose(gmap.cpu().numpy(), 2 * g, rtol=1e-5, atol=1e-6 * np.abs(g).max())
+    b.close()
+
+
+def test_subpix_grad_given_orientation(pkg, detector, oracle):
+    """A map with a given orientation map (set_orientation): the direction is a
+    constant input of the adjoint; FD of the frozen loss with the same FP32
+    orientation values."""
+    z = np.load(os.path.join(GOLD, "subpix_ref_bezier_128x160_s4_orient.npz"))
+    thr, sig, nbr = (float(v) for v in z["params"])
+    E32 = np.ascontiguousarray(z["E"], np.float32)
+    th32 = np.ascontiguousarray(z["theta"], np.float32)
+    E, th = E32.astype(np.float64), th32.astype(np.float64)
+    dev = [torch.from_numpy(E32).cuda()]
+    b = pkg.SubpixBatch(detector, dev, [torch.from_numpy(th32).cuda()], thr, sig, nbr, with_index=True)
+    b.set_grad(True)
+    b.run()
+    e = b.results()[0].cpu().numpy()
+    idx = b.index_results()[0].cpu().numpy().astype(np.int64)
+    ref = oracle.subpix(E, th, thr, sig, nbr)
+    assert np.array_equal(idx, ref["idx"]) and len(idx) > 100
+    rng = np.random.default_rng(13)
+    G = rng.standard_normal((len(idx), 4)).astype(np.float32)
+    gmap = torch.zeros_like(dev[0])
+    b.backward([torch.from_numpy(G).cuda()], [gmap])
+    g = gmap.cpu().numpy().astype(np.float64)
+    _check_map(oracle, E, idx, G.astype(np.float64), g, sig, rng, "given orientation 128x160", rows=e, theta=th)
     b.close()
 
 
@@ -97,7 +139,8 @@ def test_subpix_grad_batch_and_autograd(pkg, detector, oracle, scene):
     b.backward([torch.from_numpy(G).cuda() for G in Gs], gmaps)
     for m in range(B):
         _check_map(oracle, maps[m].astype(np.float64), idx[m].cpu().numpy().astype(np.int64),
-                   Gs[m].astype(np.float64), gmaps[m].cpu().numpy().astype(np.float64), 1.5, rng, "map %d" % m)
+                   Gs[m].astype(np.float64), gmaps[m].cpu().numpy().astype(np.float64), 1.5, rng, "map %d" % m,
+                   rows=res[m].cpu().numpy())
     b.close()
     # the autograd wrapper on a [B, H, W] stack gives the same as the plan API
     E = scene.bezier_map(73, 96, 128, 10)
@@ -114,6 +157,7 @@ def test_subpix_grad_batch_and_autograd(pkg, detector, oracle, scene):
         fr = oracle.SubpixFrozen(Em, ref["idx"], 2.0)
         d = np.random.default_rng(m).standard_normal(Em.shape)
         wn = w.cpu().numpy().astype(np.float64)
+        _check_rows(fr.loss(Em, wn, outputs=True)[1], rows.detach().cpu().numpy(), "autograd map %d" % m)
         fd = _fd(fr, Em, wn, d)
         an = float((gx[m] * d).sum())
         l1 = float(np.abs(gx[m] * d).sum())
@@ -146,5 +190,5 @@ def test_subpix_plan_rerun_on_sparser_map(pkg, detector, oracle, scene):
     b.backward([torch.from_numpy(G).cuda()], [gmap])
     g = gmap.cpu().numpy().astype(np.float64)
     assert np.isfinite(g).all()
-    _check_map(oracle, sparse.astype(np.float64), idx, G.astype(np.float64), g, 2.0, rng, "rerun 320x384")
+    _check_map(oracle, sparse.astype(np.float64), idx, G.astype(np.float64), g, 2.0, rng, "rerun 320x384", rows=e)
     b.close()

@@ -177,3 +177,49 @@ def test_maps_last_writer_in_list_order():
         TE[Y[k], X[k]] = e[k, 3]
     np.testing.assert_array_equal(TE, o["TO_E"])
     assert (o["TO_E"] != 0).sum() == len(e) - 1    # one collision in this map
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[11:-4] for p in GOLDEN])
+def test_frozen_loss_reproduces_reference_at_forward_point(path):
+    """oracle.SubpixFrozen, the finite-difference reference of the GPU adjoint,
+    is the forward path with its discrete decisions held: at the forward point
+    its outputs are the reference's own list (subpix_to_correction.cpp:623-736,
+    the golden rows) to 1e-12, its loss is sum w . rows, and moving the map
+    moves the rows (they are recomputed from E, not replayed)."""
+    z = np.load(path)
+    thr, sig, nbr = z["params"]
+    theta = z["theta"] if "theta" in z.files else None
+    E = z["E"].astype(np.float64)
+    ref = oracle.subpix(E, theta, thresh=thr, sigma=sig, nbr_radius=nbr)
+    np.testing.assert_array_equal(ref["edges"], z["edges"])
+    fr = oracle.SubpixFrozen(E, ref["idx"], sig, theta)
+    w = np.random.default_rng(1).standard_normal((ref["n"], 4))
+    L, outs = fr.loss(E, w, outputs=True)
+    assert outs.shape == z["edges"].shape
+    np.testing.assert_allclose(outs, z["edges"], rtol=0, atol=1e-12)
+    assert abs(L - float((w * outs).sum())) <= 1e-12 * float(np.abs(w * outs).sum())
+    # one shared weight row gives the same loss as that row repeated
+    w1 = np.array([0.3, -1.1, 0.8, 2.0])
+    assert abs(fr.loss(E, w1) - fr.loss(E, np.tile(w1, (ref["n"], 1)))) <= 1e-12 * abs(fr.loss(E, w1))
+    # a perturbed map gives other rows, of the size of the perturbation
+    d = np.random.default_rng(2).standard_normal(E.shape)
+    _, outs2 = fr.loss(E + 1e-6 * d, w, outputs=True)
+    delta = np.abs(outs2 - outs).max()
+    assert 1e-10 < delta < 1e-2
+
+
+def test_frozen_loss_given_orientation_is_constant_direction():
+    """With a given orientation map the direction is an input: the frozen
+    rows do not follow the map's Hessian (the estimated-orientation form on the
+    same map gives other rows)."""
+    z = np.load([g for g in GOLDEN if "orient" in g][0])
+    thr, sig, nbr = z["params"]
+    E = z["E"].astype(np.float64)
+    ref = oracle.subpix(E, z["theta"], thresh=thr, sigma=sig, nbr_radius=nbr)
+    given = oracle.SubpixFrozen(E, ref["idx"], sig, z["theta"])
+    est = oracle.SubpixFrozen(E, ref["idx"], sig, None)
+    w = np.ones(4)
+    _, a = given.loss(E, w, outputs=True)
+    _, b = est.loss(E, w, outputs=True)
+    np.testing.assert_allclose(a, z["edges"], rtol=0, atol=1e-12)
+    assert np.abs(a - b).max() > 1e-3

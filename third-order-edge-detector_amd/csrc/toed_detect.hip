@@ -1018,7 +1018,7 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
     __shared__ double sdp[3][N + 2][4];
     __shared__ float4 stf4[3][N + 2];
     __shared__ double sgs[NT / toedadj::G];
-    __shared__ int s_rc[BW_CHUNK];           // interp (row << 16 | col) of a chunk's edgels, -1: no output row
+    __shared__ int s_rc[BW_CHUNK];           // tile-local interp (row | col << 8) of a chunk's edgels, -1: no output row
     __shared__ float4 s_up[BW_CHUNK];        // their upstream gradient rows
     const int tile = tile_begin + blockIdx.x;
     const int m = job_index(jobs, njobs, tile);
@@ -1078,7 +1078,7 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
             const long long ce = J.counts_base + (long long)R * J.ntc + tc;
             const long long idx =
                 (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
-            s_rc[k] = idx < J.capacity ? (R << 16) | C : -1;
+            s_rc[k] = idx < J.capacity ? (r | (c << 8)) : -1;
             s_up[k] = idx < J.capacity ? reinterpret_cast<const float4 *>(O.gout)[idx] : make_float4(0, 0, 0, 0);
         }
         __syncthreads();
@@ -1086,7 +1086,7 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
         for (int t = g; t < S * GPW; t += NT / toedadj::G) {
             const int k = (t % GPW) * S + t / GPW;
             if (k >= cn || s_rc[k] < 0) continue;
-            const int R = s_rc[k] >> 16, C = s_rc[k] & 0xffff;
+            const int R = 2 * i0 + (s_rc[k] & 0xff), C = 2 * j0 + (s_rc[k] >> 8);
             double gs = 0.0;
             toedadj::edgel_backward<R1>(R, C, s_up[k], l, J.H, J.W, stp, sdp, stf4, img, sink, &gs);
             if (l == 0) sgs[g] += gs;
