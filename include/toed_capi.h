@@ -116,6 +116,20 @@ int toed_last_times(toed_ctx *ctx, float *ms_tiles, float *ms_finalize);
 int toed_detect(toed_ctx *ctx, const toed_image_t *img, int *d_count, int *d_flags);
 /* Synchronous convenience: returns N in *n_out and maps flags to a status. */
 int toed_detect_sync(toed_ctx *ctx, const toed_image_t *img, long long *n_out);
+/* toed_detect without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as toed_detect; its scratch (taps, counts,
+ * staging) comes from one array of TOED_NOALLOC_SCRATCH_BYTES that the library
+ * holds on each GPU until the process ends, and staging is capped at 64
+ * records per tile and phase (TOED_FLAG_STAGING when a tile has more).  That
+ * holds a 2160 x 3840 frame at the default 16-row tiles; an image needing more
+ * returns TOED_ERR_ARG.  Every call on a GPU uses the same array: calls that
+ * overlap on one GPU (two streams, two host threads) race and must be
+ * serialised by the caller.  The context must have been created by
+ * toed_create, which resolves the array's address on its device. */
+#define TOED_NOALLOC_SCRATCH_BYTES (96u << 20)
+int toed_detect_noalloc(toed_ctx *ctx, const toed_image_t *img, int *d_count, int *d_flags);
 
 /* Batch: many images of uneven sizes and sigma in one launch sequence ------- */
 int toed_batch_plan(toed_ctx *ctx, int n, const toed_image_t *imgs, toed_batch **out);

@@ -113,6 +113,7 @@ SIGNATURES = [
     ("toed_last_times", _I, [_VP, _PF, _PF]),
     ("toed_detect", _I, [_VP, ctypes.POINTER(toed_image_t), _VP, _VP]),
     ("toed_detect_sync", _I, [_VP, ctypes.POINTER(toed_image_t), ctypes.POINTER(_LL)]),
+    ("toed_detect_noalloc", _I, [_VP, ctypes.POINTER(toed_image_t), _VP, _VP]),
     ("toed_batch_plan", _I, [_VP, _I, ctypes.POINTER(toed_image_t), ctypes.POINTER(_VP)]),
     ("toed_batch_set_persistent", _I, [_VP, _I]),
     ("toed_batch_run", _I, [_VP, _VP, _VP]),

@@ -661,6 +661,34 @@ __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const Imag
     }
 }
 
+// Taps of one sigma passed by value (the forward detection's part of a
+// TapSet), so that the allocation-free entry point can place them in its
+// static scratch array with a kernel launch instead of a copy.
+struct TapArgs {
+    int R0, R1;
+    float fu[2][MAXTAPS];
+    float fs[2][MAXTAPS];
+    double du[4][MAXTAPS];
+    double ds[4][MAXTAPS];
+};
+
+__global__ __launch_bounds__(256) void toed_put_taps_kernel(TapSet *dst, TapArgs a) {
+    const int t = threadIdx.x;
+    if (t == 0) { dst->R0 = a.R0; dst->R1 = a.R1; dst->pad0 = 0; dst->pad1 = 0; }
+    if (t < 2 * MAXTAPS) {
+        const int f = t / MAXTAPS, k = t % MAXTAPS;
+        dst->fu[f][k] = a.fu[f][k];
+        dst->fs[f][k] = a.fs[f][k];
+    }
+    if (t < 4 * MAXTAPS) {
+        const int f = t / MAXTAPS, k = t % MAXTAPS;
+        dst->du[f][k] = a.du[f][k];
+        dst->ds[f][k] = a.ds[f][k];
+        dst->dsu[f][k] = 0.0;    // the sigma derivatives belong to the backward, which this form has not
+        dst->dss[f][k] = 0.0;
+    }
+}
+
 // K0: per-image flag reset (a kernel, not a memset: safe inside captured graphs)
 __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1343,6 +1371,20 @@ struct toed_batch {
 
 static std::mutex g_mu;
 
+// Scratch of the allocation-free entry point (toed_detect_noalloc): one
+// statically sized array per GPU, loaded with the code object and held until
+// the process ends.  The host never passes `scratch` itself to a kernel (in
+// host code the name is a stand-in, not a device address): each device's
+// address comes from hipGetSymbolAddress once, at context creation, and is
+// kept in g_noalloc_ptr.  Its size holds a 2160 x 3840 frame at 16-row tiles
+// (taps, counts and their scan, per-tile counts, 64 staging records per tile
+// and phase: 87.4 MB).
+constexpr size_t NOALLOC_BYTES = TOED_NOALLOC_SCRATCH_BYTES;
+constexpr int NOALLOC_STAGE_CAP = 64;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static unsigned char *g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
 #define HIPCHK(x)                                              \
     do {                                                       \
         hipError_t e_ = (x);                                   \
@@ -1447,13 +1489,14 @@ static const void *tiles_kernel_ptr() { return (const void *)toed_tiles_kernel<I
 
 // enqueue the kernels for jobs (device table) or job0 (single image)
 static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, int njobs, const ImageJob &job0,
-                   const std::vector<TileGroup> &groups, long long ntiles, long long nchunks, int *counts,
+                   const TileGroup *groups, int ngroups, long long ntiles, long long nchunks, int *counts,
                    int *offsets, int *chunks, StageBuf staging, int *tilecnt, int *tickets, hipEvent_t *ev,
                    bool qad) {
     if (ntiles <= 0) return TOED_OK;
     hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
-    for (const TileGroup &g : groups) {
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const TileGroup &g = groups[gi];
         if (g.tile_end <= g.tile_begin) continue;
         int *ticket = tickets ? tickets + group_of(g.R1) : nullptr;
         const dim3 grid(ticket ? (unsigned)g.grid_persistent : (unsigned)(g.tile_end - g.tile_begin));
@@ -1511,6 +1554,16 @@ int toed_create(int device, toed_ctx **out) {
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return TOED_ERR_ARG;
     HIPCHK(hipSetDevice(device));
+    if (device < NOALLOC_MAX_DEVICES) {
+        // the allocation-free entry point's array on this device: its address,
+        // once per device, with the device current
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!g_noalloc_ptr[device]) {
+            void *p = nullptr;
+            HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_noalloc_ptr[device] = (unsigned char *)p;
+        }
+    }
     toed_ctx *c = new toed_ctx();
     c->device = device;
     // blocking stream: orders with the legacy default stream, so work launched
@@ -1607,7 +1660,7 @@ static int detect_impl(toed_ctx *c, const toed_image_t *im, int *d_count, int *d
     J.counts_base = 0; J.chunk_base = 0; J.stage_base = 0; J.tile_begin = 0;
     TileGroup g;
     g.R1 = J.R1; g.tile_begin = 0; g.tile_end = (int)ntiles;
-    return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, std::vector<TileGroup>{g}, ntiles, J.nchunks,
+    return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, &g, 1, ntiles, J.nchunks,
                    c->d_counts, c->d_offsets, c->d_chunks, StageBuf{c->d_srec, c->d_skey}, c->d_tilecnt, nullptr,
                    timed ? c->ev : nullptr, J.qad != 0);
 }
@@ -1629,6 +1682,66 @@ int toed_detect_sync(toed_ctx *c, const toed_image_t *im, long long *n_out) {
     if (h[1] & TOED_FLAG_STAGING) return TOED_ERR_STAGING;
     if (h[1] & TOED_FLAG_CAPACITY) return TOED_ERR_CAPACITY;
     return TOED_OK;
+}
+
+// toed_detect with every scratch buffer carved from the device's static array
+// (no hipMalloc, no copy): taps by a kernel with by-value arguments, then the
+// same kernels as toed_detect.  Nothing in the array has to start at zero: the
+// tile kernel writes every count and per-tile count the scans read, staging
+// records are written before they are read, and the flags are reset by a
+// kernel.  Overlapping calls on one GPU share the array and race.
+int toed_detect_noalloc(toed_ctx *c, const toed_image_t *im, int *d_count, int *d_flags) {
+    if (!c || !d_count) return TOED_ERR_ARG;
+    int st = check_image(im);
+    if (st) return st;
+    if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES || !g_noalloc_ptr[c->device]) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    int R0, R1;
+    if ((st = radius_for(im->sigma, &R0, &R1))) return st;
+    const int TH = c->tile_rows;
+    ImageJob J;
+    std::memset(&J, 0, sizeof J);
+    J.img = im->d_img; J.out = im->d_edges; J.count_out = d_count; J.flags_out = d_flags;
+    J.index_out = im->d_index;
+    J.capacity = im->capacity; J.in_type = im->in_type; J.H = im->H; J.W = im->W; J.pitch = im->pitch;
+    J.R0 = R0; J.R1 = R1;
+    J.border = im->border; J.thresh = im->thresh;
+    J.qad = c->orient_mode;
+    job_geometry(J, TH, std::min(NOALLOC_STAGE_CAP, stage_cap_for(c->stage_frac, TH)));
+    const long long ntiles = (long long)J.ntr * J.ntc;
+    const long long ncounts = (long long)J.nchunks * CHUNK;
+    // carve the array: 256-byte aligned pieces
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t o_taps = 0;
+    const size_t o_counts = o_taps + up(sizeof(TapSet));
+    const size_t o_offsets = o_counts + up(sizeof(int) * ncounts);
+    const size_t o_chunks = o_offsets + up(sizeof(int) * ncounts);
+    const size_t o_tilecnt = o_chunks + up(sizeof(int) * J.nchunks);
+    const size_t o_srec = o_tilecnt + up(sizeof(int) * ntiles * 4);
+    const size_t o_skey = o_srec + up(sizeof(float4) * ntiles * 4 * J.stage_cap);
+    const size_t end = o_skey + up(sizeof(uint32_t) * ntiles * 4 * J.stage_cap);
+    if (end > NOALLOC_BYTES) {
+        std::fprintf(stderr, "toed_detect_noalloc: a %d x %d image needs %zu bytes of scratch, more than the %zu "
+                             "the library holds\n", J.H, J.W, end, NOALLOC_BYTES);
+        return TOED_ERR_ARG;
+    }
+    unsigned char *base = g_noalloc_ptr[c->device];
+    TapSet h;
+    fill_taps(im->sigma, R0, R1, &h);
+    TapArgs a;
+    a.R0 = R0; a.R1 = R1;
+    std::memcpy(a.fu, h.fu, sizeof a.fu); std::memcpy(a.fs, h.fs, sizeof a.fs);
+    std::memcpy(a.du, h.du, sizeof a.du); std::memcpy(a.ds, h.ds, sizeof a.ds);
+    TapSet *taps = (TapSet *)(base + o_taps);
+    hipLaunchKernelGGL(toed_put_taps_kernel, dim3(1), dim3(256), 0, c->stream, taps, a);
+    J.taps = taps;
+    J.counts_base = 0; J.chunk_base = 0; J.stage_base = 0; J.tile_begin = 0;
+    TileGroup g;
+    g.R1 = J.R1; g.tile_begin = 0; g.tile_end = (int)ntiles;
+    return enqueue(c->stream, J.in_type, TH, nullptr, 1, J, &g, 1, ntiles, J.nchunks,
+                   (int *)(base + o_counts), (int *)(base + o_offsets), (int *)(base + o_chunks),
+                   StageBuf{(float4 *)(base + o_srec), (uint32_t *)(base + o_skey)}, (int *)(base + o_tilecnt),
+                   nullptr, nullptr, J.qad != 0);
 }
 
 // persistent grid of each radius group: wg_per_cu workgroups on every CU
@@ -1757,7 +1870,8 @@ static int batch_upload(toed_batch *b, int *d_counts, int *d_flags) {
 static int batch_enqueue(toed_batch *b, hipStream_t s) {
     ImageJob dummy;
     std::memset(&dummy, 0, sizeof dummy);
-    return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->groups, b->ntiles, b->nchunks,
+    return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->groups.data(), (int)b->groups.size(),
+                   b->ntiles, b->nchunks,
                    b->d_counts, b->d_offsets, b->d_chunks, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
                    b->persistent ? b->d_ticket : nullptr, nullptr, b->qad);
 }
