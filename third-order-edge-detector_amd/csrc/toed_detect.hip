@@ -17,19 +17,20 @@
 //        staged once in LDS; the 2-D filters are applied separably (row pass,
 //        then column pass) in FP32 in three rounds that reuse two LDS planes:
 //        (R0-window G, G') -> phase (0,0); + end taps -> phase (1,0);
-//        (shifted G, G') -> phases (0,1), (1,1).  Ix, Iy, |grad I| of the tile
-//        plus a one-pixel interp halo stay in LDS; each wavefront runs NMS on one
+//        (shifted G, G') -> phases (0,1), (1,1).  Ix, Iy of the tile plus a
+//        one-pixel interp halo stay in LDS; each wavefront runs NMS on one
 //        64-wide interp row at a time, decides near-peaks and the parabola in
 //        FP64, and marks survivors in per-row masks: per-(row, tile) counts go to
-//        HBM, records to a per-tile, per-phase staging area; then the nine
-//        responses at each survivor in FP64 from the staged image (8 lanes per
-//        edgel) and its third-order orientation.
+//        HBM, records (position, strength) and raster keys to a per-tile,
+//        per-phase staging area.
 //   K2 toed_scan_chunks / toed_scan_images   exclusive scan of the counts
 //        (row-major over (interp row, tile column)), segmented per image.
 //   K2b toed_qad_kernel   orientation mode 1 only: the orientation evaluated at
 //        each edgel's subpixel location instead (one wavefront per edgel).
-//   K3 toed_finalize_kernel   one wavefront per tile: each staged edgel to
-//        offset[row, tile] + rank (raster order).
+//   K3 toed_finalize_kernel   one workgroup per tile: the nine responses of each
+//        staged edgel in FP64 from the tile's image region (8 lanes per edgel),
+//        its third-order orientation, and the edgel to offset[row, tile] + rank
+//        (raster order).
 // No MFMA: the filters are short separable stencils, not a contraction.
 
 #include <hip/hip_runtime.h>
@@ -183,7 +184,7 @@ template <int R1, int TH> struct Geo {
     static constexpr int LDR = RW;
     // LDS layout (floats): staged image | Ix, Iy with 1-px halo | two row-pass
     // planes, later reused for the near-peak queue (which becomes the
-    // survivor queue) + FP64 taps | row masks.  |grad I| is recomputed from
+    // survivor queue) | row masks.  |grad I| is recomputed from
     // Ix, Iy where needed: without its plane the u8 16-row tile fits in 35.9 KB
     // and four workgroups share a CU
     static constexpr int OFF_IMG = 0;
@@ -193,7 +194,7 @@ template <int R1, int TH> struct Geo {
     static constexpr int OFF_P1 = OFF_P0 + NRI * LDP;
     static constexpr int QMAX = 2 * TH * 2 * TW;                     // survivor queue (ints)
     static constexpr int P_END = OFF_P1 + NRI * LDP;
-    static constexpr int Q_END = OFF_P0 + QMAX + 2 * 12 * (2 * R1 + 1);
+    static constexpr int Q_END = OFF_P0 + QMAX;
     static constexpr int OFF_CAND = OFF_P0;      // near-peak queue (ints), then the survivor queue
     static constexpr int OFF_MASK = ((P_END > Q_END ? P_END : Q_END) + 1) & ~1;   // 2TH u64 row masks
     static constexpr int TOTAL = OFF_MASK + 2 * (2 * TH);
@@ -432,78 +433,81 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
     }
 }
 
-// Third-order orientation of the tile's survivors (cpu_toed.cpp:227-232) in
-// FP64 from the LDS image tile: 8 lanes per edgel, lane l the window rows
-// p = -R1 + l + 8k (the fewest idle lanes for 2R1+1 = 11, 15, 19 rows), the row
-// sums for the four x families over the columns (unrolled by 4; the four taps
-// of a column are adjacent: tapd[(fam * N + t) * 4 + f]), then the nine products
-// with the row's y taps, summed across the 8 lanes (DPP).  Phase (0,0) uses the R0
-// window through taps whose end values are zero.
-template <int R1, int TH>
-__device__ __forceinline__ void theta_tile(float *lds, const ImageJob &J, long long tile_local, StageBuf stg,
-                                  const int *queue, int qn, const double *tapd, const unsigned long long *rowmask) {
-    using Gm = Geo<R1, TH>;
+// Third-order orientation (cpu_toed.cpp:227-232) of staged edgels, in FP64,
+// run by the scatter kernel on the tile's image region, which the wavefront
+// has staged in LDS (rows and columns from 1 + R1 before the tile, zero outside
+// the image, cpu_toed.cpp:207-208; row pitch FLDI): 8 lanes per edgel, lane l
+// the window rows p = -R1 + l + 8k (the fewest idle lanes for 2R1+1 = 11, 15,
+// 19 rows), the row sums for the four x families over the columns (unrolled by
+// 4), then the nine products with the row's y taps, summed across the 8 lanes
+// (DPP).  Phase (0,0) uses the R0 window through taps whose end values are
+// zero.  tapd: [family][tap][f] with families 0 unshifted, 1 unshifted with
+// zero end taps, 2 shifted.  (ci, cj): the edgel's image pixel in the region.
+constexpr int FLDI = (TW + 2 + 2 * MAXR1) | 1;            // region row pitch (floats)
+constexpr int FROWS = 16 + 2 + 2 * MAXR1;                 // region rows at most (16-row tiles)
+
+template <int R1, int GT>
+__device__ __forceinline__ double theta_group(const float *img, int ci, int cj, int a, int b, int l,
+                                              const double (*tapd)[MAXTAPS][4]) {
     constexpr int N = 2 * R1 + 1;
-    constexpr int GT = 8, NK = (N + GT - 1) / GT;
-    const float *img = lds + Gm::OFF_IMG;
-    const int g = threadIdx.x / GT, l = threadIdx.x % GT;
-    for (int e0 = 0; e0 < qn; e0 += NT / GT) {
-        const int e = e0 + g;
-        const bool valid = e < qn;
-        const int q = valid ? queue[e] : 0;
-        const int r = q & 63, c = (q >> 6) & 63, slot = q >> 12;
-        const int a = r & 1, b = c & 1, phase = 2 * a + b;
-        // tap families: 0 unshifted, 1 unshifted with zero end taps, 2 shifted
-        const double *X = tapd + (b ? 2 : (phase == 0 ? 1 : 0)) * 4 * N;
-        const double *Y = tapd + (a ? 2 : (phase == 0 ? 1 : 0)) * 4 * N;
-        const int ci = (r >> 1) + 1 + R1, cj = (c >> 1) + 1 + R1;
-        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int NK = (N + GT - 1) / GT;
+    const int phase = 2 * a + b;
+    const double(*X)[4] = tapd[b ? 2 : (phase == 0 ? 1 : 0)];
+    const double(*Y)[4] = tapd[a ? 2 : (phase == 0 ? 1 : 0)];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // row sums of the lane's rows (rows beyond the window read a valid row and
+    // are dropped below), each column's taps read once for all of them
+    double h[NK][4];
+    const float *rows[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const int p = min(-R1 + l + GT * k, R1);
+        rows[k] = img + (ci - p) * FLDI + cj;
+        h[k][0] = h[k][1] = h[k][2] = h[k][3] = 0.0;
+    }
+#pragma unroll 2
+    for (int qq = -R1; qq <= R1; ++qq) {
+        const double *x = X[qq + R1];
+        const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-            const int p = -R1 + l + GT * k;
-            if (valid && p <= R1) {
-                const float *row = img + (ci - p) * Gm::LDI + cj;
-                double h0 = 0, h1 = 0, h2 = 0, h3 = 0;
-#pragma unroll 4
-                for (int qq = -R1; qq <= R1; ++qq) {
-                    const double v = (double)row[-qq];
-                    const double *x = X + 4 * (qq + R1);
-                    h0 = fma(v, x[0], h0);
-                    h1 = fma(v, x[1], h1);
-                    h2 = fma(v, x[2], h2);
-                    h3 = fma(v, x[3], h3);
-                }
-                const double *y = Y + 4 * (p + R1);
-                const double y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3];
-                acc[0] = fma(h1, y0, acc[0]);   // fx
-                acc[1] = fma(h0, y1, acc[1]);   // fy
-                acc[2] = fma(h2, y0, acc[2]);   // fxx
-                acc[3] = fma(h1, y1, acc[3]);   // fxy
-                acc[4] = fma(h0, y2, acc[4]);   // fyy
-                acc[5] = fma(h2, y1, acc[5]);   // fxxy
-                acc[6] = fma(h1, y2, acc[6]);   // fxyy
-                acc[7] = fma(h3, y0, acc[7]);   // fxxx
-                acc[8] = fma(h0, y3, acc[8]);   // fyyy
-            }
-        }
-#pragma unroll
-        for (int d = 0; d < 9; ++d) acc[d] = toeddpp::group_sum<GT>(acc[d]);
-        if (valid && l == 0) {
-            const double fx = acc[0], fy = acc[1], fxx = acc[2], fxy = acc[3], fyy = acc[4];
-            const double fxxy = acc[5], fxyy = acc[6], fxxx = acc[7], fyyy = acc[8];
-            double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
-                      + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
-            double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
-                      + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
-            const double m = sqrt(tx * tx + ty * ty);
-            tx /= m;
-            ty /= m;
-            const long long gi = J.stage_base + tile_local * 4 * J.stage_cap + slot;
-            stg.rec[gi].z = (float)atan2(tx, -ty);
-            const int rank = __popcll(rowmask[r] & ((1ull << c) - 1ull));
-            stg.key[gi] = (uint32_t)r | ((uint32_t)c << 8) | ((uint32_t)rank << 16);
+            const double v = (double)rows[k][-qq];
+            h[k][0] = fma(v, x0, h[k][0]);
+            h[k][1] = fma(v, x1, h[k][1]);
+            h[k][2] = fma(v, x2, h[k][2]);
+            h[k][3] = fma(v, x3, h[k][3]);
         }
     }
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const int p = -R1 + l + GT * k;
+        if (p <= R1) {
+            const double h0 = h[k][0], h1 = h[k][1], h2 = h[k][2], h3 = h[k][3];
+            const double *y = Y[p + R1];
+            const double y0 = y[0], y1 = y[1], y2 = y[2], y3 = y[3];
+            acc[0] = fma(h1, y0, acc[0]);   // fx
+            acc[1] = fma(h0, y1, acc[1]);   // fy
+            acc[2] = fma(h2, y0, acc[2]);   // fxx
+            acc[3] = fma(h1, y1, acc[3]);   // fxy
+            acc[4] = fma(h0, y2, acc[4]);   // fyy
+            acc[5] = fma(h2, y1, acc[5]);   // fxxy
+            acc[6] = fma(h1, y2, acc[6]);   // fxyy
+            acc[7] = fma(h3, y0, acc[7]);   // fxxx
+            acc[8] = fma(h0, y3, acc[8]);   // fyyy
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < 9; ++d) acc[d] = toeddpp::group_sum<GT>(acc[d]);
+    const double fx = acc[0], fy = acc[1], fxx = acc[2], fxy = acc[3], fyy = acc[4];
+    const double fxxy = acc[5], fxyy = acc[6], fxxx = acc[7], fyyy = acc[8];
+    double tx = fx * (2 * fxx * fxx + 2 * fxy * fxy) + fy * (2 * fxx * fxy + 2 * fyy * fxy)
+              + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
+    double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
+              + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
+    const double m = sqrt(tx * tx + ty * ty);
+    tx /= m;
+    ty /= m;
+    return atan2(tx, -ty);
 }
 
 // Image staging of a tile (the tile + 1 + R1 halo; zero outside the image,
@@ -639,21 +643,21 @@ __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const Imag
     __syncthreads();
     col_pass<R1, TH, R1, 2>(lds, 1, tp->fu[0], tp->fu[1], 0, tp->fs[0], tp->fs[1], 1);
     __syncthreads();
-    // the row-pass planes are dead now: the survivor queue and the FP64 taps
-    // of the orientation pass live in their place
+    // the row-pass planes are dead now: the survivor queue lives in their place
     int *queue = (int *)(lds + Gm::OFF_P0);
-    double *tapd = (double *)(lds + Gm::OFF_P0 + Gm::QMAX);
+    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
+    // each survivor's raster key (local row, local column, rank in its row):
+    // the row masks are final after the barrier inside nms_tile
     {
-        constexpr int N = 2 * R1 + 1;
-        for (int k = threadIdx.x; k < 12 * N; k += NT) {
-            const int fam = k / (4 * N), f = (k / N) & 3, t = k % N;
-            double v = (fam == 2) ? tp->ds[f][t] : tp->du[f][t];
-            if (fam == 1 && (t == 0 || t == N - 1)) v = 0.0;
-            tapd[(fam * N + t) * 4 + f] = v;
+        const int qn = sh_cnt[4];
+        for (int e = threadIdx.x; e < qn; e += NT) {
+            const int q = queue[e];
+            const int r = q & 63, c = (q >> 6) & 63, slot = q >> 12;
+            const int rank = __popcll(rowmask[r] & ((1ull << c) - 1ull));
+            stg.key[J.stage_base + tile_local * 4 * J.stage_cap + slot] =
+                (uint32_t)r | ((uint32_t)c << 8) | ((uint32_t)rank << 16);
         }
     }
-    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
-    theta_tile<R1, TH>(lds, J, tile_local, stg, queue, sh_cnt[4], tapd, rowmask);
     if (threadIdx.x < 4) {
         const int n = sh_cnt[threadIdx.x];
         tile_cnt[(long long)tile * 4 + threadIdx.x] = n;
@@ -988,37 +992,140 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
     }
 }
 
-// K3: scatter every staged edgel to offset[row, tile column] + rank, the
-// reference's raster order of the interp grid (cpu_toed.cpp:510-538).  One
-// wavefront per tile (a tile stages a handful of edgels per phase).
-__global__ __launch_bounds__(NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0, int TH,
-                                                           const int *offsets, const int *chunk_pref,
-                                                           StageBuf stg, const int *tile_cnt, int ntiles,
-                                                           int *ticket) {
-    const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-    if (tile < ntiles) {
-        const ImageJob J = load_job(jobs, njobs, job0, tile);
-        const long long tile_local = tile - J.tile_begin;
-        const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
-        const int IW = 2 * J.W;
-        for (int phase = 0; phase < 4; ++phase) {
-            const int n = min(tile_cnt[(long long)tile * 4 + phase], J.stage_cap);
-            const long long base = J.stage_base + (tile_local * 4 + phase) * J.stage_cap;
-            for (int e = lane; e < n; e += 64) {
-                const uint32_t key = stg.key[base + e];
-                const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
-                const int R = 2 * tr * TH + r, C = 2 * tc * TW + c;
-                const long long ce = J.counts_base + (long long)R * J.ntc + tc;
-                const long long idx = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
-                if (idx >= J.capacity) continue;
-                reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[base + e];
-                if (J.index_out) J.index_out[idx] = R * IW + C;
+// K3: the third-order orientation of every staged edgel (theta_group, 8 lanes
+// per edgel; in orientation mode 1 toed_qad_kernel has already set it), then
+// the edgel to offset[row, tile column] + rank, the reference's raster order of
+// the interp grid (cpu_toed.cpp:510-538).  One wavefront per tile (a tile
+// stages a handful of edgels per phase).
+__device__ __forceinline__ long long out_index(const ImageJob &J, const int *offsets, const int *chunk_pref, int R,
+                                               int tc, int rank) {
+    const long long ce = J.counts_base + (long long)R * J.ntc + tc;
+    return (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
+}
+
+// One workgroup of four wavefronts per tile: a tile without edgels ends at
+// once; a tile's edgels are dealt to its 32 groups of 8 lanes, so the tiles
+// with many edgels (along the scene's contours) finish in few rounds.
+constexpr int FIN_NT = 256;
+constexpr int FIN_GT = 8;                  // lanes per edgel
+constexpr int FIN_G = FIN_NT / FIN_GT;     // lane groups per workgroup
+
+__global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0, int TH,
+                                                               const int *offsets, const int *chunk_pref,
+                                                               StageBuf stg, const int *tile_cnt, int ntiles,
+                                                               int *ticket) {
+    __shared__ double tapd[3][MAXTAPS][4];
+    __shared__ float fimg[FROWS * FLDI];
+    __shared__ float4 s_rec[FIN_NT];       // a chunk of staged edgels: record,
+    __shared__ long long s_idx[FIN_NT];    // output row (-1: none),
+    __shared__ int s_rc[FIN_NT];           // tile-local interp row | col << 8
+    // the persistent tile kernels' tickets are re-armed once every tile is final
+    if (ticket && blockIdx.x == gridDim.x - 1 && threadIdx.x < NGROUPS) ticket[threadIdx.x] = 0;
+    const int tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const ImageJob J = load_job(jobs, njobs, job0, tile);
+    const long long tile_local = tile - J.tile_begin;
+    const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
+    const int IW = 2 * J.W;
+    int nph[4];
+#pragma unroll
+    for (int ph = 0; ph < 4; ++ph) nph[ph] = min(tile_cnt[(long long)tile * 4 + ph], J.stage_cap);
+    const int total = nph[0] + nph[1] + nph[2] + nph[3];
+    if (total == 0) return;
+    if (J.qad) {
+        // orientation mode 1: toed_qad_kernel has set it; scatter only
+        for (int e = threadIdx.x; e < total; e += FIN_NT) {
+            int k = e, phase = 0;
+            while (phase < 3 && k >= nph[phase]) { k -= nph[phase]; ++phase; }
+            const long long gi = J.stage_base + (tile_local * 4 + phase) * J.stage_cap + k;
+            const uint32_t key = stg.key[gi];
+            const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
+            const int R = 2 * tr * TH + r, C = 2 * tc * TW + c;
+            const long long idx = out_index(J, offsets, chunk_pref, R, tc, rank);
+            if (idx >= J.capacity) continue;
+            reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[gi];
+            if (J.index_out) J.index_out[idx] = R * IW + C;
+        }
+        return;
+    }
+    // the FP64 taps of this tile's sigma and the tile's image region (rows and
+    // columns from 1 + R1 before the tile, zero outside the image)
+    const TOED_CONST TapSet *tp = as_const(J.taps);
+    const int R1 = J.R1, N = 2 * R1 + 1;
+    for (int k = threadIdx.x; k < 12 * N; k += FIN_NT) {
+        const int fam = k / (4 * N), f = (k / N) & 3, t = k % N;
+        double v = (fam == 2) ? tp->ds[f][t] : tp->du[f][t];
+        if (fam == 1 && (t == 0 || t == N - 1)) v = 0.0;
+        tapd[fam][t][f] = v;
+    }
+    {
+        const int nri = TH + 2 + 2 * R1, nci = TW + 2 + 2 * R1;
+        const int gi0 = tr * TH - 1 - R1, gj0 = tc * TW - 1 - R1;
+        const int col = threadIdx.x & 63, r0 = threadIdx.x >> 6;      // 4 rows per pass, a lane per column
+        const int gj = gj0 + col;
+        const bool colok = col < nci && gj >= 0 && gj < J.W;
+        const int gjc = min(max(gj, 0), J.W - 1);
+        constexpr int NPASS = (FROWS + 3) / 4;
+        float vals[NPASS];      // every row's load issued before the first store
+        if (J.in_type == TOED_U8) {
+            const uint8_t *src = (const uint8_t *)J.img + gjc;
+#pragma unroll
+            for (int u = 0; u < NPASS; ++u)
+                vals[u] = (float)src[(long long)min(max(gi0 + r0 + 4 * u, 0), J.H - 1) * J.pitch];
+        } else {
+            const float *src = (const float *)J.img + gjc;
+#pragma unroll
+            for (int u = 0; u < NPASS; ++u) vals[u] = src[(long long)min(max(gi0 + r0 + 4 * u, 0), J.H - 1) * J.pitch];
+        }
+        if (col < nci) {
+#pragma unroll
+            for (int u = 0; u < NPASS; ++u) {
+                const int rr = r0 + 4 * u, gi = gi0 + rr;
+                if (rr < nri) fimg[rr * FLDI + col] = (colok && gi >= 0 && gi < J.H) ? vals[u] : 0.f;
             }
         }
     }
-    // the persistent tile kernels' tickets are re-armed once every tile is final
-    if (ticket && blockIdx.x == gridDim.x - 1 && threadIdx.x < NGROUPS) ticket[threadIdx.x] = 0;
+    const int g = threadIdx.x / FIN_GT, l = threadIdx.x % FIN_GT;
+    for (int c0 = 0; c0 < total; c0 += FIN_NT) {
+        // one round of loads for a chunk of edgels: key, record and output row
+        {
+            const int e = c0 + threadIdx.x;
+            if (e < total) {
+                int k = e, phase = 0;
+                while (phase < 3 && k >= nph[phase]) { k -= nph[phase]; ++phase; }
+                const long long gi = J.stage_base + (tile_local * 4 + phase) * J.stage_cap + k;
+                const uint32_t key = stg.key[gi];
+                const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
+                const long long idx = out_index(J, offsets, chunk_pref, 2 * tr * TH + r, tc, rank);
+                s_rec[threadIdx.x] = stg.rec[gi];
+                s_idx[threadIdx.x] = idx < J.capacity ? idx : -1;
+                s_rc[threadIdx.x] = r | (c << 8);
+            }
+        }
+        __syncthreads();
+        const int cn = min(FIN_NT, total - c0);
+        for (int e0 = 0; e0 < cn; e0 += FIN_G) {
+            const int e = e0 + g;
+            if (e >= cn) break;                 // the same on the lanes of a group
+            const int rc = s_rc[e];
+            const int r = rc & 0xff, c = rc >> 8;
+            const int ci = (r >> 1) + 1 + R1, cj = (c >> 1) + 1 + R1, a = r & 1, b = c & 1;
+            double th;
+            if (R1 == 5) th = theta_group<5, FIN_GT>(fimg, ci, cj, a, b, l, tapd);
+            else if (R1 == 7) th = theta_group<7, FIN_GT>(fimg, ci, cj, a, b, l, tapd);
+            else th = theta_group<9, FIN_GT>(fimg, ci, cj, a, b, l, tapd);
+            if (l == 0) {
+                const long long idx = s_idx[e];
+                if (idx >= 0) {
+                    float4 rec = s_rec[e];
+                    rec.z = (float)th;
+                    reinterpret_cast<float4 *>(J.out)[idx] = rec;
+                    if (J.index_out) J.index_out[idx] = (2 * tr * TH + r) * IW + 2 * tc * TW + c;
+                }
+            }
+        }
+        __syncthreads();   // the next chunk overwrites this chunk's slots only after they were read
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1520,8 +1627,8 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
     hipLaunchKernelGGL(toed_scan_chunks, dim3((unsigned)nchunks), dim3(256), 0, s, d_jobs, njobs, job0, counts,
                        offsets, chunks);
     hipLaunchKernelGGL(toed_scan_images, dim3(njobs), dim3(1024), 0, s, d_jobs, njobs, job0, chunks);
-    constexpr int TPB = NT / 64;    // finalize: one wavefront per tile
-    hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)((ntiles + TPB - 1) / TPB)), dim3(NT), 0, s, d_jobs,
+    // finalize: one workgroup per tile
+    hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)ntiles), dim3(FIN_NT), 0, s, d_jobs,
                        njobs, job0, TH, offsets, chunks, staging, tilecnt, (int)ntiles, tickets);
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     HIPCHK(hipGetLastError());
