@@ -258,28 +258,58 @@ int toed_subpix_set_grad(toed_subpix *p, int on);
 int toed_subpix_backward(toed_subpix *p, const toed_subpix_grad_t *items);
 
 /* Reference launch layer (unfused, for drop-in and debugging) ---------------
+ * The reference's gpu_convolve / gpu_nms overloads (gpu_kernels.hpp:24-63)
+ * with their argument lists unchanged; only the name carries the precision
+ * suffix (a C ABI cannot overload) and the result is a status code (the
+ * reference returns void).  As in the reference, the window is fixed at 17 / 19
+ * taps (R0 = 8, R1 = 9) and the taps come in as eight device arrays of 19
+ * values: the first, second and third derivative of the Gaussian and the
+ * Gaussian at x = k - 9, then the same four at x = k - 8.5 (gpu_toed.hpp:
+ * 368-386); NMS uses the reference's threshold 2 and border 10
+ * (gpu_nms.cu:97,208).  Launches go to the null stream, like the reference's.
  * toed_gpu_convolve_f32: gpu_kernels.hpp:24-33.  Writes the four interp planes
  * Ix, Iy, |grad I| and TO orientation ((2h) x (2w), row-major) from a float
- * image; the 8 tap pointers of the reference are replaced by sigma.
- * toed_gpu_nms_f32: gpu_kernels.hpp:48-54.  Writes the subpixel position maps
- * (zero where no edgel, reference convention) from the three planes. */
+ * image.  toed_gpu_nms_f32: gpu_kernels.hpp:48-54.  Writes the subpixel
+ * position maps (zero where no edgel, reference convention) from the three
+ * planes.  The _f64 pair (gpu_kernels.hpp:35-45, :56-63) computes wholly in
+ * FP64: image, taps, sums, orientation and the NMS decision. */
 int toed_gpu_convolve_f32(int device_id, int h, int w, int interp_h, int interp_w,
                           const float *dev_img, float *dev_Ix, float *dev_Iy,
-                          float *dev_I_grad_mag, float *dev_I_orient, float sigma);
-int toed_gpu_nms_f32(int device_id, int interp_h, int interp_w,
-                     const float *dev_Ix, const float *dev_Iy, const float *dev_I_grad_mag,
-                     float *dev_subpix_pos_x_map, float *dev_subpix_pos_y_map,
-                     float thresh, int border);
-/* toed_gpu_convolve_f64 / toed_gpu_nms_f64: the T = double overloads,
- * gpu_kernels.hpp:35-45 and :56-63.  The same planes and maps in FP64: FP64
- * image, taps, sums and orientation; the NMS decision wholly in FP64. */
+                          float *dev_I_grad_mag, float *dev_I_orient,
+                          const float *dev_Gx, const float *dev_Gxx,
+                          const float *dev_Gxxx, const float *dev_G_of_x,
+                          const float *dev_Gx_sh, const float *dev_Gxx_sh,
+                          const float *dev_Gxxx_sh, const float *dev_G_of_x_sh);
 int toed_gpu_convolve_f64(int device_id, int h, int w, int interp_h, int interp_w,
                           const double *dev_img, double *dev_Ix, double *dev_Iy,
-                          double *dev_I_grad_mag, double *dev_I_orient, double sigma);
+                          double *dev_I_grad_mag, double *dev_I_orient,
+                          const double *dev_Gx, const double *dev_Gxx,
+                          const double *dev_Gxxx, const double *dev_G_of_x,
+                          const double *dev_Gx_sh, const double *dev_Gxx_sh,
+                          const double *dev_Gxxx_sh, const double *dev_G_of_x_sh);
+int toed_gpu_nms_f32(int device_id, int interp_h, int interp_w,
+                     const float *dev_Ix, const float *dev_Iy, const float *dev_I_grad_mag,
+                     float *dev_subpix_pos_x_map, float *dev_subpix_pos_y_map);
 int toed_gpu_nms_f64(int device_id, int interp_h, int interp_w,
                      const double *dev_Ix, const double *dev_Iy, const double *dev_I_grad_mag,
-                     double *dev_subpix_pos_x_map, double *dev_subpix_pos_y_map,
-                     double thresh, int border);
+                     double *dev_subpix_pos_x_map, double *dev_subpix_pos_y_map);
+/* The same layer for any supported sigma (no reference counterpart: the
+ * reference's taps are literals for sigma = 2): taps from sigma (windows
+ * R0 = ceil(4 sigma), R1 = R0 + 1), and NMS with a given threshold and border. */
+int toed_gpu_convolve_sigma_f32(int device_id, int h, int w, int interp_h, int interp_w,
+                                const float *dev_img, float *dev_Ix, float *dev_Iy,
+                                float *dev_I_grad_mag, float *dev_I_orient, float sigma);
+int toed_gpu_convolve_sigma_f64(int device_id, int h, int w, int interp_h, int interp_w,
+                                const double *dev_img, double *dev_Ix, double *dev_Iy,
+                                double *dev_I_grad_mag, double *dev_I_orient, double sigma);
+int toed_gpu_nms_thresh_f32(int device_id, int interp_h, int interp_w,
+                            const float *dev_Ix, const float *dev_Iy, const float *dev_I_grad_mag,
+                            float *dev_subpix_pos_x_map, float *dev_subpix_pos_y_map,
+                            float thresh, int border);
+int toed_gpu_nms_thresh_f64(int device_id, int interp_h, int interp_w,
+                            const double *dev_Ix, const double *dev_Iy, const double *dev_I_grad_mag,
+                            double *dev_subpix_pos_x_map, double *dev_subpix_pos_y_map,
+                            double thresh, int border);
 
 #ifdef __cplusplus
 }

@@ -121,3 +121,111 @@ def test_header_class_float_and_double(pkg, detector, scene, tmp_path):
     for t in ("f32", "f64"):
         got = _read_list(str(tmp_path / t / "data_final_output_gpu.txt"))
         np.testing.assert_allclose(got, want.astype(np.float64), rtol=1e-9, atol=1e-9)
+
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GOLDEN_DET = sorted(__import__("glob").glob(os.path.join(GOLDEN_DIR, "toed_ref_*.npz")))
+GOLDEN_SP = sorted(__import__("glob").glob(os.path.join(GOLDEN_DIR, "subpix_ref_*.npz")))
+SP_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "toed_subpix_driver")
+
+
+@pytest.mark.parametrize("path", GOLDEN_DET, ids=[os.path.basename(p)[9:-4] for p in GOLDEN_DET])
+def test_cli_against_reference_golden_lists(oracle, path, tmp_path):
+    """toed_detect_cli with the reference binary's argv on each image of the
+    golden set: its data_final_output_gpu.txt against the list the reference's
+    own cpu_toed.cpp wrote for that image (tests/golden).  Rows are matched by
+    position (within 1e-2 px) and must keep the reference's order; matched rows
+    meet the parity tolerances (1e-3 px, 1e-3 rad, 1e-4 relative strength); a
+    row found on one side only must sit at a pixel whose threshold or peak-test
+    margin in the FP64 oracle is below 2e-3 (the FP32 filters can decide such a
+    pixel either way)."""
+    from scipy.spatial import cKDTree
+    d = np.load(path)
+    img, gold = d["img"], d["edges"]
+    H, W = img.shape
+    p = str(tmp_path / "img.pgm")
+    _pgm(p, img)
+    out_dir = str(tmp_path / "out")
+    r = _run([p, "1", "0", out_dir])
+    assert r.returncode == 0, r.stdout
+    got = _read_list(os.path.join(out_dir, "data_final_output_gpu.txt"))
+    assert "Number of GPU edges = %d" % len(got) in r.stdout
+    ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
+    assert ref["n"] == len(gold)
+    np.testing.assert_allclose(ref["edges"][:, [0, 1, 3]], gold[:, [0, 1, 3]], rtol=1e-9, atol=1e-9)
+    dist, j = cKDTree(gold[:, :2]).query(got[:, :2], distance_upper_bound=1e-2)
+    ok = np.isfinite(dist)
+    jm = j[ok]
+    assert np.all(np.diff(jm) > 0), "CLI rows out of the reference's order"
+    a, b = gold[jm], got[ok]
+    dxy = np.abs(a[:, :2] - b[:, :2]).max()
+    both = ~np.isnan(a[:, 2]) & ~np.isnan(b[:, 2])
+    assert np.array_equal(np.isnan(a[:, 2]), np.isnan(b[:, 2]))
+    dth = np.abs(np.angle(np.exp(1j * (a[both, 2] - b[both, 2])))).max() if both.any() else 0.0
+    ds = (np.abs(a[:, 3] - b[:, 3]) / np.abs(a[:, 3])).max()
+    mm, ms = ref["mmag"].reshape(-1), ref["msub"].reshape(-1)
+    IW = 2 * W
+    only_gold = np.setdiff1d(np.arange(len(gold)), jm)
+    gold_expl = (mm[ref["eidx"][only_gold]] < 2e-3) | (ms[ref["eidx"][only_gold]] < 2e-3)
+    # a CLI-only row: the interp pixels around its subpixel position
+    only_cli = got[~ok]
+    cli_expl = []
+    for x, y in only_cli[:, :2]:
+        C, R = int(round(2 * x + 1)), int(round(2 * y + 1))
+        win = [(R + dr) * IW + (C + dc) for dr in (-2, -1, 0, 1, 2) for dc in (-2, -1, 0, 1, 2)
+               if 0 <= R + dr < 2 * H and 0 <= C + dc < IW]
+        cli_expl.append(bool(((mm[win] < 2e-3) | (ms[win] < 2e-3)).any()))
+    print("%s: %d reference rows, %d CLI rows, %d matched, only reference %d, only CLI %d, max |dxy| %.2e px, "
+          "|dtheta| %.2e rad, strength %.2e rel" % (os.path.basename(path), len(gold), len(got), ok.sum(),
+                                                   len(only_gold), len(only_cli), dxy, dth, ds), flush=True)
+    assert ok.sum() >= 0.99 * len(gold) and len(gold) > 100
+    assert gold_expl.all() and all(cli_expl)
+    assert dxy <= 1e-3 and dth <= 1e-3 and ds <= 1e-4
+
+
+@pytest.mark.parametrize("T", ["double", "float"])
+@pytest.mark.parametrize("path", GOLDEN_SP, ids=[os.path.basename(p)[11:-4] for p in GOLDEN_SP])
+def test_cpp_subpix_class_against_reference_golden(path, T, tmp_path):
+    """The header-only C++ SubpixelTOCorrectionGPU<T> (T = double as the
+    reference's driver instantiates it, and float), driven as
+    main_subpix_to_correction.cpp:279-293 drives SubpixelTOCorrectionCPU<double>,
+    on each golden map: the edgel list equals the reference's own list (same
+    edgels, in order) within 1e-3 px, 1e-3 rad and 1e-4 relative strength;
+    TO_edgemap / TO_orientation have the reference's support and values within
+    the same tolerances; the text list is written."""
+    assert os.path.exists(SP_DRIVER), "toed_subpix_driver is not built"
+    z = np.load(path)
+    thr, sig, nbr = (float(v) for v in z["params"])
+    E = np.ascontiguousarray(z["E"], np.float64)
+    H, W = E.shape
+    mp = str(tmp_path / "map.f64")
+    E.tofile(mp)
+    op = "-"
+    if "theta" in z.files:
+        op = str(tmp_path / "orient.f64")
+        np.ascontiguousarray(z["theta"], np.float64).tofile(op)
+    out, od = str(tmp_path / "out.bin"), str(tmp_path / "o")
+    os.makedirs(od)
+    args = [SP_DRIVER, mp, op, str(H), str(W), repr(thr), repr(sig), repr(nbr), T, out, od]
+    print("driver:", " ".join(args[3:9]), flush=True)
+    r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    with open(out, "rb") as f:
+        n = int(np.frombuffer(f.read(8), np.int64)[0])
+        e = np.frombuffer(f.read(n * 32), np.float64).reshape(n, 4)
+        te = np.frombuffer(f.read(H * W * 8), np.float64).reshape(H, W)
+        to = np.frombuffer(f.read(H * W * 8), np.float64).reshape(H, W)
+    gold = z["edges"]
+    assert "Number of corrected edgels = %d" % n in r.stdout
+    assert n == len(gold) > 100
+    assert os.path.exists(os.path.join(od, "TO_corrected_edges.txt"))
+    dxy = np.abs(e[:, :2] - gold[:, :2]).max()
+    dth = np.abs(np.angle(np.exp(1j * (e[:, 2] - gold[:, 2])))).max()
+    ds = (np.abs(e[:, 3] - gold[:, 3]) / np.abs(gold[:, 3])).max()
+    print("%s <%s>: %d edgels, max |dxy| %.2e px, |dtheta| %.2e rad, strength %.2e rel"
+          % (os.path.basename(path), T, n, dxy, dth, ds), flush=True)
+    assert dxy <= 1e-3 and dth <= 1e-3 and ds <= 1e-4
+    assert np.array_equal(te != 0, z["TO_E"] != 0)
+    sup = z["TO_E"] != 0
+    assert (np.abs(te[sup] - z["TO_E"][sup]) / np.abs(z["TO_E"][sup])).max() <= 1e-4
+    assert np.abs(np.angle(np.exp(1j * (to[sup] - z["TO_O"][sup])))).max() <= 1e-3

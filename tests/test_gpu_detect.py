@@ -255,8 +255,9 @@ def test_reference_class_mirror(pkg, oracle, scene, tmp_path):
 
 
 def test_unfused_launch_layer_f64(pkg, oracle, scene):
-    """toed_gpu_convolve_f64 / toed_gpu_nms_f64 (the T = double overloads of
-    the reference's launch layer) against the oracle's FP64 planes and list:
+    """toed_gpu_convolve_sigma_f64 / toed_gpu_nms_thresh_f64 (the T = double
+    form of the launch layer, taps from sigma) against the oracle's FP64 planes
+    and list:
     planes to 1e-9 of their range, the same edgel set apart from decisions
     within 1e-9 of a tie, positions to 1e-8 interp px."""
     import ctypes
@@ -266,7 +267,7 @@ def test_unfused_launch_layer_f64(pkg, oracle, scene):
     t = torch.from_numpy(img.astype(np.float64)).cuda()
     planes = [torch.zeros((2 * H, 2 * W), dtype=torch.float64, device="cuda") for _ in range(4)]
     ptr = lambda x: ctypes.c_void_p(x.data_ptr())
-    assert L.toed_gpu_convolve_f64(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in planes], 2.0) == 0
+    assert L.toed_gpu_convolve_sigma_f64(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in planes], 2.0) == 0
     torch.cuda.synchronize()
     c = oracle.convolve(img.astype(np.float64), 2.0)
     Ix, Iy, mag, ori = (p.cpu().numpy() for p in planes)
@@ -278,8 +279,8 @@ def test_unfused_launch_layer_f64(pkg, oracle, scene):
     assert np.nanmax(d) < 1e-6
     px = torch.zeros((2 * H, 2 * W), dtype=torch.float64, device="cuda")
     py = torch.zeros_like(px)
-    assert L.toed_gpu_nms_f64(0, 2 * H, 2 * W, ptr(planes[0]), ptr(planes[1]), ptr(planes[2]), ptr(px), ptr(py),
-                              2.0, 10) == 0
+    assert L.toed_gpu_nms_thresh_f64(0, 2 * H, 2 * W, ptr(planes[0]), ptr(planes[1]), ptr(planes[2]), ptr(px),
+                                     ptr(py), 2.0, 10) == 0
     torch.cuda.synchronize()
     pxm, pym = px.cpu().numpy().reshape(-1), py.cpu().numpy().reshape(-1)
     sel = np.flatnonzero(pxm)
@@ -295,8 +296,8 @@ def test_unfused_launch_layer_f64(pkg, oracle, scene):
 
 
 def test_unfused_launch_layer(pkg, oracle, scene):
-    """toed_gpu_convolve_f32 / toed_gpu_nms_f32 (the reference's launch layer)
-    against the oracle's planes and its subpixel maps."""
+    """toed_gpu_convolve_sigma_f32 / toed_gpu_nms_thresh_f32 (the launch layer
+    with taps from sigma) against the oracle's planes and its subpixel maps."""
     import ctypes
     L = pkg._lib.lib()
     H, W = 90, 120
@@ -304,7 +305,7 @@ def test_unfused_launch_layer(pkg, oracle, scene):
     t = torch.from_numpy(img.astype(np.float32)).cuda()
     planes = [torch.zeros((2 * H, 2 * W), dtype=torch.float32, device="cuda") for _ in range(4)]
     ptr = lambda x: ctypes.c_void_p(x.data_ptr())
-    assert L.toed_gpu_convolve_f32(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in planes], 2.0) == 0
+    assert L.toed_gpu_convolve_sigma_f32(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in planes], 2.0) == 0
     torch.cuda.synchronize()
     c = oracle.convolve(img.astype(np.float64), 2.0)
     Ix, Iy, mag, ori = (p.cpu().numpy().astype(np.float64) for p in planes)
@@ -316,10 +317,64 @@ def test_unfused_launch_layer(pkg, oracle, scene):
     assert np.nanquantile(d, 0.99) < 1e-2
     px = torch.zeros((2 * H, 2 * W), dtype=torch.float32, device="cuda")
     py = torch.zeros_like(px)
-    assert L.toed_gpu_nms_f32(0, 2 * H, 2 * W, ptr(planes[0]), ptr(planes[1]), ptr(planes[2]), ptr(px), ptr(py),
-                              2.0, 10) == 0
+    assert L.toed_gpu_nms_thresh_f32(0, 2 * H, 2 * W, ptr(planes[0]), ptr(planes[1]), ptr(planes[2]), ptr(px),
+                                     ptr(py), 2.0, 10) == 0
     torch.cuda.synchronize()
     sel = np.flatnonzero(px.cpu().numpy().reshape(-1))
+    ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
+    only = np.setxor1d(sel, ref["eidx"])
+    mm, ms = ref["mmag"].reshape(-1), ref["msub"].reshape(-1)
+    assert np.all((mm[only] < 2e-3) | (ms[only] < 2e-3))
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_launch_layer_reference_arguments(pkg, oracle, scene, dtype):
+    """toed_gpu_convolve_f32/_f64 and toed_gpu_nms_f32/_f64 called with the
+    reference's own argument lists (gpu_kernels.hpp:24-63), the way
+    gpu_toed.hpp:379-394 and :420 call them: eight device tap arrays (Gx, Gxx,
+    Gxxx, G_of_x, then the shifted four; the oracle's sigma = 2 taps, which
+    test_oracle pins to the reference's literals) and an NMS with the
+    reference's threshold 2 and border 10 built in.  Planes match the oracle's
+    (FP32: to 1e-3; FP64: to 1e-9 of their range) and the maps the sigma /
+    threshold forms' bit for bit."""
+    import ctypes
+    L = pkg._lib.lib()
+    H, W = 90, 120
+    img = scene.shapes_scene(43, H, W)
+    tdt = getattr(torch, dtype)
+    t = torch.from_numpy(img.astype(dtype)).cuda()
+    _, _, taps = oracle.taps(2.0)
+    order = [1, 2, 3, 0, 5, 6, 7, 4]       # Gx, Gxx, Gxxx, G_of_x, Gx_sh, Gxx_sh, Gxxx_sh, G_of_x_sh
+    dev_taps = [torch.from_numpy(np.ascontiguousarray(taps[k], dtype)).cuda() for k in order]
+    assert all(d.numel() == 19 for d in dev_taps)
+    ptr = lambda x: ctypes.c_void_p(x.data_ptr())
+    planes = [torch.zeros((2 * H, 2 * W), dtype=tdt, device="cuda") for _ in range(4)]
+    conv = L.toed_gpu_convolve_f32 if dtype == "float32" else L.toed_gpu_convolve_f64
+    nms = L.toed_gpu_nms_f32 if dtype == "float32" else L.toed_gpu_nms_f64
+    assert conv(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in planes], *[ptr(d) for d in dev_taps]) == 0
+    px = torch.zeros((2 * H, 2 * W), dtype=tdt, device="cuda")
+    py = torch.zeros_like(px)
+    assert nms(0, 2 * H, 2 * W, ptr(planes[0]), ptr(planes[1]), ptr(planes[2]), ptr(px), ptr(py)) == 0
+    torch.cuda.synchronize()
+    c = oracle.convolve(img.astype(np.float64), 2.0)
+    tol = 1e-3 if dtype == "float32" else 1e-9
+    for k, key in enumerate(("Ix", "Iy", "mag")):
+        got = planes[k].cpu().numpy().astype(np.float64)
+        scale = 1.0 if dtype == "float32" else np.abs(c[key]).max()
+        assert np.abs(got - c[key]).max() < tol * scale, key
+    # the sigma / threshold forms on the same image: the same maps
+    sp = [torch.zeros_like(p) for p in planes]
+    sconv = L.toed_gpu_convolve_sigma_f32 if dtype == "float32" else L.toed_gpu_convolve_sigma_f64
+    snms = L.toed_gpu_nms_thresh_f32 if dtype == "float32" else L.toed_gpu_nms_thresh_f64
+    assert sconv(0, H, W, 2 * H, 2 * W, ptr(t), *[ptr(p) for p in sp], 2.0) == 0
+    qx, qy = torch.zeros_like(px), torch.zeros_like(py)
+    assert snms(0, 2 * H, 2 * W, ptr(sp[0]), ptr(sp[1]), ptr(sp[2]), ptr(qx), ptr(qy), 2.0, 10) == 0
+    torch.cuda.synchronize()
+    sel = np.flatnonzero(px.cpu().numpy().reshape(-1))
+    assert len(sel) > 100
+    assert np.array_equal(sel, np.flatnonzero(qx.cpu().numpy().reshape(-1)))
+    np.testing.assert_allclose(px.cpu().numpy(), qx.cpu().numpy(), rtol=0, atol=1e-9 if dtype == "float64" else 1e-4)
+    # the edgel set is the oracle's (decisions within the FP32 margin aside)
     ref = oracle.detect(img.astype(np.float64), 2.0, margins=True)
     only = np.setxor1d(sel, ref["eidx"])
     mm, ms = ref["mmag"].reshape(-1), ref["msub"].reshape(-1)

@@ -133,10 +133,14 @@ SIGNATURES = [
     ("toed_subpix_last_times", _I, [_VP, _PF, _PF, _PF]),
     ("toed_subpix_set_grad", _I, [_VP, _I]),
     ("toed_subpix_backward", _I, [_VP, ctypes.POINTER(toed_subpix_grad_t)]),
-    ("toed_gpu_convolve_f32", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F]),
-    ("toed_gpu_nms_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F, _I]),
-    ("toed_gpu_convolve_f64", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_double]),
-    ("toed_gpu_nms_f64", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_double, _I]),
+    ("toed_gpu_convolve_f32", _I, [_I, _I, _I, _I, _I] + [_VP] * 13),
+    ("toed_gpu_nms_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+    ("toed_gpu_convolve_f64", _I, [_I, _I, _I, _I, _I] + [_VP] * 13),
+    ("toed_gpu_nms_f64", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+    ("toed_gpu_convolve_sigma_f32", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F]),
+    ("toed_gpu_nms_thresh_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F, _I]),
+    ("toed_gpu_convolve_sigma_f64", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_double]),
+    ("toed_gpu_nms_thresh_f64", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_double, _I]),
 ]
 
 _LIB = None

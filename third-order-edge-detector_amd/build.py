@@ -6,6 +6,8 @@
                       script (host C++ over the C ABI)
     tests/cpp/toed_class_driver   the header-only C++ class run for T = float
                       and double (test program)
+    tests/cpp/toed_subpix_driver  the header-only C++ subpix class, driven as the
+                      reference's subpix driver drives its class (test program)
 
 Run:  python third-order-edge-detector_amd/build.py
 """
@@ -68,10 +70,11 @@ def build_cli():
 
 def build_class_driver():
     d = os.path.join(ROOT, "tests", "cpp")
-    out = os.path.join(d, "toed_class_driver")
-    _run([HIPCC, "-O2", "-std=c++17", "-I" + INC, os.path.join(d, "toed_class_driver.cpp"),
-          "-L" + PKG, "-ltoed_hip", "-Wl,-rpath,$ORIGIN/../../third-order-edge-detector_amd", "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
+    for name in ("toed_class_driver", "toed_subpix_driver"):
+        out = os.path.join(d, name)
+        _run([HIPCC, "-O2", "-std=c++17", "-I" + INC, os.path.join(d, name + ".cpp"),
+              "-L" + PKG, "-ltoed_hip", "-Wl,-rpath,$ORIGIN/../../third-order-edge-detector_amd", "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
 
 
 def build_scene():

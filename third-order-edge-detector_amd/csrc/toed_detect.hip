@@ -693,6 +693,26 @@ __global__ __launch_bounds__(256) void toed_put_taps_kernel(TapSet *dst, TapArgs
     }
 }
 
+// The reference launch layer's eight device tap arrays (gpu_kernels.hpp:24-45:
+// G', G'', G''', G unshifted, then shifted, 19 taps each for its fixed 17-tap
+// window, gpu_toed.hpp:368-386) packed into a TapSet (R0 = 8, R1 = 9).
+template <typename T>
+__global__ __launch_bounds__(64) void toed_pack_taps_kernel(TapSet *dst, const T *Gx, const T *Gxx, const T *Gxxx,
+                                                            const T *G, const T *Gx_sh, const T *Gxx_sh,
+                                                            const T *Gxxx_sh, const T *G_sh) {
+    const int k = threadIdx.x;
+    if (k == 0) { dst->R0 = MAXR1 - 1; dst->R1 = MAXR1; dst->pad0 = 0; dst->pad1 = 0; }
+    if (k < MAXTAPS) {
+        dst->du[0][k] = (double)G[k];    dst->du[1][k] = (double)Gx[k];
+        dst->du[2][k] = (double)Gxx[k];  dst->du[3][k] = (double)Gxxx[k];
+        dst->ds[0][k] = (double)G_sh[k]; dst->ds[1][k] = (double)Gx_sh[k];
+        dst->ds[2][k] = (double)Gxx_sh[k]; dst->ds[3][k] = (double)Gxxx_sh[k];
+        dst->fu[0][k] = (float)G[k];     dst->fu[1][k] = (float)Gx[k];
+        dst->fs[0][k] = (float)G_sh[k];  dst->fs[1][k] = (float)Gx_sh[k];
+        for (int f = 0; f < 4; ++f) { dst->dsu[f][k] = 0.0; dst->dss[f][k] = 0.0; }
+    }
+}
+
 // K0: per-image flag reset (a kernel, not a memset: safe inside captured graphs)
 __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2166,8 +2186,8 @@ static TapSet *compat_taps(float sigma, int *st) {
     return d;
 }
 
-int toed_gpu_convolve_f32(int device_id, int h, int w, int ih, int iw, const float *img, float *Ix, float *Iy,
-                          float *mag, float *orient, float sigma) {
+int toed_gpu_convolve_sigma_f32(int device_id, int h, int w, int ih, int iw, const float *img, float *Ix, float *Iy,
+                                float *mag, float *orient, float sigma) {
     if (h <= 0 || w <= 0 || ih != 2 * h || iw != 2 * w || !img || !Ix || !Iy || !mag || !orient) return TOED_ERR_ARG;
     if ((long long)h * w > (1LL << 28)) return TOED_ERR_ARG;
     HIPCHK(hipSetDevice(device_id));
@@ -2181,8 +2201,8 @@ int toed_gpu_convolve_f32(int device_id, int h, int w, int ih, int iw, const flo
     return TOED_OK;
 }
 
-int toed_gpu_nms_f32(int device_id, int ih, int iw, const float *Ix, const float *Iy, const float *mag, float *px,
-                     float *py, float thresh, int border) {
+int toed_gpu_nms_thresh_f32(int device_id, int ih, int iw, const float *Ix, const float *Iy, const float *mag, float *px,
+                            float *py, float thresh, int border) {
     if (ih <= 0 || iw <= 0 || !Ix || !Iy || !mag || !px || !py || border < 1) return TOED_ERR_ARG;
     if ((long long)ih * iw > (1LL << 30)) return TOED_ERR_ARG;
     HIPCHK(hipSetDevice(device_id));
@@ -2193,8 +2213,8 @@ int toed_gpu_nms_f32(int device_id, int ih, int iw, const float *Ix, const float
     return TOED_OK;
 }
 
-int toed_gpu_convolve_f64(int device_id, int h, int w, int ih, int iw, const double *img, double *Ix, double *Iy,
-                          double *mag, double *orient, double sigma) {
+int toed_gpu_convolve_sigma_f64(int device_id, int h, int w, int ih, int iw, const double *img, double *Ix,
+                                double *Iy, double *mag, double *orient, double sigma) {
     if (h <= 0 || w <= 0 || ih != 2 * h || iw != 2 * w || !img || !Ix || !Iy || !mag || !orient) return TOED_ERR_ARG;
     if ((long long)h * w > (1LL << 28)) return TOED_ERR_ARG;
     HIPCHK(hipSetDevice(device_id));
@@ -2208,8 +2228,8 @@ int toed_gpu_convolve_f64(int device_id, int h, int w, int ih, int iw, const dou
     return TOED_OK;
 }
 
-int toed_gpu_nms_f64(int device_id, int ih, int iw, const double *Ix, const double *Iy, const double *mag,
-                     double *px, double *py, double thresh, int border) {
+int toed_gpu_nms_thresh_f64(int device_id, int ih, int iw, const double *Ix, const double *Iy, const double *mag,
+                            double *px, double *py, double thresh, int border) {
     if (ih <= 0 || iw <= 0 || !Ix || !Iy || !mag || !px || !py || border < 1) return TOED_ERR_ARG;
     if ((long long)ih * iw > (1LL << 30)) return TOED_ERR_ARG;
     HIPCHK(hipSetDevice(device_id));
@@ -2218,6 +2238,85 @@ int toed_gpu_nms_f64(int device_id, int ih, int iw, const double *Ix, const doub
                        Ix, Iy, mag, px, py, thresh, border);
     HIPCHK(hipGetLastError());
     return TOED_OK;
+}
+
+
+// The reference's own argument lists (gpu_kernels.hpp:24-63): eight device
+// tap arrays for its fixed window (R0 = 8, R1 = 9), NMS with its literal
+// threshold 2 and border 10 (gpu_nms.cu:97,208).  The taps go through one
+// TapSet per device, packed on the null stream before the launch.
+}  // extern "C"
+
+static TapSet *g_ref_taps[NOALLOC_MAX_DEVICES];
+
+static int ref_taps(int device_id, TapSet **out) {
+    if (device_id < 0 || device_id >= NOALLOC_MAX_DEVICES) return TOED_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_ref_taps[device_id]) {
+        TapSet *d = nullptr;
+        if (hipMalloc(&d, sizeof(TapSet)) != hipSuccess) return TOED_ERR_ALLOC;
+        g_ref_taps[device_id] = d;
+    }
+    *out = g_ref_taps[device_id];
+    return TOED_OK;
+}
+
+template <typename T>
+static int convolve_taps(int device_id, int h, int w, int ih, int iw, const T *img, T *Ix, T *Iy, T *mag, T *orient,
+                         const T *Gx, const T *Gxx, const T *Gxxx, const T *G, const T *Gx_sh, const T *Gxx_sh,
+                         const T *Gxxx_sh, const T *G_sh) {
+    if (h <= 0 || w <= 0 || ih != 2 * h || iw != 2 * w || !img || !Ix || !Iy || !mag || !orient) return TOED_ERR_ARG;
+    if (!Gx || !Gxx || !Gxxx || !G || !Gx_sh || !Gxx_sh || !Gxxx_sh || !G_sh) return TOED_ERR_ARG;
+    if ((long long)h * w > (1LL << 28)) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(device_id));
+    TapSet *tp = nullptr;
+    int st = ref_taps(device_id, &tp);
+    if (st) return st;
+    hipLaunchKernelGGL(toed_pack_taps_kernel<T>, dim3(1), dim3(64), 0, nullptr, tp, Gx, Gxx, Gxxx, G, Gx_sh, Gxx_sh,
+                       Gxxx_sh, G_sh);
+    const long long n = 4LL * h * w;
+    if constexpr (std::is_same<T, float>::value)
+        hipLaunchKernelGGL(toed_planes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, h, w, img, Ix,
+                           Iy, mag, orient, tp);
+    else
+        hipLaunchKernelGGL(toed_planes_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr, h, w, img,
+                           Ix, Iy, mag, orient, tp);
+    HIPCHK(hipGetLastError());
+    return TOED_OK;
+}
+
+extern "C" {
+
+int toed_gpu_convolve_f32(int device_id, int h, int w, int interp_h, int interp_w, const float *dev_img,
+                          float *dev_Ix, float *dev_Iy, float *dev_I_grad_mag, float *dev_I_orient,
+                          const float *dev_Gx, const float *dev_Gxx, const float *dev_Gxxx, const float *dev_G_of_x,
+                          const float *dev_Gx_sh, const float *dev_Gxx_sh, const float *dev_Gxxx_sh,
+                          const float *dev_G_of_x_sh) {
+    return convolve_taps<float>(device_id, h, w, interp_h, interp_w, dev_img, dev_Ix, dev_Iy, dev_I_grad_mag,
+                                dev_I_orient, dev_Gx, dev_Gxx, dev_Gxxx, dev_G_of_x, dev_Gx_sh, dev_Gxx_sh,
+                                dev_Gxxx_sh, dev_G_of_x_sh);
+}
+
+int toed_gpu_convolve_f64(int device_id, int h, int w, int interp_h, int interp_w, const double *dev_img,
+                          double *dev_Ix, double *dev_Iy, double *dev_I_grad_mag, double *dev_I_orient,
+                          const double *dev_Gx, const double *dev_Gxx, const double *dev_Gxxx,
+                          const double *dev_G_of_x, const double *dev_Gx_sh, const double *dev_Gxx_sh,
+                          const double *dev_Gxxx_sh, const double *dev_G_of_x_sh) {
+    return convolve_taps<double>(device_id, h, w, interp_h, interp_w, dev_img, dev_Ix, dev_Iy, dev_I_grad_mag,
+                                 dev_I_orient, dev_Gx, dev_Gxx, dev_Gxxx, dev_G_of_x, dev_Gx_sh, dev_Gxx_sh,
+                                 dev_Gxxx_sh, dev_G_of_x_sh);
+}
+
+int toed_gpu_nms_f32(int device_id, int interp_h, int interp_w, const float *dev_Ix, const float *dev_Iy,
+                     const float *dev_I_grad_mag, float *dev_subpix_pos_x_map, float *dev_subpix_pos_y_map) {
+    return toed_gpu_nms_thresh_f32(device_id, interp_h, interp_w, dev_Ix, dev_Iy, dev_I_grad_mag,
+                                   dev_subpix_pos_x_map, dev_subpix_pos_y_map, 2.0f, 10);
+}
+
+int toed_gpu_nms_f64(int device_id, int interp_h, int interp_w, const double *dev_Ix, const double *dev_Iy,
+                     const double *dev_I_grad_mag, double *dev_subpix_pos_x_map, double *dev_subpix_pos_y_map) {
+    return toed_gpu_nms_thresh_f64(device_id, interp_h, interp_w, dev_Ix, dev_Iy, dev_I_grad_mag,
+                                   dev_subpix_pos_x_map, dev_subpix_pos_y_map, 2.0, 10);
 }
 
 }  // extern "C"
