@@ -141,7 +141,7 @@ def bench_fwd_bwd(torch, pkg, scene, steps, warmup, B=64, H=512, W=512, sigma=1.
     x = torch.from_numpy(imgs).cuda().requires_grad_(True)
     sig = torch.tensor(sigma, dtype=torch.float64, requires_grad=True)
     w = torch.tensor(scene.loss_weights(12345), dtype=torch.float32, device="cuda")
-    det = pkg.Detector(0, staging_fraction=0.5)
+    det = pkg.Detector(0, staging_fraction=0.25)
     state = {}
 
     def fwd():
@@ -157,8 +157,34 @@ def bench_fwd_bwd(torch, pkg, scene, steps, warmup, B=64, H=512, W=512, sigma=1.
 
     t_f = time_loop(torch, fwd, steps, warmup)
     t_fb = time_loop(torch, fwd_bwd, steps, warmup)
+    # kernel time of the same plan, HIP events on the library's stream (torch's
+    # current stream): its forward launch sequence, and its backward
+    autograd = importlib.import_module(PKG_NAME + ".autograd")
+    plan = next(reversed(autograd._PLANS.values()))
+    plan.point_at(x.detach())
+    bd = plan.bd
+    bd.run()
+    counts, _ = bd.counts_flags()
+    offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+    g = (torch.ones((max(int(offs[-1]), 1), 4), device="cuda") * w).contiguous()
+    gimg = torch.zeros((B, H, W), dtype=torch.float32, device="cuda")
+    gsig = torch.zeros(1, dtype=torch.float64, device="cuda")
+    gp = [g.data_ptr() + 16 * int(offs[b]) for b in range(B)]
+    ip = [gimg.data_ptr() + 4 * H * W * b for b in range(B)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    kf, kb = [], []
+    for _ in range(max(3, steps)):
+        ev[0].record()
+        bd.run()
+        ev[1].record()
+        bd.backward_ptrs(gp, ip, [W] * B, gsig.data_ptr())
+        ev[2].record()
+        torch.cuda.synchronize()
+        kf.append(ev[0].elapsed_time(ev[1]))
+        kb.append(ev[1].elapsed_time(ev[2]))
     det.close()
-    return dict(px=float(B * H * W), t_f=t_f, t_fb=t_fb, n=state["n"])
+    return dict(px=float(B * H * W), t_f=t_f, t_fb=t_fb, n=state["n"], k_f=float(np.median(kf)) * 1e-3,
+                k_b=float(np.median(kb)) * 1e-3)
 
 
 def bench_subpix(torch, pkg, scene, steps, warmup, B=32, S=2048):
@@ -205,7 +231,7 @@ def cpu_baseline_subpix(map_np, gpu_edges=None):
     subpix workload; the oracle port when the reference harness is not built.
     Its first run's list is also the checker of the GPU list of that map."""
     import oracle
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = host_cpu()["threads_used"]
     H, W = map_np.shape
     sec, runs = 0.0, 0
     first = None
@@ -248,35 +274,81 @@ def accuracy(gpu, ref):
     return out
 
 
-def cpu_baseline(img_np, gpu_edges=None):
-    """The reference CPU path (cpu_toed.cpp, FP64, OpenMP) on one frame of the
-    same workload, on this host's cores; the oracle port when the reference
-    harness is not built.  Its first run's list is also the checker of the GPU
-    list of the same frame (``accuracy``)."""
+def host_cpu():
+    """The host's CPU as lscpu reports it, and the threads this process may use:
+    the CPUs in its affinity mask, limited by OMP_NUM_THREADS where the job
+    sets one (a GPU box allots each GPU a share of the host's cores)."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        kv = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name", "?")
+        sockets = int(kv.get("Socket(s)", "1") or 1)
+        cps = int(kv.get("Core(s) per socket", "0") or 0)
+        info["physical_cores"] = sockets * cps if cps else None
+        info["logical_cpus"] = int(kv.get("CPU(s)", "0") or 0)
+    except (OSError, ValueError):
+        info["model"] = "?"
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    n = avail
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    info["cpus_available"] = avail
+    info["threads_used"] = max(1, n)
+    return info
+
+
+def _time_ref(img_np, threads, budget, max_runs, env):
+    """Repeated runs of the reference CPU path (or the oracle port) on one
+    image until about `budget` seconds of measured time: (Mpixel/s, kind, runs,
+    seconds, first list).  The reference's runs are timed with its own
+    omp_get_wtime intervals around convolve_img and non_maximum_suppresion
+    (cpu_toed.cpp:171/359, 390/499)."""
     import oracle
-    cores = max(1, min(16, os.cpu_count() or 1))
     H, W = img_np.shape
-    sec = wall = 0.0
-    runs = 0
-    first = None
-    while runs < CPU_MAX_RUNS and sec < CPU_SAMPLE_SEC:   # a bounded sample: about 10 s of CPU work
-        t0 = time.perf_counter()
+    sec, runs, first, kind = 0.0, 0, None, "port"
+    while runs < max_runs and (runs == 0 or sec < budget):
         if oracle.ref_available():
-            edges, tc, tn = oracle.run_ref(img_np, nthreads=cores)
+            edges, tc, tn = oracle.run_ref(img_np, nthreads=threads, env=env)
             kind = "reference"
             sec += tc + tn
         else:
-            edges = oracle.detect(img_np.astype(np.float64), 2.0, nthreads=cores)["edges"]
-            kind = "port"
+            t0 = time.perf_counter()
+            edges = oracle.detect(img_np.astype(np.float64), 2.0, nthreads=threads)["edges"]
             sec += time.perf_counter() - t0
-        wall += time.perf_counter() - t0
         if first is None:
             first = edges
         runs += 1
-    res = {"value": round(runs * H * W / sec / 1e6, 4), "unit": "Mpixel/s", "cores": cores, "kind": kind,
-           "sample": "%d runs on one %dx%d frame of the same workload, FP64, convolution + NMS timed at the "
-                     "reference's omp_get_wtime boundaries (%.2f s in total; %.2f s wall incl. list build and I/O)"
-                     % (runs, H, W, sec, wall)}
+    return runs * H * W / sec / 1e6, kind, runs, sec, first
+
+
+def cpu_baseline(img_np, gpu_edges=None, scene=None):
+    """The reference CPU path (cpu_toed.cpp, FP64, OpenMP, schedule(dynamic))
+    on this host, as the baseline protocol states it: the 480x752 scene at one
+    thread and at N threads, and one frame of the 2160x3840 workload at N
+    threads (the reported value), N = the threads available to this process,
+    OMP_PROC_BIND=close; the oracle port when the reference harness is not
+    built.  Its first 2160x3840 list is also the checker of the GPU list of the
+    same frame (``accuracy``)."""
+    host = host_cpu()
+    n = host["threads_used"]
+    env = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}
+    H, W = img_np.shape
+    v, kind, runs, sec, first = _time_ref(img_np, n, CPU_SAMPLE_SEC, CPU_MAX_RUNS, env)
+    res = {"value": round(v, 4), "unit": "Mpixel/s", "cores": n, "kind": kind,
+           "sample": "%d runs on one %dx%d frame of the same workload at %d threads (OMP_PROC_BIND=close), FP64, "
+                     "timed at the reference's omp_get_wtime boundaries of convolve_img + non_maximum_suppresion "
+                     "(%.2f s in total)" % (runs, H, W, n, sec),
+           "host": host}
+    if scene is not None:
+        c1 = scene.shapes_scene(480 * 752, 480, 752)
+        v1, _, r1, s1, _ = _time_ref(c1, 1, 4.0, 3, env)
+        vn, _, rn, sn, _ = _time_ref(c1, n, 2.0, 20, env)
+        res["c1_480x752"] = {"threads_1": {"mpx_s": round(v1, 4), "runs": r1, "sec": round(s1, 3)},
+                             "threads_n": {"threads": n, "mpx_s": round(vn, 4), "runs": rn, "sec": round(sn, 3)},
+                             "speedup": round(vn / v1, 2)}
     acc = accuracy(gpu_edges, first) if gpu_edges is not None else None
     return res, acc
 
@@ -343,6 +415,18 @@ def main():
         forms["batch_loop_of_plain"] = {"mpx_s": round(bt["px"] / bt["sec_loop"] / 1e6, 1),
                                         "ms": round(bt["sec_loop"] * 1e3, 3)}
         forms["batch_config"] = {"images": bt["n_images"], "mpx": round(bt["px"] / 1e6, 2), "edgels": bt["edgels"]}
+        # the images' sigma differ (windows of 9/11, 13/15, 17/19 taps): algorithmic work weighted per image
+        bflops = sum(float(h * w) * dense_flops_per_px(sg, pkg) for (h, w), sg in zip(bt["shapes"], bt["sigmas"]))
+        ach_b = bflops / bt["sec_graph"] / 1e12
+        forms["batched_graph"]["roofline"] = {
+            "bound": "valu", "achieved": round(ach_b, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(ach_b / PEAK_FP32_TFLOPS, 4),
+            "algorithmic": "sum over images of H*W*2*(10*W1+2*W0) flop (sigma-weighted: 256/352/448 flop/px "
+                           "at sigma 1/1.5/2), %.3g GFLOP per batch, whole launch sequence timed" % (bflops / 1e9)}
+        forms["batched_graph"]["vs_loop_of_plain"] = round(bt["sec_loop"] / bt["sec_graph"], 3)
+        forms["batched_graph"]["hbm_note"] = (
+            "the fused path reads each pixel once and is bound by FP32 VALU work (about 300-450 flop per byte "
+            "against a ridge of 20): it cannot reach 50% of HBM peak without adding traffic")
         tb = bt["tune"]["best"]
         forms["batch_autotune"] = {"tile_rows": tb["tile_rows"], "persistent": tb["persistent"],
                                    "wg_per_cu": tb["wg_per_cu"], "waves_per_cu": 4 * tb["wg_per_cu"],
@@ -360,13 +444,17 @@ def main():
         fb = bench_fwd_bwd(torch, pkg, scene, max(3, args.steps // 4), 1)
         forms["fwd_only_64x512x512"] = {"mpx_s": round(fb["px"] / fb["t_f"] / 1e6, 1), "ms": round(fb["t_f"] * 1e3, 3)}
         forms["fwd_bwd_64x512x512"] = {"mpx_s": round(fb["px"] / fb["t_fb"] / 1e6, 1), "ms": round(fb["t_fb"] * 1e3, 3),
-                                       "edgels": fb["n"], "ratio_to_fwd": round(fb["t_fb"] / fb["t_f"], 3)}
+                                       "edgels": fb["n"], "ratio_to_fwd": round(fb["t_fb"] / fb["t_f"], 3),
+                                       "kernels_ms": {"forward": round(fb["k_f"] * 1e3, 3),
+                                                      "backward": round(fb["k_b"] * 1e3, 3)},
+                                       "ratio_kernels": round((fb["k_f"] + fb["k_b"]) / fb["k_f"], 3)}
         if "value" not in out:
             out["value"] = forms["fwd_bwd_64x512x512"]["mpx_s"]
             out["ms_per_step"] = round(fb["t_fb"] * 1e3, 3)
             out["config"] = {"workload": "differentiable TOED fwd+bwd, 64 x 512x512 f32, sigma 1.5", "global_batch": 64}
-        log("fwd+bwd: %.1f Mpx/s (fwd %.1f), ratio %.2f" % (forms["fwd_bwd_64x512x512"]["mpx_s"],
-                                                            forms["fwd_only_64x512x512"]["mpx_s"], fb["t_fb"] / fb["t_f"]))
+        log("fwd+bwd: %.1f Mpx/s (fwd %.1f), ratio %.2f wall clock, %.2f kernels (fwd %.3f ms, bwd %.3f ms)"
+            % (forms["fwd_bwd_64x512x512"]["mpx_s"], forms["fwd_only_64x512x512"]["mpx_s"], fb["t_fb"] / fb["t_f"],
+               (fb["k_f"] + fb["k_b"]) / fb["k_f"], fb["k_f"] * 1e3, fb["k_b"] * 1e3))
     if args.workload in ("all", "subpix"):
         log("subpix 32 x 2048x2048 ...")
         sp = bench_subpix(torch, pkg, scene, max(3, args.steps // 4), 1)
@@ -392,7 +480,7 @@ def main():
     out["forms"] = forms
     if plain is not None and not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"], acc = cpu_baseline(plain["img_np"], plain["edges_np"])
+        out["cpu_baseline"], acc = cpu_baseline(plain["img_np"], plain["edges_np"], scene)
         out["accuracy_vs_cpu_fp64"] = acc
         log("accuracy vs the FP64 CPU path: %s" % json.dumps(acc))
     print(json.dumps(out))

@@ -144,6 +144,19 @@ int toed_batch_destroy(toed_batch *b);
 /* Persistent grid size: workgroups per CU for the persistent tile kernel
  * (0 = as many as fit on a CU, the default).  No reference counterpart. */
 int toed_batch_set_grid(toed_batch *b, int wg_per_cu);
+/* Point the plan's images at other device buffers of the same type, sizes and
+ * pitches (d_imgs[m]: image m, the order of toed_batch_plan), for the next
+ * runs: a plan is reused for new inputs without being rebuilt.  No reference
+ * counterpart. */
+int toed_batch_set_images(toed_batch *b, const void *const *d_imgs);
+/* Compact output for the next runs: every image's rows go to one buffer of
+ * `capacity` rows, image m's rows right after those of images 0 .. m-1 (its
+ * first row is the sum of their counts, computed on the device), so the
+ * concatenated list needs no host step; d_index (may be NULL) likewise.  The
+ * per-image d_edges / d_index / capacity of the plan are no longer used; an
+ * image whose rows would pass `capacity` gets TOED_FLAG_CAPACITY.  Runs then
+ * need d_counts.  No reference counterpart. */
+int toed_batch_set_compact_output(toed_batch *b, float *d_edges, int *d_index, long long capacity);
 
 /* Autotuner of the batched form (no reference counterpart: the reference runs
  * a process per image, run_gpu_batch.sh:152-167).  Plans the batch with 8 and

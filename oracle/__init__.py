@@ -198,17 +198,21 @@ def ref_available():
     return os.path.exists(REF_BIN)
 
 
-def run_ref(img_u8, nthreads=1):
+def run_ref(img_u8, nthreads=1, env=None):
     """Run the reference's own cpu_toed.cpp (compiled into oracle/_ref) on a
-    uint8 image.  Returns (edges N x 4 float64, time_conv_s, time_nms_s)."""
+    uint8 image.  Returns (edges N x 4 float64, time_conv_s, time_nms_s): the
+    reference's own omp_get_wtime intervals around its convolution and NMS
+    (cpu_toed.cpp:171/359, 390/499).  env: extra environment variables for the
+    run (e.g. OMP_PROC_BIND)."""
     img_u8 = np.ascontiguousarray(img_u8, np.uint8)
     H, W = img_u8.shape
+    full_env = dict(os.environ, **(env or {}))
     with tempfile.TemporaryDirectory() as td:
         raw = os.path.join(td, "img.raw")
         out = os.path.join(td, "out.bin")
         img_u8.tofile(raw)
         subprocess.run([REF_BIN, raw, str(H), str(W), str(int(nthreads)), out, td],
-                       check=True, stdout=subprocess.DEVNULL)
+                       check=True, stdout=subprocess.DEVNULL, env=full_env)
         with open(out, "rb") as f:
             n = int(np.frombuffer(f.read(8), np.int64)[0])
             tc, tn = np.frombuffer(f.read(16), np.float64)

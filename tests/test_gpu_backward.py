@@ -131,3 +131,48 @@ def test_full_batch_64x512_tile_and_index_agree(pkg, detector, scene):
     # FP64 sums of the same 550k per-edgel terms in two orders
     assert s_tile == pytest.approx(s_idx, rel=1e-7)
     assert bool((g_tile.abs().amax(dim=(1, 2)) > 0).all())
+
+
+def test_new_tensor_each_step_reuses_the_plan(pkg, detector, scene):
+    """A training loop passes a new input tensor every step: the cached plan
+    (keyed by shape, strides and parameters, not by address) is re-pointed at
+    it instead of rebuilt, at most two plans are kept, and each step's rows
+    and gradients equal those of a call on a fresh copy of the same data."""
+    autograd = __import__(pkg.__name__ + ".autograd", fromlist=["_PLANS"])
+    base = np.stack([scene.shapes_scene(60 + k, 96, 128) for k in range(2)]).astype(np.float32)
+    w = torch.tensor([0.3, -0.7, 1.3, 0.5], device="cuda")
+    sig = torch.tensor(1.5, dtype=torch.float64)
+    plans = set()
+    ref_e, ref_g = None, None
+    for step in range(4):
+        x = torch.from_numpy(base + (step % 2) * 1.0).cuda().requires_grad_(True)   # a new tensor each step
+        e, _ = pkg.detect_differentiable(x, sig, detector=detector)
+        (e * w).sum().backward()
+        plans.add(id(next(reversed(autograd._PLANS.values()))))
+        if step % 2 == 0:
+            if ref_e is None:
+                ref_e, ref_g = e.detach().clone(), x.grad.clone()
+            else:
+                # rows bit for bit; gradients up to the order of the float atomics that sum them
+                assert torch.equal(e, ref_e)
+                assert (x.grad - ref_g).abs().max().item() <= 1e-5 * ref_g.abs().max().item()
+    assert len(plans) == 1 and len(autograd._PLANS) <= 2
+    # a call on another shape adds one plan; the cache never holds more than two
+    y = torch.from_numpy(base[:, :80, :100].copy()).cuda()
+    pkg.detect_differentiable(y, sig, detector=detector)
+    pkg.detect_differentiable(y[:1].clone(), sig, detector=detector)
+    assert len(autograd._PLANS) <= 2
+
+
+def test_in_place_edit_before_backward_is_rejected(pkg, detector, scene):
+    """The input is saved for the backward: editing it in place between the
+    forward and the backward is an error (torch's version check), not a
+    silent gradient of the edited image."""
+    x = torch.from_numpy(np.stack([scene.shapes_scene(70, 96, 128)]).astype(np.float32)).cuda()
+    xr = x.clone().requires_grad_(True)
+    y = xr * 1.0                       # a non-leaf input, edited in place below
+    e, _ = pkg.detect_differentiable(y, 2.0, detector=detector)
+    with torch.no_grad():
+        y.add_(1.0)
+    with pytest.raises(RuntimeError):
+        e.sum().backward()

@@ -224,22 +224,28 @@ class BatchDetector:
     start at row ``offsets[m]``; ``counts[m]`` is its edgel count."""
 
     def __init__(self, detector, images, sigmas, thresh=DEFAULT_THRESH, border=DEFAULT_BORDER, capacities=None,
-                 persistent=False, with_index=False, wg_per_cu=0):
+                 persistent=False, with_index=False, wg_per_cu=0, compact_capacity=None):
         torch = _torch()
         self.det = detector
         self.images = list(images)
-        n = len(self.images)
+        n = self.n = len(self.images)
         if n == 0:
             raise ValueError("empty batch")
         sig = list(sigmas) if np.ndim(sigmas) else [float(sigmas)] * n
-        caps = capacities or [4 * int(im.shape[0]) * int(im.shape[1]) for im in self.images]
+        self.compact = compact_capacity is not None
+        if self.compact:
+            caps = [1] * n      # the per-image buffers are not used with compact output
+        else:
+            caps = capacities or [4 * int(im.shape[0]) * int(im.shape[1]) for im in self.images]
         self.capacities = [int(c) for c in caps]
         self.offsets = np.concatenate([[0], np.cumsum(self.capacities)]).astype(np.int64)
         dev = self.images[0].device
         self.edges = torch.empty((int(self.offsets[-1]), 4), dtype=torch.float32, device=dev)
         self.index = torch.empty(int(self.offsets[-1]), dtype=torch.int32, device=dev) if with_index else None
-        self.counts = torch.zeros(n, dtype=torch.int32, device=dev)
-        self.flags = torch.zeros(n, dtype=torch.int32, device=dev)
+        # counts and flags side by side: one device-to-host copy reads both
+        self.cf = torch.zeros(2 * n, dtype=torch.int32, device=dev)
+        self.counts = self.cf[:n]
+        self.flags = self.cf[n:]
         arr = (_lib.toed_image_t * n)()
         for m, im in enumerate(self.images):
             o = int(self.offsets[m])
@@ -254,32 +260,60 @@ class BatchDetector:
             check(lib().toed_batch_set_persistent(self._b, 1), "toed_batch_set_persistent")
             if wg_per_cu:
                 check(lib().toed_batch_set_grid(self._b, int(wg_per_cu)), "toed_batch_set_grid")
+        if self.compact:
+            cap = int(compact_capacity)
+            self.cedges = torch.empty((max(cap, 1), 4), dtype=torch.float32, device=dev)
+            self.cindex = torch.empty(max(cap, 1), dtype=torch.int32, device=dev) if with_index else None
+            check(lib().toed_batch_set_compact_output(self._b, _ptr(self.cedges), _ptr(self.cindex), cap),
+                  "toed_batch_set_compact_output")
+
+    def set_images(self, images):
+        """Point the plan at other device images of the same shapes, dtypes and
+        row strides (toed_batch_set_images): no new plan, no new buffers."""
+        images = list(images)
+        if len(images) != self.n:
+            raise ValueError("one image per planned image")
+        for a, b in zip(images, self.images):
+            if a.shape != b.shape or a.dtype != b.dtype or a.stride() != b.stride() or a.device != b.device:
+                raise ValueError("images must match the planned shapes, dtypes and strides")
+        arr = (ctypes.c_void_p * self.n)(*[im.data_ptr() for im in images])
+        check(lib().toed_batch_set_images(self._b, arr), "toed_batch_set_images")
+        self.images = images
 
     def run(self, graph=False):
         self.det._bind_stream()
         fn = lib().toed_batch_graph_run if graph else lib().toed_batch_run
         check(fn(self._b, _ptr(self.counts), _ptr(self.flags)), "toed_batch_run")
 
-    def results(self):
-        """Synchronise and return the per-image N x 4 device tensors."""
-        counts = self.counts.cpu().numpy()
-        flags = self.flags.cpu().numpy()
+    def counts_flags(self):
+        """Synchronise and return (counts, flags) as numpy arrays (one copy);
+        raises on staging or capacity overflow."""
+        cf = self.cf.cpu().numpy()
+        counts, flags = cf[:self.n], cf[self.n:]
         if (flags & FLAG_STAGING).any():
             raise ToedError(6, "batch (raise staging_fraction)")
         if (flags & FLAG_CAPACITY).any():
             raise ToedError(5, "batch (edgel capacity)")
-        out = []
-        for m in range(len(self.images)):
-            o = int(self.offsets[m])
-            out.append(self.edges[o:o + int(counts[m])])
-        return out
+        return counts, flags
+
+    def _starts(self, counts):
+        if self.compact:
+            return np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+        return self.offsets[:-1]
+
+    def results(self):
+        """Synchronise and return the per-image N x 4 device tensors."""
+        counts, _ = self.counts_flags()
+        buf = self.cedges if self.compact else self.edges
+        st = self._starts(counts)
+        return [buf[int(st[m]):int(st[m]) + int(counts[m])] for m in range(self.n)]
 
     def backward(self, gouts, gimgs, gsigma=None):
         """Adjoint of the last run (edgel set frozen): gouts[m] holds image m's
         dL/d[x, y, orientation, strength] rows in its output layout, gimgs[m]
         receives dL/dimage (accumulated), gsigma (float64 device tensor, one
         element, optional) dL/dsigma summed over the images."""
-        n = len(self.images)
+        n = self.n
         arr = (_lib.toed_grad_out_t * n)()
         for m in range(n):
             arr[m].d_gout = gouts[m].data_ptr()
@@ -295,7 +329,7 @@ class BatchDetector:
         elements; gsigma_ptr the one-element float64 accumulator or None.  For
         callers that already hold the batch in single tensors (no per-image
         views)."""
-        n = len(self.images)
+        n = self.n
         arr = getattr(self, "_grad_arr", None)
         if arr is None:
             arr = self._grad_arr = (_lib.toed_grad_out_t * n)()
@@ -310,7 +344,9 @@ class BatchDetector:
 
     def index_results(self):
         counts = self.counts.cpu().numpy()
-        return [self.index[int(self.offsets[m]):int(self.offsets[m]) + int(counts[m])] for m in range(len(counts))]
+        buf = self.cindex if self.compact else self.index
+        st = self._starts(counts)
+        return [buf[int(st[m]):int(st[m]) + int(counts[m])] for m in range(len(counts))]
 
     def close(self):
         if getattr(self, "_b", None):

@@ -120,6 +120,8 @@ SIGNATURES = [
     ("toed_batch_graph_run", _I, [_VP, _VP, _VP]),
     ("toed_batch_destroy", _I, [_VP]),
     ("toed_batch_set_grid", _I, [_VP, _I]),
+    ("toed_batch_set_images", _I, [_VP, ctypes.POINTER(_VP)]),
+    ("toed_batch_set_compact_output", _I, [_VP, _VP, _VP, _LL]),
     ("toed_image_load", _I, [ctypes.c_char_p, _I, ctypes.POINTER(ctypes.POINTER(ctypes.c_ubyte)), _PI, _PI]),
     ("toed_image_free", None, [ctypes.POINTER(ctypes.c_ubyte)]),
     ("toed_batch_autotune", _I, [_VP, _I, ctypes.POINTER(toed_image_t), _I, ctypes.POINTER(toed_tune_t),

@@ -13,6 +13,7 @@ dL/d[x, y, orientation, strength] to every pixel and to sigma (window radii
 held fixed) through toed_backward: FP64 recomputation and hand-written adjoint
 per edgel, transposed filters scattered into dL/dimage.
 """
+import collections
 import ctypes
 
 import numpy as np
@@ -27,29 +28,39 @@ def _torch():
 
 
 class _Plan:
-    """A cached forward plan for one (images tensor, sigma) pair."""
+    """A cached forward plan for one (shape, strides, sigma, thresh, border,
+    capacity): its images are re-pointed at each forward's tensor
+    (toed_batch_set_images), and its rows go to one compact buffer whose
+    per-image starts are computed on the device."""
 
     def __init__(self, det, images, sigma, thresh, border, cap):
         from . import BatchDetector
         B = images.shape[0]
-        self.images = images
         self.gen = 0    # forward runs of this plan; the tile backward needs the latest
-        self.bd = BatchDetector(det, [images[b] for b in range(B)], [sigma] * B, thresh, border,
-                                capacities=[cap] * B, with_index=True)
+        self.ptr = images.data_ptr()
+        self.bd = BatchDetector(det, [images[b] for b in range(B)], [sigma] * B, thresh, border, with_index=True,
+                                compact_capacity=B * cap)
+
+    def point_at(self, images):
+        if images.data_ptr() != self.ptr:
+            self.bd.set_images([images[b] for b in range(images.shape[0])])
+            self.ptr = images.data_ptr()
+        self.bd.images = []     # no reference to the inputs outlives the call
 
 
-_PLANS = {}
+_PLANS = collections.OrderedDict()
+_MAX_PLANS = 2
 
 
 def _plan_for(det, images, sigma, thresh, border, cap):
-    key = (id(det), images.data_ptr(), tuple(images.shape), tuple(images.stride()), float(sigma), float(thresh),
+    key = (id(det), str(images.device), tuple(images.shape), tuple(images.stride()), float(sigma), float(thresh),
            int(border), int(cap))
-    p = _PLANS.get(key)
-    if p is None or p.images is not images:
-        if len(_PLANS) > 16:
-            _PLANS.clear()
+    p = _PLANS.pop(key, None)
+    if p is None:
         p = _Plan(det, images, sigma, thresh, border, cap)
-        _PLANS[key] = p
+    _PLANS[key] = p                   # most recently used last
+    while len(_PLANS) > _MAX_PLANS:   # a forward still waiting for its backward keeps its plan alive
+        _PLANS.popitem(last=False)
     return p
 
 
@@ -61,21 +72,20 @@ def _make_function():
         def forward(ctx, images, sigma_t, det, thresh, border, cap):
             sigma = float(sigma_t)
             plan = _plan_for(det, images, sigma, thresh, border, cap)
+            plan.point_at(images)
             bd = plan.bd
             bd.run()
             plan.gen += 1
             ctx.gen = plan.gen
-            counts = bd.counts.cpu().numpy().astype(np.int64)
-            flags = bd.flags.cpu().numpy()
-            if flags.any():
-                raise _lib.ToedError(6 if (flags & _lib.FLAG_STAGING).any() else 5, "differentiable forward")
+            counts, _ = bd.counts_flags()          # the one host read of the forward
+            counts = counts.astype(np.int64)
             offsets = np.concatenate([[0], np.cumsum(counts)])
-            parts = [bd.edges[int(bd.offsets[b]):int(bd.offsets[b]) + int(counts[b])] for b in range(len(counts))]
-            out = torch.cat(parts, 0) if parts else bd.edges[:0]
-            # compact copies, so a later forward on the same plan cannot change this backward
-            iparts = [bd.index[int(bd.offsets[b]):int(bd.offsets[b]) + int(counts[b])] for b in range(len(counts))]
-            ctx.index = torch.cat(iparts, 0) if iparts else bd.index[:0].clone()
+            n = int(offsets[-1])
+            # copies, so a later forward on the same plan cannot change this output or backward
+            out = bd.cedges[:n].clone()
+            ctx.index = bd.cindex[:n].clone()
             ctx.counts_dev = bd.counts.clone()
+            ctx.save_for_backward(images)
             ctx.plan = plan
             ctx.counts = counts
             ctx.offsets = offsets
@@ -89,13 +99,13 @@ def _make_function():
 
         @staticmethod
         def backward(ctx, g_out, g_offsets):
+            (images,) = ctx.saved_tensors     # raises if the input was modified in place
             plan, counts, offsets = ctx.plan, ctx.counts, ctx.offsets
-            images = plan.images
             B, H, W = images.shape
             g_out = g_out.contiguous().float()
             gimg = torch.zeros((B, H, W), dtype=torch.float32, device=images.device)
             gsig = torch.zeros(1, dtype=torch.float64, device=images.device)
-            if plan.gen == ctx.gen:
+            if plan.gen == ctx.gen and plan.ptr == images.data_ptr():
                 # the plan's staging still holds this forward's edgels: tile-driven backward,
                 # per-image addresses computed from the two batch tensors (an image with
                 # no edgels reads no upstream row)

@@ -95,6 +95,7 @@ struct ImageJob {
     float thresh;
     int qad;                   // 1: orientation at the subpixel location (toed_set_orientation_mode)
     int *index_out;            // optional interp index per edgel
+    long long *cbase;          // compact output: the device slot holding this image's first row (else null)
 };
 
 // Survivors of a tile, per phase: the finished edgel [x, y, theta, strength]
@@ -830,7 +831,7 @@ __global__ __launch_bounds__(1024) void toed_scan_images(const ImageJob *jobs, i
     if (threadIdx.x == 0) {
         const int n = carry;
         if (J.count_out) *J.count_out = n;
-        if (n > J.capacity && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_CAPACITY);
+        if (!J.cbase && n > J.capacity && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_CAPACITY);
     }
 }
 
@@ -1023,6 +1024,45 @@ __device__ __forceinline__ long long out_index(const ImageJob &J, const int *off
     return (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
 }
 
+// Output row of an edgel: its rank in the image's list, plus, with compact
+// output, the image's first row in the shared buffer.
+__device__ __forceinline__ long long out_row(const ImageJob &J, long long idx) {
+    return J.cbase ? idx + *J.cbase : idx;
+}
+
+// Compact output: image m's rows start after those of images 0 .. m-1 (the
+// caller's order).  One workgroup scans the per-image counts; an image whose
+// rows would pass the shared capacity gets TOED_FLAG_CAPACITY.
+__global__ __launch_bounds__(1024) void toed_compact_base_kernel(const int *counts, int *flags, int n,
+                                                                 long long capacity, long long *cbase) {
+    __shared__ long long wsum[16];
+    __shared__ long long carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int base = 0; base < n; base += 1024) {
+        const int m = base + threadIdx.x;
+        const long long x = m < n ? counts[m] : 0;
+        long long inc = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const long long y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        long long wb = carry;
+        for (int w = 0; w < wave; ++w) wb += wsum[w];
+        if (m < n) {
+            cbase[m] = wb + inc - x;
+            if (wb + inc > capacity && flags) atomicOr(flags + m, (int)TOED_FLAG_CAPACITY);
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = wb + inc;
+        __syncthreads();
+    }
+}
+
 // One workgroup of four wavefronts per tile: a tile without edgels ends at
 // once; a tile's edgels are dealt to its 32 groups of 8 lanes, so the tiles
 // with many edgels (along the scene's contours) finish in few rounds.
@@ -1061,7 +1101,7 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
             const uint32_t key = stg.key[gi];
             const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
             const int R = 2 * tr * TH + r, C = 2 * tc * TW + c;
-            const long long idx = out_index(J, offsets, chunk_pref, R, tc, rank);
+            const long long idx = out_row(J, out_index(J, offsets, chunk_pref, R, tc, rank));
             if (idx >= J.capacity) continue;
             reinterpret_cast<float4 *>(J.out)[idx] = stg.rec[gi];
             if (J.index_out) J.index_out[idx] = R * IW + C;
@@ -1116,7 +1156,7 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
                 const long long gi = J.stage_base + (tile_local * 4 + phase) * J.stage_cap + k;
                 const uint32_t key = stg.key[gi];
                 const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
-                const long long idx = out_index(J, offsets, chunk_pref, 2 * tr * TH + r, tc, rank);
+                const long long idx = out_row(J, out_index(J, offsets, chunk_pref, 2 * tr * TH + r, tc, rank));
                 s_rec[threadIdx.x] = stg.rec[gi];
                 s_idx[threadIdx.x] = idx < J.capacity ? idx : -1;
                 s_rc[threadIdx.x] = r | (c << 8);
@@ -1494,6 +1534,10 @@ struct toed_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int *g_counts = nullptr, *g_flags = nullptr;
+    bool dirty = false;                // the job table changed on the host (images re-pointed, compact output)
+    bool uploaded = false;             // the device job table holds the host's
+    long long *d_cbase = nullptr;      // compact output: each image's first row (caller's order)
+    long long ccap = 0;
 };
 
 static std::mutex g_mu;
@@ -1618,7 +1662,8 @@ static const void *tiles_kernel_ptr() { return (const void *)toed_tiles_kernel<I
 static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, int njobs, const ImageJob &job0,
                    const TileGroup *groups, int ngroups, long long ntiles, long long nchunks, int *counts,
                    int *offsets, int *chunks, StageBuf staging, int *tilecnt, int *tickets, hipEvent_t *ev,
-                   bool qad) {
+                   bool qad, const int *ccounts = nullptr, int *cflags = nullptr, long long ccap = 0,
+                   long long *cbase = nullptr) {
     if (ntiles <= 0) return TOED_OK;
     hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
@@ -1647,6 +1692,8 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
     hipLaunchKernelGGL(toed_scan_chunks, dim3((unsigned)nchunks), dim3(256), 0, s, d_jobs, njobs, job0, counts,
                        offsets, chunks);
     hipLaunchKernelGGL(toed_scan_images, dim3(njobs), dim3(1024), 0, s, d_jobs, njobs, job0, chunks);
+    if (cbase)
+        hipLaunchKernelGGL(toed_compact_base_kernel, dim3(1), dim3(1024), 0, s, ccounts, cflags, njobs, ccap, cbase);
     // finalize: one workgroup per tile
     hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)ntiles), dim3(FIN_NT), 0, s, d_jobs,
                        njobs, job0, TH, offsets, chunks, staging, tilecnt, (int)ntiles, tickets);
@@ -1984,6 +2031,34 @@ int toed_batch_set_grid(toed_batch *b, int wg_per_cu) {
     return TOED_OK;
 }
 
+int toed_batch_set_images(toed_batch *b, const void *const *d_imgs) {
+    if (!b || !d_imgs) return TOED_ERR_ARG;
+    for (int k = 0; k < b->n; ++k)
+        if (!d_imgs[b->order[k]]) return TOED_ERR_ARG;
+    for (int k = 0; k < b->n; ++k) b->jobs[k].img = d_imgs[b->order[k]];
+    b->dirty = true;
+    return TOED_OK;
+}
+
+int toed_batch_set_compact_output(toed_batch *b, float *d_edges, int *d_index, long long capacity) {
+    if (!b || !d_edges || capacity <= 0) return TOED_ERR_ARG;
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (!b->d_cbase && hipMalloc(&b->d_cbase, sizeof(long long) * b->n) != hipSuccess) {
+        b->d_cbase = nullptr;
+        return TOED_ERR_ALLOC;
+    }
+    for (int k = 0; k < b->n; ++k) {
+        ImageJob &J = b->jobs[k];
+        J.out = d_edges;
+        J.index_out = d_index;
+        J.capacity = capacity;
+        J.cbase = b->d_cbase + b->order[k];
+    }
+    b->ccap = capacity;
+    b->dirty = true;
+    return TOED_OK;
+}
+
 static int batch_upload(toed_batch *b, int *d_counts, int *d_flags) {
     for (int k = 0; k < b->n; ++k) {
         const int m = b->order[k];
@@ -2000,16 +2075,20 @@ static int batch_enqueue(toed_batch *b, hipStream_t s) {
     return enqueue(s, b->in_type, b->tile_rows, b->d_jobs, b->n, dummy, b->groups.data(), (int)b->groups.size(),
                    b->ntiles, b->nchunks,
                    b->d_counts, b->d_offsets, b->d_chunks, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
-                   b->persistent ? b->d_ticket : nullptr, nullptr, b->qad);
+                   b->persistent ? b->d_ticket : nullptr, nullptr, b->qad, b->g_counts, b->g_flags, b->ccap,
+                   b->d_cbase);
 }
 
 int toed_batch_run(toed_batch *b, int *d_counts, int *d_flags) {
     if (!b) return TOED_ERR_ARG;
     HIPCHK(hipSetDevice(b->ctx->device));
-    if (b->g_counts != d_counts || b->g_flags != d_flags || !b->gexec) {
+    if (b->d_cbase && !d_counts) return TOED_ERR_ARG;    // compact output needs the per-image counts
+    if (!b->uploaded || b->g_counts != d_counts || b->g_flags != d_flags || b->dirty) {
+        b->dirty = false;
         HIPCHK(hipStreamSynchronize(b->ctx->stream));
         int st = batch_upload(b, d_counts, d_flags);
         if (st) return st;
+        b->uploaded = true;
         if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
         if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
         b->g_counts = d_counts; b->g_flags = d_flags;
@@ -2023,12 +2102,16 @@ int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags) {
     // the legacy default stream cannot be captured: capture and replay on the
     // context's own (blocking) stream, which orders with the default stream
     hipStream_t s = b->ctx->stream ? b->ctx->stream : b->ctx->own_stream;
-    if (!b->gexec || b->g_counts != d_counts || b->g_flags != d_flags) {
+    if (b->d_cbase && !d_counts) return TOED_ERR_ARG;
+    if (!b->gexec || !b->uploaded || b->g_counts != d_counts || b->g_flags != d_flags || b->dirty) {
+        b->dirty = false;
         HIPCHK(hipStreamSynchronize(s));
         int st = batch_upload(b, d_counts, d_flags);
         if (st) return st;
+        b->uploaded = true;
         if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
         if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+        b->g_counts = d_counts; b->g_flags = d_flags;     // the captured launches read these
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         st = batch_enqueue(b, s);
         hipGraph_t g = nullptr;
@@ -2037,7 +2120,6 @@ int toed_batch_graph_run(toed_batch *b, int *d_counts, int *d_flags) {
         HIPCHK(e);
         b->graph = g;
         HIPCHK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-        b->g_counts = d_counts; b->g_flags = d_flags;
     }
     HIPCHK(hipGraphLaunch(b->gexec, s));
     return TOED_OK;
@@ -2164,6 +2246,7 @@ int toed_batch_destroy(toed_batch *b) {
     (void)hipFree(b->d_jobs); (void)hipFree(b->d_counts); (void)hipFree(b->d_offsets); (void)hipFree(b->d_chunks);
     (void)hipFree(b->d_tilecnt); (void)hipFree(b->d_ticket); (void)hipFree(b->d_srec); (void)hipFree(b->d_skey);
     (void)hipFree(b->d_gouts);
+    (void)hipFree(b->d_cbase);
     delete b;
     return TOED_OK;
 }
