@@ -245,6 +245,9 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
  * d_counts[n] / d_flags[n] (device, either may be NULL). */
 int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags);
 int toed_subpix_destroy(toed_subpix *p);
+/* Point the plan's maps at other device buffers of the same sizes and pitches
+ * (d_maps[m]: map m) for the next runs.  No reference counterpart. */
+int toed_subpix_set_maps(toed_subpix *p, const float *const *d_maps);
 /* Per-stage timing of later runs with HIP events (the reference's time_nms,
  * time_conv + time_correct, subpix_to_correction.hpp:76-78), in ms:
  * orientation + NMS kernel; isolation and convolution + correction kernels

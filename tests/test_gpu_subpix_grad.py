@@ -192,3 +192,80 @@ def test_subpix_plan_rerun_on_sparser_map(pkg, detector, oracle, scene):
     assert np.isfinite(g).all()
     _check_map(oracle, sparse.astype(np.float64), idx, G.astype(np.float64), g, 2.0, rng, "rerun 320x384", rows=e)
     b.close()
+
+
+def test_subpix_grad_full_size_maps(pkg, detector, oracle, scene):
+    """The differentiable correction at the workload's map size: two Bezier
+    maps of 2048 x 2048 (40 curves, the benchmark's seeds), thresh 0.1, sigma 2,
+    orientation estimated.  The plan API's gradient is linear in the upstream
+    rows, the autograd wrapper gives the plan API's rows and gradient, the
+    frozen FP64 loss reproduces map 0's rows, and a random directional
+    derivative of it matches the GPU gradient to 1e-4 of sum |g d|."""
+    S, B, cap = 2048, 2, 2048 * 2048 // 8
+    maps = [scene.bezier_map(5000 + m, S, S, 40) for m in range(B)]
+    x = torch.from_numpy(np.stack(maps)).cuda()
+    b = pkg.SubpixBatch(detector, [x[m] for m in range(B)], None, 0.1, 2.0, 2.0, capacities=[cap] * B,
+                        with_index=True)
+    b.set_grad(True)
+    b.run()
+    res = [r.cpu().numpy() for r in b.results()]
+    idx = [i.cpu().numpy().astype(np.int64) for i in b.index_results()]
+    assert all(len(r) > 50000 for r in res)
+    rng = np.random.default_rng(21)
+    G1 = [rng.standard_normal((len(r), 4)).astype(np.float32) for r in res]
+    G2 = [rng.standard_normal((len(r), 4)).astype(np.float32) for r in res]
+
+    def grad(Gs):
+        gm = [torch.zeros((S, S), dtype=torch.float32, device="cuda") for _ in range(B)]
+        b.backward([torch.from_numpy(np.ascontiguousarray(G)).cuda() for G in Gs], gm)
+        return [g.cpu().numpy().astype(np.float64) for g in gm]
+
+    g1, g2 = grad(G1), grad(G2)
+    g12 = grad([2 * a + 0.5 * c for a, c in zip(G1, G2)])
+    for m in range(B):
+        scale = np.abs(g12[m]).max()
+        assert scale > 0
+        assert np.abs(g12[m] - (2 * g1[m] + 0.5 * g2[m])).max() <= 1e-5 * scale
+    b.close()
+    # the autograd wrapper on the [B, S, S] stack
+    xg = x.clone().requires_grad_(True)
+    e, off = pkg.subpix_differentiable(xg, 0.1, 2.0, 2.0, detector=detector, capacity=cap)
+    assert np.array_equal(e.detach().cpu().numpy(), np.concatenate(res))
+    (e * torch.from_numpy(np.concatenate(G1)).cuda()).sum().backward()
+    for m in range(B):
+        ga = xg.grad[m].cpu().numpy().astype(np.float64)
+        assert np.abs(ga - g1[m]).max() <= 1e-5 * np.abs(g1[m]).max()
+    # FP64 frozen loss of map 0: its rows, then one random directional derivative
+    E0 = maps[0].astype(np.float64)
+    fr = oracle.SubpixFrozen(E0, idx[0], 2.0)
+    _check_rows(fr.loss(E0, G1[0].astype(np.float64), outputs=True)[1], res[0], "2048 map 0")
+    d = rng.standard_normal(E0.shape)
+    fd = _fd(fr, E0, G1[0].astype(np.float64), d)
+    an = float((g1[0] * d).sum())
+    l1 = float(np.abs(g1[0] * d).sum())
+    print("2048 map 0: %d edgels, directional fd %.8g gpu %.8g |diff|/sum|g d| %.2e"
+          % (len(idx[0]), fd, an, abs(fd - an) / l1), flush=True)
+    assert abs(fd - an) <= TOL * l1
+
+
+def test_subpix_autograd_plan_reuse_and_rerun(pkg, detector, scene):
+    """The subpix autograd plan is keyed by shape and parameters: a new maps
+    tensor each step reuses it, and a backward whose plan has run again since
+    its forward (a second forward in between) re-runs that forward's maps
+    first and gives the same gradient as right after it."""
+    sp = __import__(pkg.__name__ + ".subpix", fromlist=["_SP_PLANS"])
+    base = np.stack([scene.bezier_map(90 + m, 96, 128, 8) for m in range(2)])
+    w = torch.tensor([0.5, -1.1, 0.8, 1.7], device="cuda")
+    x1 = torch.from_numpy(base.copy()).cuda().requires_grad_(True)
+    e1, _ = pkg.subpix_differentiable(x1, 0.1, 2.0, 2.0, detector=detector)
+    (e1 * w).sum().backward()
+    g_ref = x1.grad.clone()
+    plan = next(reversed(sp._SP_PLANS.values()))
+    x2 = torch.from_numpy(base.copy()).cuda().requires_grad_(True)     # a new tensor, same data
+    e2, _ = pkg.subpix_differentiable(x2, 0.1, 2.0, 2.0, detector=detector)
+    x3 = torch.from_numpy(base[::-1].copy()).cuda()                   # other data through the same plan
+    pkg.subpix_differentiable(x3, 0.1, 2.0, 2.0, detector=detector)
+    assert next(reversed(sp._SP_PLANS.values())) is plan and len(sp._SP_PLANS) <= 2
+    (e2 * w).sum().backward()                                            # re-runs x2's forward first
+    assert torch.equal(e1, e2)
+    assert (x2.grad - g_ref).abs().max().item() <= 1e-5 * g_ref.abs().max().item()

@@ -131,6 +131,7 @@ SIGNATURES = [
                               ctypes.POINTER(_VP)]),
     ("toed_subpix_run", _I, [_VP, _VP, _VP]),
     ("toed_subpix_destroy", _I, [_VP]),
+    ("toed_subpix_set_maps", _I, [_VP, ctypes.POINTER(_VP)]),
     ("toed_subpix_set_timing", _I, [_VP, _I]),
     ("toed_subpix_last_times", _I, [_VP, _PF, _PF, _PF]),
     ("toed_subpix_set_grad", _I, [_VP, _I]),

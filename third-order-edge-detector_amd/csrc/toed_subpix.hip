@@ -1458,6 +1458,7 @@ struct toed_subpix {
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // differentiable form
     bool grad = false, grad_ran = false;
+    bool dirty = false;         // maps re-pointed on the host since the last upload
     char *d_gwork = nullptr;
     std::vector<size_t> goff;
     int eblocks = 0;            // edgel-kernel workgroups per map
@@ -1630,7 +1631,9 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
         p->jobs[m].flags_out = f;
     }
     hipStream_t s = (hipStream_t)toed_get_stream(p->ctx);
-    if (changed) SPCHK(hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(SpJob) * p->n, hipMemcpyHostToDevice, s));
+    if (changed || p->dirty)
+        SPCHK(hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(SpJob) * p->n, hipMemcpyHostToDevice, s));
+    p->dirty = false;
     hipLaunchKernelGGL(sp_reset_kernel, dim3((p->n + 255) / 256), dim3(256), 0, s, p->d_jobs, p->n);
     if (p->timing) SPCHK(hipEventRecord(p->ev[0], s));
     hipLaunchKernelGGL(sp_orient_nms_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
@@ -1650,6 +1653,15 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     if (p->timing) SPCHK(hipEventRecord(p->ev[5], s));
     SPCHK(hipGetLastError());
     p->grad_ran = p->grad;
+    return TOED_OK;
+}
+
+int toed_subpix_set_maps(toed_subpix *p, const float *const *d_maps) {
+    if (!p || !d_maps) return TOED_ERR_ARG;
+    for (int m = 0; m < p->n; ++m)
+        if (!d_maps[m]) return TOED_ERR_ARG;
+    for (int m = 0; m < p->n; ++m) p->jobs[m].map = d_maps[m];
+    p->dirty = true;
     return TOED_OK;
 }
 

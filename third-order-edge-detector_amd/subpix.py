@@ -8,6 +8,7 @@ reference's subpix_TO_correction/ path) over the C ABI of libtoed_hip.so.
     SubpixelTOCorrectionGPU   mirror of SubpixelTOCorrectionCPU<T>
                               (subpix_to_correction.hpp:20-96)
 """
+import collections
 import ctypes
 import os
 
@@ -100,6 +101,19 @@ class SubpixBatch:
         self.det._bind_stream()
         check(lib().toed_subpix_run(self._p, ctypes.c_void_p(self.counts.data_ptr()),
                                     ctypes.c_void_p(self.flags.data_ptr())), "toed_subpix_run")
+
+    def set_maps(self, maps):
+        """Point the plan at other device maps of the same shapes and row
+        strides (toed_subpix_set_maps): no new plan, no new buffers."""
+        maps = list(maps)
+        if len(maps) != len(self.maps):
+            raise ValueError("one map per planned map")
+        for a, b in zip(maps, self.maps):
+            if b is not None and (a.shape != b.shape or a.dtype != b.dtype or a.stride() != b.stride()):
+                raise ValueError("maps must match the planned shapes, dtypes and strides")
+        arr = (ctypes.c_void_p * len(maps))(*[E.data_ptr() for E in maps])
+        check(lib().toed_subpix_set_maps(self._p, arr), "toed_subpix_set_maps")
+        self.maps = maps
 
     def set_timing(self, on=True):
         check(lib().toed_subpix_set_timing(self._p, int(bool(on))), "toed_subpix_set_timing")
@@ -257,26 +271,36 @@ class SubpixelTOCorrectionGPU:
 
 
 class _SubpixPlan:
+    """A cached differentiable plan for one (shape, strides, threshold, sigma,
+    radius, capacity), re-pointed at each forward's maps."""
+
     def __init__(self, det, maps, thresh, sigma, nbr, cap):
-        self.maps = maps
         self.gen = 0
+        self.ptr = maps.data_ptr()
         self.b = SubpixBatch(det, [maps[k] for k in range(maps.shape[0])], None, thresh, sigma, nbr,
                              capacities=[cap] * maps.shape[0], with_index=True)
         self.b.set_grad(True)
 
+    def point_at(self, maps):
+        if maps.data_ptr() != self.ptr:
+            self.b.set_maps([maps[k] for k in range(maps.shape[0])])
+            self.ptr = maps.data_ptr()
+        self.b.maps = [None] * maps.shape[0]    # no reference to the inputs outlives the call
 
-_SP_PLANS = {}
+
+_SP_PLANS = collections.OrderedDict()
+_SP_MAX_PLANS = 2
 
 
 def _subpix_plan(det, maps, thresh, sigma, nbr, cap):
-    key = (id(det), maps.data_ptr(), tuple(maps.shape), tuple(maps.stride()), float(thresh), float(sigma),
+    key = (id(det), str(maps.device), tuple(maps.shape), tuple(maps.stride()), float(thresh), float(sigma),
            float(nbr), int(cap))
-    p = _SP_PLANS.get(key)
-    if p is None or p.maps is not maps:
-        if len(_SP_PLANS) > 8:
-            _SP_PLANS.clear()
+    p = _SP_PLANS.pop(key, None)
+    if p is None:
         p = _SubpixPlan(det, maps, thresh, sigma, nbr, cap)
-        _SP_PLANS[key] = p
+    _SP_PLANS[key] = p
+    while len(_SP_PLANS) > _SP_MAX_PLANS:
+        _SP_PLANS.popitem(last=False)
     return p
 
 
@@ -290,6 +314,7 @@ def _make_subpix_function():
         @staticmethod
         def forward(ctx, maps, det, thresh, sigma, nbr, cap):
             plan = _subpix_plan(det, maps, thresh, sigma, nbr, cap)
+            plan.point_at(maps)
             b = plan.b
             b.run()
             plan.gen += 1
@@ -298,6 +323,7 @@ def _make_subpix_function():
             offsets = np.concatenate([[0], np.cumsum(counts)])
             parts = [b.edges[int(b.offsets[m]):int(b.offsets[m]) + int(counts[m])] for m in range(len(counts))]
             out = torch.cat(parts, 0) if parts else b.edges[:0]
+            ctx.save_for_backward(maps)
             ctx.plan, ctx.counts, ctx.offsets = plan, counts, offsets
             off_t = torch.from_numpy(offsets)
             ctx.mark_non_differentiable(off_t)
@@ -305,10 +331,15 @@ def _make_subpix_function():
 
         @staticmethod
         def backward(ctx, g_out, g_offsets):
+            (maps,) = ctx.saved_tensors           # raises if the maps were modified in place
             plan, counts, offsets = ctx.plan, ctx.counts, ctx.offsets
-            if plan.gen != ctx.gen:
-                raise RuntimeError("subpix_differentiable: the plan ran again before this backward")
-            maps = plan.maps
+            if plan.gen != ctx.gen or plan.ptr != maps.data_ptr():
+                # the plan ran again since this forward: run it once more on this
+                # forward's maps (the same decisions and records), then take the adjoint
+                plan.point_at(maps)
+                plan.b.run()
+                plan.gen += 1
+                assert np.array_equal(plan.b._counts().astype(np.int64), counts)
             B, H, W = maps.shape
             g_out = g_out.contiguous().float()
             if g_out.numel() == 0:
