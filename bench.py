@@ -7,8 +7,8 @@ seeded scenes of mixed size and sigma in one plan, replayed as a hipGraph)
 against a loop of plain launches over the same images, the differentiable
 forward+backward form, the subpixel third-order correction of 32 Bezier edge
 maps of 2048x2048, kernel durations from HIP events on the library's stream,
-the VALU/HBM roofline of the dominant kernel and the reference CPU paths timed
-on this host.
+the VALU/HBM roofline of the dominant kernel, the curvelet formation stage on
+the frame's edgel list, and the reference CPU paths timed on this host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload all|plain|batch|fwdbwd|subpix]
 """
@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the plain-frame tile kernel (2160x3840 u8, sigma 2), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, FETCH_SIZE scaled by
 # 2.0 (what the counter reads for a 256 MiB copy); the summary is kept under profiles/.
-TRAFFIC_TILES_PLAIN_BYTES = (2.0 * 24196.1 + 19272.1) * 1024
+TRAFFIC_TILES_PLAIN_BYTES = (2.0 * 24162.1 + 19272.1) * 1024
 BATCH_SIZES = [(480, 640), (480, 752), (720, 1280), (1080, 1920), (2160, 3840)]
 BATCH_SIGMAS = [1.0, 1.5, 2.0]
 
@@ -250,6 +250,48 @@ def cpu_baseline_subpix(map_np, gpu_edges=None):
            "sample": "%d runs on one %dx%d Bezier map, FP64, orientation estimate through the edgel list "
                      "(%.2f s in total)" % (runs, H, W, sec)}
     return res, (accuracy(gpu_edges, first) if gpu_edges is not None else None)
+
+
+def bench_curvelets(pkg, rows, H, W, threads, reps=5, ref_band=540):
+    """Curvelet formation on the headline frame's GPU list (widened to double,
+    as the reference's driver hands its FP64 list over, main_gpu_cpu.cpp:
+    211-281) with the drivers' settings: the library on `threads` host
+    threads (best of `reps` runs of execute()), and -- when the reference's
+    own curvelet code is built under oracle/_ref -- that code on a bounded
+    sample: the edgels of the frame's top `ref_band` rows, timed around its
+    execute(), with the two outputs compared bit for bit on the sample."""
+    e = np.ascontiguousarray(rows, np.float64)
+    best = None
+    for _ in range(reps):
+        cf = pkg.CurveletFormation(e, H, W)
+        t0 = time.perf_counter()
+        cf.execute(threads)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    h, _ = cf.get_output_size(0)
+    res = {"edgels": int(len(e)), "curvelets": int(h), "ms": round(best * 1e3, 2),
+           "edgels_s": round(len(e) / best, 1), "threads": threads,
+           "settings": "nrad 3.5, gap 1.5, dx 0.4, dt 15 deg, max_k 0.3, style 2, groups of 4; all four stages"}
+    import oracle
+    if oracle.curvelet_ref_available():
+        band = e[e[:, 1] < ref_band]
+        secs, ref = oracle.run_curvelet_ref(band, ref_band, W, types="01",
+                                            binary=os.path.join(ROOT, "oracle", "_ref", "curvelet_ref"))
+        cb = pkg.CurveletFormation(band, ref_band, W)
+        t0 = time.perf_counter()
+        cb.execute(threads)
+        ours_band = time.perf_counter() - t0
+        same = True
+        for t in (0, 1):
+            c, i = cb.get_output_arrary(t)
+            same = same and np.array_equal(c, ref[t][0]) and np.array_equal(i.view(np.int64), ref[t][1].view(np.int64))
+        res["cpu_reference"] = {"edgels_s": round(len(band) / secs, 1), "threads": 1, "kind": "reference",
+                                "sample": "%d edgels of the frame's top %d rows, one run of execute() (%.2f s)"
+                                          % (len(band), ref_band, secs),
+                                "library_on_sample_ms": round(ours_band * 1e3, 2),
+                                "bit_exact_on_sample": bool(same)}
+        res["vs_reference"] = round(res["edgels_s"] / res["cpu_reference"]["edgels_s"], 1)
+    return res
 
 
 def accuracy(gpu, ref):
@@ -477,6 +519,15 @@ def main():
             out["config"] = {"workload": "subpixel TO correction, 32 x 2048x2048 f32 Bezier maps, thresh 0.1, "
                                          "sigma 2, orientation estimated", "global_batch": 32}
         log("subpix: %.1f Mpx/s, %.3f ms per batch" % (forms["subpix_32x2048x2048"]["mpx_s"], sp["sec"] * 1e3))
+    if plain is not None and args.workload == "all":
+        log("curvelets on the 2160x3840 list ...")
+        threads = host_cpu()["threads_used"]
+        cv = bench_curvelets(pkg, plain["edges_np"], plain["H"], plain["W"], threads)
+        forms["curvelets_2160x3840"] = cv
+        log("curvelets: %d edgels -> %d curvelets in %.1f ms on %d threads%s"
+            % (cv["edgels"], cv["curvelets"], cv["ms"], threads,
+               "" if "cpu_reference" not in cv else ", reference %.0f edgels/s (x%.1f), bit-exact %s"
+               % (cv["cpu_reference"]["edgels_s"], cv["vs_reference"], cv["cpu_reference"]["bit_exact_on_sample"])))
     out["forms"] = forms
     if plain is not None and not args.no_cpu:
         log("cpu baseline ...")

@@ -13,6 +13,7 @@
  *                           launch sequence producing the raster-ordered N x 4 list on the GPU)
  *   run_gpu_batch.sh:152-167 one process per image              -> toed_batch_* (one plan,
  *                           one launch sequence, optionally one captured hipGraph)
+ *   curvelet/form_curvelet_process.hpp:23-82 form_curvelet_process (host) -> toed_curvelet_*
  * A C ABI cannot overload, so the f32 variants carry a suffix.
  *
  * Every function returns a toed_status (0 = success).  Launch functions are
@@ -326,6 +327,39 @@ int toed_gpu_nms_thresh_f64(int device_id, int interp_h, int interp_w,
                             const double *dev_Ix, const double *dev_Iy, const double *dev_I_grad_mag,
                             double *dev_subpix_pos_x_map, double *dev_subpix_pos_y_map,
                             double thresh, int border);
+
+/* Curvelet formation (host code) ---------------------------------------------
+ * The consumer the reference runs on its FP64 edgel list (main_gpu_cpu.cpp:
+ * 211-281): curvelet/form_curvelet_process.hpp:23-82 form_curvelet_process.
+ *   toed_curvelet_create      the constructor (:23-31, :101-113): edgeinfo is
+ *                             the list in column major, n_edgels rows of
+ *                             [x, y, orientation, strength] (Array.hpp layout,
+ *                             as main_gpu_cpu.cpp:231-236 hands it over);
+ *                             rad, gap, dx, dt (radians), token_len, max_k,
+ *                             max_size_to_group and the centered /
+ *                             bidirectional grouping flags as there.  The
+ *                             list is copied.
+ *   toed_curvelet_execute     execute() (:120-163): grouping, link graph,
+ *                             curve fragments, poly-arcs.  nthreads = 0 uses
+ *                             every hardware thread; results do not depend on it.
+ *   toed_curvelet_output_size get_output_size (:437-457); info_w is the info
+ *                             width the reference's driver allocates (10, 1, 12).
+ *   toed_curvelet_output      get_output_arrary (:459-485): chain (h x w ints)
+ *                             and info (h x info_w doubles) in column major;
+ *                             either may be NULL.  Cells the reference leaves
+ *                             unwritten are 0.  Type 2 (poly-arcs) is written
+ *                             in full; the reference's own build stops at the
+ *                             assert in curvelet::set_output for that type.
+ * Results are bit-identical to the reference's on the same list. */
+typedef struct toed_curvelets toed_curvelets;
+int toed_curvelet_create(const double *edgeinfo, unsigned n_edgels, unsigned nrows, unsigned ncols, double rad,
+                         double gap, double dx, double dt, double token_len, double max_k,
+                         unsigned max_size_to_group, int centered, int bidirectional, toed_curvelets **out);
+int toed_curvelet_execute(toed_curvelets *h, int nthreads);
+int toed_curvelet_output_size(const toed_curvelets *h, unsigned output_type, unsigned *out_h, unsigned *out_w,
+                              unsigned *info_w);
+int toed_curvelet_output(const toed_curvelets *h, unsigned output_type, int *chain, double *info);
+int toed_curvelet_destroy(toed_curvelets *h);
 
 #ifdef __cplusplus
 }

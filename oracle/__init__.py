@@ -5,6 +5,9 @@ may import this package, and only as the checker (or the timed CPU baseline).
 The C sources restate the reference's ``cpu_toed.cpp`` (see the header of
 ``toed_oracle.c`` for the file:line map) and its
 ``subpix_TO_correction/subpix_to_correction.cpp`` (``subpix_oracle.c``).
+The curvelet formation stage has no restatement here: its checker is the
+reference's own header-only curvelet code, compiled by ``make ref`` into
+``_ref/`` (``run_curvelet_ref``), and the golden hashes it wrote.
 """
 import ctypes
 import os
@@ -404,3 +407,42 @@ class SubpixFrozen:
                                           self.oct.ctypes.data_as(ip), self.base.ctypes.data_as(ip),
                                           self.fb.ctypes.data_as(ip), _p(outs))
         return (L, outs) if outputs else L
+
+
+CURVELET_REF_BIN = os.path.join(HERE, "_ref", "curvelet_ref_ndebug")
+
+
+def curvelet_ref_available():
+    return os.path.exists(CURVELET_REF_BIN)
+
+
+def run_curvelet_ref(edges, H, W, types="012", rad=3.5, gap=1.5, dx=0.4, dt=15.0 / 180.0 * np.pi, token_len=1.0,
+                     max_k=0.3, style=2, group=4, binary=None):
+    """Run the reference's own curvelet code (curvelet/*.hpp compiled into
+    oracle/_ref by oracle/Makefile) on an N x 4 list, driven as
+    main_gpu_cpu.cpp:211-281 drives it.  Returns (seconds of execute(),
+    {type: (chain int32 h x w, info float64 h x info_w)}); the arrays are the
+    reference's column-major outputs, returned as row-major views.  The
+    default binary is the -DNDEBUG build (output type 2 stops at an assert in
+    the reference's own build); types 0 and 1 are the same in both."""
+    e = np.ascontiguousarray(np.asarray(edges, np.float64)[:, :4]) if len(edges) else np.zeros((0, 4))
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "edges.f64")
+        out = os.path.join(td, "out.bin")
+        e.tofile(inp)
+        subprocess.run([binary or CURVELET_REF_BIN, inp, str(len(e)), str(int(H)), str(int(W)), repr(float(rad)),
+                        repr(float(gap)), repr(float(dx)), repr(float(dt)), repr(float(token_len)),
+                        repr(float(max_k)), str(int(style)), str(int(group)), types, out], check=True,
+                       stdout=subprocess.DEVNULL)
+        b = open(out, "rb").read()
+    secs = float(np.frombuffer(b, np.float64, 1, 0)[0])
+    o, res = 8, {}
+    while o < len(b):
+        ty, h, w, iw = (int(v) for v in np.frombuffer(b, np.int32, 4, o))
+        o += 16
+        chain = np.frombuffer(b, np.int32, h * w, o).reshape(w, h).T.copy()
+        o += 4 * h * w
+        info = np.frombuffer(b, np.float64, h * iw, o).reshape(iw, h).T.copy()
+        o += 8 * h * iw
+        res[ty] = (chain, info)
+    return secs, res

@@ -17,6 +17,9 @@ in the package's own HIP kernels.
                                 (subpix_to_correction.hpp:20-96)
     subpix_differentiable       torch.autograd.Function over the correction path:
                                 gradients of the edgel rows w.r.t. the maps
+    CurveletFormation           curvelet formation on an edgel list, mirror of
+                                form_curvelet_process (host code, bit-identical)
+    form_curvelets              mirror of curvelet_formation (form_curvelet_main.cpp)
 """
 import ctypes
 import os
@@ -467,3 +470,4 @@ def detect_differentiable(images, sigma, detector=None, thresh=DEFAULT_THRESH, b
 
 
 from .subpix import SubpixBatch, SubpixelTOCorrectionGPU, subpix_differentiable, subpix_radius  # noqa: E402
+from .curvelet import CurveletFormation, form_curvelets  # noqa: E402

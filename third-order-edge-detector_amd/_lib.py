@@ -96,6 +96,7 @@ class toed_tune_t(ctypes.Structure):
 # exported symbols and their signatures: (name, restype, argtypes)
 _VP, _I, _F, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
 _PI, _PF = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
+_U, _PU = ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)
 SIGNATURES = [
     ("toed_version", _I, []),
     ("toed_sigma_radius", _I, [_F, _PI, _PI]),
@@ -144,6 +145,11 @@ SIGNATURES = [
     ("toed_gpu_nms_thresh_f32", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, _F, _I]),
     ("toed_gpu_convolve_sigma_f64", _I, [_I, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_double]),
     ("toed_gpu_nms_thresh_f64", _I, [_I, _I, _I, _VP, _VP, _VP, _VP, _VP, ctypes.c_double, _I]),
+    ("toed_curvelet_create", _I, [_VP, _U, _U, _U] + [ctypes.c_double] * 6 + [_U, _I, _I, ctypes.POINTER(_VP)]),
+    ("toed_curvelet_execute", _I, [_VP, _I]),
+    ("toed_curvelet_output_size", _I, [_VP, _U, _PU, _PU, _PU]),
+    ("toed_curvelet_output", _I, [_VP, _U, _VP, _VP]),
+    ("toed_curvelet_destroy", _I, [_VP]),
 ]
 
 _LIB = None

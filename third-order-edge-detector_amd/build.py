@@ -29,8 +29,12 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "
 HIP_LIBS = {
     "libtoed_hip.so": ["toed_detect.hip", "toed_backward.hip", "toed_subpix.hip"],
 }
-HOST_SRCS = {"libtoed_hip.so": ["toed_image.cpp"]}   # host-only parts of the same library
-HOST_LIBS = ["-lz"]
+HOST_SRCS = {"libtoed_hip.so": ["toed_image.cpp", "toed_curvelet.cpp"]}   # host-only parts of the same library
+HOST_LIBS = ["-lz", "-lpthread"]
+# the curvelet code chooses between sincos and separate sin / cos calls per
+# site (it follows the reference's build bit for bit); the compiler must not
+# merge the separate calls
+HOST_FLAGS = {"toed_curvelet.cpp": ["-fno-builtin-sin", "-fno-builtin-cos", "-fno-builtin-sincos"]}
 
 
 def _run(cmd):
@@ -48,7 +52,8 @@ def build_hip(verbose=False):
         objs = []
         for hs in HOST_SRCS.get(lib, []):
             obj = os.path.join(PKG, os.path.splitext(hs)[0] + ".host.o")
-            _run(["g++", "-O2", "-std=c++17", "-fPIC", "-I" + INC, "-c", os.path.join(CSRC, hs), "-o", obj])
+            _run(["g++", "-O3", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off"] + HOST_FLAGS.get(hs, []) +
+                 ["-I" + INC, "-c", os.path.join(CSRC, hs), "-o", obj])
             objs.append(obj)
         # "-x none": the host objects are linker inputs, not HIP sources
         cmd = ([HIPCC] + HIP_FLAGS + ["-I" + INC] + [os.path.join(CSRC, s) for s in srcs] + ["-x", "none"] + objs +

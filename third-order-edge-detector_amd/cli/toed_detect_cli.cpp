@@ -12,6 +12,14 @@
 //       plan replayed as a hipGraph, each list written to
 //       DIR/<stem>_edges_gpu.txt (the name the script gives it).
 //
+//   toed_detect_cli <input_image> ... --curvelets
+//       also runs the curvelet formation the reference's driver runs on its
+//       FP64 list when built with CurvelFormation (main_gpu_cpu.cpp:211-281):
+//       the list widened to double, nrad 3.5, gap 1.5, dx 0.4, dt 15 degrees,
+//       token_len 1, max_k 0.3, style 2, groups of 4, output type 0, written
+//       to output_dir/chain.txt and info.txt as main_gpu_cpu.cpp:69-97 writes
+//       them (default stream format, a tab after every value).
+//
 // sigma 2, threshold 2, border 10 (main_gpu_cpu.cpp:158-159, cpu_toed.cpp:
 // 400-403).  Images: 8-bit PGM (P5) or PNG via toed_image_load; with
 // --pgm-raw-offset a PGM is read as the reference's no-OpenCV path reads it
@@ -30,8 +38,11 @@
 #include <vector>
 
 #include "toed_capi.h"
+#include "toed_curvelet.hpp"
 
 namespace {
+
+bool g_curvelets = false;
 
 const float kSigma = 2.0f, kThresh = 2.0f;
 const int kBorder = 10;
@@ -78,6 +89,53 @@ bool write_list(const std::string &path, const std::vector<float> &rows, long lo
         out << "\n";
     }
     return (bool)out;
+}
+
+// main_gpu_cpu.cpp:69-97 _write_array_to_file_colmajor: column-major data,
+// one row per line, a tab after every value, default stream format
+template <class T>
+bool write_colmajor(const std::string &path, const T *data, unsigned h, unsigned w) {
+    std::ofstream out(path);
+    if (!out.is_open()) return false;
+    for (unsigned i = 0; i < h; ++i) {
+        for (unsigned j = 0; j < w; ++j) out << data[i + (size_t)h * j] << "\t";
+        out << "\n";
+    }
+    return (bool)out;
+}
+
+// main_gpu_cpu.cpp:211-281: curvelets of the list, chain.txt and info.txt
+bool write_curvelets(const std::string &out_dir, const std::vector<float> &rows, long long n, int H, int W) {
+    std::vector<double> cols((size_t)n * 4 + 1);
+    for (long long i = 0; i < n; ++i)
+        for (int j = 0; j < 4; ++j) cols[(size_t)j * n + i] = (double)rows[(size_t)i * 4 + j];
+    toed_cv::arrayd edgeinfo;
+    edgeinfo._data = cols.data();
+    edgeinfo.set_h((unsigned)n);
+    edgeinfo.set_w(4);
+    const double pi = 3.14159265358979323846;
+    const unsigned style = 2, group = 4, output_type = 0;
+    toed_cv::form_curvelet_process pro(edgeinfo, (unsigned)H, (unsigned)W, 3.5, 1.5, 0.4, (15.0 / 180.0) * pi, 1, 0.3,
+                                       group, style == 0 || style == 1, style == 0 || style == 2);
+    const auto t0 = std::chrono::steady_clock::now();
+    pro.execute();
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    unsigned out_h = 0, out_w = 0;
+    pro.get_output_size(out_h, out_w, output_type);
+    std::vector<int> chain((size_t)out_h * out_w + 1);
+    std::vector<double> info((size_t)out_h * 10 + 1);
+    toed_cv::arrayi ca;
+    ca._data = chain.data();
+    ca.set_h(out_h);
+    ca.set_w(out_w);
+    toed_cv::arrayd ia;
+    ia._data = info.data();
+    ia.set_h(out_h);
+    ia.set_w(10);
+    pro.get_output_arrary(ca, ia, output_type);
+    std::cout << "Curvelet formation: " << out_h << " curvelets in " << ms << " ms" << std::endl;
+    return write_colmajor(join(out_dir, "chain.txt"), chain.data(), out_h, out_w) &&
+           write_colmajor(join(out_dir, "info.txt"), info.data(), out_h, 10u);
 }
 
 struct Image {
@@ -165,9 +223,17 @@ int run_single(int argc, char **argv) {
     std::cout << "Number of GPU edges = " << n << std::endl;
     const std::string path = join(out_dir, "data_final_output_gpu.txt");
     std::cout << "writing data to a file " << path << " ..." << std::endl;
-    const bool ok = write_list(path, rows, n);
+    bool ok = write_list(path, rows, n);
     toed_destroy(ctx);
     release(im);
+    if (ok && g_curvelets) {
+        try {
+            ok = write_curvelets(out_dir, rows, n, im.H, im.W);
+        } catch (const std::exception &e) {
+            std::cerr << "Error: " << e.what() << std::endl;
+            ok = false;
+        }
+    }
     return ok ? 0 : 1;
 }
 
@@ -254,8 +320,17 @@ int run_batch(int argc, char **argv) {
 }  // namespace
 
 int main(int argc, char **argv) {
+    // --curvelets may appear anywhere in the single-image form
+    int m = 1;
+    for (int a = 1; a < argc; ++a) {
+        if (!std::strcmp(argv[a], "--curvelets"))
+            g_curvelets = true;
+        else
+            argv[m++] = argv[a];
+    }
+    argc = m;
     if (argc < 2) {
-        std::cout << "Usage: " << argv[0] << " <input_image> [nthreads] [gpu_id] [output_dir]\n"
+        std::cout << "Usage: " << argv[0] << " <input_image> [nthreads] [gpu_id] [output_dir] [--curvelets]\n"
                   << "       " << argv[0]
                   << " --batch [--gpu N] [--out DIR] [--autotune] [--pgm-raw-offset] image...  "
                      "(writes DIR/<stem>_edges_gpu.txt)" << std::endl;
