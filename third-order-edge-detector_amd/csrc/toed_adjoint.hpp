@@ -15,44 +15,61 @@
 
 namespace toedadj {
 
-constexpr int G = 8;    // lanes per edgel (window rows split over the group)
-
-__device__ __forceinline__ double redG(double v) { return toeddpp::group_sum<G>(v); }
+constexpr int G = 8;    // default lanes per edgel (window rows split over the group)
 
 // response k (fx, fy, fxx, fxy, fyy, fxxy, fxyy, fxxx, fyyy) -> x (column) and
 // y (row) filter family (0 G, 1 G', 2 G'', 3 G''')
 __host__ __device__ constexpr int XF(int k) { return k == 0 || k == 3 || k == 6 ? 1 : (k == 2 || k == 5 ? 2 : (k == 7 ? 3 : 0)); }
 __host__ __device__ constexpr int YF(int k) { return k == 1 || k == 3 || k == 5 ? 1 : (k == 4 || k == 6 ? 2 : (k == 8 ? 3 : 0)); }
 
-// One edgel at interp pixel (R, C) of an H x W image, upstream row `up` =
-// dL/d[x, y, orientation, strength], group lane l.  Tap tables, variants 0
-// unshifted, 1 unshifted with zero end taps (window R0 of phase (0,0)),
-// 2 shifted, each with one zero tap on either side so that index t + R1 + 1
-// is tap t = -R1-1 .. R1+1 (tap_tables_fill builds them): stp / sdp the FP64
-// taps and their sigma-derivatives, the four families (G to G''') of one tap
-// together; stf4 the FP32 copy of stp used by the scatter (its contributions
-// are rounded to FP32 for the atomics anyway).  img(i, j) returns the pixel
-// (0 outside the image); sink(i, j, v) adds v to dL/dimage.  Lane 0 writes
-// the edgel's dL/dsigma to *gs_out.
+// Geometry of an edgel's five points for the scatter: the edgel (m = 0) and
+// its four NMS neighbours (m = 1..4), +-1 interp px along the gradient octant
+// (cpu_toed.cpp:415-462): x-major iff |gx| >= |gy|; one step along +-gradient,
+// axis and diagonal.  oct packs (x-major, gx < 0, gy < 0) in bits 0..2.
+__device__ __forceinline__ void octant_neighbours(int oct, int nbr[4][2]) {
+    const bool xm = oct & 1;
+    const int sx = (oct & 2) ? -1 : 1, sy = (oct & 4) ? -1 : 1;
+    nbr[0][0] = xm ? 0 : sy;   nbr[0][1] = xm ? sx : 0;
+    nbr[1][0] = sy;            nbr[1][1] = sx;
+    nbr[2][0] = xm ? 0 : -sy;  nbr[2][1] = xm ? -sx : 0;
+    nbr[3][0] = -sy;           nbr[3][1] = -sx;
+}
+
+// Number of adjoint coefficients of an edgel: dL/d(nine responses at the
+// edgel) (fx, fy, fxx, fxy, fyy, fxxy, fxyy, fxxx, fyyy), then dL/d(Ix, Iy)
+// at the four neighbours.
+constexpr int NADJ = 17;
+
+// One edgel at interp pixel (R, C), upstream row `up` = dL/d[x, y,
+// orientation, strength], lane l of a GL-lane group: the reverse pass from
+// the edgel's outputs to the adjoints of the responses its outputs depend on.
+// Tap tables, variants 0 unshifted, 1 unshifted with zero end taps (window R0
+// of phase (0,0)), 2 shifted, each with one zero tap on either side so that
+// index t + R1 + 1 is tap t = -R1-1 .. R1+1 (tap_tables_fill builds them):
+// stp / sdp the FP64 taps and their sigma-derivatives, the four families (G
+// to G''') of one tap together.  img(i, j) returns the pixel (0 outside the
+// image).  Writes the NADJ coefficients (FP32, as the scatter uses them) to
+// adjf and the octant to *oct on every lane of the group; lane 0 writes the
+// edgel's dL/dsigma to *gs_out.  Returns false, writing nothing, for an edgel
+// whose parabola is degenerate: it contributes no gradient.
 //
 // The edgel and its four neighbours (+-1 interp px = 1/2 image px) have their
 // windows, and so their transposed filters, inside the P x P patch
 // (P = 2 R1 + 2) whose first row is (R >> 1) - U + (R & 1) and first column
 // (C >> 1) - U + (C & 1): over the patch every tap index is within
-// -R1-1 .. R1+1.  Lane l owns patch rows l + G rr.
-template <int R1, class Img, class Sink>
-__device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, int H, int W,
-                                               const double (*stp)[2 * R1 + 3][4], const double (*sdp)[2 * R1 + 3][4],
-                                               const float4 (*stf4)[2 * R1 + 3], const Img &img, const Sink &sink,
-                                               double *gs_out) {
+// -R1-1 .. R1+1.  Lane l of the group owns patch rows l + GL rr.
+template <int R1, int GL = G, class Img>
+__device__ __forceinline__ bool edgel_adjoint(int R, int C, float4 up, int l,
+                                              const double (*stp)[2 * R1 + 3][4], const double (*sdp)[2 * R1 + 3][4],
+                                              const Img &img, float *adjf, int *oct, double *gs_out) {
     constexpr int U = R1 + 1;
     constexpr int P = 2 * U;
-    constexpr int NSR = (P + G - 1) / G;       // patch rows per lane
+    constexpr int NSR = (P + GL - 1) / GL;       // patch rows per lane
     const int pr0 = (R >> 1) - U + (R & 1), pc0 = (C >> 1) - U + (C & 1);
 
-    // Each lane owns window rows p = -R1 + l + G*rr (NRR of them); the column
+    // Each lane owns window rows p = -R1 + l + GL*rr (NRR of them); the column
     // loop is outermost so every tap read from LDS serves all of the lane's rows.
-    constexpr int NRR = (2 * R1 + G) / G;
+    constexpr int NRR = (2 * R1 + GL) / GL;
 
     // ---- 1a. nine responses at the edgel and their sigma-derivatives
     double v9[9], s9[9];
@@ -71,7 +88,7 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
             for (int f = 0; f < 4; ++f) { xt[f] = stp[vx][q + R1 + 1][f]; xs[f] = sdp[vx][q + R1 + 1][f]; }
 #pragma unroll
             for (int rr = 0; rr < NRR; ++rr) {
-                const int p = -R1 + l + G * rr;
+                const int p = -R1 + l + GL * rr;
                 if (p > R1) continue;
                 const double px = img(i - p, j - q);
 #pragma unroll
@@ -85,7 +102,7 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
         for (int k = 0; k < 9; ++k) { v9[k] = 0.0; s9[k] = 0.0; }
 #pragma unroll
         for (int rr = 0; rr < NRR; ++rr) {
-            const int p = -R1 + l + G * rr;
+            const int p = -R1 + l + GL * rr;
             if (p > R1) continue;
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
@@ -95,7 +112,7 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
             }
         }
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { v9[k] = redG(v9[k]); s9[k] = redG(s9[k]); }
+        for (int k = 0; k < 9; ++k) { v9[k] = toeddpp::group_sum<GL>(v9[k]); s9[k] = toeddpp::group_sum<GL>(s9[k]); }
     }
     // ---- 2a. reverse pass of the third-order orientation theta = atan2(TOx, -TOy)
     // (cpu_toed.cpp:227-232), which depends on the nine responses only:
@@ -145,7 +162,9 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
     const double gx = v9[0], gy = v9[1];
     const int sx = gx >= 0.0 ? 1 : -1, sy = gy >= 0.0 ? 1 : -1;
     const bool xm = fabs(gx) >= fabs(gy);
-    const int nbr[4][2] = {{xm ? 0 : sy, xm ? sx : 0}, {sy, sx}, {xm ? 0 : -sy, xm ? -sx : 0}, {-sy, -sx}};
+    const int oc = (xm ? 1 : 0) | (sx < 0 ? 2 : 0) | (sy < 0 ? 4 : 0);
+    int nbr[4][2];
+    octant_neighbours(oc, nbr);
 
     // ---- 1b. Ix, Iy at the neighbours (+ sigma-derivatives), one pass over
     // the patch columns: a lane row's image value serves all four neighbours
@@ -168,7 +187,7 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
             double px[NSR];
 #pragma unroll
             for (int rr = 0; rr < NSR; ++rr) {
-                const int r = l + G * rr;
+                const int r = l + GL * rr;
                 px[rr] = r < P ? img(pr0 + r, pc0 + c) : 0.0;
             }
 #pragma unroll
@@ -191,7 +210,7 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
             double ix = 0, iy = 0, ixs = 0, iys = 0;
 #pragma unroll
             for (int rr = 0; rr < NSR; ++rr) {
-                const int r = l + G * rr;
+                const int r = l + GL * rr;
                 if (r >= P) continue;
                 const double *yt = stp[vy][in - (pr0 + r) + R1 + 1], *ys = sdp[vy][in - (pr0 + r) + R1 + 1];
                 const double y0 = yt[0], y1 = yt[1], ys0 = ys[0], ys1 = ys[1];
@@ -200,8 +219,8 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
                 ixs = fma(hs1[m][rr], y0, fma(h1[m][rr], ys0, ixs));
                 iys = fma(hs0[m][rr], y1, fma(h0[m][rr], ys1, iys));
             }
-            nv[m][0] = redG(ix); nv[m][1] = redG(iy);
-            ns[m][0] = redG(ixs); ns[m][1] = redG(iys);
+            nv[m][0] = toeddpp::group_sum<GL>(ix); nv[m][1] = toeddpp::group_sum<GL>(iy);
+            ns[m][0] = toeddpp::group_sum<GL>(ixs); ns[m][1] = toeddpp::group_sum<GL>(iys);
         }
     }
 
@@ -217,7 +236,7 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
     const double fp = M[0] + s * (M[1] - M[0]), fm = M[2] + s * (M[3] - M[2]);
     const double q = 1.0 + s * s, sq = sqrt(q), rq = 1.0 / q, rsq = 1.0 / sq;
     const double A = (fm + fp - 2.0 * f) * (0.5 * rq);
-    if (!(isfinite(A) && A != 0.0 && f > 0.0)) return;
+    if (!(isfinite(A) && A != 0.0 && f > 0.0)) return false;
     const double rA = 1.0 / A;
     const double B = (fp - fm) * (0.5 * rsq);
     const double t = -0.5 * B * rA;
@@ -260,47 +279,78 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
         *gs_out = gs;
     }
 
-    // ---- 4. scatter the transposed filters into dL/dimage over the patch:
-    // per point and patch column one float4 of x taps serves all of the
-    // lane's rows.  The edgel uses all four x families, a neighbour two
-    // (Ix: G' in x, G in y; Iy: G in x, G' in y).
-    float adjf[17];
     adjf[0] = (float)gxbar;
     adjf[1] = (float)gybar;
 #pragma unroll
     for (int k = 2; k < 9; ++k) adjf[k] = ao[k];
 #pragma unroll
-    for (int k = 9; k < 17; ++k) adjf[k] = (float)an[k - 9];
-    // per point m (0 the edgel, 1..4 the neighbours) and lane row: y taps
-    // times the adjoints, per x family; the point's x tap base and variant
-    float cf[5][NSR][4];
-    int qb[5], vxm[5];
+    for (int k = 9; k < NADJ; ++k) adjf[k] = (float)an[k - 9];
+    *oct = oc;
+    return true;
+}
+
+// The y-tap coefficients of point m of an edgel at one patch row, per x
+// family: the point's y taps at the row (yv, families G .. G''') times its
+// adjoints.  The edgel uses all four x families, a neighbour two (Ix is G'
+// in x and G in y, Iy G in x and G' in y).
+__device__ __forceinline__ void point_row_coefs(int m, const float *adjf, const float yv[4], float cf[4]) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) cf[f] = 0.f;
+    if (m == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) cf[XF(k)] = fmaf(adjf[k], yv[YF(k)], cf[XF(k)]);
+    } else {
+        cf[1] = adjf[7 + 2 * m] * yv[0];
+        cf[0] = adjf[8 + 2 * m] * yv[1];
+    }
+}
+
+// Tap-table indices of the five points relative to the edgel's P x P patch
+// (first row pr0, first column pc0): variant and base of the x taps (tap
+// column index q = jp - image col is stored at q + R1 + 1, so patch column c
+// reads qb - c) and of the y taps (patch row r reads yb - r).
+template <int R1>
+__device__ __forceinline__ void point_taps(int R, int C, int oct, int vx[5], int qb[5], int vy[5], int yb[5]) {
+    constexpr int U = R1 + 1;
+    const int pr0 = (R >> 1) - U + (R & 1), pc0 = (C >> 1) - U + (C & 1);
+    int nbr[4][2];
+    octant_neighbours(oct, nbr);
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         const int Rp = m == 0 ? R : R + nbr[m - 1][0], Cp = m == 0 ? C : C + nbr[m - 1][1];
         const int ip = Rp >> 1, jp = Cp >> 1, a = Rp & 1, b = Cp & 1, ph = 2 * a + b;
-        vxm[m] = b ? 2 : (ph == 0 ? 1 : 0);
-        const int vy = a ? 2 : (ph == 0 ? 1 : 0);
-        qb[m] = jp - pc0 + R1 + 1;                  // + (-c): tap column index q = jp - image col
+        vx[m] = b ? 2 : (ph == 0 ? 1 : 0);
+        vy[m] = a ? 2 : (ph == 0 ? 1 : 0);
+        qb[m] = jp - pc0 + R1 + 1;
+        yb[m] = ip - pr0 + R1 + 1;
+    }
+}
+
+// Scatter of one edgel's transposed filters into dL/dimage over its patch by
+// its lane group: per point and patch column one float4 of x taps serves all
+// of the lane's rows; each column's sums go to sink(i, j, v) (pixels of the
+// H x W image only).  stf4: the FP32 copy of stp (the contributions are
+// rounded to FP32 anyway).
+template <int R1, int GL = G, class Sink>
+__device__ __forceinline__ void edgel_scatter(int R, int C, int oct, const float *adjf, int l, int H, int W,
+                                              const float4 (*stf4)[2 * R1 + 3], const Sink &sink) {
+    constexpr int U = R1 + 1;
+    constexpr int P = 2 * U;
+    constexpr int NSR = (P + GL - 1) / GL;
+    const int pr0 = (R >> 1) - U + (R & 1), pc0 = (C >> 1) - U + (C & 1);
+    int vxm[5], qb[5], vym[5], yb[5];
+    point_taps<R1>(R, C, oct, vxm, qb, vym, yb);
+    float cf[5][NSR][4];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
 #pragma unroll
         for (int rr = 0; rr < NSR; ++rr) {
-            const int r = l + G * rr;
-            // tap row index p = ip - image row, stored at p + R1 + 1
-            const float4 y4 = r < P ? stf4[vy][ip - (pr0 + r) + R1 + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const int r = l + GL * rr;
+            const float4 y4 = r < P ? stf4[vym[m]][yb[m] - r] : make_float4(0.f, 0.f, 0.f, 0.f);
             const float yv[4] = {y4.x, y4.y, y4.z, y4.w};
-#pragma unroll
-            for (int f = 0; f < 4; ++f) cf[m][rr][f] = 0.f;
-            if (m == 0) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) cf[0][rr][XF(k)] = fmaf(adjf[k], yv[YF(k)], cf[0][rr][XF(k)]);
-            } else {
-                cf[m][rr][1] = adjf[7 + 2 * m] * yv[0];
-                cf[m][rr][0] = adjf[8 + 2 * m] * yv[1];
-            }
+            point_row_coefs(m, adjf, yv, cf[m][rr]);
         }
     }
-    // patch columns: per column five float4 tap reads serve all of the
-    // lane's rows, and the column's sums go straight to the sink
 #pragma unroll 2
     for (int c = 0; c < P; ++c) {
         float acc[NSR];
@@ -321,10 +371,42 @@ __device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, i
         if (jj < 0 || jj >= W) continue;
 #pragma unroll
         for (int rr = 0; rr < NSR; ++rr) {
-            const int r = l + G * rr, ii = pr0 + r;
+            const int r = l + GL * rr, ii = pr0 + r;
             if (r < P && ii >= 0 && ii < H && acc[rr] != 0.f) sink(ii, jj, acc[rr]);
         }
     }
+}
+
+// The contribution to one patch pixel from its row's coefficients cf (from
+// point_row_coefs, per point) and the x taps at its column c: the same FP32
+// operations in the same order as edgel_scatter, for scatters that give a
+// lane pixels of one edgel at a time.  vx, qb from point_taps.
+template <int R1>
+__device__ __forceinline__ float patch_column(const float (*cf)[4], const int vx[5], const int qb[5], int c,
+                                              const float4 (*stf4)[2 * R1 + 3]) {
+    float acc = 0.f;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const float4 x4 = stf4[vx[m]][qb[m] - c];
+        if (m == 0)
+            acc = fmaf(cf[0][0], x4.x, fmaf(cf[0][1], x4.y, fmaf(cf[0][2], x4.z, fmaf(cf[0][3], x4.w, acc))));
+        else
+            acc = fmaf(cf[m][0], x4.x, fmaf(cf[m][1], x4.y, acc));
+    }
+    return acc;
+}
+
+// Adjoint and scatter of one edgel by its lane group; sink(i, j, v) adds v to
+// dL/dimage.
+template <int R1, int GL = G, class Img, class Sink>
+__device__ __forceinline__ void edgel_backward(int R, int C, float4 up, int l, int H, int W,
+                                               const double (*stp)[2 * R1 + 3][4], const double (*sdp)[2 * R1 + 3][4],
+                                               const float4 (*stf4)[2 * R1 + 3], const Img &img, const Sink &sink,
+                                               double *gs_out) {
+    float adjf[NADJ];
+    int oct;
+    if (edgel_adjoint<R1, GL>(R, C, up, l, stp, sdp, img, adjf, &oct, gs_out))
+        edgel_scatter<R1, GL>(R, C, oct, adjf, l, H, W, stf4, sink);
 }
 
 // The tap tables of edgel_backward from tap(v, f, t) and dtap(v, f, t)

@@ -55,7 +55,8 @@ constexpr int MAXR1 = 9;            // largest supported R1 (sigma <= 2)
 constexpr int MAXTAPS = 2 * MAXR1 + 1;
 constexpr int CHUNK = 4096;         // scan chunk (entries)
 constexpr int NGROUPS = 3;          // supported window radii R1 = 5, 7, 9 (sigma 1, 1.5, 2)
-constexpr int BW_CHUNK = 256;       // edgels per pre-pass chunk of the tile backward
+constexpr int BW_CHUNK = 64;        // edgels per pre-pass chunk of the tile backward
+constexpr int BW_G = 8;             // lanes per edgel in the tile backward
 
 // Filter taps for one sigma.  Families: 0 G, 1 G', 2 G'', 3 G''' sampled at
 // x = k - R1 (unshifted) and x = k - R1 + 1/2 (shifted), k = 0..2R1
@@ -1207,14 +1208,18 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
     constexpr int N = 2 * R1 + 1;
     constexpr int U = R1 + 1;
     constexpr int PR = TH + 2 * U, PC = TW + 2 * U, LDG = PC | 1;
+    constexpr int NW = NT / 64;              // wavefronts
     __shared__ float simg[PR * LDG];
-    __shared__ float sgrad[PR * LDG];
+    __shared__ float sgrad[NW][PR * LDG];    // one gradient patch per wavefront
     __shared__ double stp[3][N + 2][4];
     __shared__ double sdp[3][N + 2][4];
     __shared__ float4 stf4[3][N + 2];
-    __shared__ double sgs[NT / toedadj::G];
+    __shared__ double sgs[NT / BW_G];
     __shared__ int s_rc[BW_CHUNK];           // tile-local interp (row | col << 8) of a chunk's edgels, -1: no output row
     __shared__ float4 s_up[BW_CHUNK];        // their upstream gradient rows
+    constexpr int NG = NT / BW_G;            // edgels per round (one per lane group)
+    __shared__ float s_adj[NG][toedadj::NADJ];
+    __shared__ int s_geo[NG];                // tile-local (row | col << 8 | octant << 16) of a round's edgels, -1: none
     const int tile = tile_begin + blockIdx.x;
     const int m = job_index(jobs, njobs, tile);
     const ImageJob J = load_job(jobs, njobs, job0, tile);
@@ -1230,7 +1235,8 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
         float v = 0.f;
         if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
         simg[r * LDG + c] = v;
-        sgrad[r * LDG + c] = 0.f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sgrad[w][r * LDG + c] = 0.f;
     }
     // variant 1 (window R0) is variant 0 with zero end taps
     toedadj::tap_tables_fill<R1>(
@@ -1241,27 +1247,35 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
         [&](int v, int f, int t) {
             return (v == 1 && (t == 0 || t == N - 1)) ? 0.0 : ((v == 2) ? tp->dss[f][t] : tp->dsu[f][t]);
         });
-    const int g = threadIdx.x / toedadj::G, l = threadIdx.x % toedadj::G;
+    const int g = threadIdx.x / BW_G, l = threadIdx.x % BW_G;
     if (l == 0) sgs[g] = 0.0;
     __syncthreads();
     struct LdsImg {
         const float *p; int gi0, gj0;
         __device__ double operator()(int i, int j) const { return (double)p[(i - gi0) * LDG + (j - gj0)]; }
     } img{simg, gi0, gj0};
-    struct LdsSink {
-        float *p; int gi0, gj0;
-        __device__ void operator()(int i, int j, float v) const { atomicAdd(&p[(i - gi0) * LDG + (j - gj0)], v); }
-    } sink{sgrad, gi0, gj0};
     // All four phases form one work list, processed in chunks: a cooperative
     // pre-pass resolves each edgel's interp pixel, output row and upstream
-    // gradient into LDS (no global-load chains inside the per-edgel work);
-    // the groups of one wavefront then take edgels spread across the chunk
-    // (their patches overlap less, so their LDS atomics collide less).
+    // gradient into LDS (no global-load chains inside the per-edgel work).
+    // Each chunk goes in rounds of NG edgels, one per lane group, spread over
+    // the wavefronts.  A group runs the FP64 reverse pass of its edgel and
+    // leaves the adjoint coefficients in LDS; then its wavefront scatters its
+    // groups' edgels one after the other into its own gradient patch, each
+    // edgel by all 64 lanes (a lane takes a row and four columns of the
+    // edgel's patch), with plain read-add-writes: the lanes of one edgel
+    // touch distinct pixels, a wavefront's LDS operations complete in order,
+    // and no other wavefront writes that patch (LDS float atomics cost
+    // several cycles per lane).  The patches are summed at the end.
     int nph[4];
 #pragma unroll
     for (int ph = 0; ph < 4; ++ph) nph[ph] = min(tile_cnt[(long long)tile * 4 + ph], J.stage_cap);
     const int total = nph[0] + nph[1] + nph[2] + nph[3];
-    constexpr int GPW = 64 / toedadj::G;                 // groups per wavefront
+    constexpr int GPW = 64 / BW_G;                 // groups per wavefront
+    constexpr int P = 2 * U;                       // patch side of one edgel
+    constexpr int PQ = (P + 3) / 4;                // four-column blocks per patch row
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = (g % GPW) * NW + wave;        // this group's edgel within a round
+    float *wgrad = sgrad[wave];
     for (int c0 = 0; c0 < total; c0 += BW_CHUNK) {
         const int cn = min(BW_CHUNK, total - c0);
         for (int k = threadIdx.x; k < cn; k += NT) {
@@ -1277,28 +1291,76 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
             s_up[k] = idx < J.capacity ? reinterpret_cast<const float4 *>(O.gout)[idx] : make_float4(0, 0, 0, 0);
         }
         __syncthreads();
-        const int S = (cn + GPW - 1) / GPW;
-        for (int t = g; t < S * GPW; t += NT / toedadj::G) {
-            const int k = (t % GPW) * S + t / GPW;
-            if (k >= cn || s_rc[k] < 0) continue;
-            const int R = 2 * i0 + (s_rc[k] & 0xff), C = 2 * j0 + (s_rc[k] >> 8);
-            double gs = 0.0;
-            toedadj::edgel_backward<R1>(R, C, s_up[k], l, J.H, J.W, stp, sdp, stf4, img, sink, &gs);
-            if (l == 0) sgs[g] += gs;
+        for (int r0 = 0; r0 < cn; r0 += NG) {
+            const int nr = min(NG, cn - r0);
+            if (slot < nr) {
+                const int k = r0 + slot;
+                int geo = -1;
+                if (s_rc[k] >= 0) {
+                    const int R = 2 * i0 + (s_rc[k] & 0xff), C = 2 * j0 + (s_rc[k] >> 8);
+                    double gs = 0.0;
+                    float adjf[toedadj::NADJ];
+                    int oct;
+                    if (toedadj::edgel_adjoint<R1, BW_G>(R, C, s_up[k], l, stp, sdp, img, adjf, &oct, &gs)) {
+                        if (l == 0)
+#pragma unroll
+                            for (int q = 0; q < toedadj::NADJ; ++q) s_adj[slot][q] = adjf[q];
+                        geo = s_rc[k] | (oct << 16);
+                    }
+                    if (l == 0) sgs[g] += gs;
+                }
+                if (l == 0) s_geo[slot] = geo;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // this wavefront's edgels of the round: slots j * NW + wave
+            for (int j = 0; j < GPW; ++j) {
+                const int q = j * NW + wave;
+                if (q >= nr) break;
+                const int geo = s_geo[q];
+                if (geo < 0) continue;
+                const int R = 2 * i0 + (geo & 0xff), C = 2 * j0 + ((geo >> 8) & 0xff), oct = geo >> 16;
+                const int pr0 = (R >> 1) - U + (R & 1), pc0 = (C >> 1) - U + (C & 1);
+                int vx[5], qb[5], vy[5], yb[5];
+                toedadj::point_taps<R1>(R, C, oct, vx, qb, vy, yb);
+                for (int t = lane; t < P * PQ; t += 64) {
+                    const int r = t / PQ, cb = (t - r * PQ) * 4;
+                    float cf[5][4];
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) {
+                        const float4 y4 = stf4[vy[m]][yb[m] - r];
+                        const float yv[4] = {y4.x, y4.y, y4.z, y4.w};
+                        toedadj::point_row_coefs(m, s_adj[q], yv, cf[m]);
+                    }
+                    float *row = wgrad + (pr0 + r - gi0) * LDG + (pc0 - gj0);
+#pragma unroll
+                    for (int cc = 0; cc < 4; ++cc) {
+                        const int c = cb + cc;
+                        if (c >= P) break;
+                        row[c] += toedadj::patch_column<R1>(cf, vx, qb, c, stf4);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
         }
-        __syncthreads();
+        __syncthreads();     // the chunk's s_rc / s_up are rewritten next
     }
     __syncthreads();
     for (int e = threadIdx.x; e < PR * PC; e += NT) {
         const int r = e / PC, c = e % PC;
         const int gi = gi0 + r, gj = gj0 + c;
-        const float v = sgrad[r * LDG + c];
+        float v = sgrad[0][r * LDG + c];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) v += sgrad[w][r * LDG + c];
         if (v != 0.f && gi >= 0 && gi < J.H && gj >= 0 && gj < J.W)
             unsafeAtomicAdd(O.gimg + (long long)gi * O.gpitch + gj, v);
     }
     if (threadIdx.x == 0 && O.gsigma) {
         double t = 0.0;
-        for (int k = 0; k < NT / toedadj::G; ++k) t += sgs[k];
+        for (int k = 0; k < NT / BW_G; ++k) t += sgs[k];
         if (t != 0.0) unsafeAtomicAdd(O.gsigma, t);
     }
 }
