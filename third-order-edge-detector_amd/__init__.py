@@ -335,15 +335,21 @@ class BatchDetector:
         n = self.n
         arr = getattr(self, "_grad_arr", None)
         if arr is None:
-            arr = self._grad_arr = (_lib.toed_grad_out_t * n)()
-        for m in range(n):
-            it = arr[m]
-            it.d_gout = gout_ptrs[m]
-            it.d_gimg = gimg_ptrs[m]
-            it.gpitch = gpitches[m]
-            it.d_gsigma = gsigma_ptr
+            # toed_grad_out_t as a numpy record array: filled with vector
+            # assignments instead of one ctypes field write per image
+            dt = np.dtype({"names": ["d_gout", "d_gimg", "gpitch", "d_gsigma"],
+                           "formats": [np.uint64, np.uint64, np.int32, np.uint64],
+                           "offsets": [_lib.toed_grad_out_t.d_gout.offset, _lib.toed_grad_out_t.d_gimg.offset,
+                                       _lib.toed_grad_out_t.gpitch.offset, _lib.toed_grad_out_t.d_gsigma.offset],
+                           "itemsize": ctypes.sizeof(_lib.toed_grad_out_t)})
+            arr = self._grad_arr = np.zeros(n, dt)
+        arr["d_gout"] = gout_ptrs
+        arr["d_gimg"] = gimg_ptrs
+        arr["gpitch"] = gpitches
+        arr["d_gsigma"] = gsigma_ptr or 0
         self.det._bind_stream()
-        check(lib().toed_batch_backward(self._b, arr), "toed_batch_backward")
+        check(lib().toed_batch_backward(self._b, arr.ctypes.data_as(ctypes.POINTER(_lib.toed_grad_out_t))),
+              "toed_batch_backward")
 
     def index_results(self):
         counts = self.counts.cpu().numpy()

@@ -112,9 +112,9 @@ def _make_function():
                 if g_out.numel() == 0:
                     g_out = torch.zeros((1, 4), dtype=torch.float32, device=images.device)
                 g0, i0 = g_out.data_ptr(), gimg.data_ptr()
-                gout_ptrs = [g0 + 16 * int(offsets[b]) if counts[b] else g0 for b in range(B)]
-                gimg_ptrs = [i0 + 4 * H * W * b for b in range(B)]
-                plan.bd.backward_ptrs(gout_ptrs, gimg_ptrs, [W] * B, gsig.data_ptr())
+                gout_ptrs = np.where(counts > 0, g0 + 16 * offsets[:-1], g0).astype(np.uint64)
+                gimg_ptrs = i0 + 4 * H * W * np.arange(B, dtype=np.uint64)
+                plan.bd.backward_ptrs(gout_ptrs, gimg_ptrs, W, gsig.data_ptr())
                 gs = None
                 if ctx.sigma_shape is not None:
                     gs = gsig.reshape(()).to(ctx.sigma_dtype).reshape(ctx.sigma_shape)
