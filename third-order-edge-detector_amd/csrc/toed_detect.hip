@@ -1203,8 +1203,9 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
 template <typename IN_T, int TH, int R1>
 __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
                                                                 const GradOut *gouts, StageBuf stg,
-                                                                const int *tile_cnt, const int *offsets,
-                                                                const int *chunk_pref, int tile_begin) {
+                                                                const int *tile_cnt, const int *counts,
+                                                                const int *offsets, const int *chunk_pref,
+                                                                int tile_begin) {
     constexpr int N = 2 * R1 + 1;
     constexpr int U = R1 + 1;
     constexpr int PR = TH + 2 * U, PC = TW + 2 * U, LDG = PC | 1;
@@ -1220,6 +1221,8 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
     constexpr int NG = NT / BW_G;            // edgels per round (one per lane group)
     __shared__ float s_adj[NG][toedadj::NADJ];
     __shared__ int s_geo[NG];                // tile-local (row | col << 8 | octant << 16) of a round's edgels, -1: none
+    __shared__ int s_rowpre[2 * TH + 1];     // edgels of the tile's interp rows before each row
+    __shared__ long long s_rowstart[2 * TH]; // output row (image-relative) of each interp row's first edgel
     const int tile = tile_begin + blockIdx.x;
     const int m = job_index(jobs, njobs, tile);
     const ImageJob J = load_job(jobs, njobs, job0, tile);
@@ -1229,6 +1232,29 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
     const int i0 = tr * TH, j0 = tc * TW;
     const int gi0 = i0 - U, gj0 = j0 - U;        // patch origin (image rows/cols)
     const TOED_CONST TapSet *tp = as_const(J.taps);
+    // Prologue: the tile's independent global loads in one pass so that their
+    // latencies overlap: the tap tables (threads < 3 (2 R1 + 3); variant 1,
+    // window R0, is variant 0 with zero end taps), the image region, and --
+    // when the plan keeps each edgel's interp index -- the count and output
+    // start of each of the tile's interp rows.
+    constexpr int NP = N + 2;
+    const bool by_rows = J.index_out != nullptr;
+    int rcnt = 0;
+    if (by_rows && threadIdx.x < 2 * TH && 2 * i0 + (int)threadIdx.x < 2 * J.H) {   // rows of the image only
+        const long long ce = J.counts_base + (long long)(2 * i0 + threadIdx.x) * J.ntc + tc;
+        rcnt = counts[ce];
+        s_rowstart[threadIdx.x] = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK];
+    }
+    double ta[4], td[4];
+    const int tv = threadIdx.x / NP, ti = threadIdx.x - tv * NP, tt = ti - 1;
+    if (threadIdx.x < 3 * NP) {
+        const bool in = tt >= 0 && tt <= 2 * R1 && !(tv == 1 && (tt == 0 || tt == N - 1));
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            ta[f] = in ? (tv == 2 ? tp->ds[f][tt] : tp->du[f][tt]) : 0.0;
+            td[f] = in ? (tv == 2 ? tp->dss[f][tt] : tp->dsu[f][tt]) : 0.0;
+        }
+    }
     for (int e = threadIdx.x; e < PR * PC; e += NT) {
         const int r = e / PC, c = e % PC;
         const int gi = gi0 + r, gj = gj0 + c;
@@ -1238,15 +1264,16 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
 #pragma unroll
         for (int w = 0; w < NW; ++w) sgrad[w][r * LDG + c] = 0.f;
     }
-    // variant 1 (window R0) is variant 0 with zero end taps
-    toedadj::tap_tables_fill<R1>(
-        stp, sdp, stf4, threadIdx.x, NT,
-        [&](int v, int f, int t) {
-            return (v == 1 && (t == 0 || t == N - 1)) ? 0.0 : ((v == 2) ? tp->ds[f][t] : tp->du[f][t]);
-        },
-        [&](int v, int f, int t) {
-            return (v == 1 && (t == 0 || t == N - 1)) ? 0.0 : ((v == 2) ? tp->dss[f][t] : tp->dsu[f][t]);
-        });
+    if (threadIdx.x < 3 * NP) {
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            stp[tv][ti][f] = ta[f];
+            sdp[tv][ti][f] = td[f];
+        }
+        stf4[tv][ti] = make_float4((float)ta[0], (float)ta[1], (float)ta[2], (float)ta[3]);
+    }
+    if (by_rows && threadIdx.x < 2 * TH) s_rowpre[threadIdx.x + 1] = rcnt;
+    if (threadIdx.x == 0) s_rowpre[0] = 0;
     const int g = threadIdx.x / BW_G, l = threadIdx.x % BW_G;
     if (l == 0) sgs[g] = 0.0;
     __syncthreads();
@@ -1269,7 +1296,17 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
     int nph[4];
 #pragma unroll
     for (int ph = 0; ph < 4; ++ph) nph[ph] = min(tile_cnt[(long long)tile * 4 + ph], J.stage_cap);
-    const int total = nph[0] + nph[1] + nph[2] + nph[3];
+    int total = nph[0] + nph[1] + nph[2] + nph[3];
+    if (by_rows) {
+        // inclusive prefix of the row counts: each row's thread sums the rows before it
+        int pre = 0;
+        if (threadIdx.x <= 2 * TH)
+            for (int u = 1; u <= threadIdx.x; ++u) pre += s_rowpre[u];
+        __syncthreads();
+        if (threadIdx.x <= 2 * TH) s_rowpre[threadIdx.x] = pre;
+        __syncthreads();
+        total = s_rowpre[2 * TH];
+    }
     constexpr int GPW = 64 / BW_G;                 // groups per wavefront
     constexpr int P = 2 * U;                       // patch side of one edgel
     constexpr int PQ = (P + 3) / 4;                // four-column blocks per patch row
@@ -1279,14 +1316,27 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
     for (int c0 = 0; c0 < total; c0 += BW_CHUNK) {
         const int cn = min(BW_CHUNK, total - c0);
         for (int k = threadIdx.x; k < cn; k += NT) {
-            int e = c0 + k, phase = 0;
-            while (e >= nph[phase]) { e -= nph[phase]; ++phase; }
-            const uint32_t key = stg.key[J.stage_base + (tile_local * 4 + phase) * J.stage_cap + e];
-            const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
-            const int R = 2 * i0 + r, C = 2 * j0 + c;
-            const long long ce = J.counts_base + (long long)R * J.ntc + tc;
-            const long long idx =
-                (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK] + rank;
+            const int e = c0 + k;
+            long long idx;
+            int r = 0, c = 0;
+            if (by_rows) {
+                // the interp row holding edgel e of the tile, then its output row
+                int lo = 0, hi = 2 * TH - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_rowpre[mid] <= e) lo = mid; else hi = mid - 1;
+                }
+                r = lo;
+                idx = s_rowstart[lo] + (e - s_rowpre[lo]);
+                if (idx < J.capacity) c = J.index_out[out_row(J, idx)] % (2 * J.W) - 2 * j0;
+            } else {
+                int ee = e, phase = 0;
+                while (ee >= nph[phase]) { ee -= nph[phase]; ++phase; }
+                const uint32_t key = stg.key[J.stage_base + (tile_local * 4 + phase) * J.stage_cap + ee];
+                r = key & 0xff;
+                c = (key >> 8) & 0xff;
+                idx = out_index(J, offsets, chunk_pref, 2 * i0 + r, tc, key >> 16);
+            }
             s_rc[k] = idx < J.capacity ? (r | (c << 8)) : -1;
             s_up[k] = idx < J.capacity ? reinterpret_cast<const float4 *>(O.gout)[idx] : make_float4(0, 0, 0, 0);
         }
@@ -2296,11 +2346,11 @@ int toed_batch_backward(toed_batch *b, const toed_grad_out_t *outs) {
                 if (b->in_type == TOED_U8)
                     hipLaunchKernelGGL((toed_tile_backward_kernel<uint8_t, T, R>), grid, dim3(NT), 0, s, b->d_jobs,
                                        b->n, dummy, b->d_gouts, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
-                                       b->d_offsets, b->d_chunks, g.tile_begin);
+                                       b->d_counts, b->d_offsets, b->d_chunks, g.tile_begin);
                 else
                     hipLaunchKernelGGL((toed_tile_backward_kernel<float, T, R>), grid, dim3(NT), 0, s, b->d_jobs,
                                        b->n, dummy, b->d_gouts, StageBuf{b->d_srec, b->d_skey}, b->d_tilecnt,
-                                       b->d_offsets, b->d_chunks, g.tile_begin);
+                                       b->d_counts, b->d_offsets, b->d_chunks, g.tile_begin);
             });
         });
     }
