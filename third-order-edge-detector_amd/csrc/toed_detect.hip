@@ -207,6 +207,9 @@ template <int R1, int TH> struct Geo {
     static constexpr int NBV = (NVR + KV - 1) / KV;
     static constexpr int KH = 3;
     static constexpr int NBH = (NCP + KH - 1) / KH;
+    // the passes' over-reads of a partial last block stay inside the LDS
+    static_assert(OFF_IMG + (NBV * KV - 1 + 2 * R1) * LDI + NCI <= TOTAL, "vertical pass over-read");
+    static_assert(OFF_P1 + (NVR - 1) * LDI + NBH * KH + 2 * R1 <= TOTAL, "horizontal pass over-read");
 };
 
 
@@ -239,9 +242,12 @@ __device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, co
                 P1[(v0 + d) * Gm::LDI + u] += hi * t1[0] + lo * t1[2 * R1];
             }
         } else {
+            // the last block may read up to KV - 1 rows past the staged image
+            // (into the Ix plane that follows it); those rows feed only the
+            // outputs past NVR, which are not stored
             float w[KV + 2 * R1];
 #pragma unroll
-            for (int t = 0; t < KV + 2 * R1; ++t) w[t] = col[min(t, Gm::NRI - 1 - v0) * Gm::LDI];
+            for (int t = 0; t < KV + 2 * R1; ++t) w[t] = col[t * Gm::LDI];
 #pragma unroll
             for (int d = 0; d < KV; ++d) {
                 // the first term as a product (an fma onto +0 would cost one more op)
@@ -279,11 +285,13 @@ __device__ __forceinline__ void hpass(float *lds, int A,
         const int v = it % Gm::NVR;
         const int u0 = (it / Gm::NVR) * KH;
         float w0[NW], w1[NW];
+        // the last block may read past the plane row (into the next row, or
+        // past P1 into the queue area); those values feed only outputs past
+        // NCP, which are not stored
 #pragma unroll
         for (int t = 0; t < NW; ++t) {
-            const int u = min(u0 + t, Gm::NCI - 1);     // the last block may be partial
-            w0[t] = P0[v * Gm::LDI + u];
-            w1[t] = P1[v * Gm::LDI + u];
+            w0[t] = P0[v * Gm::LDI + u0 + t];
+            w1[t] = P1[v * Gm::LDI + u0 + t];
         }
 #pragma unroll
         for (int ph = 0; ph < NPH; ++ph) {
