@@ -14,8 +14,9 @@
 // MI355X design:
 //   K1 toed_tiles_kernel   one workgroup (4 wavefronts) per TH x 32 tile of
 //        original pixels = 2TH x 64 interp pixels.  The image tile plus halo is
-//        staged once in LDS; the 2-D filters are applied separably (row pass,
-//        then column pass) in FP32 in three rounds that reuse two LDS planes:
+//        staged once in LDS; the 2-D filters are applied separably (vertical
+//        pass, then horizontal pass) in FP32 in three rounds that reuse two
+//        LDS planes:
 //        (R0-window G, G') -> phase (0,0); + end taps -> phase (1,0);
 //        (shifted G, G') -> phases (0,1), (1,1).  Ix, Iy of the tile plus a
 //        one-pixel interp halo stay in LDS; each wavefront runs NMS on one
@@ -664,7 +665,7 @@ __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const Imag
     __syncthreads();
     hpass<R1, TH, R1, 2>(lds, 1, tp->fu[0], tp->fu[1], 0, tp->fs[0], tp->fs[1], 1);
     __syncthreads();
-    // the row-pass planes are dead now: the survivor queue lives in their place
+    // the vertical-pass planes are dead now: the survivor queue lives in their place
     int *queue = (int *)(lds + Gm::OFF_P0);
     nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
     // each survivor's raster key (local row, local column, rank in its row):
@@ -1212,11 +1213,12 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
 // Backward of a batch plan's last forward, tile by tile (differentiable form)
 // ---------------------------------------------------------------------------
 // One workgroup per forward tile: the tile's image plus a (R1+1)-pixel halo is
-// staged in LDS once, every survivor of the tile (from the plan's staging, so
-// no index list is needed) runs the shared FP64 reverse pass on a lane group,
-// its transposed filters are accumulated with LDS atomics into a
-// (TH + 2R1 + 2) x (32 + 2R1 + 2) gradient patch, and the patch is added to
-// dL/dimage once per tile.  dL/dsigma is summed per workgroup.
+// staged in LDS once, every survivor of the tile (from the plan's staging, or
+// from its interp rows when the plan keeps the interp index) runs the shared
+// FP64 reverse pass on a lane group, its transposed filters are added by one
+// wavefront into that wavefront's own (TH + 2R1 + 2) x (32 + 2R1 + 2)
+// gradient patch, and the summed patches are added to dL/dimage once per
+// tile.  dL/dsigma is summed per workgroup.
 // Occupancy 4 waves per SIMD (128 VGPRs, a few spills to scratch) measures
 // faster than the 2 that the unconstrained allocation (about 230 VGPRs) gives.
 template <typename IN_T, int TH, int R1>
