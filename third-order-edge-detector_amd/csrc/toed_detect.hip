@@ -175,11 +175,22 @@ template <typename T> __device__ inline float load_px(const void *p, long long i
 // ---------------------------------------------------------------------------
 // LDS geometry of one tile
 // ---------------------------------------------------------------------------
-template <int R1, int TH> struct Geo {
+// EB: bytes per staged image pixel, 1 for u8 images (the pixels are exact in
+// a byte and are widened as the vertical pass reads them: the image region
+// shrinks by three quarters, which lets a fifth workgroup onto a CU) and 4
+// for f32 images.
+template <int R1, int TH, int EB = 4> struct Geo {
+    static_assert(EB == 1 || EB == 4, "staged pixel size");
     static constexpr int R0 = R1 - 1;
     static constexpr int NRI = TH + 2 + 2 * R1;            // staged image rows
     static constexpr int NCI = TW + 2 + 2 * R1;            // staged image cols
-    static constexpr int LDI = NCI | 1;                    // odd pitch: conflict-free columns
+    static constexpr int LDI = NCI | 1;                    // odd pitch (floats): conflict-free columns
+    // u8 rows: the dword-aligned global columns from gj0 & ~3 land on LDS
+    // dwords, so image column c sits at byte SHB + c of its row; gj0 =
+    // 32 tc - 1 - R1, so SHB = gj0 & 3 is the same for every tile
+    static constexpr int SHB = (-1 - R1) & 3;
+    static constexpr int LDB = 4 * (((SHB + NCI + 3) / 4) | 1);    // row pitch in bytes (odd in dwords)
+    static constexpr int IMG_FLOATS = EB == 4 ? NRI * LDI : NRI * LDB / 4;
     static constexpr int NCP = TW + 2;                     // horizontal-pass output cols
     static constexpr int NVR = TH + 2;                     // vertical-pass output rows
     static constexpr int RH = 2 * TH + 2, RW = 2 * TW + 2; // interp results incl. 1-px halo
@@ -187,10 +198,9 @@ template <int R1, int TH> struct Geo {
     // LDS layout (floats): staged image | Ix, Iy with 1-px halo | two
     // vertical-pass planes (NVR rows x NCI cols), later reused for the
     // near-peak queue (which becomes the survivor queue) | row masks.
-    // |grad I| is recomputed from Ix, Iy where needed: without its plane the
-    // u8 16-row tile fits in four workgroups per CU
+    // |grad I| is recomputed from Ix, Iy where needed (no third plane)
     static constexpr int OFF_IMG = 0;
-    static constexpr int OFF_IX = (NRI * LDI + 1) & ~1;
+    static constexpr int OFF_IX = (IMG_FLOATS + 1) & ~1;
     static constexpr int OFF_IY = OFF_IX + RH * LDR;
     static constexpr int OFF_P0 = (OFF_IY + RH * LDR + 1) & ~1;      // 8-byte aligned
     static constexpr int OFF_P1 = OFF_P0 + NVR * LDI;
@@ -209,9 +219,13 @@ template <int R1, int TH> struct Geo {
     static constexpr int KH = 3;
     static constexpr int NBH = (NCP + KH - 1) / KH;
     // the passes' over-reads of a partial last block stay inside the LDS
-    static_assert(OFF_IMG + (NBV * KV - 1 + 2 * R1) * LDI + NCI <= TOTAL, "vertical pass over-read");
+    static_assert(EB == 1 || OFF_IMG + (NBV * KV - 1 + 2 * R1) * LDI + NCI <= TOTAL, "vertical pass over-read");
+    static_assert(EB == 4 || 4 * OFF_IMG + (NBV * KV - 1 + 2 * R1) * LDB + SHB + NCI <= 4 * TOTAL,
+                  "vertical pass over-read");
     static_assert(OFF_P1 + (NVR - 1) * LDI + NBH * KH + 2 * R1 <= TOTAL, "horizontal pass over-read");
 };
+
+template <typename IN_T> constexpr int stage_bytes() { return std::is_same<IN_T, uint8_t>::value ? 1 : 4; }
 
 
 // The 2-D filters run vertical pass first: a 16 x 32 tile is wider than it
@@ -222,10 +236,9 @@ template <int R1, int TH> struct Geo {
 // Vertical pass over the staged image columns into the two planes (P0: G in
 // y, P1: G' in y).  mode 0: R0-window taps (fresh); mode 1: add the two end
 // taps (R0 -> R1 window); mode 2: shifted taps (fresh).
-template <int R1, int TH, int MODE>
+template <int R1, int TH, int MODE, int EB>
 __device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, const TOED_CONST float *t1) {
-    using Gm = Geo<R1, TH>;
-    const float *img = lds + Gm::OFF_IMG;
+    using Gm = Geo<R1, TH, EB>;
     float *P0 = lds + Gm::OFF_P0;
     float *P1 = lds + Gm::OFF_P1;
     constexpr int R = (MODE == 0) ? Gm::R0 : R1;
@@ -233,12 +246,18 @@ __device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, co
     for (int it = threadIdx.x; it < Gm::NCI * Gm::NBV; it += NT) {
         const int u = it % Gm::NCI;
         const int v0 = (it / Gm::NCI) * KV;
-        const float *col = img + v0 * Gm::LDI + u;
+        // image row v0 + t, column u
+        const float *colf = lds + Gm::OFF_IMG + v0 * Gm::LDI + u;
+        const uint8_t *colb = (const uint8_t *)(lds + Gm::OFF_IMG) + v0 * Gm::LDB + Gm::SHB + u;
+        auto px = [&](int t) -> float {
+            if constexpr (EB == 4) return colf[t * Gm::LDI];
+            else return (float)colb[t * Gm::LDB];
+        };
         if (MODE == 1) {
 #pragma unroll
             for (int d = 0; d < KV; ++d) {
                 if (v0 + d >= Gm::NVR) break;
-                const float lo = col[d * Gm::LDI], hi = col[(d + 2 * R1) * Gm::LDI];
+                const float lo = px(d), hi = px(d + 2 * R1);
                 P0[(v0 + d) * Gm::LDI + u] += hi * t0[0] + lo * t0[2 * R1];
                 P1[(v0 + d) * Gm::LDI + u] += hi * t1[0] + lo * t1[2 * R1];
             }
@@ -248,7 +267,7 @@ __device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, co
             // outputs past NVR, which are not stored
             float w[KV + 2 * R1];
 #pragma unroll
-            for (int t = 0; t < KV + 2 * R1; ++t) w[t] = col[t * Gm::LDI];
+            for (int t = 0; t < KV + 2 * R1; ++t) w[t] = px(t);
 #pragma unroll
             for (int d = 0; d < KV; ++d) {
                 // the first term as a product (an fma onto +0 would cost one more op)
@@ -271,11 +290,11 @@ __device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, co
 // Horizontal pass: for phases with y shift A (0/1) and x shift B (0/1), from
 // the current planes: Ix = sum_x P0 * X1, Iy = sum_x P1 * X0, into the interp
 // result planes.  NPH = 1 or 2 phases (x taps xa, xb) from the same planes.
-template <int R1, int TH, int R, int NPH>
+template <int R1, int TH, int R, int NPH, int EB>
 __device__ __forceinline__ void hpass(float *lds, int A,
                                       const TOED_CONST float *xa0, const TOED_CONST float *xa1, int B0,
                                       const TOED_CONST float *xb0, const TOED_CONST float *xb1, int B1) {
-    using Gm = Geo<R1, TH>;
+    using Gm = Geo<R1, TH, EB>;
     const float *P0 = lds + Gm::OFF_P0;
     const float *P1 = lds + Gm::OFF_P1;
     float *RX = lds + Gm::OFF_IX;
@@ -353,11 +372,11 @@ __device__ __forceinline__ int octant(T gx, T gy, T nx, T ny, MF M, T &slope, T 
 //     queue and set their bit in the row's 64-bit mask;
 //  3. per-(row, tile) counts from the masks (the raster ranks are taken from
 //     the same masks by the orientation pass).
-template <int R1, int TH>
+template <int R1, int TH, int EB>
 __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, int tc, long long tile_local,
                                 int *__restrict__ counts, StageBuf stg, int *sh_cnt, int *cand, int *queue,
                                 unsigned long long *rowmask) {
-    using Gm = Geo<R1, TH>;
+    using Gm = Geo<R1, TH, EB>;
     const float *RX = lds + Gm::OFF_IX;
     const float *RY = lds + Gm::OFF_IY;
     // |grad I| from Ix, Iy (hardware square root, 1 ulp) wherever it is needed
@@ -580,8 +599,9 @@ __device__ __forceinline__ void stage_fetch_u8(const ImageJob &J, int tile, Stag
 
 template <int R1, int TH>
 __device__ __forceinline__ void stage_store_u8(float *lds, const ImageJob &J, int tile, const StageRegs<R1, TH> &sr) {
-    using Gm = Geo<R1, TH>;
-    float *img = lds + Gm::OFF_IMG;
+    // the fetched dwords are the LDS dwords of the byte rows (column c at byte SHB + c)
+    using Gm = Geo<R1, TH, 1>;
+    uint32_t *img = (uint32_t *)(lds + Gm::OFF_IMG);
     const long long tile_local = tile - J.tile_begin;
     const int tc = (int)(tile_local % J.ntc);
     const int gj0 = tc * TW - 1 - R1;
@@ -591,12 +611,8 @@ __device__ __forceinline__ void stage_store_u8(float *lds, const ImageJob &J, in
     for (int u = 0; u < StageRegs<R1, TH>::NU; ++u) {
         const int e = threadIdx.x + u * NT;
         if (e >= Gm::NRI * nd) continue;
-        const int rr = e / nd, k = e - rr * nd, col0 = gc0 + 4 * k;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int lc = col0 + b - gj0;
-            if (lc >= 0 && lc < Gm::NCI) img[rr * Gm::LDI + lc] = (float)((sr.v[u] >> (8 * b)) & 255u);
-        }
+        const int rr = e / nd, k = e - rr * nd;
+        img[rr * (Gm::LDB / 4) + k] = sr.v[u];
     }
 }
 
@@ -604,17 +620,22 @@ __device__ __forceinline__ void stage_store_u8(float *lds, const ImageJob &J, in
 // straight to LDS
 template <typename IN_T, int R1, int TH>
 __device__ __forceinline__ void stage_direct(float *lds, const ImageJob &J, int tile) {
-    using Gm = Geo<R1, TH>;
+    constexpr int EB = stage_bytes<IN_T>();
+    using Gm = Geo<R1, TH, EB>;
     float *img = lds + Gm::OFF_IMG;
+    uint8_t *imgb = (uint8_t *)(lds + Gm::OFF_IMG) + Gm::SHB;
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
     const int gi0 = tr * TH - 1 - R1, gj0 = tc * TW - 1 - R1;
     for (int e = threadIdx.x; e < Gm::NRI * Gm::NCI; e += NT) {
         const int rr = e / Gm::NCI, cc = e % Gm::NCI;
         const int gi = gi0 + rr, gj = gj0 + cc;
-        float v = 0.f;
-        if (gi >= 0 && gi < J.H && gj >= 0 && gj < J.W) v = load_px<IN_T>(J.img, (long long)gi * J.pitch + gj);
-        img[rr * Gm::LDI + cc] = v;
+        const bool in = gi >= 0 && gi < J.H && gj >= 0 && gj < J.W;
+        if constexpr (EB == 1) {
+            imgb[rr * Gm::LDB + cc] = in ? ((const uint8_t *)J.img)[(long long)gi * J.pitch + gj] : (uint8_t)0;
+        } else {
+            img[rr * Gm::LDI + cc] = in ? load_px<IN_T>(J.img, (long long)gi * J.pitch + gj) : 0.f;
+        }
     }
 }
 
@@ -633,8 +654,8 @@ __device__ __forceinline__ void stage_tile(float *lds, const ImageJob &J, int ti
 
 // per-tile LDS counters: 4 phase counters, orientation-queue length,
 // near-peak count; the survivor row masks
-template <int R1, int TH> __device__ __forceinline__ void tile_setup(float *lds, int *sh_cnt) {
-    using Gm = Geo<R1, TH>;
+template <int R1, int TH, int EB> __device__ __forceinline__ void tile_setup(float *lds, int *sh_cnt) {
+    using Gm = Geo<R1, TH, EB>;
     if (threadIdx.x < 6) sh_cnt[threadIdx.x] = 0;
     unsigned long long *rowmask = (unsigned long long *)(lds + Gm::OFF_MASK);
     if (threadIdx.x < 2 * TH) rowmask[threadIdx.x] = 0ull;
@@ -645,29 +666,30 @@ template <int R1, int TH> __device__ __forceinline__ void tile_setup(float *lds,
 template <typename IN_T, int R1, int TH>
 __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const ImageJob &J, int tile,
                           int *__restrict__ counts, StageBuf stg, int *__restrict__ tile_cnt) {
-    using Gm = Geo<R1, TH>;
+    constexpr int EB = stage_bytes<IN_T>();
+    using Gm = Geo<R1, TH, EB>;
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
     const TOED_CONST TapSet *tp = as_const(J.taps);
     unsigned long long *rowmask = (unsigned long long *)(lds + Gm::OFF_MASK);
     // round A: R0 window, unshifted -> phase (0,0)
-    vpass<R1, TH, 0>(lds, tp->fu[0], tp->fu[1]);
+    vpass<R1, TH, 0, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1 - 1, 1>(lds, 0, tp->fu[0], tp->fu[1], 0, tp->fu[0], tp->fu[1], 0);
+    hpass<R1, TH, R1 - 1, 1, EB>(lds, 0, tp->fu[0], tp->fu[1], 0, tp->fu[0], tp->fu[1], 0);
     __syncthreads();
     // round B: R1 window, unshifted in y -> phase (0,1) with shifted x taps
-    vpass<R1, TH, 1>(lds, tp->fu[0], tp->fu[1]);
+    vpass<R1, TH, 1, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 1>(lds, 0, tp->fs[0], tp->fs[1], 1, tp->fs[0], tp->fs[1], 1);
+    hpass<R1, TH, R1, 1, EB>(lds, 0, tp->fs[0], tp->fs[1], 1, tp->fs[0], tp->fs[1], 1);
     __syncthreads();
     // round C: shifted in y -> phases (1,0) and (1,1)
-    vpass<R1, TH, 2>(lds, tp->fs[0], tp->fs[1]);
+    vpass<R1, TH, 2, EB>(lds, tp->fs[0], tp->fs[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 2>(lds, 1, tp->fu[0], tp->fu[1], 0, tp->fs[0], tp->fs[1], 1);
+    hpass<R1, TH, R1, 2, EB>(lds, 1, tp->fu[0], tp->fu[1], 0, tp->fs[0], tp->fs[1], 1);
     __syncthreads();
     // the vertical-pass planes are dead now: the survivor queue lives in their place
     int *queue = (int *)(lds + Gm::OFF_P0);
-    nms_tile<R1, TH>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
+    nms_tile<R1, TH, EB>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
     // each survivor's raster key (local row, local column, rank in its row):
     // the row masks are final after the barrier inside nms_tile
     {
@@ -747,20 +769,20 @@ __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0
 // from a device counter (re-armed by K3's last workgroup).  One
 // instantiation per window radius R1; a launch covers the tiles
 // [tile_begin, tile_end) of the images with that radius.
-// Four workgroups per CU (16 waves) for 16-row tiles: registers capped at 128
-// (a few spills), 0.310 -> 0.265 ms per 2160x3840 frame against three
-// workgroups.  Five for 8-row tiles (96 registers; 0.372 -> 0.334 ms there).
+// Five workgroups per CU for u8 images (byte-staged image) and for 8-row
+// tiles; four for 16-row tiles of f32 images at sigma 2 (LDS).
 template <typename IN_T, int TH, int R1>
-__global__ __launch_bounds__(NT, TH == 8 ? 5 : 4) void toed_tiles_kernel(const ImageJob *jobs, int njobs, ImageJob job0,
-                                                        int *counts, StageBuf staging, int *tile_cnt,
-                                                        int tile_begin, int tile_end, int *ticket) {
-    __shared__ float lds[Geo<R1, TH>::TOTAL];
+__global__ __launch_bounds__(NT, (TH == 8 || stage_bytes<IN_T>() == 1) ? 5 : 4) void toed_tiles_kernel(
+    const ImageJob *jobs, int njobs, ImageJob job0, int *counts, StageBuf staging, int *tile_cnt, int tile_begin,
+    int tile_end, int *ticket) {
+    constexpr int EB = stage_bytes<IN_T>();
+    __shared__ float lds[Geo<R1, TH, EB>::TOTAL];
     __shared__ int sh_cnt[6];
     __shared__ int sh_tile;
     if (ticket == nullptr) {
         const int tile = tile_begin + blockIdx.x;
         const ImageJob J = load_job(jobs, njobs, job0, tile);
-        tile_setup<R1, TH>(lds, sh_cnt);
+        tile_setup<R1, TH, EB>(lds, sh_cnt);
         stage_tile<IN_T, R1, TH>(lds, J, tile);
         __syncthreads();
         tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
@@ -773,7 +795,7 @@ __global__ __launch_bounds__(NT, TH == 8 ? 5 : 4) void toed_tiles_kernel(const I
         __syncthreads();
         if (tile >= tile_end) break;
         const ImageJob J = load_job(jobs, njobs, job0, tile);
-        tile_setup<R1, TH>(lds, sh_cnt);
+        tile_setup<R1, TH, EB>(lds, sh_cnt);
         stage_tile<IN_T, R1, TH>(lds, J, tile);
         __syncthreads();
         tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
