@@ -215,7 +215,7 @@ int toed_batch_backward(toed_batch *b, const toed_grad_out_t *outs);
 
 /* Subpixel third-order correction of dense edge maps ------------------------
  * Replaces SubpixelTOCorrectionCPU<T> (subpix_TO_correction/
- * subpix_to_correction.hpp:20-96) with interp_n = 0: set_edgemap (:86),
+ * subpix_to_correction.hpp:20-96) (interp_n: toed_subpix_plan_n): set_edgemap (:86),
  * set_orientation (:87) or estimate_orientation (:88), then run (:95), which
  * returns the N x 4 list [x, y, orientation, strength] and fills TO_edgemap /
  * TO_orientation (:71-72).  Many maps of any sizes go through one plan and
@@ -239,9 +239,24 @@ typedef struct {
 /* sigma -> filter radius ceil(4 sigma) (:107); TOED_ERR_SIGMA above 12 */
 int toed_subpix_radius(double sigma, int *R);
 /* nbr_radius: drop edgels with no other edgel within this distance (:558-621,
- * reference default 2; <= 0 keeps all; at most 5). */
+ * reference default 2; <= 0 keeps all; at most 9). */
 int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double sigma, double nbr_radius,
                      toed_subpix **out);
+/* The constructor's n = interp_n (subpix_to_correction.cpp:92-105): each map
+ * (and its given orientation map) is resized to the work grid
+ * ((H - 1) 2^n + 1) x ((W - 1) 2^n + 1) by bilinear interpolation with the
+ * corners aligned (:69-90, set_edgemap / set_orientation :226-240; FP64, stored
+ * as float) at every run; the path runs there with the NMS margin 3 * 2^n
+ * (:627-629); x and y of the list are work-grid positions / 2^n (:694-695) and
+ * the isolation radius applies to them; TO_edgemap / TO_orientation stay
+ * H x W.  capacity counts rows (at most one per work pixel); d_index holds
+ * work-grid pixels i * W' + j.  interp_n at most TOED_SUBPIX_MAX_INTERP, and
+ * nbr_radius * 2^n at most 9 (the isolation reach on the work grid);
+ * toed_subpix_set_grad refuses plans with interp_n > 0.  toed_subpix_plan is
+ * this entry point with interp_n = 0. */
+#define TOED_SUBPIX_MAX_INTERP 4
+int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double sigma, double nbr_radius,
+                       int interp_n, toed_subpix **out);
 /* Enqueue on the context's stream; per-map counts and TOED_FLAG_* bits land in
  * d_counts[n] / d_flags[n] (device, either may be NULL). */
 int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags);

@@ -15,7 +15,9 @@
 //     cubic sampling and the orientation (T = double inputs are rounded to
 //     float once when uploaded; outputs are converted back to T).
 //   * cpu_nthreads is accepted and ignored; the GPU is the current HIP device.
-//   * n (interp_n, the bilinear work-grid upsampling of :226-232) must be 0.
+//   * n = interp_n > 0 (the bilinear work grid of :92-105, :226-240) at most
+//     TOED_SUBPIX_MAX_INTERP; the map is resized on the GPU from its float
+//     upload (FP64 arithmetic, the work grid stored as float).
 //   * time_nms / time_conv / time_correct are in ms (GPU events): the
 //     orientation + NMS kernel, the isolation and convolution + correction
 //     kernels (which write the list rows), and the rank scans + TO maps
@@ -39,6 +41,7 @@ template <typename T>
 class SubpixelTOCorrectionGPU {
     static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value, "T = float or double");
     int img_height, img_width;
+    int interp_n, work_height, work_width;
     T thresh, sigma, neighbor_radius;
     std::string output_dir;
     toed_ctx *ctx = nullptr;
@@ -71,10 +74,12 @@ class SubpixelTOCorrectionGPU {
     double time_nms = 0, time_conv = 0, time_correct = 0;
 
     SubpixelTOCorrectionGPU(int H, int W, T threshold, T g_sigma, int cpu_nthreads, int n = 0)
-        : img_height(H), img_width(W), thresh(threshold), sigma(g_sigma), neighbor_radius(2),
-          output_dir("./output_files") {
+        : img_height(H), img_width(W), interp_n(n < 0 ? 0 : n), thresh(threshold), sigma(g_sigma),
+          neighbor_radius(2), output_dir("./output_files") {
         (void)cpu_nthreads;
-        if (n != 0) check(TOED_ERR_ARG, "interp_n > 0 is not supported on the GPU path");
+        if (interp_n > TOED_SUBPIX_MAX_INTERP) check(TOED_ERR_ARG, "interp_n");
+        work_height = (H - 1) * (1 << interp_n) + 1;
+        work_width = (W - 1) * (1 << interp_n) + 1;
         int R = 0;
         check(toed_subpix_radius((double)g_sigma, &R), "sigma");
         int dev = 0;
@@ -83,12 +88,13 @@ class SubpixelTOCorrectionGPU {
         const size_t px = (size_t)H * W;
         hcheck(hipMalloc(&d_map, px * sizeof(float)), "hipMalloc");
         hcheck(hipMalloc(&d_orient, px * sizeof(float)), "hipMalloc");
-        hcheck(hipMalloc(&d_edges, px * 4 * sizeof(float)), "hipMalloc");
+        const size_t wpx = (size_t)work_height * work_width;    // one list row per work pixel at most
+        hcheck(hipMalloc(&d_edges, wpx * 4 * sizeof(float)), "hipMalloc");
         hcheck(hipMalloc(&d_te, px * sizeof(float)), "hipMalloc");
         hcheck(hipMalloc(&d_to, px * sizeof(float)), "hipMalloc");
         hcheck(hipMalloc(&d_count, sizeof(int)), "hipMalloc");
         hcheck(hipMemset(d_map, 0, px * sizeof(float)), "hipMemset");
-        edges_host.assign(px * 4, T(0));
+        edges_host.assign(wpx * 4, T(0));
         te_host.assign(px, T(0));
         to_host.assign(px, T(0));
         subpix_edge_pts_final = edges_host.data();
@@ -125,11 +131,12 @@ class SubpixelTOCorrectionGPU {
         m.pitch = m.opitch = m.tpitch = img_width;
         m.thresh = (double)thresh;
         m.d_edges = d_edges;
-        m.capacity = (long long)img_height * img_width;
+        m.capacity = (long long)work_height * work_width;
         m.d_to_edgemap = d_te;
         m.d_to_orientation = d_to;
         toed_subpix *p = nullptr;
-        check(toed_subpix_plan(ctx, 1, &m, (double)sigma, (double)neighbor_radius, &p), "toed_subpix_plan");
+        check(toed_subpix_plan_n(ctx, 1, &m, (double)sigma, (double)neighbor_radius, interp_n, &p),
+              "toed_subpix_plan_n");
         check(toed_subpix_set_timing(p, 1), "toed_subpix_set_timing");
         int st = toed_subpix_run(p, d_count, nullptr);
         float a = 0.f, b = 0.f, c = 0.f;

@@ -65,6 +65,12 @@ def lib():
         L.toed_subpix_oracle.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int, _dp, ctypes.c_long, _i64p, _dp, _dp]
         L.toed_subpix_oracle.restype = ctypes.c_long
+        L.toed_subpix_oracle_n.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                           ctypes.c_long, _i64p, _dp, _dp]
+        L.toed_subpix_oracle_n.restype = ctypes.c_long
+        L.toed_subpix_resize.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int]
         _i8p = ctypes.POINTER(ctypes.c_byte)
         L.toed_subpix_frozen_state.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp, _i64p, ctypes.c_long, _i8p,
                                                _i8p, _ip, _ip, _ip, ctypes.c_double]
@@ -313,20 +319,40 @@ def subpix_interp_cubic(src, x, y):
     return lib().toed_subpix_interp_cubic(_p(src), src.shape[0], src.shape[1], float(x), float(y))
 
 
-def subpix(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthreads=0, maps=False):
+def work_shape(h, w, interp_n=0):
+    """Work grid of a map with interp_n > 0 (subpix_to_correction.cpp:102-104)."""
+    s = 1 << int(interp_n)
+    return (h - 1) * s + 1, (w - 1) * s + 1
+
+
+def resize(src, dh, dw, f32=False):
+    """Bilinear resize with aligned corners (subpix_to_correction.cpp:69-90), FP64."""
+    src = np.ascontiguousarray(src, np.float64)
+    out = np.empty((dh, dw), np.float64)
+    lib().toed_subpix_resize(_p(src), src.shape[0], src.shape[1], _p(out), dh, dw, int(bool(f32)))
+    return out
+
+
+def subpix(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthreads=0, maps=False, interp_n=0,
+           work_f32=False):
     """FP64 subpixel third-order correction of an edge map.  Returns dict with
-    ``edges`` (N x 4: x, y, orientation, strength), ``idx`` (map pixel i*W+j
-    of each edgel) and, with maps=True, ``TO_E`` / ``TO_O``."""
+    ``edges`` (N x 4: x, y, orientation, strength), ``idx`` (work-grid pixel
+    i*W'+j of each edgel; the map pixel when interp_n = 0) and, with
+    maps=True, ``TO_E`` / ``TO_O`` (map size).  interp_n > 0 runs the path on
+    the bilinearly upsampled work grid; work_f32 rounds the resized maps to
+    float, as the GPU stores them."""
     E = np.ascontiguousarray(E, np.float64)
     h, w = E.shape
     th = None if theta is None else np.ascontiguousarray(theta, np.float64)
-    cap = h * w
+    wh, ww = work_shape(h, w, interp_n)
+    cap = wh * ww
     edges = np.empty((cap, 4), np.float64)
     idx = np.empty(cap, np.int64)
     te = np.empty((h, w), np.float64) if maps else None
     to = np.empty((h, w), np.float64) if maps else None
-    n = lib().toed_subpix_oracle(_p(E), h, w, _p(th), float(thresh), float(sigma), float(nbr_radius),
-                                 int(nthreads), _p(edges), cap, _p(idx, _i64p), _p(te), _p(to))
+    n = lib().toed_subpix_oracle_n(_p(E), h, w, _p(th), float(thresh), float(sigma), float(nbr_radius),
+                                   int(nthreads), int(interp_n), int(bool(work_f32)), _p(edges), cap,
+                                   _p(idx, _i64p), _p(te), _p(to))
     if n < 0:
         raise ValueError("subpix oracle failed")
     res = dict(edges=edges[:n].copy(), idx=idx[:n].copy(), n=int(n))
@@ -342,10 +368,10 @@ def subpix_ref_available():
     return os.path.exists(SUBPIX_REF_BIN)
 
 
-def run_subpix_ref(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthreads=1):
+def run_subpix_ref(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthreads=1, interp_n=0):
     """Run the reference's own subpix_to_correction.cpp (compiled into
-    oracle/_ref).  Returns dict(edges, TO_E, TO_O, times=(nms, conv, correct,
-    total) seconds)."""
+    oracle/_ref), constructed with n = interp_n.  Returns dict(edges, TO_E,
+    TO_O, times=(nms, conv, correct, total) seconds)."""
     E = np.ascontiguousarray(E, np.float64)
     h, w = E.shape
     with tempfile.TemporaryDirectory() as td:
@@ -357,7 +383,7 @@ def run_subpix_ref(E, theta=None, thresh=0.2, sigma=2.0, nbr_radius=2.0, nthread
             np.ascontiguousarray(theta, np.float64).tofile(op)
         out = os.path.join(td, "out.bin")
         subprocess.run([SUBPIX_REF_BIN, mp, op, str(h), str(w), repr(float(thresh)), repr(float(sigma)),
-                        str(int(nthreads)), repr(float(nbr_radius)), out, td], check=True,
+                        str(int(nthreads)), repr(float(nbr_radius)), out, td, str(int(interp_n))], check=True,
                        stdout=subprocess.DEVNULL)
         with open(out, "rb") as f:
             n = int(np.frombuffer(f.read(8), np.int64)[0])

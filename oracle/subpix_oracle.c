@@ -1,13 +1,18 @@
 /*
  * TEST INFRASTRUCTURE ONLY -- FP64 oracle of the subpixel third-order
  * correction path (reference: subpix_TO_correction/subpix_to_correction.cpp,
- * class SubpixelTOCorrectionCPU<double>, interp_n = 0).
+ * class SubpixelTOCorrectionCPU<double>, any interp_n).
  *
  *   estimate_orientation  :307-375  triangle smoothing (r = 4, half-sample
  *                                   symmetric boundary, :47-55), central
  *                                   differences (one-sided at the borders),
  *                                   theta = mod(atan(Oyy sgn(-Oxy) / (Oxx + 1e-5)), pi)
  *   compute_dir_from_theta :296-305 dir = (cos, sin)(theta mod pi), :216-222
+ *   set_edgemap / set_orientation with interp_n = n > 0 (:226-240): bilinear
+ *                         :69-90    resize (corners aligned) to the work grid
+ *                                   ((h - 1) 2^n + 1) x ((w - 1) 2^n + 1); the
+ *                                   steps below run on it, NMS margin 3 * 2^n
+ *                                   (:627-629), x, y divided by 2^n (:694-695)
  *   nms_token(margin 3)   :413-519  octant NMS on the map along dir, parabola,
  *                                   |A| < 1e-16 rejected, strength |max_f|
  *   run                   :623-736  Hx = E dirx, Hy = E diry; 8 circular
@@ -171,20 +176,79 @@ double toed_subpix_interp_cubic(const double *src, int h, int w, double x, doubl
 }
 
 /*
- * Whole path.  E: h x w map; theta: given orientation or NULL (estimated).
- * out: N x 4 [x, y, orientation, strength] after isolated-edgel removal;
- * nms_idx (optional): the map pixel i*w+j of each output edgel; TO_E / TO_O
- * (optional) h x w maps.  Returns N (rows beyond cap counted, not written).
+ * Bilinear resize with the corners aligned (:69-90): work pixel (i, j) samples
+ * the map at (i (sh - 1) / (dh - 1), j (sw - 1) / (dw - 1)), the right and
+ * lower neighbours clamped to the last row / column, in the reference's
+ * operation order (built with -ffp-contract=off).  f32: each result rounded to
+ * float, as the GPU stores its work maps.
  */
-long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, double thresh, double sigma,
-                        double nbr_radius, int nthreads, double *out, long cap, int64_t *nms_idx, double *TO_E,
-                        double *TO_O) {
-    if (h <= 0 || w <= 0 || !E || !(sigma > 0)) return -1;
+void toed_subpix_resize(const double *src, int sh, int sw, double *dst, int dh, int dw, int f32) {
+    for (int i = 0; i < dh; ++i) {
+        const double y = dh == 1 ? 0.0 : (double)i * (sh - 1) / (double)(dh - 1);
+        const int ya = (int)floor(y), yb = ya + 1 < sh ? ya + 1 : sh - 1;
+        const double fy = y - ya;
+        const double *ra = src + (size_t)ya * sw, *rb = src + (size_t)yb * sw;
+        for (int j = 0; j < dw; ++j) {
+            const double x = dw == 1 ? 0.0 : (double)j * (sw - 1) / (double)(dw - 1);
+            const int xa = (int)floor(x), xb = xa + 1 < sw ? xa + 1 : sw - 1;
+            const double fx = x - xa;
+            const double v = (1 - fy) * ((1 - fx) * ra[xa] + fx * ra[xb]) + fy * ((1 - fx) * rb[xa] + fx * rb[xb]);
+            dst[(size_t)i * dw + j] = f32 ? (double)(float)v : v;
+        }
+    }
+}
+
+static long subpix_work(const double *E, int h, int w, const double *theta_in, double thresh, double sigma,
+                        double nbr_radius, double *out, long cap, int64_t *nms_idx, double *TO_E, double *TO_O,
+                        int margin, double scale, int ih, int iw);
+
+/*
+ * Whole path.  E: ih x iw map; theta: given orientation or NULL (estimated).
+ * interp_n = n > 0: E and theta are resized to the work grid
+ * ((ih - 1) 2^n + 1) x ((iw - 1) 2^n + 1) first (set_edgemap / set_orientation,
+ * :226-240; work_f32 rounds the resized maps to float); the NMS margin is
+ * 3 * 2^n (:627-629) and x, y are divided by 2^n (:694-695).
+ * out: N x 4 [x, y, orientation, strength] after isolated-edgel removal;
+ * nms_idx (optional): the work-grid pixel i*w'+j of each output edgel; TO_E /
+ * TO_O (optional) ih x iw maps.  Returns N (rows beyond cap counted, not written).
+ */
+long toed_subpix_oracle_n(const double *E, int ih, int iw, const double *theta_in, double thresh, double sigma,
+                          double nbr_radius, int nthreads, int interp_n, int work_f32, double *out, long cap,
+                          int64_t *nms_idx, double *TO_E, double *TO_O) {
+    if (ih <= 0 || iw <= 0 || !E || !(sigma > 0) || interp_n < 0 || interp_n > 8) return -1;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
     (void)nthreads;
 #endif
+    if (interp_n == 0 && !work_f32)
+        return subpix_work(E, ih, iw, theta_in, thresh, sigma, nbr_radius, out, cap, nms_idx, TO_E, TO_O, 3, 1.0, ih,
+                           iw);
+    const int s = 1 << interp_n;
+    const int h = (ih - 1) * s + 1, w = (iw - 1) * s + 1;
+    double *Ew = (double *)malloc(sizeof(double) * (size_t)h * w);
+    double *Tw = theta_in ? (double *)malloc(sizeof(double) * (size_t)h * w) : NULL;
+    toed_subpix_resize(E, ih, iw, Ew, h, w, work_f32);
+    if (Tw) toed_subpix_resize(theta_in, ih, iw, Tw, h, w, work_f32);
+    const long N = subpix_work(Ew, h, w, Tw, thresh, sigma, nbr_radius, out, cap, nms_idx, TO_E, TO_O, 3 * s,
+                               (double)s, ih, iw);
+    free(Ew);
+    free(Tw);
+    return N;
+}
+
+long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, double thresh, double sigma,
+                        double nbr_radius, int nthreads, double *out, long cap, int64_t *nms_idx, double *TO_E,
+                        double *TO_O) {
+    return toed_subpix_oracle_n(E, h, w, theta_in, thresh, sigma, nbr_radius, nthreads, 0, 0, out, cap, nms_idx,
+                                TO_E, TO_O);
+}
+
+/* The path on an h x w work grid; TO maps ih x iw (image grid), positions
+ * divided by `scale` (a power of two: exact) before isolation and output. */
+static long subpix_work(const double *E, int h, int w, const double *theta_in, double thresh, double sigma,
+                        double nbr_radius, double *out, long cap, int64_t *nms_idx, double *TO_E, double *TO_O,
+                        int margin, double scale, int ih, int iw) {
     const size_t n = (size_t)h * w;
     int rad;
     double G[129], dG[129], d2G[129];
@@ -204,8 +268,7 @@ long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, d
         dirx[k] = cos(a);
         diry[k] = sin(a);
     }
-    /* S4: nms_token(3): rows/cols [4, n - 5) */
-    const int margin = 3;
+    /* S4: nms_token(margin): rows/cols [margin + 1, n - margin - 2) */
     const int j0 = margin + 1, j1 = w - (margin + 2), i0 = margin + 1, i1 = h - (margin + 2);
 #define EM(a, b) E[(size_t)(a) * w + (b)]
 #pragma omp parallel for schedule(dynamic)
@@ -278,7 +341,7 @@ long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, d
             double ori;
             if (Fm < 1e-12) ori = atan2(dirx[o], -diry[o]);
             else { Fx /= Fm; Fy /= Fm; ori = atan2(Fx, -Fy); }
-            ex[4 * m + 0] = sx; ex[4 * m + 1] = sy; ex[4 * m + 2] = ori; ex[4 * m + 3] = nm[o];
+            ex[4 * m + 0] = sx / scale; ex[4 * m + 1] = sy / scale; ex[4 * m + 2] = ori; ex[4 * m + 3] = nm[o];
             eidx[m] = (int64_t)o;
             ++m;
         }
@@ -317,17 +380,17 @@ long toed_subpix_oracle(const double *E, int h, int w, const double *theta_in, d
         for (long a = 0; a < ne; ++a) keep[a] = 1;
     }
     /* S8 */
-    if (TO_E) memset(TO_E, 0, sizeof(double) * n);
-    if (TO_O) memset(TO_O, 0, sizeof(double) * n);
+    if (TO_E) memset(TO_E, 0, sizeof(double) * (size_t)ih * iw);
+    if (TO_O) memset(TO_O, 0, sizeof(double) * (size_t)ih * iw);
     long N = 0;
     for (long a = 0; a < ne; ++a) {
         if (!keep[a]) continue;
         if (TO_E || TO_O) {
             int X = (int)llround(ex[4 * a]), Y = (int)llround(ex[4 * a + 1]);
-            X = X < 0 ? 0 : (X > w - 1 ? w - 1 : X);
-            Y = Y < 0 ? 0 : (Y > h - 1 ? h - 1 : Y);
-            if (TO_E) TO_E[(size_t)Y * w + X] = ex[4 * a + 3];
-            if (TO_O) TO_O[(size_t)Y * w + X] = ex[4 * a + 2];
+            X = X < 0 ? 0 : (X > iw - 1 ? iw - 1 : X);
+            Y = Y < 0 ? 0 : (Y > ih - 1 ? ih - 1 : Y);
+            if (TO_E) TO_E[(size_t)Y * iw + X] = ex[4 * a + 3];
+            if (TO_O) TO_O[(size_t)Y * iw + X] = ex[4 * a + 2];
         }
         if (N < cap && out) memcpy(out + 4 * N, ex + 4 * a, 4 * sizeof(double));
         if (N < cap && nms_idx) nms_idx[N] = eidx[a];

@@ -1,12 +1,13 @@
 // Drives the header-only SubpixelTOCorrectionGPU<T> (include/toed_subpix_gpu.hpp)
 // the way the reference's main_subpix_to_correction.cpp:279-293 drives its
-// SubpixelTOCorrectionCPU<double>: ctor(H, W, thresh, sigma, nthreads, 0),
+// SubpixelTOCorrectionCPU<double>: ctor(H, W, thresh, sigma, nthreads, n),
 // set_output_dir, set_neighbor_radius, set_edgemap, set_orientation or
-// estimate_orientation, run(edginfo) with a caller buffer of H*W x 4 rows, then
+// estimate_orientation, run(edginfo) with a caller buffer of one row per
+// work-grid pixel (H*W x 4 rows when n = 0, as :271), then
 // write_array_to_file of the list.
 //
 //     toed_subpix_driver <map.f64> <orient.f64|-> <H> <W> <thresh> <sigma>
-//                        <nbr_radius> <float|double> <out.bin> <out_dir>
+//                        <nbr_radius> <float|double> <out.bin> <out_dir> [interp_n]
 //
 // out.bin: int64 N, N x 4 doubles (the returned rows, as T, widened), then
 // TO_edgemap and TO_orientation (H x W doubles each).
@@ -29,10 +30,11 @@ static bool read_f64(const char *path, std::vector<double> &v, size_t n) {
 
 template <typename T>
 static int run(const std::vector<double> &E, const std::vector<double> &O, bool have_o, int H, int W, double thresh,
-               double sigma, double nbr, const char *out, const char *dir) {
+               double sigma, double nbr, const char *out, const char *dir, int interp_n) {
     std::vector<T> e(E.begin(), E.end()), o(O.begin(), O.end());
-    std::vector<T> edginfo((size_t)H * W * 4, T(0));
-    SubpixelTOCorrectionGPU<T> to_corr(H, W, (T)thresh, (T)sigma, 1, 0);
+    const size_t wpx = (size_t)((H - 1) * (1 << interp_n) + 1) * (size_t)((W - 1) * (1 << interp_n) + 1);
+    std::vector<T> edginfo(wpx * 4, T(0));
+    SubpixelTOCorrectionGPU<T> to_corr(H, W, (T)thresh, (T)sigma, 1, interp_n);
     to_corr.set_output_dir(dir);
     to_corr.set_neighbor_radius((T)nbr);
     to_corr.set_edgemap(e.data());
@@ -56,11 +58,14 @@ static int run(const std::vector<double> &E, const std::vector<double> &O, bool 
 }
 
 int main(int argc, char **argv) {
-    if (argc != 11) {
-        std::fprintf(stderr, "usage: %s map orient|- H W thresh sigma nbr_radius float|double out.bin out_dir\n",
+    if (argc != 11 && argc != 12) {
+        std::fprintf(stderr,
+                     "usage: %s map orient|- H W thresh sigma nbr_radius float|double out.bin out_dir [interp_n]\n",
                      argv[0]);
         return 2;
     }
+    const int interp_n = argc == 12 ? std::atoi(argv[11]) : 0;
+    if (interp_n < 0 || interp_n > TOED_SUBPIX_MAX_INTERP) return 2;
     const int H = std::atoi(argv[3]), W = std::atoi(argv[4]);
     const double thresh = std::atof(argv[5]), sigma = std::atof(argv[6]), nbr = std::atof(argv[7]);
     if (H <= 0 || W <= 0 || !(sigma > 0)) return 2;
@@ -69,8 +74,9 @@ int main(int argc, char **argv) {
     const bool have_o = std::strcmp(argv[2], "-") != 0;
     if (have_o && !read_f64(argv[2], O, (size_t)H * W)) return 2;
     try {
-        if (std::strcmp(argv[8], "float") == 0) return run<float>(E, O, have_o, H, W, thresh, sigma, nbr, argv[9], argv[10]);
-        return run<double>(E, O, have_o, H, W, thresh, sigma, nbr, argv[9], argv[10]);
+        if (std::strcmp(argv[8], "float") == 0)
+            return run<float>(E, O, have_o, H, W, thresh, sigma, nbr, argv[9], argv[10], interp_n);
+        return run<double>(E, O, have_o, H, W, thresh, sigma, nbr, argv[9], argv[10], interp_n);
     } catch (const std::exception &ex) {
         std::fprintf(stderr, "toed_subpix_driver: %s\n", ex.what());
         return 4;

@@ -20,9 +20,11 @@ oracle/_ref/subpix_ref, run on seeded Bezier edge maps from the project's
 generator (the reference ships no edge maps of its own), with the orientation
 estimated from the map or given as a seeded random field.  Each stores the
 float32 map, the orientation field if given, the parameters, the N x 4 list
-and the TO_edgemap / TO_orientation maps in float64.
+and the TO_edgemap / TO_orientation maps in float64.  The SUBPIX_INTERP cases
+construct the class with n = interp_n > 0 (the bilinear work grid) and also
+store interp_n.
 
-Run from the repo root:  python tests/golden/make_golden.py
+Run from the repo root:  python tests/golden/make_golden.py [--interp-only]
 """
 import importlib
 import os
@@ -48,6 +50,11 @@ SUBPIX = [("bezier_96x128_s1", 1, 96, 128, 12, 0.1, 2.0, 2.0, False),
           ("bezier_160x200_s2_sig1.5", 2, 160, 200, 12, 0.1, 1.5, 2.0, False),
           ("bezier_256x256_s3_t0.2_r0", 3, 256, 256, 12, 0.2, 2.0, 0.0, False),
           ("bezier_512x512_s5", 5, 512, 512, 40, 0.1, 2.0, 2.0, False)]
+# the reference class constructed with n = interp_n > 0 (bilinear work grid):
+# (name, seed, H, W, curves, thresh, sigma, nbr_radius, given orientation, interp_n)
+SUBPIX_INTERP = [("bezier_96x128_s6_n1", 6, 96, 128, 10, 0.1, 2.0, 2.0, False, 1),
+                 ("bezier_80x100_s7_orient_n1", 7, 80, 100, 8, 0.1, 1.5, 2.0, True, 1),
+                 ("bezier_64x80_s8_n2", 8, 64, 80, 6, 0.1, 2.0, 2.0, False, 2)]
 
 
 def read_pgm_p5(path):
@@ -113,8 +120,26 @@ def read_png_gray8(path):
     return out
 
 
+def write_subpix(name, seed, H, W, nc, thr, sig, nbr, given, interp_n=0):
+    E = scene.bezier_map(seed, H, W, nc)
+    theta = np.random.default_rng(seed).uniform(-4.0, 4.0, (H, W)) if given else None
+    r = oracle.run_subpix_ref(E.astype(np.float64), theta, thresh=thr, sigma=sig, nbr_radius=nbr, nthreads=8,
+                              interp_n=interp_n)
+    path = os.path.join(OUT, "subpix_ref_%s.npz" % name)
+    extra = {} if theta is None else {"theta": theta}
+    if interp_n:
+        extra["interp_n"] = np.array(interp_n)
+    np.savez_compressed(path, E=E, params=np.array([thr, sig, nbr]), edges=r["edges"], TO_E=r["TO_E"],
+                        TO_O=r["TO_O"], **extra)
+    print("%-28s %4dx%-4d N=%6d total %.3fs -> %s" % (name, H, W, r["n"], r["times"][3], os.path.basename(path)))
+
+
 def main():
     oracle.build(ref=True)
+    if "--interp-only" in sys.argv:
+        for case in SUBPIX_INTERP:
+            write_subpix(*case)
+        return
     items = [(name, scene.shapes_scene(seed, H, W)) for name, seed, H, W in SCENES]
     if os.path.isdir(REF_IMAGES):
         items.append(("img_2018_pgm", read_pgm_p5(os.path.join(REF_IMAGES, "2018.pgm"))))
@@ -126,15 +151,8 @@ def main():
         np.savez_compressed(path, img=img, edges=edges)
         print("%-22s %4dx%-4d N=%6d conv %.3fs nms %.4fs -> %s" % (name, img.shape[0], img.shape[1], len(edges),
                                                                     tc, tn, os.path.basename(path)))
-    for name, seed, H, W, nc, thr, sig, nbr, given in SUBPIX:
-        E = scene.bezier_map(seed, H, W, nc)
-        theta = np.random.default_rng(seed).uniform(-4.0, 4.0, (H, W)) if given else None
-        r = oracle.run_subpix_ref(E.astype(np.float64), theta, thresh=thr, sigma=sig, nbr_radius=nbr, nthreads=8)
-        path = os.path.join(OUT, "subpix_ref_%s.npz" % name)
-        extra = {} if theta is None else {"theta": theta}
-        np.savez_compressed(path, E=E, params=np.array([thr, sig, nbr]), edges=r["edges"], TO_E=r["TO_E"],
-                            TO_O=r["TO_O"], **extra)
-        print("%-28s %4dx%-4d N=%6d total %.3fs -> %s" % (name, H, W, r["n"], r["times"][3], os.path.basename(path)))
+    for case in SUBPIX + SUBPIX_INTERP:
+        write_subpix(*case)
 
 
 if __name__ == "__main__":

@@ -189,8 +189,9 @@ def test_cpp_subpix_class_against_reference_golden(path, T, tmp_path):
     """The header-only C++ SubpixelTOCorrectionGPU<T> (T = double as the
     reference's driver instantiates it, and float), driven as
     main_subpix_to_correction.cpp:279-293 drives SubpixelTOCorrectionCPU<double>,
-    on each golden map: the edgel list equals the reference's own list (same
-    edgels, in order) within 1e-3 px, 1e-3 rad and 1e-4 relative strength;
+    on each golden map (constructed with the fixture's n = interp_n): the
+    edgel list equals the reference's own list (same edgels, in order) within
+    1e-3 px, 1e-3 rad and 1e-4 relative strength;
     TO_edgemap / TO_orientation have the reference's support and values within
     the same tolerances; the text list is written."""
     assert os.path.exists(SP_DRIVER), "toed_subpix_driver is not built"
@@ -206,8 +207,9 @@ def test_cpp_subpix_class_against_reference_golden(path, T, tmp_path):
         np.ascontiguousarray(z["theta"], np.float64).tofile(op)
     out, od = str(tmp_path / "out.bin"), str(tmp_path / "o")
     os.makedirs(od)
-    args = [SP_DRIVER, mp, op, str(H), str(W), repr(thr), repr(sig), repr(nbr), T, out, od]
-    print("driver:", " ".join(args[3:9]), flush=True)
+    interp_n = int(z["interp_n"]) if "interp_n" in z.files else 0
+    args = [SP_DRIVER, mp, op, str(H), str(W), repr(thr), repr(sig), repr(nbr), T, out, od, str(interp_n)]
+    print("driver:", " ".join(args[3:9] + args[11:]), flush=True)
     r = subprocess.run(args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode == 0, r.stdout
     with open(out, "rb") as f:

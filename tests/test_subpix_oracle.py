@@ -19,6 +19,8 @@ import pytest
 import oracle
 
 GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "subpix_ref_*.npz")))
+# the maps of the class constructed with n = 0 (the differentiable form covers those)
+GOLDEN0 = [g for g in GOLDEN if "interp_n" not in np.load(g).files]
 
 
 def test_golden_files_present():
@@ -30,15 +32,73 @@ def test_oracle_matches_reference_golden(path):
     z = np.load(path)
     thr, sig, nbr = z["params"]
     theta = z["theta"] if "theta" in z.files else None
+    interp_n = int(z["interp_n"]) if "interp_n" in z.files else 0
     o = oracle.subpix(z["E"].astype(np.float64), theta, thresh=thr, sigma=sig, nbr_radius=nbr, nthreads=4,
-                      maps=True)
+                      maps=True, interp_n=interp_n)
     assert o["n"] == len(z["edges"]) > 100
+    assert o["TO_E"].shape == z["E"].shape
     # same arithmetic in the same order: bit-identical lists and maps
     np.testing.assert_array_equal(o["edges"], z["edges"])
     np.testing.assert_array_equal(o["TO_E"], z["TO_E"])
     np.testing.assert_array_equal(o["TO_O"], z["TO_O"])
     # raster order of the map pixels
     assert np.all(np.diff(o["idx"]) > 0)
+
+
+def test_interp_goldens_present():
+    assert sum("interp_n" in np.load(g).files for g in GOLDEN) >= 3
+
+
+def test_resize_reproduces_bilinear_functions():
+    """The work grid of interp_n = n samples the map at (i / 2^n, j / 2^n):
+    corners aligned, original pixels kept exactly, and any function
+    a + b y + c x + d x y reproduced between them."""
+    rng = np.random.default_rng(11)
+    h, w = 7, 9
+    a, b, c, d = rng.standard_normal(4)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    src = a + b * yy + c * xx + d * xx * yy
+    for n in (1, 2, 3):
+        dh, dw = oracle.work_shape(h, w, n)
+        assert (dh, dw) == ((h - 1) * 2 ** n + 1, (w - 1) * 2 ** n + 1)
+        out = oracle.resize(src, dh, dw)
+        s = 2 ** n
+        np.testing.assert_array_equal(out[::s, ::s], src)
+        Y, X = np.mgrid[0:dh, 0:dw].astype(np.float64) / s
+        np.testing.assert_allclose(out, a + b * Y + c * X + d * X * Y, rtol=0, atol=1e-12)
+        # float rounding of the work map, as the GPU stores it
+        np.testing.assert_array_equal(oracle.resize(src, dh, dw, f32=True), out.astype(np.float32))
+    # a single row / column stays put
+    np.testing.assert_array_equal(oracle.resize(src[:1], 1, 17)[0, ::2], src[0])
+
+
+def test_interp_zero_is_the_plain_path():
+    z = np.load(GOLDEN[0])
+    E = z["E"].astype(np.float64)
+    a = oracle.subpix(E, None, 0.1, 2.0, 2.0, maps=True)
+    b = oracle.subpix(E, None, 0.1, 2.0, 2.0, maps=True, interp_n=0, work_f32=True)
+    np.testing.assert_array_equal(a["edges"], b["edges"])    # the map is float already
+    np.testing.assert_array_equal(a["TO_E"], b["TO_E"])
+
+
+def test_interp_straight_ridge_keeps_its_place():
+    """A straight vertical ridge at column c of the map: on the work grid of
+    interp_n = 1 the edgels lie near x = c in map coordinates (the list is
+    divided by 2^n; the parabola on the piecewise-linear upsampled profile is
+    biased by about 0.15 px here), on every half row, with the ridge's tangent
+    orientation, and each map row's TO pixel holds one of them."""
+    h, w, c = 48, 40, 19.25
+    xx = np.arange(w, dtype=np.float64)
+    E = np.tile(np.exp(-0.5 * ((xx - c) / 1.2) ** 2), (h, 1))
+    for n in (0, 1):
+        o = oracle.subpix(E, None, 0.1, 2.0, 2.0, maps=True, interp_n=n)
+        e = o["edges"]
+        assert o["n"] > 20
+        np.testing.assert_allclose(e[:, 0], c, atol=0.2)
+        np.testing.assert_array_equal(np.diff(e[:, 1]), 0.5 ** n)
+        np.testing.assert_allclose(np.abs(np.sin(e[:, 2])), 1.0, atol=1e-6)    # tangent along y: +-pi/2
+        rows = np.unique(np.round(e[:, 1]).astype(int))
+        assert np.all(o["TO_E"][rows].astype(bool).sum(1) == 1)
 
 
 def test_taps_closed_forms():
@@ -179,7 +239,7 @@ def test_maps_last_writer_in_list_order():
     assert (o["TO_E"] != 0).sum() == len(e) - 1    # one collision in this map
 
 
-@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[11:-4] for p in GOLDEN])
+@pytest.mark.parametrize("path", GOLDEN0, ids=[os.path.basename(p)[11:-4] for p in GOLDEN0])
 def test_frozen_loss_reproduces_reference_at_forward_point(path):
     """oracle.SubpixFrozen, the finite-difference reference of the GPU adjoint,
     is the forward path with its discrete decisions held: at the forward point

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(PKG, "libtoed_hip.so")
 
 TOED_U8, TOED_F32 = 0, 1
 FLAG_CAPACITY, FLAG_STAGING = 1, 2
+TOED_SUBPIX_MAX_INTERP = 4     # toed_capi.h
 
 STATUS = {
     0: "TOED_OK", 1: "TOED_ERR_ARG", 2: "TOED_ERR_SIGMA", 3: "TOED_ERR_ALLOC", 4: "TOED_ERR_HIP",
@@ -130,6 +131,8 @@ SIGNATURES = [
     ("toed_subpix_radius", _I, [ctypes.c_double, _PI]),
     ("toed_subpix_plan", _I, [_VP, _I, ctypes.POINTER(toed_subpix_map_t), ctypes.c_double, ctypes.c_double,
                               ctypes.POINTER(_VP)]),
+    ("toed_subpix_plan_n", _I, [_VP, _I, ctypes.POINTER(toed_subpix_map_t), ctypes.c_double, ctypes.c_double, _I,
+                                ctypes.POINTER(_VP)]),
     ("toed_subpix_run", _I, [_VP, _VP, _VP]),
     ("toed_subpix_destroy", _I, [_VP]),
     ("toed_subpix_set_maps", _I, [_VP, ctypes.POINTER(_VP)]),

@@ -1,8 +1,13 @@
 // Subpixel third-order correction of dense edge maps on MI355X (gfx950):
 // the GPU form of the reference's SubpixelTOCorrectionCPU<T>
-// (subpix_TO_correction/subpix_to_correction.cpp, interp_n = 0).
+// (subpix_TO_correction/subpix_to_correction.cpp).
 //
 // Per map (any number of maps of any sizes in one plan, one sigma):
+//   R  sp_resize_kernel       interp_n = n > 0 only: the map (and a given
+//        orientation) bilinearly resized to the work grid
+//        ((H - 1) 2^n + 1) x ((W - 1) 2^n + 1) (:69-90, :226-240); every step
+//        below then runs on the work grid (NMS margin 3 * 2^n, list x, y / 2^n,
+//        isolation radius 2^n r, TO maps gathered onto the H x W grid).
 //   A  sp_orient_nms_kernel   one workgroup per 16 x 32 tile.  Orientation:
 //        given, or estimated from the map (:307-375): triangle smoothing r = 4
 //        with half-sample-symmetric borders (row pass, column pass, FP64, same
@@ -49,7 +54,8 @@ constexpr int TH = 16;          // tile rows
 constexpr int TW = 32;          // tile columns = one mask word per tile row
 constexpr int NT = 256;         // threads per workgroup
 constexpr int MAXR = 12;        // largest filter radius ceil(4 sigma)
-constexpr int MAXK = 8;         // largest isolation reach in pixels
+constexpr int MAXK = 8;         // isolation reach in work pixels of the default isolation kernel
+constexpr int MAXK_WIDE = 12;   // ... and of the wide one (radius 2^n r on an interp_n = n work grid)
 constexpr int CHUNKW = 4096;    // mask words per scan chunk
 constexpr int HALO = 3;         // cubic stencil reach around an edgel pixel
 
@@ -86,6 +92,14 @@ struct SpJob {
                                 // stencil's fraction (fx, fy) and base (int bits of xf, yf)
     int *gcount;                // kept edgels of the map (device int)
     float *ghx, *ghy, *gth;     // H x W scratch: dL/dHx, dL/dHy, dL/dtheta at edgels
+    // work grid (interp_n = slog2 > 0, :92-105, :226-240): H x W above is the
+    // work grid ((IH - 1) 2^n + 1) x ((IW - 1) 2^n + 1); map / orient point at
+    // the plan's work planes, resized from the caller's IH x IW arrays src_map /
+    // src_orient (rows of spitch / sopitch); the TO maps stay IH x IW.  With
+    // slog2 = 0, IH = H and IW = W.
+    const float *src_map, *src_orient;
+    int IH, IW, spitch, sopitch;
+    int margin, slog2;          // NMS margin 3 * 2^n (:627-629)
 };
 
 // upstream gradient and output of one map's backward pass
@@ -287,8 +301,8 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
             const long long o = (long long)gi * W + gj;
             J.hx[o] = (float)(e * dx);
             J.hy[o] = (float)(e * dy);
-            // NMS (:446-516): rows/cols [margin + 1, n - margin - 2), margin 3
-            if (gi >= 4 && gi < H - 5 && gj >= 4 && gj < W - 5 && !(e <= J.thresh) &&
+            // NMS (:446-516): rows/cols [margin + 1, n - margin - 2), margin 3 * 2^n
+            if (gi > J.margin && gi < H - J.margin - 2 && gj > J.margin && gj < W - J.margin - 2 && !(e <= J.thresh) &&
                 !(fabs(dx) < 1e-5 && fabs(dy) < 1e-5)) {
                 auto M = [&](int di, int dj) -> double { return (double)Es[li + 6 + di][lj + 6 + dj]; };
                 const double md = sqrt(dx * dx + dy * dy);
@@ -501,9 +515,12 @@ __device__ __forceinline__ void conv_group(const float *In, float *X, float *Y, 
 // when another survivor lies within the radius; survivors within
 // floor(r + 2 sqrt 2) pixels are candidates (an edgel lies within sqrt 2 of
 // its pixel).  Small LDS, so many tiles are in flight to hide the loads.
+// KM: the largest reach K the instantiation holds (LDS is sized for it).
+template <int KM>
 __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTaps tp) {
-    __shared__ uint32_t vm[TH + 2 * MAXK][3];
-    __shared__ double2 posw[(TH + 2 * MAXK) * (TW + 2 * MAXK)];
+    static_assert(2 * KM + 1 < 32 && KM <= TW, "one mask word each side");
+    __shared__ uint32_t vm[TH + 2 * KM][3];
+    __shared__ double2 posw[(TH + 2 * KM) * (TW + 2 * KM)];
     __shared__ int s_flag;
 
     const SpJob J = load_job(jobs);
@@ -537,7 +554,7 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
     };
     if (any_surv && !keep_all) {
         // all of a thread's loads are issued before its LDS stores
-        constexpr int UW = ((TH + 2 * MAXK) * (TW + 2 * MAXK) + NT - 1) / NT;
+        constexpr int UW = ((TH + 2 * KM) * (TW + 2 * KM) + NT - 1) / NT;
         double2 pv[UW];
         bool sv[UW];
 #pragma unroll
@@ -722,7 +739,8 @@ __global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, in
                 J.val[o].y = ori;
             }
             if (rank < J.capacity) {
-                out[rank] = make_float4((float)ps.x, (float)ps.y, ori, vv.x);
+                // work-grid position / 2^n (exact), :694-695
+                out[rank] = make_float4((float)ldexp(ps.x, -J.slog2), (float)ldexp(ps.y, -J.slog2), ori, vv.x);
                 if (J.index) J.index[rank] = (int)o;
             }
         }
@@ -846,29 +864,32 @@ __global__ __launch_bounds__(256) void sp_prefix_kernel(const SpJob *jobs, int n
     }
 }
 
-// E: TO_edgemap / TO_orientation by gather
+// E: TO_edgemap / TO_orientation by gather, on the IH x IW map grid.  The
+// edgel of work pixel (i, j) lies within sqrt 2 of it, so the edgels whose
+// rounded map position (x / 2^n, clamped, :708-713) is (Y, X) come from work
+// pixels within D = floor(2^n / 2 + sqrt 2) of (2^n Y, 2^n X): the 3 x 3
+// neighbourhood when n = 0.
 __global__ __launch_bounds__(NT) void sp_maps_kernel(const SpJob *jobs, int njobs) {
     const SpJob J = load_job(jobs);
     const int lt = blockIdx.x;
-    if ((!J.to_e && !J.to_o) || lt >= J.ntx * J.nty) return;
-    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
-    const int H = J.H, W = J.W;
+    const int intx = (J.IW + TW - 1) / TW, inty = (J.IH + TH - 1) / TH;
+    if ((!J.to_e && !J.to_o) || lt >= intx * inty) return;
+    const int ty = lt / intx, tx = lt - ty * intx;
+    const int H = J.H, W = J.W, sc = 1 << J.slog2;
+    const int D = (int)floor(0.5 * sc + 1.4142135623730951);
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
         const int p = threadIdx.x + k * NT;
         const int Y = ty * TH + p / TW, X = tx * TW + (p % TW);
-        if (Y >= H || X >= W) continue;
+        if (Y >= J.IH || X >= J.IW) continue;
         long long win = -1;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int i = Y + dy;
-            if (i < 0 || i >= H) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int j = X + dx;
-                if (j < 0 || j >= W) continue;
+        for (int i = max(sc * Y - D, 0); i <= min(sc * Y + D, H - 1); ++i) {
+            for (int j = max(sc * X - D, 0); j <= min(sc * X + D, W - 1); ++j) {
                 if (!((J.kmask[(long long)i * J.nw + (j >> 5)] >> (j & 31)) & 1u)) continue;
                 const long long o = (long long)i * W + j;
                 const double2 ps = J.pos[o];
-                const int rx = min(max((int)round(ps.x), 0), W - 1), ry = min(max((int)round(ps.y), 0), H - 1);
+                const int rx = min(max((int)round(ldexp(ps.x, -J.slog2)), 0), J.IW - 1);
+                const int ry = min(max((int)round(ldexp(ps.y, -J.slog2)), 0), J.IH - 1);
                 if (rx == X && ry == Y && o > win) win = o;
             }
         }
@@ -1439,6 +1460,41 @@ __global__ void sp_reset_kernel(const SpJob *jobs, int njobs) {
     if (jobs[m].count_out) *jobs[m].count_out = 0;
 }
 
+// R: interp_n > 0 only: the caller's IH x IW map (and given orientation) to
+// the H x W work grid by bilinear interpolation with the corners aligned
+// (:69-90, set_edgemap / set_orientation :226-240): work pixel (i, j) samples
+// (i (IH - 1) / (H - 1), j (IW - 1) / (W - 1)), the lower / right neighbour
+// clamped to the last row / column.  FP64 in the reference's operation order
+// (contraction off, correctly rounded division), stored as float.
+__global__ __launch_bounds__(NT) void sp_resize_kernel(const SpJob *jobs, int njobs) {
+#pragma clang fp contract(off)
+    const SpJob J = load_job(jobs);
+    const int lt = blockIdx.x;
+    if (J.slog2 == 0 || lt >= J.ntx * J.nty) return;
+    const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
+    float *dmap = const_cast<float *>(J.map);
+    float *dori = const_cast<float *>(J.orient);
+#pragma unroll 1
+    for (int k = 0; k < 2; ++k) {
+        const int p = threadIdx.x + k * NT;
+        const int i = ty * TH + p / TW, j = tx * TW + (p % TW);
+        if (i >= J.H || j >= J.W) continue;
+        const double y = (double)i * (J.IH - 1) / (double)(J.H - 1);
+        const double x = (double)j * (J.IW - 1) / (double)(J.W - 1);
+        const int ya = (int)floor(y), yb = min(ya + 1, J.IH - 1);
+        const int xa = (int)floor(x), xb = min(xa + 1, J.IW - 1);
+        const double fy = y - ya, fx = x - xa;
+        auto bil = [&](const float *src, int pitch) -> float {
+            const float *ra = src + (long long)ya * pitch, *rb = src + (long long)yb * pitch;
+            const double v = (1 - fy) * ((1 - fx) * (double)ra[xa] + fx * (double)ra[xb]) +
+                             fy * ((1 - fx) * (double)rb[xa] + fx * (double)rb[xb]);
+            return (float)v;
+        };
+        dmap[(long long)i * J.W + j] = bil(J.src_map, J.spitch);
+        if (J.src_orient) dori[(long long)i * J.W + j] = bil(J.src_orient, J.sopitch);
+    }
+}
+
 }  // namespace toedsp
 
 using namespace toedsp;
@@ -1454,6 +1510,7 @@ struct toed_subpix {
     int *d_nsurv = nullptr;
     long long ntiles = 0, nchunks = 0;
     int max_tiles = 0, max_chunks = 0;
+    int slog2 = 0;              // interp_n: the maps run on their 2^n work grids
     bool timing = false;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // differentiable form
@@ -1519,26 +1576,38 @@ int toed_subpix_radius(double sigma, int *R) {
 
 int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double sigma, double nbr_radius,
                      toed_subpix **out) {
-    if (!ctx || !out || n <= 0 || n > 65535 || !maps) return TOED_ERR_ARG;
+    return toed_subpix_plan_n(ctx, n, maps, sigma, nbr_radius, 0, out);
+}
+
+int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double sigma, double nbr_radius,
+                       int interp_n, toed_subpix **out) {
+    if (!ctx || !out || n <= 0 || n > 65535 || !maps || interp_n < 0 || interp_n > TOED_SUBPIX_MAX_INTERP)
+        return TOED_ERR_ARG;
     *out = nullptr;
     int R = 0;
     const int st = toed_subpix_radius(sigma, &R);
     if (st) return st;
     if (!(nbr_radius == nbr_radius)) return TOED_ERR_ARG;
-    const int K = nbr_radius > 0 ? (int)std::floor(nbr_radius + 2.8284271247461903 + 1e-9) : 0;
-    if (K > MAXK) return TOED_ERR_ARG;
+    const int sc = 1 << interp_n;
+    // the isolation radius on the work grid: positions there are 2^n times the
+    // list's, so distances compare exactly the same way (powers of two)
+    const double rw = nbr_radius * sc;
+    const int K = nbr_radius > 0 ? (int)std::floor(rw + 2.8284271247461903 + 1e-9) : 0;
+    if (K > MAXK_WIDE) return TOED_ERR_ARG;
+    auto work = [sc](int d) { return (long long)(d - 1) * sc + 1; };
     for (int m = 0; m < n; ++m) {
         const toed_subpix_map_t &M = maps[m];
         if (!M.d_map || M.H < 2 || M.W < 2 || M.pitch < M.W || !M.d_edges || M.capacity < 0) return TOED_ERR_ARG;
         if (M.d_orient && M.opitch < M.W) return TOED_ERR_ARG;
         if ((M.d_to_edgemap || M.d_to_orientation) && M.tpitch < M.W) return TOED_ERR_ARG;
-        if ((long long)M.H * M.W >= (1LL << 31)) return TOED_ERR_ARG;
+        if (work(M.H) * work(M.W) >= (1LL << 31)) return TOED_ERR_ARG;
     }
     toed_subpix *p = new toed_subpix();
     p->ctx = ctx;
     p->device = toed_get_device(ctx);
     p->n = n;
     p->R = R;
+    p->slog2 = interp_n;
     // taps (:194-214), FP64 closed forms rounded once to FP32
     {
         const double s2 = sigma * sigma, s3 = s2 * sigma, s5 = s2 * s3, nrm = std::sqrt(2.0 * M_PI);
@@ -1551,15 +1620,17 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
         p->taps.R = R;
         p->taps.K = K;
         p->taps.keep_all = nbr_radius > 0 ? 0 : 1;
-        p->taps.r2 = nbr_radius * nbr_radius;
+        p->taps.r2 = rw * rw;
     }
-    // work planes per map
+    // work planes per map (and, with interp_n > 0, the resized map and orientation)
     size_t total = 0;
     std::vector<size_t> off(n);
     for (int m = 0; m < n; ++m) {
-        const size_t px = (size_t)maps[m].H * maps[m].W, nw = (size_t)(maps[m].W + 31) / 32;
+        const size_t wh = (size_t)work(maps[m].H), ww = (size_t)work(maps[m].W);
+        const size_t px = wh * ww, nw = (ww + 31) / 32;
         off[m] = total;
-        total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(maps[m].H * nw * 4) * 3;
+        total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(wh * nw * 4) * 3;
+        if (interp_n) total += align_up(px * 4) * (maps[m].d_orient ? 2 : 1);
     }
     if (hipSetDevice(p->device) != hipSuccess) { delete p; return TOED_ERR_HIP; }
     if (hipMalloc(&p->d_work, total) != hipSuccess) { delete p; return TOED_ERR_ALLOC; }
@@ -1570,35 +1641,50 @@ int toed_subpix_plan(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, double
         const toed_subpix_map_t &M = maps[m];
         SpJob &J = p->jobs[m];
         std::memset(&J, 0, sizeof(J));
-        const size_t px = (size_t)M.H * M.W;
+        const int wh = (int)work(M.H), ww = (int)work(M.W);
+        const size_t px = (size_t)wh * ww;
         char *b = p->d_work + off[m];
         J.hx = (float *)b; b += align_up(px * 4);
         J.hy = (float *)b; b += align_up(px * 4);
         J.pos = (double2 *)b; b += align_up(px * 16);
         J.val = (float2 *)b; b += align_up(px * 8);
-        J.nw = (M.W + 31) / 32;
-        J.vmask = (uint32_t *)b; b += align_up((size_t)M.H * J.nw * 4);
-        J.kmask = (uint32_t *)b; b += align_up((size_t)M.H * J.nw * 4);
-        J.wpre = (uint32_t *)b;
+        J.nw = (ww + 31) / 32;
+        J.vmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
+        J.kmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
+        J.wpre = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
         J.nsurv = p->d_nsurv + 64 * m;
-        J.map = M.d_map;
-        J.orient = M.d_orient;
+        J.src_map = M.d_map;
+        J.src_orient = M.d_orient;
+        J.IH = M.H;
+        J.IW = M.W;
+        J.spitch = M.pitch;
+        J.sopitch = M.d_orient ? M.opitch : M.W;
+        J.slog2 = interp_n;
+        J.margin = 3 * sc;
+        if (interp_n) {
+            J.map = (const float *)b; b += align_up(px * 4);
+            J.orient = M.d_orient ? (const float *)b : nullptr;
+            J.pitch = J.opitch = ww;
+        } else {
+            J.map = M.d_map;
+            J.orient = M.d_orient;
+            J.pitch = M.pitch;
+            J.opitch = M.d_orient ? M.opitch : M.W;
+        }
         J.edges = M.d_edges;
         J.index = M.d_index;
         J.to_e = M.d_to_edgemap;
         J.to_o = M.d_to_orientation;
         J.capacity = M.capacity;
         J.thresh = M.thresh;
-        J.H = M.H;
-        J.W = M.W;
-        J.pitch = M.pitch;
-        J.opitch = M.d_orient ? M.opitch : M.W;
+        J.H = wh;
+        J.W = ww;
         J.tpitch = (M.d_to_edgemap || M.d_to_orientation) ? M.tpitch : M.W;
-        J.ntx = (M.W + TW - 1) / TW;
-        J.nty = (M.H + TH - 1) / TH;
+        J.ntx = (ww + TW - 1) / TW;
+        J.nty = (wh + TH - 1) / TH;
         J.tile_begin = (int)tiles;
         tiles += (long long)J.ntx * J.nty;
-        J.nchunks = (int)(((long long)M.H * J.nw + CHUNKW - 1) / CHUNKW);
+        J.nchunks = (int)(((long long)wh * J.nw + CHUNKW - 1) / CHUNKW);
         J.chunk_begin = (int)chunks;
         chunks += J.nchunks;
         p->max_tiles = std::max(p->max_tiles, J.ntx * J.nty);
@@ -1636,9 +1722,14 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     p->dirty = false;
     hipLaunchKernelGGL(sp_reset_kernel, dim3((p->n + 255) / 256), dim3(256), 0, s, p->d_jobs, p->n);
     if (p->timing) SPCHK(hipEventRecord(p->ev[0], s));
+    if (p->slog2) hipLaunchKernelGGL(sp_resize_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
     hipLaunchKernelGGL(sp_orient_nms_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
     if (p->timing) SPCHK(hipEventRecord(p->ev[1], s));
-    hipLaunchKernelGGL(sp_isolate_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->taps);
+    if (p->taps.K <= MAXK)
+        hipLaunchKernelGGL(sp_isolate_kernel<MAXK>, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->taps);
+    else
+        hipLaunchKernelGGL(sp_isolate_kernel<MAXK_WIDE>, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs,
+                           p->taps);
     if (p->timing) SPCHK(hipEventRecord(p->ev[2], s));
     hipLaunchKernelGGL(sp_scan_chunks, dim3(p->max_chunks, p->n), dim3(256), 0, s, p->d_jobs, p->n, p->d_chunk);
     hipLaunchKernelGGL(sp_scan_maps, dim3(p->n), dim3(1024), 0, s, p->d_jobs, p->d_chunk);
@@ -1660,13 +1751,16 @@ int toed_subpix_set_maps(toed_subpix *p, const float *const *d_maps) {
     if (!p || !d_maps) return TOED_ERR_ARG;
     for (int m = 0; m < p->n; ++m)
         if (!d_maps[m]) return TOED_ERR_ARG;
-    for (int m = 0; m < p->n; ++m) p->jobs[m].map = d_maps[m];
+    for (int m = 0; m < p->n; ++m) {
+        if (p->slog2) p->jobs[m].src_map = d_maps[m];   // resized into the work plane at each run
+        else p->jobs[m].map = d_maps[m];
+    }
     p->dirty = true;
     return TOED_OK;
 }
 
 int toed_subpix_set_grad(toed_subpix *p, int on) {
-    if (!p) return TOED_ERR_ARG;
+    if (!p || (on && p->slog2)) return TOED_ERR_ARG;     // the adjoint covers interp_n = 0
     SPCHK(hipSetDevice(p->device));
     // per map: tq, dirs (H x W x 16 B), ghx, ghy, gth (H x W x 4 B), the edgel
     // count and the edgel records (min(capacity, H W) x (128 + 64) B: an edgel

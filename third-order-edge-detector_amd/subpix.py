@@ -35,16 +35,27 @@ def subpix_radius(sigma):
     return r.value
 
 
+def work_shape(H, W, interp_n=0):
+    """Work grid of an H x W map with interp_n > 0 (subpix_to_correction.cpp:102-104)."""
+    s = 1 << int(interp_n)
+    return (H - 1) * s + 1, (W - 1) * s + 1
+
+
 class SubpixBatch:
     """Edge maps (2-D float32 device tensors, row stride >= width, unit column
     stride) corrected by one plan.  ``orients`` (optional, same shapes, radians)
     gives the orientation of every map, or None per map to estimate it from
-    the map.  Outputs: one device buffer of sum(capacity) x 4 floats; map m's
-    edgels start at row ``offsets[m]``, ``counts[m]`` of them."""
+    the map.  ``interp_n`` > 0 runs every map on its bilinearly upsampled work
+    grid ((H - 1) 2^n + 1) x ((W - 1) 2^n + 1) (toed_subpix_plan_n); x, y stay
+    in map pixels and ``index`` holds work-grid pixels.  Outputs: one device
+    buffer of sum(capacity) x 4 floats (capacity defaults to the work grid's
+    pixel count); map m's edgels start at row ``offsets[m]``, ``counts[m]`` of
+    them."""
 
     def __init__(self, detector, maps, orients=None, thresh=DEFAULT_THRESH, sigma=DEFAULT_SIGMA,
-                 nbr_radius=DEFAULT_NBR_RADIUS, capacities=None, with_index=False, with_maps=False):
+                 nbr_radius=DEFAULT_NBR_RADIUS, capacities=None, with_index=False, with_maps=False, interp_n=0):
         torch = _torch()
+        self.interp_n = int(interp_n)
         self.det = detector
         self.maps = list(maps)
         n = len(self.maps)
@@ -60,7 +71,8 @@ class SubpixBatch:
             if ors[m] is not None and (ors[m].shape != E.shape or ors[m].dtype != torch.float32
                                        or ors[m].stride(1) != 1):
                 raise TypeError("orientation maps must match their edge maps (float32)")
-        caps = capacities or [int(E.shape[0]) * int(E.shape[1]) for E in self.maps]
+        caps = capacities or [int(np.prod(work_shape(int(E.shape[0]), int(E.shape[1]), self.interp_n)))
+                              for E in self.maps]
         self.capacities = [int(c) for c in caps]
         self.offsets = np.concatenate([[0], np.cumsum(self.capacities)]).astype(np.int64)
         dev = self.maps[0].device
@@ -93,8 +105,8 @@ class SubpixBatch:
         self._arr = arr
         self._p = ctypes.c_void_p()
         detector._bind_stream()
-        check(lib().toed_subpix_plan(detector._ctx, n, arr, float(sigma), float(nbr_radius), ctypes.byref(self._p)),
-              "toed_subpix_plan")
+        check(lib().toed_subpix_plan_n(detector._ctx, n, arr, float(sigma), float(nbr_radius), self.interp_n,
+                                       ctypes.byref(self._p)), "toed_subpix_plan_n")
         detector._batches.add(self)
 
     def run(self):
@@ -195,17 +207,20 @@ class SubpixelTOCorrectionGPU:
     279-293): ctor (H, W, threshold, g_sigma, device, n), set_edgemap,
     set_orientation or estimate_orientation, run(edginfo_out) -> N, public
     TO_edgemap / TO_orientation / subpix_edge_pts_final (host arrays),
-    set_neighbor_radius, set_output_dir, write_array_to_file.  Only n = 0 (no
-    bilinear upsampling of the map, :226-232) is supported."""
+    set_neighbor_radius, set_output_dir, write_array_to_file.  n = interp_n
+    > 0 runs on the bilinearly upsampled work grid (:92-105, :226-240): the
+    list holds up to one row per work pixel, TO maps stay H x W."""
 
     num_of_edge_data = 4
 
     def __init__(self, H, W, threshold=DEFAULT_THRESH, g_sigma=DEFAULT_SIGMA, device=0, n=0):
-        if n not in (0, None):
-            raise NotImplementedError("interp_n > 0 (upsampled work grid) is not part of the GPU path")
         from . import Detector
         torch = _torch()
+        self.interp_n = max(0, int(n or 0))        # the reference clamps n < 0 to 0 (:94)
+        if self.interp_n > _lib.TOED_SUBPIX_MAX_INTERP:
+            raise ValueError("interp_n above %d" % _lib.TOED_SUBPIX_MAX_INTERP)
         self.img_height, self.img_width = int(H), int(W)
+        self.work_height, self.work_width = work_shape(self.img_height, self.img_width, self.interp_n)
         self.thresh, self.sigma = float(threshold), float(g_sigma)
         subpix_radius(self.sigma)   # validates sigma
         self.neighbor_radius = DEFAULT_NBR_RADIUS
@@ -240,7 +255,7 @@ class SubpixelTOCorrectionGPU:
     def run(self, edginfo_out=None):
         torch = _torch()
         b = SubpixBatch(self._det, [self._map], [self._orient], self.thresh, self.sigma, self.neighbor_radius,
-                        with_maps=True)
+                        with_maps=True, interp_n=self.interp_n)
         try:
             b.set_timing(True)
             b.run()
