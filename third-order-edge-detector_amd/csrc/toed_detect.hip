@@ -191,7 +191,6 @@ template <int R1, int TH, int EB = 4> struct Geo {
     static constexpr int SHB = (-1 - R1) & 3;
     static constexpr int LDB = 4 * (((SHB + NCI + 3) / 4) | 1);    // row pitch in bytes (odd in dwords)
     static constexpr int IMG_FLOATS = EB == 4 ? NRI * LDI : NRI * LDB / 4;
-    static constexpr int NCP = TW + 2;                     // horizontal-pass output cols
     static constexpr int NVR = TH + 2;                     // vertical-pass output rows
     static constexpr int RH = 2 * TH + 2, RW = 2 * TW + 2; // interp results incl. 1-px halo
     static constexpr int LDR = RW;
@@ -210,19 +209,22 @@ template <int R1, int TH, int EB = 4> struct Geo {
     static constexpr int OFF_CAND = OFF_P0;      // near-peak queue (ints), then the survivor queue
     static constexpr int OFF_MASK = ((P_END > Q_END ? P_END : Q_END) + 1) & ~1;   // 2TH u64 row masks
     static constexpr int TOTAL = OFF_MASK + 2 * (2 * TH);
-    // vertical pass: items = (col, block of KV rows); horizontal pass: items =
-    // (row, block of KH columns).  Sized so that each pass is one round of
-    // items over the workgroup (16-row tiles: 4 x NCI <= 208 and
-    // 18 x 12 = 216 items for 256 threads) with a long FMA run per loaded value
-    static constexpr int KV = TH >= 16 ? 5 : 3;
+    // Work items.  Vertical pass: (col, block of KV rows), the NVR rows exactly.
+    // Horizontal pass, one phase at a time: (row, block of KH cols) over the
+    // HR = TH + 1 rows and HC = TW + 1 columns whose results land inside the
+    // interp planes (17 x 33 at TH = 16: no output outside them is computed).
+    // Both stay within three wavefronts (the fourth idles at the barrier;
+    // which one rotates with the tile), with a long FMA run per loaded value.
+    static constexpr int KV = TH >= 16 ? 6 : 5;
     static constexpr int NBV = (NVR + KV - 1) / KV;
     static constexpr int KH = 3;
-    static constexpr int NBH = (NCP + KH - 1) / KH;
-    // the passes' over-reads of a partial last block stay inside the LDS
-    static_assert(EB == 1 || OFF_IMG + (NBV * KV - 1 + 2 * R1) * LDI + NCI <= TOTAL, "vertical pass over-read");
-    static_assert(EB == 4 || 4 * OFF_IMG + (NBV * KV - 1 + 2 * R1) * LDB + SHB + NCI <= 4 * TOTAL,
-                  "vertical pass over-read");
-    static_assert(OFF_P1 + (NVR - 1) * LDI + NBH * KH + 2 * R1 <= TOTAL, "horizontal pass over-read");
+    static constexpr int HR = TH + 1, HC = TW + 1;
+    static constexpr int NBH = (HC + KH - 1) / KH;
+    static_assert(NBV * KV == NVR && NBH * KH == HC, "blocks cover the pass outputs exactly");
+    static_assert(NCI * NBV <= 3 * 64 && HR * NBH <= 3 * 64, "three wavefronts per pass");
+    // every read stays inside its array: the vertical pass reads image rows
+    // up to NVR - 1 + 2 R1 = NRI - 1, the horizontal pass plane rows up to
+    // HR = NVR - 1 and columns up to HC + 2 R1 = NCI - 1
 };
 
 template <typename IN_T> constexpr int stage_bytes() { return std::is_same<IN_T, uint8_t>::value ? 1 : 4; }
@@ -235,105 +237,94 @@ template <typename IN_T> constexpr int stage_bytes() { return std::is_same<IN_T,
 //
 // Vertical pass over the staged image columns into the two planes (P0: G in
 // y, P1: G' in y).  mode 0: R0-window taps (fresh); mode 1: add the two end
-// taps (R0 -> R1 window); mode 2: shifted taps (fresh).
+// taps (R0 -> R1 window); mode 2: shifted taps (fresh).  rot permutes the
+// work items over the wavefronts (a multiple of 64).
 template <int R1, int TH, int MODE, int EB>
-__device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, const TOED_CONST float *t1) {
+__device__ __forceinline__ void vpass(float *lds, int rot, const TOED_CONST float *t0, const TOED_CONST float *t1) {
     using Gm = Geo<R1, TH, EB>;
     float *P0 = lds + Gm::OFF_P0;
     float *P1 = lds + Gm::OFF_P1;
     constexpr int R = (MODE == 0) ? Gm::R0 : R1;
     constexpr int KV = Gm::KV;
-    for (int it = threadIdx.x; it < Gm::NCI * Gm::NBV; it += NT) {
-        const int u = it % Gm::NCI;
-        const int v0 = (it / Gm::NCI) * KV;
-        // image row v0 + t, column u
-        const float *colf = lds + Gm::OFF_IMG + v0 * Gm::LDI + u;
-        const uint8_t *colb = (const uint8_t *)(lds + Gm::OFF_IMG) + v0 * Gm::LDB + Gm::SHB + u;
-        auto px = [&](int t) -> float {
-            if constexpr (EB == 4) return colf[t * Gm::LDI];
-            else return (float)colb[t * Gm::LDB];
-        };
-        if (MODE == 1) {
+    const int it = threadIdx.x ^ rot;
+    if (it >= Gm::NCI * Gm::NBV) return;
+    const int u = it % Gm::NCI;
+    const int v0 = (it / Gm::NCI) * KV;
+    // image row v0 + t, column u
+    const float *colf = lds + Gm::OFF_IMG + v0 * Gm::LDI + u;
+    const uint8_t *colb = (const uint8_t *)(lds + Gm::OFF_IMG) + v0 * Gm::LDB + Gm::SHB + u;
+    auto px = [&](int t) -> float {
+        if constexpr (EB == 4) return colf[t * Gm::LDI];
+        else return (float)colb[t * Gm::LDB];
+    };
+    if (MODE == 1) {
 #pragma unroll
-            for (int d = 0; d < KV; ++d) {
-                if (v0 + d >= Gm::NVR) break;
-                const float lo = px(d), hi = px(d + 2 * R1);
-                P0[(v0 + d) * Gm::LDI + u] += hi * t0[0] + lo * t0[2 * R1];
-                P1[(v0 + d) * Gm::LDI + u] += hi * t1[0] + lo * t1[2 * R1];
+        for (int d = 0; d < KV; ++d) {
+            const float lo = px(d), hi = px(d + 2 * R1);
+            P0[(v0 + d) * Gm::LDI + u] += hi * t0[0] + lo * t0[2 * R1];
+            P1[(v0 + d) * Gm::LDI + u] += hi * t1[0] + lo * t1[2 * R1];
+        }
+    } else {
+        float w[KV + 2 * R1];
+#pragma unroll
+        for (int t = 0; t < KV + 2 * R1; ++t) w[t] = px(t);
+#pragma unroll
+        for (int d = 0; d < KV; ++d) {
+            // the first term as a product (an fma onto +0 would cost one more op)
+            float a0 = w[d + R1 + R] * t0[R1 - R], a1 = w[d + R1 + R] * t1[R1 - R];
+#pragma unroll
+            for (int q = -R + 1; q <= R; ++q) {
+                const float v = w[d + R1 - q];
+                a0 = fmaf(v, t0[q + R1], a0);
+                a1 = fmaf(v, t1[q + R1], a1);
             }
-        } else {
-            // the last block may read up to KV - 1 rows past the staged image
-            // (into the Ix plane that follows it); those rows feed only the
-            // outputs past NVR, which are not stored
-            float w[KV + 2 * R1];
-#pragma unroll
-            for (int t = 0; t < KV + 2 * R1; ++t) w[t] = px(t);
-#pragma unroll
-            for (int d = 0; d < KV; ++d) {
-                // the first term as a product (an fma onto +0 would cost one more op)
-                float a0 = w[d + R1 + R] * t0[R1 - R], a1 = w[d + R1 + R] * t1[R1 - R];
-#pragma unroll
-                for (int q = -R + 1; q <= R; ++q) {
-                    const float v = w[d + R1 - q];
-                    a0 = fmaf(v, t0[q + R1], a0);
-                    a1 = fmaf(v, t1[q + R1], a1);
-                }
-                if (v0 + d < Gm::NVR) {
-                    P0[(v0 + d) * Gm::LDI + u] = a0;
-                    P1[(v0 + d) * Gm::LDI + u] = a1;
-                }
-            }
+            P0[(v0 + d) * Gm::LDI + u] = a0;
+            P1[(v0 + d) * Gm::LDI + u] = a1;
         }
     }
 }
 
-// Horizontal pass: for phases with y shift A (0/1) and x shift B (0/1), from
-// the current planes: Ix = sum_x P0 * X1, Iy = sum_x P1 * X0, into the interp
-// result planes.  NPH = 1 or 2 phases (x taps xa, xb) from the same planes.
-template <int R1, int TH, int R, int NPH, int EB>
-__device__ __forceinline__ void hpass(float *lds, int A,
-                                      const TOED_CONST float *xa0, const TOED_CONST float *xa1, int B0,
-                                      const TOED_CONST float *xb0, const TOED_CONST float *xb1, int B1) {
+// Horizontal pass for the phase with y shift A and x shift B (0/1) from the
+// current planes: Ix = sum_x P0 * X1, Iy = sum_x P1 * X0 (x taps X0, X1) at
+// interp row 2v - 1 + A, column 2u - 1 + B, for the plane rows v = 1 - A ..
+// TH + 1 - A and columns u = 1 - B .. TW + 1 - B: exactly the results inside
+// the interp planes.
+template <int R1, int TH, int R, int A, int B, int EB>
+__device__ __forceinline__ void hpass(float *lds, int rot, const TOED_CONST float *x0, const TOED_CONST float *x1) {
     using Gm = Geo<R1, TH, EB>;
-    const float *P0 = lds + Gm::OFF_P0;
-    const float *P1 = lds + Gm::OFF_P1;
     float *RX = lds + Gm::OFF_IX;
     float *RY = lds + Gm::OFF_IY;
     constexpr int KH = Gm::KH;
     constexpr int NW = KH + 2 * R1;
-    for (int it = threadIdx.x; it < Gm::NVR * Gm::NBH; it += NT) {
-        const int v = it % Gm::NVR;
-        const int u0 = (it / Gm::NVR) * KH;
-        float w0[NW], w1[NW];
-        // the last block may read past the plane row (into the next row, or
-        // past P1 into the queue area); those values feed only outputs past
-        // NCP, which are not stored
+    const int it = threadIdx.x ^ rot;
+    if (it >= Gm::HR * Gm::NBH) return;
+    const int v = (1 - A) + it % Gm::HR;
+    const int u0 = (1 - B) + (it / Gm::HR) * KH;
+    // the windows' first elements as opaque registers: the loads then take
+    // their small constant offsets from one base per plane (a visible constant
+    // base makes the compiler add each large plane offset separately)
+    int base0 = Gm::OFF_P0 + v * Gm::LDI + u0, base1 = base0 + (Gm::OFF_P1 - Gm::OFF_P0);
+    asm volatile("" : "+v"(base0), "+v"(base1));
+    const float *p0 = lds + base0;
+    const float *p1 = lds + base1;
+    float w0[NW], w1[NW];
 #pragma unroll
-        for (int t = 0; t < NW; ++t) {
-            w0[t] = P0[v * Gm::LDI + u0 + t];
-            w1[t] = P1[v * Gm::LDI + u0 + t];
+    for (int t = 0; t < NW; ++t) {
+        w0[t] = p0[t];
+        w1[t] = p1[t];
+    }
+    const int rr = 2 * v - 1 + A;
+#pragma unroll
+    for (int d = 0; d < KH; ++d) {
+        float ix = w0[d + R1 + R] * x1[R1 - R], iy = w1[d + R1 + R] * x0[R1 - R];
+#pragma unroll
+        for (int q = -R + 1; q <= R; ++q) {
+            ix = fmaf(w0[d + R1 - q], x1[q + R1], ix);
+            iy = fmaf(w1[d + R1 - q], x0[q + R1], iy);
         }
-#pragma unroll
-        for (int ph = 0; ph < NPH; ++ph) {
-            const TOED_CONST float *x0 = ph ? xb0 : xa0;
-            const TOED_CONST float *x1 = ph ? xb1 : xa1;
-            const int B = ph ? B1 : B0;
-#pragma unroll
-            for (int d = 0; d < KH; ++d) {
-                const int u = u0 + d;
-                float ix = w0[d + R1 + R] * x1[R1 - R], iy = w1[d + R1 + R] * x0[R1 - R];
-#pragma unroll
-                for (int q = -R + 1; q <= R; ++q) {
-                    ix = fmaf(w0[d + R1 - q], x1[q + R1], ix);
-                    iy = fmaf(w1[d + R1 - q], x0[q + R1], iy);
-                }
-                const int rr = 2 * v - 1 + A, cc = 2 * u - 1 + B;
-                if (u < Gm::NCP && rr >= 0 && rr < Gm::RH && cc >= 0 && cc < Gm::RW) {
-                    RX[rr * Gm::LDR + cc] = ix;
-                    RY[rr * Gm::LDR + cc] = iy;
-                }
-            }
-        }
+        const int cc = 2 * (u0 + d) - 1 + B;
+        RX[rr * Gm::LDR + cc] = ix;
+        RY[rr * Gm::LDR + cc] = iy;
     }
 }
 
@@ -672,20 +663,24 @@ __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const Imag
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
     const TOED_CONST TapSet *tp = as_const(J.taps);
     unsigned long long *rowmask = (unsigned long long *)(lds + Gm::OFF_MASK);
+    // the wavefront left idle by each pass rotates with the tile, so that the
+    // workgroups on a CU share the idle slots across its SIMDs
+    const int rot = (tile & 3) << 6;
     // round A: R0 window, unshifted -> phase (0,0)
-    vpass<R1, TH, 0, EB>(lds, tp->fu[0], tp->fu[1]);
+    vpass<R1, TH, 0, EB>(lds, rot, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1 - 1, 1, EB>(lds, 0, tp->fu[0], tp->fu[1], 0, tp->fu[0], tp->fu[1], 0);
+    hpass<R1, TH, R1 - 1, 0, 0, EB>(lds, rot, tp->fu[0], tp->fu[1]);
     __syncthreads();
     // round B: R1 window, unshifted in y -> phase (0,1) with shifted x taps
-    vpass<R1, TH, 1, EB>(lds, tp->fu[0], tp->fu[1]);
+    vpass<R1, TH, 1, EB>(lds, rot, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 1, EB>(lds, 0, tp->fs[0], tp->fs[1], 1, tp->fs[0], tp->fs[1], 1);
+    hpass<R1, TH, R1, 0, 1, EB>(lds, rot, tp->fs[0], tp->fs[1]);
     __syncthreads();
-    // round C: shifted in y -> phases (1,0) and (1,1)
-    vpass<R1, TH, 2, EB>(lds, tp->fs[0], tp->fs[1]);
+    // round C: shifted in y -> phases (1,0) and (1,1), one x family at a time
+    vpass<R1, TH, 2, EB>(lds, rot, tp->fs[0], tp->fs[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 2, EB>(lds, 1, tp->fu[0], tp->fu[1], 0, tp->fs[0], tp->fs[1], 1);
+    hpass<R1, TH, R1, 1, 0, EB>(lds, rot, tp->fu[0], tp->fu[1]);
+    hpass<R1, TH, R1, 1, 1, EB>(lds, rot, tp->fs[0], tp->fs[1]);
     __syncthreads();
     // the vertical-pass planes are dead now: the survivor queue lives in their place
     int *queue = (int *)(lds + Gm::OFF_P0);
