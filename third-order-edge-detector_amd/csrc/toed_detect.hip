@@ -195,16 +195,20 @@ template <int R1, int TH, int EB = 4> struct Geo {
     static constexpr int RH = 2 * TH + 2, RW = 2 * TW + 2; // interp results incl. 1-px halo
     static constexpr int LDR = RW;
     // LDS layout (floats): staged image | Ix, Iy with 1-px halo | two
-    // vertical-pass planes (NVR rows x NCI cols), later reused for the
+    // vertical-pass planes (NVR rows x NCI cols, pitch LDP), later reused for the
     // near-peak queue (which becomes the survivor queue) | row masks.
     // |grad I| is recomputed from Ix, Iy where needed (no third plane)
     static constexpr int OFF_IMG = 0;
     static constexpr int OFF_IX = (IMG_FLOATS + 1) & ~1;
     static constexpr int OFF_IY = OFF_IX + RH * LDR;
     static constexpr int OFF_P0 = (OFF_IY + RH * LDR + 1) & ~1;      // 8-byte aligned
-    static constexpr int OFF_P1 = OFF_P0 + NVR * LDI;
+    // vertical-pass plane pitch: the smallest >= NCI that is 1 mod 32, so that
+    // the horizontal pass's window loads (lanes along a row 3 columns apart,
+    // the next row 11 lanes on) hit 32 distinct banks in each half-wavefront
+    static constexpr int LDP = (NCI + 30) / 32 * 32 + 1;
+    static constexpr int OFF_P1 = OFF_P0 + NVR * LDP;
     static constexpr int QMAX = 2 * TH * 2 * TW;                     // survivor queue (ints)
-    static constexpr int P_END = OFF_P1 + NVR * LDI;
+    static constexpr int P_END = OFF_P1 + NVR * LDP;
     static constexpr int Q_END = OFF_P0 + QMAX;
     static constexpr int OFF_CAND = OFF_P0;      // near-peak queue (ints), then the survivor queue
     static constexpr int OFF_MASK = ((P_END > Q_END ? P_END : Q_END) + 1) & ~1;   // 2TH u64 row masks
@@ -261,8 +265,8 @@ __device__ __forceinline__ void vpass(float *lds, int rot, const TOED_CONST floa
 #pragma unroll
         for (int d = 0; d < KV; ++d) {
             const float lo = px(d), hi = px(d + 2 * R1);
-            P0[(v0 + d) * Gm::LDI + u] += hi * t0[0] + lo * t0[2 * R1];
-            P1[(v0 + d) * Gm::LDI + u] += hi * t1[0] + lo * t1[2 * R1];
+            P0[(v0 + d) * Gm::LDP + u] += hi * t0[0] + lo * t0[2 * R1];
+            P1[(v0 + d) * Gm::LDP + u] += hi * t1[0] + lo * t1[2 * R1];
         }
     } else {
         float w[KV + 2 * R1];
@@ -278,8 +282,8 @@ __device__ __forceinline__ void vpass(float *lds, int rot, const TOED_CONST floa
                 a0 = fmaf(v, t0[q + R1], a0);
                 a1 = fmaf(v, t1[q + R1], a1);
             }
-            P0[(v0 + d) * Gm::LDI + u] = a0;
-            P1[(v0 + d) * Gm::LDI + u] = a1;
+            P0[(v0 + d) * Gm::LDP + u] = a0;
+            P1[(v0 + d) * Gm::LDP + u] = a1;
         }
     }
 }
@@ -298,12 +302,13 @@ __device__ __forceinline__ void hpass(float *lds, int rot, const TOED_CONST floa
     constexpr int NW = KH + 2 * R1;
     const int it = threadIdx.x ^ rot;
     if (it >= Gm::HR * Gm::NBH) return;
-    const int v = (1 - A) + it % Gm::HR;
-    const int u0 = (1 - B) + (it / Gm::HR) * KH;
+    // column blocks fastest: 11 lanes per plane row
+    const int v = (1 - A) + it / Gm::NBH;
+    const int u0 = (1 - B) + (it % Gm::NBH) * KH;
     // the windows' first elements as opaque registers: the loads then take
     // their small constant offsets from one base per plane (a visible constant
     // base makes the compiler add each large plane offset separately)
-    int base0 = Gm::OFF_P0 + v * Gm::LDI + u0, base1 = base0 + (Gm::OFF_P1 - Gm::OFF_P0);
+    int base0 = Gm::OFF_P0 + v * Gm::LDP + u0, base1 = base0 + (Gm::OFF_P1 - Gm::OFF_P0);
     asm volatile("" : "+v"(base0), "+v"(base1));
     const float *p0 = lds + base0;
     const float *p1 = lds + base1;
