@@ -385,13 +385,29 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
     const int IH = 2 * J.H, IW = 2 * J.W;
     const int R0g = 2 * tr * TH, C0g = 2 * tc * TW;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    // step 1: FP32 filter
-    for (int r = wave; r < 2 * TH; r += NT / 64) {
+    // step 1: FP32 filter over the wavefront's RPW rows r = wave + 4k (four
+    // wavefronts): the
+    // centre values of all its rows are loaded up front (their latencies
+    // overlap), each row's near-peaks are balloted, and the wavefront takes
+    // its queue slots with one LDS atomic for all its rows
+    constexpr int RPW = 2 * TH / (NT / 64);
+    float gxs[RPW], gys[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int o = (wave + (NT / 64) * k + 1) * Gm::LDR + lane + 1;
+        gxs[k] = RX[o];
+        gys[k] = RY[o];
+    }
+    unsigned long long bal[RPW];
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int r = wave + (NT / 64) * k;
         const int R = R0g + r, C = C0g + lane;
         const int rr = r + 1, cc = lane + 1;
         bool cnd = false;
         const bool inside = R >= J.border && R < IH - J.border && C >= J.border && C < IW - J.border;
-        const float gx = RX[rr * Gm::LDR + cc], gy = RY[rr * Gm::LDR + cc];
+        const float gx = gxs[k], gy = gys[k];
         const float f = __builtin_amdgcn_sqrtf(gx * gx + gy * gy);
         if (inside && f > J.thresh) {
             const float ax = fabsf(gx), ay = fabsf(gy);
@@ -407,12 +423,17 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
                 cnd = f >= fmaxf(fp, fm) * (1.0f - 1e-5f);
             }
         }
-        const unsigned long long bal = __ballot(cnd);
-        if (bal) {
-            int b0 = 0;
-            if (lane == 0) b0 = atomicAdd(&sh_cnt[5], __popcll(bal));
-            b0 = __shfl(b0, 0);
-            if (cnd) cand[b0 + __popcll(bal & lt)] = r | (lane << 6);
+        bal[k] = __ballot(cnd);
+        tot += __popcll(bal[k]);
+    }
+    if (tot) {
+        int b0 = 0;
+        if (lane == 0) b0 = atomicAdd(&sh_cnt[5], tot);
+        b0 = __shfl(b0, 0);
+#pragma unroll
+        for (int k = 0; k < RPW; ++k) {
+            if ((bal[k] >> lane) & 1ull) cand[b0 + __popcll(bal[k] & lt)] = (wave + (NT / 64) * k) | (lane << 6);
+            b0 += __popcll(bal[k]);
         }
     }
     __syncthreads();
