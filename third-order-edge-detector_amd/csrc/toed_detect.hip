@@ -1296,11 +1296,11 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
     const int i0 = tr * TH, j0 = tc * TW;
     const int gi0 = i0 - U, gj0 = j0 - U;        // patch origin (image rows/cols)
     const TOED_CONST TapSet *tp = as_const(J.taps);
-    // Prologue: the tile's independent global loads in one pass so that their
-    // latencies overlap: the tap tables (threads < 3 (2 R1 + 3); variant 1,
-    // window R0, is variant 0 with zero end taps), the image region, and --
-    // when the plan keeps each edgel's interp index -- the count and output
-    // start of each of the tile's interp rows.
+    // Prologue: when the plan keeps each edgel's interp index, the count and
+    // output start of each of the tile's interp rows (an empty tile returns
+    // here); then the tile's independent global loads in one pass so that
+    // their latencies overlap: the tap tables (threads < 3 (2 R1 + 3); variant
+    // 1, window R0, is variant 0 with zero end taps) and the image region.
     constexpr int NP = N + 2;
     const bool by_rows = J.index_out != nullptr;
     int rcnt = 0;
@@ -1308,6 +1308,19 @@ __global__ __launch_bounds__(NT, 4) void toed_tile_backward_kernel(const ImageJo
         const long long ce = J.counts_base + (long long)(2 * i0 + threadIdx.x) * J.ntc + tc;
         rcnt = counts[ce];
         s_rowstart[threadIdx.x] = (long long)offsets[ce] + chunk_pref[J.chunk_base + (ce - J.counts_base) / CHUNK];
+    }
+    // a tile without edgels adds nothing: it leaves before staging anything
+    {
+        bool any;
+        if (by_rows) {
+            any = __syncthreads_or(rcnt > 0) != 0;
+        } else {
+            int t = 0;
+#pragma unroll
+            for (int ph = 0; ph < 4; ++ph) t += tile_cnt[(long long)tile * 4 + ph];
+            any = t > 0;
+        }
+        if (!any) return;
     }
     double ta[4], td[4];
     const int tv = threadIdx.x / NP, ti = threadIdx.x - tv * NP, tt = ti - 1;
