@@ -1172,8 +1172,20 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
         }
         return;
     }
-    // the FP64 taps of this tile's sigma and the tile's image region (rows and
-    // columns from 1 + R1 before the tile, zero outside the image)
+    // staging slot of the tile's edgel e (phases in order)
+    auto stage_slot = [&](int e) -> long long {
+        int k = e, phase = 0;
+        while (phase < 3 && k >= nph[phase]) { k -= nph[phase]; ++phase; }
+        return J.stage_base + (tile_local * 4 + phase) * J.stage_cap + k;
+    };
+    // The first chunk's keys and records are loaded in the same round as the
+    // FP64 taps of this tile's sigma and the tile's image region (rows and
+    // columns from 1 + R1 before the tile, zero outside the image); only the
+    // output rows, which need the keys, take a second round.
+    const bool have0 = (int)threadIdx.x < total;
+    const long long slot0 = stage_slot(have0 ? (int)threadIdx.x : 0);
+    const uint32_t key0 = stg.key[slot0];
+    const float4 rec0 = stg.rec[slot0];
     const TOED_CONST TapSet *tp = as_const(J.taps);
     const int R1 = J.R1, N = 2 * R1 + 1;
     for (int k = threadIdx.x; k < 12 * N; k += FIN_NT) {
@@ -1209,21 +1221,22 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
             }
         }
     }
+    // a chunk of staged edgels to LDS: record, output row and tile-local pixel
+    auto put_edgel = [&](uint32_t key, float4 rec) {
+        const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
+        const long long idx = out_row(J, out_index(J, offsets, chunk_pref, 2 * tr * TH + r, tc, rank));
+        s_rec[threadIdx.x] = rec;
+        s_idx[threadIdx.x] = idx < J.capacity ? idx : -1;
+        s_rc[threadIdx.x] = r | (c << 8);
+    };
+    if (have0) put_edgel(key0, rec0);
     const int g = threadIdx.x / FIN_GT, l = threadIdx.x % FIN_GT;
     for (int c0 = 0; c0 < total; c0 += FIN_NT) {
-        // one round of loads for a chunk of edgels: key, record and output row
-        {
+        if (c0 > 0) {
             const int e = c0 + threadIdx.x;
             if (e < total) {
-                int k = e, phase = 0;
-                while (phase < 3 && k >= nph[phase]) { k -= nph[phase]; ++phase; }
-                const long long gi = J.stage_base + (tile_local * 4 + phase) * J.stage_cap + k;
-                const uint32_t key = stg.key[gi];
-                const int r = key & 0xff, c = (key >> 8) & 0xff, rank = key >> 16;
-                const long long idx = out_row(J, out_index(J, offsets, chunk_pref, 2 * tr * TH + r, tc, rank));
-                s_rec[threadIdx.x] = stg.rec[gi];
-                s_idx[threadIdx.x] = idx < J.capacity ? idx : -1;
-                s_rc[threadIdx.x] = r | (c << 8);
+                const long long gi = stage_slot(e);
+                put_edgel(stg.key[gi], stg.rec[gi]);
             }
         }
         __syncthreads();
