@@ -217,8 +217,8 @@ template <int R1, int TH, int EB = 4> struct Geo {
     // Horizontal pass, one phase at a time: (row, block of KH cols) over the
     // HR = TH + 1 rows and HC = TW + 1 columns whose results land inside the
     // interp planes (17 x 33 at TH = 16: no output outside them is computed).
-    // Both stay within three wavefronts (the fourth idles at the barrier;
-    // which one rotates with the tile), with a long FMA run per loaded value.
+    // Both stay within three wavefronts (the fourth waits at the barrier),
+    // with a long FMA run per loaded value.
     static constexpr int KV = TH >= 16 ? 6 : 5;
     static constexpr int NBV = (NVR + KV - 1) / KV;
     static constexpr int KH = 3;
@@ -241,16 +241,15 @@ template <typename IN_T> constexpr int stage_bytes() { return std::is_same<IN_T,
 //
 // Vertical pass over the staged image columns into the two planes (P0: G in
 // y, P1: G' in y).  mode 0: R0-window taps (fresh); mode 1: add the two end
-// taps (R0 -> R1 window); mode 2: shifted taps (fresh).  rot permutes the
-// work items over the wavefronts (a multiple of 64).
+// taps (R0 -> R1 window); mode 2: shifted taps (fresh).
 template <int R1, int TH, int MODE, int EB>
-__device__ __forceinline__ void vpass(float *lds, int rot, const TOED_CONST float *t0, const TOED_CONST float *t1) {
+__device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, const TOED_CONST float *t1) {
     using Gm = Geo<R1, TH, EB>;
     float *P0 = lds + Gm::OFF_P0;
     float *P1 = lds + Gm::OFF_P1;
     constexpr int R = (MODE == 0) ? Gm::R0 : R1;
     constexpr int KV = Gm::KV;
-    const int it = threadIdx.x ^ rot;
+    const int it = threadIdx.x;
     if (it >= Gm::NCI * Gm::NBV) return;
     const int u = it % Gm::NCI;
     const int v0 = (it / Gm::NCI) * KV;
@@ -294,13 +293,13 @@ __device__ __forceinline__ void vpass(float *lds, int rot, const TOED_CONST floa
 // TH + 1 - A and columns u = 1 - B .. TW + 1 - B: exactly the results inside
 // the interp planes.
 template <int R1, int TH, int R, int A, int B, int EB>
-__device__ __forceinline__ void hpass(float *lds, int rot, const TOED_CONST float *x0, const TOED_CONST float *x1) {
+__device__ __forceinline__ void hpass(float *lds, const TOED_CONST float *x0, const TOED_CONST float *x1) {
     using Gm = Geo<R1, TH, EB>;
     float *RX = lds + Gm::OFF_IX;
     float *RY = lds + Gm::OFF_IY;
     constexpr int KH = Gm::KH;
     constexpr int NW = KH + 2 * R1;
-    const int it = threadIdx.x ^ rot;
+    const int it = threadIdx.x;
     if (it >= Gm::HR * Gm::NBH) return;
     // column blocks fastest: 11 lanes per plane row
     const int v = (1 - A) + it / Gm::NBH;
@@ -689,24 +688,21 @@ __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const Imag
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
     const TOED_CONST TapSet *tp = as_const(J.taps);
     unsigned long long *rowmask = (unsigned long long *)(lds + Gm::OFF_MASK);
-    // the wavefront left idle by each pass rotates with the tile, so that the
-    // workgroups on a CU share the idle slots across its SIMDs
-    const int rot = (tile & 3) << 6;
     // round A: R0 window, unshifted -> phase (0,0)
-    vpass<R1, TH, 0, EB>(lds, rot, tp->fu[0], tp->fu[1]);
+    vpass<R1, TH, 0, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1 - 1, 0, 0, EB>(lds, rot, tp->fu[0], tp->fu[1]);
+    hpass<R1, TH, R1 - 1, 0, 0, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
     // round B: R1 window, unshifted in y -> phase (0,1) with shifted x taps
-    vpass<R1, TH, 1, EB>(lds, rot, tp->fu[0], tp->fu[1]);
+    vpass<R1, TH, 1, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 0, 1, EB>(lds, rot, tp->fs[0], tp->fs[1]);
+    hpass<R1, TH, R1, 0, 1, EB>(lds, tp->fs[0], tp->fs[1]);
     __syncthreads();
     // round C: shifted in y -> phases (1,0) and (1,1), one x family at a time
-    vpass<R1, TH, 2, EB>(lds, rot, tp->fs[0], tp->fs[1]);
+    vpass<R1, TH, 2, EB>(lds, tp->fs[0], tp->fs[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 1, 0, EB>(lds, rot, tp->fu[0], tp->fu[1]);
-    hpass<R1, TH, R1, 1, 1, EB>(lds, rot, tp->fs[0], tp->fs[1]);
+    hpass<R1, TH, R1, 1, 0, EB>(lds, tp->fu[0], tp->fu[1]);
+    hpass<R1, TH, R1, 1, 1, EB>(lds, tp->fs[0], tp->fs[1]);
     __syncthreads();
     // the vertical-pass planes are dead now: the survivor queue lives in their place
     int *queue = (int *)(lds + Gm::OFF_P0);
