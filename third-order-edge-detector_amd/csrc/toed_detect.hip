@@ -490,6 +490,19 @@ __device__ __forceinline__ void nms_tile(float *lds, const ImageJob &J, int tr, 
     }
 }
 
+// Angle of the third-order vector (TOx, TOy): cpu_toed.cpp:227-232 divides it
+// by its norm and takes atan2(TOx, -TOy).  A positive scale leaves the angle
+// unchanged, so the FP64 vector is scaled exactly by a power of two (its
+// larger component to [0.5, 1)) and the angle is taken in FP32, the precision
+// it is stored in; a zero vector gives NaN, as the reference's 0/0 does.
+__device__ __forceinline__ float to_angle(double tx, double ty) {
+    const double m = fmax(fabs(tx), fabs(ty));
+    if (!(m > 0.0)) return __builtin_nanf("");
+    int e;
+    (void)frexp(m, &e);
+    return atan2f((float)ldexp(tx, -e), (float)ldexp(-ty, -e));
+}
+
 // Third-order orientation (cpu_toed.cpp:227-232) of staged edgels, in FP64,
 // run by the scatter kernel on the tile's image region, which the wavefront
 // has staged in LDS (rows and columns from 1 + R1 before the tile, zero outside
@@ -561,10 +574,7 @@ __device__ __forceinline__ double theta_group(const float *img, int ci, int cj, 
               + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
     double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
               + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
-    const double m = sqrt(tx * tx + ty * ty);
-    tx /= m;
-    ty /= m;
-    return atan2(tx, -ty);
+    return (double)to_angle(tx, ty);
 }
 
 // Image staging of a tile (the tile + 1 + R1 halo; zero outside the image,
@@ -1061,10 +1071,7 @@ __global__ __launch_bounds__(NT) void toed_qad_kernel(const ImageJob *jobs, int 
                       + 2 * fx * fy * fxxy + fy * fy * fxyy + fx * fx * fxxx;
             double ty = fx * (2 * fxx * fxy + 2 * fyy * fxy) + fy * (2 * fyy * fyy + 2 * fxy * fxy)
                       + 2 * fx * fy * fxyy + fx * fx * fxxy + fy * fy * fyyy;
-            const double m = sqrt(tx * tx + ty * ty);
-            tx /= m;
-            ty /= m;
-            stg.rec[base + e].z = (float)atan2(tx, -ty);
+            stg.rec[base + e].z = to_angle(tx, ty);
         }
         // the next edgel's sums overwrite this one's only after they were read
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1193,26 +1200,27 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
     {
         const int nri = TH + 2 + 2 * R1, nci = TW + 2 + 2 * R1;
         const int gi0 = tr * TH - 1 - R1, gj0 = tc * TW - 1 - R1;
-        const int col = threadIdx.x & 63, r0 = threadIdx.x >> 6;      // 4 rows per pass, a lane per column
+        constexpr int RPP = FIN_NT / 64;                              // rows per pass, a lane per column
+        const int col = threadIdx.x & 63, r0 = threadIdx.x >> 6;
         const int gj = gj0 + col;
         const bool colok = col < nci && gj >= 0 && gj < J.W;
         const int gjc = min(max(gj, 0), J.W - 1);
-        constexpr int NPASS = (FROWS + 3) / 4;
+        constexpr int NPASS = (FROWS + RPP - 1) / RPP;
         float vals[NPASS];      // every row's load issued before the first store
         if (J.in_type == TOED_U8) {
             const uint8_t *src = (const uint8_t *)J.img + gjc;
 #pragma unroll
             for (int u = 0; u < NPASS; ++u)
-                vals[u] = (float)src[(long long)min(max(gi0 + r0 + 4 * u, 0), J.H - 1) * J.pitch];
+                vals[u] = (float)src[(long long)min(max(gi0 + r0 + RPP * u, 0), J.H - 1) * J.pitch];
         } else {
             const float *src = (const float *)J.img + gjc;
 #pragma unroll
-            for (int u = 0; u < NPASS; ++u) vals[u] = src[(long long)min(max(gi0 + r0 + 4 * u, 0), J.H - 1) * J.pitch];
+            for (int u = 0; u < NPASS; ++u) vals[u] = src[(long long)min(max(gi0 + r0 + RPP * u, 0), J.H - 1) * J.pitch];
         }
         if (col < nci) {
 #pragma unroll
             for (int u = 0; u < NPASS; ++u) {
-                const int rr = r0 + 4 * u, gi = gi0 + rr;
+                const int rr = r0 + RPP * u, gi = gi0 + rr;
                 if (rr < nri) fimg[rr * FLDI + col] = (colok && gi >= 0 && gi < J.H) ? vals[u] : 0.f;
             }
         }
