@@ -201,3 +201,19 @@ def test_subpix_class_mirror_interp(pkg, detector, oracle, scene):
     np.testing.assert_allclose(out[:n, :2], ref["edges"][:, :2], atol=1e-3)
     np.testing.assert_allclose(c.TO_edgemap, ref["TO_E"], rtol=1e-4, atol=1e-7)
     c.close()
+
+
+def test_subpix_interp_large_map(pkg, detector, oracle, scene):
+    """interp_n = 1 on a 768 x 1024 map (work grid 1535 x 2047, 3.1 M work pixels): identical
+    edgel set to the oracle on the same float work map, TO maps of the map size, and
+    repeated runs bit-identical."""
+    E = scene.bezier_map(77, 768, 1024, 24)
+    print("768 x 1024, interp_n 1", flush=True)
+    b = _run(pkg, detector, [E], None, 0.1, 2.0, 2.0, with_maps=True, interp_n=1)
+    ref = oracle.subpix(E.astype(np.float64), None, 0.1, 2.0, 2.0, nthreads=8, maps=True, interp_n=1, work_f32=True)
+    _check_interp(ref, b, 0, E.shape)
+    assert ref["n"] > 20000
+    e1 = b.results()[0].clone()
+    b.run()
+    assert torch.equal(e1, b.results()[0])
+    b.close()

@@ -166,22 +166,25 @@ __device__ __forceinline__ bool edgel_adjoint(int R, int C, float4 up, int l,
     int nbr[4][2];
     octant_neighbours(oc, nbr);
 
-    // ---- 1b. Ix, Iy at the neighbours (+ sigma-derivatives), one pass over
-    // the patch columns: a lane row's image value serves all four neighbours
+    // ---- 1b. Ix, Iy at the neighbours (+ sigma-derivatives): two passes over
+    // the patch columns, two neighbours each (a lane row's image value serves
+    // both), so that the row sums of only two neighbours are live at a time
     double nv[4][2], ns[4][2];
-    {
-        int qb[4], vxn[4];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
+    for (int mp = 0; mp < 4; mp += 2) {
+        int qb[2], vxn[2];
+#pragma unroll
+        for (int mm = 0; mm < 2; ++mm) {
+            const int m = mp + mm;
             const int Cn = C + nbr[m][1], jn = Cn >> 1;
-            vxn[m] = (Cn & 1) ? 2 : (((R + nbr[m][0]) & 1) == 0 ? 1 : 0);
-            qb[m] = jn - pc0 + R1 + 1;           // - c: tap column index q = jn - image col
+            vxn[mm] = (Cn & 1) ? 2 : (((R + nbr[m][0]) & 1) == 0 ? 1 : 0);
+            qb[mm] = jn - pc0 + R1 + 1;          // - c: tap column index q = jn - image col
         }
-        double h0[4][NSR], h1[4][NSR], hs0[4][NSR], hs1[4][NSR];
+        double h0[2][NSR], h1[2][NSR], hs0[2][NSR], hs1[2][NSR];
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+        for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
-            for (int rr = 0; rr < NSR; ++rr) { h0[m][rr] = 0; h1[m][rr] = 0; hs0[m][rr] = 0; hs1[m][rr] = 0; }
+            for (int rr = 0; rr < NSR; ++rr) { h0[mm][rr] = 0; h1[mm][rr] = 0; hs0[mm][rr] = 0; hs1[mm][rr] = 0; }
 #pragma unroll 2
         for (int c = 0; c < P; ++c) {
             double px[NSR];
@@ -191,20 +194,21 @@ __device__ __forceinline__ bool edgel_adjoint(int R, int C, float4 up, int l,
                 px[rr] = r < P ? img(pr0 + r, pc0 + c) : 0.0;
             }
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const double *xt = stp[vxn[m]][qb[m] - c], *xs = sdp[vxn[m]][qb[m] - c];
+            for (int mm = 0; mm < 2; ++mm) {
+                const double *xt = stp[vxn[mm]][qb[mm] - c], *xs = sdp[vxn[mm]][qb[mm] - c];
                 const double x0 = xt[0], x1 = xt[1], xs0 = xs[0], xs1 = xs[1];
 #pragma unroll
                 for (int rr = 0; rr < NSR; ++rr) {
-                    h0[m][rr] = fma(px[rr], x0, h0[m][rr]);
-                    h1[m][rr] = fma(px[rr], x1, h1[m][rr]);
-                    hs0[m][rr] = fma(px[rr], xs0, hs0[m][rr]);
-                    hs1[m][rr] = fma(px[rr], xs1, hs1[m][rr]);
+                    h0[mm][rr] = fma(px[rr], x0, h0[mm][rr]);
+                    h1[mm][rr] = fma(px[rr], x1, h1[mm][rr]);
+                    hs0[mm][rr] = fma(px[rr], xs0, hs0[mm][rr]);
+                    hs1[mm][rr] = fma(px[rr], xs1, hs1[mm][rr]);
                 }
             }
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
+        for (int mm = 0; mm < 2; ++mm) {
+            const int m = mp + mm;
             const int Rn = R + nbr[m][0], in = Rn >> 1;
             const int vy = (Rn & 1) ? 2 : (((C + nbr[m][1]) & 1) == 0 ? 1 : 0);
             double ix = 0, iy = 0, ixs = 0, iys = 0;
@@ -214,10 +218,10 @@ __device__ __forceinline__ bool edgel_adjoint(int R, int C, float4 up, int l,
                 if (r >= P) continue;
                 const double *yt = stp[vy][in - (pr0 + r) + R1 + 1], *ys = sdp[vy][in - (pr0 + r) + R1 + 1];
                 const double y0 = yt[0], y1 = yt[1], ys0 = ys[0], ys1 = ys[1];
-                ix = fma(h1[m][rr], y0, ix);
-                iy = fma(h0[m][rr], y1, iy);
-                ixs = fma(hs1[m][rr], y0, fma(h1[m][rr], ys0, ixs));
-                iys = fma(hs0[m][rr], y1, fma(h0[m][rr], ys1, iys));
+                ix = fma(h1[mm][rr], y0, ix);
+                iy = fma(h0[mm][rr], y1, iy);
+                ixs = fma(hs1[mm][rr], y0, fma(h1[mm][rr], ys0, ixs));
+                iys = fma(hs0[mm][rr], y1, fma(h0[mm][rr], ys1, iys));
             }
             nv[m][0] = toeddpp::group_sum<GL>(ix); nv[m][1] = toeddpp::group_sum<GL>(iy);
             ns[m][0] = toeddpp::group_sum<GL>(ixs); ns[m][1] = toeddpp::group_sum<GL>(iys);
