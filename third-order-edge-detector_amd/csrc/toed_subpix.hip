@@ -332,7 +332,7 @@ __global__ __launch_bounds__(NT) void sp_orient_nms_kernel(const SpJob *jobs, in
                         if (fabs(ss) <= sqrt(2.0)) {
                             J.pos[o] = make_double2(gj + ss * nx, gi + ss * ny);
                             if (J.dirs) J.dirs[o] = make_double2(dx, dy);
-                            J.val[o] = make_float2((float)fabs(mf), (float)atan2(dx, -dy));
+                            J.val[o] = make_float2((float)fabs(mf), atan2f((float)dx, (float)-dy));   // unit direction: FP32 angle
                             surv = true;
                         }
                     }
@@ -733,9 +733,12 @@ __global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, in
             const double Fm = sqrt(Fx * Fx + Fy * Fy);
             float ori = vv.y;               // atan2(dirx, -diry) from the orientation pass
             if (!(Fm < 1e-12)) {
-                Fx /= Fm;
-                Fy /= Fm;
-                ori = (float)atan2(Fx, -Fy);
+                // atan2 of F / |F| (:686-690): a positive scale leaves the angle
+                // unchanged, so F is scaled exactly by a power of two and the angle
+                // taken in FP32, the precision it is stored in
+                int ex;
+                (void)frexp(fmax(fabs(Fx), fabs(Fy)), &ex);
+                ori = atan2f((float)ldexp(Fx, -ex), (float)ldexp(-Fy, -ex));
                 J.val[o].y = ori;
             }
             if (rank < J.capacity) {
