@@ -133,6 +133,43 @@ def test_full_batch_64x512_tile_and_index_agree(pkg, detector, scene):
     assert bool((g_tile.abs().amax(dim=(1, 2)) > 0).all())
 
 
+def test_full_batch_64x512_against_oracle(pkg, detector, scene, oracle):
+    """The differentiable workload at its full size (64 scenes of 512 x 512 f32,
+    sigma 1.5, the bench's loss weights): images 0 and 37 of the batch have the
+    oracle's edgel set, their dL/dimage equals the FP64 oracle gradient of the
+    frozen loss (an independent route: local central differences and transposed
+    filters) to 1e-4 of its largest entry, and central differences of the frozen
+    loss agree on 16 pixels of image 0."""
+    imgs = [scene.shapes_scene(9000 + b, 512, 512) for b in range(64)]
+    w = scene.loss_weights(12345)
+    print("64 x 512 x 512 forward + backward", flush=True)
+    edges, off, g, _ = _forward_backward(pkg, imgs, 1.5, w)
+    assert off[-1] > 500000
+    for b in (0, 37):
+        im = imgs[b].astype(np.float64)
+        det = oracle.detect(im, 1.5)
+        x = edges[off[b]:off[b + 1]]
+        eidx = det["eidx"]
+        assert len(eidx) == len(x) > 5000 and np.abs(det["edges"][:, :2] - x[:, :2]).max() < 1e-3
+        L, oc = oracle.frozen_loss(im, 1.5, eidx, w)
+        g_ref, _ = oracle.frozen_grad(im, 1.5, eidx, w, oc)
+        scale = np.abs(g_ref).max()
+        err = np.abs(g[b] - g_ref).max() / scale
+        print("image %d: %d edgels, max |g - g_oracle| / max|g| = %.2e" % (b, len(eidx), err), flush=True)
+        assert err < 1e-4
+        if b == 0:
+            cand = np.flatnonzero(np.abs(g_ref) > 1e-2 * scale)
+            h = 1e-2
+            for k in np.random.default_rng(3).choice(cand, size=16, replace=False):
+                i, j = divmod(int(k), im.shape[1])
+                p, m = im.copy(), im.copy()
+                p[i, j] += h
+                m[i, j] -= h
+                fd = (oracle.frozen_loss(p, 1.5, eidx, w, octants=oc)[0] -
+                      oracle.frozen_loss(m, 1.5, eidx, w, octants=oc)[0]) / (2 * h)
+                assert abs(fd - g[b][i, j]) <= 1e-4 * scale + 1e-4 * abs(fd), (i, j, fd, g[b][i, j])
+
+
 def test_new_tensor_each_step_reuses_the_plan(pkg, detector, scene):
     """A training loop passes a new input tensor every step: the cached plan
     (keyed by shape, strides and parameters, not by address) is re-pointed at
