@@ -437,6 +437,18 @@ def main():
                            "kernel": "toed_tiles_kernel<uint8_t,16>",
                            "algorithmic": "2*(10*W1+2*W0) flop/px = %d flop/px at sigma 2, %d px/launch"
                                           % (dense_flops_per_px(2.0, pkg), px)}
+        # the whole frame against the same roof, with the per-edgel term of the
+        # minimal useful work (the seven higher derivatives at each edgel,
+        # 4*W1^2 + 7*W1 FMAs, which the finalize kernel evaluates)
+        r0, r1 = pkg.sigma_radius(2.0)
+        w1 = 2 * r1 + 1
+        fflops = flops + 2.0 * (4 * w1 * w1 + 7 * w1) * plain["n"]
+        fach = fflops / plain["sec"] / 1e12
+        out["roofline_frame"] = {"achieved": round(fach, 2), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": round(fach / PEAK_FP32_TFLOPS, 4),
+                                 "algorithmic": "2*(10*W1+2*W0) flop/px + 2*(4*W1^2+7*W1) flop/edgel = %.3f GFLOP "
+                                                "per frame (%d edgels), whole launch sequence" % (fflops / 1e9,
+                                                                                                 plain["n"])}
         hb = (px * 1 + 16 * plain["n"] + 4) / plain["sec"] / 1e9
         out["hbm"] = {"achieved": round(hb, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(hb / PEAK_HBM_GBS, 5),
                       "algorithmic_bytes": "1 B/px in + 16 B/edgel + 4 B count"}

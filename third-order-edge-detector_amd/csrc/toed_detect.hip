@@ -291,16 +291,17 @@ __device__ __forceinline__ void vpass(float *lds, const TOED_CONST float *t0, co
 // current planes: Ix = sum_x P0 * X1, Iy = sum_x P1 * X0 (x taps X0, X1) at
 // interp row 2v - 1 + A, column 2u - 1 + B, for the plane rows v = 1 - A ..
 // TH + 1 - A and columns u = 1 - B .. TW + 1 - B: exactly the results inside
-// the interp planes.
+// the interp planes.  Returns the largest Ix^2 + Iy^2 among the thread's
+// results (0 for a thread without an item).
 template <int R1, int TH, int R, int A, int B, int EB>
-__device__ __forceinline__ void hpass(float *lds, const TOED_CONST float *x0, const TOED_CONST float *x1) {
+__device__ __forceinline__ float hpass(float *lds, const TOED_CONST float *x0, const TOED_CONST float *x1) {
     using Gm = Geo<R1, TH, EB>;
     float *RX = lds + Gm::OFF_IX;
     float *RY = lds + Gm::OFF_IY;
     constexpr int KH = Gm::KH;
     constexpr int NW = KH + 2 * R1;
     const int it = threadIdx.x;
-    if (it >= Gm::HR * Gm::NBH) return;
+    if (it >= Gm::HR * Gm::NBH) return 0.f;
     // column blocks fastest: 11 lanes per plane row
     const int v = (1 - A) + it / Gm::NBH;
     const int u0 = (1 - B) + (it % Gm::NBH) * KH;
@@ -318,6 +319,7 @@ __device__ __forceinline__ void hpass(float *lds, const TOED_CONST float *x0, co
         w1[t] = p1[t];
     }
     const int rr = 2 * v - 1 + A;
+    float mag2 = 0.f;
 #pragma unroll
     for (int d = 0; d < KH; ++d) {
         float ix = w0[d + R1 + R] * x1[R1 - R], iy = w1[d + R1 + R] * x0[R1 - R];
@@ -329,7 +331,9 @@ __device__ __forceinline__ void hpass(float *lds, const TOED_CONST float *x0, co
         const int cc = 2 * (u0 + d) - 1 + B;
         RX[rr * Gm::LDR + cc] = ix;
         RY[rr * Gm::LDR + cc] = iy;
+        mag2 = fmaxf(mag2, ix * ix + iy * iy);
     }
+    return mag2;
 }
 
 // Octant choice and neighbour interpolation of cpu_toed.cpp:415-462 for a pixel
@@ -701,19 +705,32 @@ __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const Imag
     // round A: R0 window, unshifted -> phase (0,0)
     vpass<R1, TH, 0, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1 - 1, 0, 0, EB>(lds, tp->fu[0], tp->fu[1]);
+    float mag2 = hpass<R1, TH, R1 - 1, 0, 0, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
     // round B: R1 window, unshifted in y -> phase (0,1) with shifted x taps
     vpass<R1, TH, 1, EB>(lds, tp->fu[0], tp->fu[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 0, 1, EB>(lds, tp->fs[0], tp->fs[1]);
+    mag2 = fmaxf(mag2, hpass<R1, TH, R1, 0, 1, EB>(lds, tp->fs[0], tp->fs[1]));
     __syncthreads();
     // round C: shifted in y -> phases (1,0) and (1,1), one x family at a time
     vpass<R1, TH, 2, EB>(lds, tp->fs[0], tp->fs[1]);
     __syncthreads();
-    hpass<R1, TH, R1, 1, 0, EB>(lds, tp->fu[0], tp->fu[1]);
-    hpass<R1, TH, R1, 1, 1, EB>(lds, tp->fs[0], tp->fs[1]);
-    __syncthreads();
+    mag2 = fmaxf(mag2, hpass<R1, TH, R1, 1, 0, EB>(lds, tp->fu[0], tp->fu[1]));
+    mag2 = fmaxf(mag2, hpass<R1, TH, R1, 1, 1, EB>(lds, tp->fs[0], tp->fs[1]));
+    // A tile whose every Ix^2 + Iy^2 (halo included) is at most
+    // thresh^2 (1 - 2^-20) has no NMS candidate: the test there is
+    // sqrt(Ix^2 + Iy^2) > thresh with the hardware square root (1 ulp), which
+    // that margin keeps below thresh.  Such a tile writes zero counts and
+    // leaves; the barrier that ends the passes takes the vote.
+    const float quiet = J.thresh * J.thresh * (1.0f - 0x1p-20f);
+    if (!__syncthreads_or(!(J.thresh > 0.f) || mag2 > quiet)) {
+        for (int r = threadIdx.x; r < 2 * TH; r += NT) {
+            const int R = 2 * tr * TH + r;
+            if (R < 2 * J.H) counts[J.counts_base + (long long)R * J.ntc + tc] = 0;
+        }
+        if (threadIdx.x < 4) tile_cnt[(long long)tile * 4 + threadIdx.x] = 0;
+        return;
+    }
     // the vertical-pass planes are dead now: the survivor queue lives in their place
     int *queue = (int *)(lds + Gm::OFF_P0);
     nms_tile<R1, TH, EB>(lds, J, tr, tc, tile_local, counts, stg, sh_cnt, (int *)(lds + Gm::OFF_CAND), queue, rowmask);
