@@ -23,15 +23,15 @@
 //        64-wide interp row at a time, decides near-peaks and the parabola in
 //        FP64, and marks survivors in per-row masks: per-(row, tile) counts go to
 //        HBM, records (position, strength) and raster keys to a per-tile,
-//        per-phase staging area.
+//        per-phase staging area; a tile with edgels appends itself to a list.
 //   K2 toed_scan_chunks / toed_scan_images   exclusive scan of the counts
 //        (row-major over (interp row, tile column)), segmented per image.
 //   K2b toed_qad_kernel   orientation mode 1 only: the orientation evaluated at
 //        each edgel's subpixel location instead (one wavefront per edgel).
-//   K3 toed_finalize_kernel   one workgroup per tile: the nine responses of each
-//        staged edgel in FP64 from the tile's image region (8 lanes per edgel),
-//        its third-order orientation, and the edgel to offset[row, tile] + rank
-//        (raster order).
+//   K3 toed_finalize_kernel   a fixed grid walking the list of tiles with
+//        edgels: the nine responses of each staged edgel in FP64 from the
+//        tile's image region (8 lanes per edgel), its third-order orientation,
+//        and the edgel to offset[row, tile] + rank (raster order).
 // No MFMA: the filters are short separable stencils, not a contraction.
 
 #include <hip/hip_runtime.h>
@@ -695,7 +695,7 @@ template <int R1, int TH, int EB> __device__ __forceinline__ void tile_setup(flo
 // the counters of the tile are in LDS (a barrier precedes the call).
 template <typename IN_T, int R1, int TH>
 __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const ImageJob &J, int tile,
-                          int *__restrict__ counts, StageBuf stg, int *__restrict__ tile_cnt) {
+                          int *__restrict__ counts, StageBuf stg, int *__restrict__ tile_cnt, int *tlist) {
     constexpr int EB = stage_bytes<IN_T>();
     using Gm = Geo<R1, TH, EB>;
     const long long tile_local = tile - J.tile_begin;
@@ -751,6 +751,9 @@ __device__ __forceinline__ void tile_compute(float *lds, int *sh_cnt, const Imag
         tile_cnt[(long long)tile * 4 + threadIdx.x] = n;
         if (n > J.stage_cap && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_STAGING);
     }
+    // a tile with edgels joins the list the finalize kernel walks
+    // (tlist[0] entries from tlist[1], in no particular order)
+    if (threadIdx.x == 0 && (sh_cnt[0] | sh_cnt[1] | sh_cnt[2] | sh_cnt[3])) tlist[1 + atomicAdd(tlist, 1)] = tile;
 }
 
 // Taps of one sigma passed by value (the forward detection's part of a
@@ -801,9 +804,11 @@ __global__ __launch_bounds__(64) void toed_pack_taps_kernel(TapSet *dst, const T
     }
 }
 
-// K0: per-image flag reset (a kernel, not a memset: safe inside captured graphs)
-__global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0) {
+// K0: per-image flag reset and the count of the list of tiles with edgels
+// (a kernel, not a memset: safe inside captured graphs)
+__global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0, int *tlist) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0) *tlist = 0;
     if (m >= njobs) return;
     const ImageJob &J = jobs ? jobs[m] : job0;
     if (J.flags_out) *J.flags_out = 0;
@@ -817,8 +822,8 @@ __global__ void toed_reset_kernel(const ImageJob *jobs, int njobs, ImageJob job0
 // tiles; four for 16-row tiles of f32 images at sigma 2 (LDS).
 template <typename IN_T, int TH, int R1>
 __global__ __launch_bounds__(NT, (TH == 8 || stage_bytes<IN_T>() == 1) ? 5 : 4) void toed_tiles_kernel(
-    const ImageJob *jobs, int njobs, ImageJob job0, int *counts, StageBuf staging, int *tile_cnt, int tile_begin,
-    int tile_end, int *ticket) {
+    const ImageJob *jobs, int njobs, ImageJob job0, int *counts, StageBuf staging, int *tile_cnt, int *tlist,
+    int tile_begin, int tile_end, int *ticket) {
     constexpr int EB = stage_bytes<IN_T>();
     __shared__ float lds[Geo<R1, TH, EB>::TOTAL];
     __shared__ int sh_cnt[6];
@@ -829,7 +834,7 @@ __global__ __launch_bounds__(NT, (TH == 8 || stage_bytes<IN_T>() == 1) ? 5 : 4) 
         tile_setup<R1, TH, EB>(lds, sh_cnt);
         stage_tile<IN_T, R1, TH>(lds, J, tile);
         __syncthreads();
-        tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt, tlist);
         return;
     }
     for (;;) {
@@ -842,7 +847,7 @@ __global__ __launch_bounds__(NT, (TH == 8 || stage_bytes<IN_T>() == 1) ? 5 : 4) 
         tile_setup<R1, TH, EB>(lds, sh_cnt);
         stage_tile<IN_T, R1, TH>(lds, J, tile);
         __syncthreads();
-        tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt);
+        tile_compute<IN_T, R1, TH>(lds, sh_cnt, J, tile, counts, staging, tile_cnt, tlist);
         __syncthreads();
     }
 }
@@ -1147,26 +1152,20 @@ __global__ __launch_bounds__(1024) void toed_compact_base_kernel(const int *coun
     }
 }
 
-// One workgroup of four wavefronts per tile: a tile without edgels ends at
-// once; a tile's edgels are dealt to its 32 groups of 8 lanes, so the tiles
-// with many edgels (along the scene's contours) finish in few rounds.
+// A grid of at most FIN_GRID workgroups of four wavefronts walks the list of
+// tiles with edgels that the tile kernel built (tiles without edgels cost
+// nothing here); a tile's edgels are dealt to the workgroup's 32 groups of 8
+// lanes, so the tiles with many edgels (along the scene's contours) finish in
+// few rounds.
 constexpr int FIN_NT = 256;
 constexpr int FIN_GT = 8;                  // lanes per edgel
 constexpr int FIN_G = FIN_NT / FIN_GT;     // lane groups per workgroup
+constexpr int FIN_GRID = 2048;
 
-__global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0, int TH,
-                                                               const int *offsets, const int *chunk_pref,
-                                                               StageBuf stg, const int *tile_cnt, int ntiles,
-                                                               int *ticket) {
-    __shared__ double tapd[3][MAXTAPS][4];
-    __shared__ float fimg[FROWS * FLDI];
-    __shared__ float4 s_rec[FIN_NT];       // a chunk of staged edgels: record,
-    __shared__ long long s_idx[FIN_NT];    // output row (-1: none),
-    __shared__ int s_rc[FIN_NT];           // tile-local interp row | col << 8
-    // the persistent tile kernels' tickets are re-armed once every tile is final
-    if (ticket && blockIdx.x == gridDim.x - 1 && threadIdx.x < NGROUPS) ticket[threadIdx.x] = 0;
-    const int tile = blockIdx.x;
-    if (tile >= ntiles) return;
+__device__ __forceinline__ void finalize_tile(const ImageJob *jobs, int njobs, const ImageJob &job0, int TH,
+                                              const int *offsets, const int *chunk_pref, StageBuf stg,
+                                              const int *tile_cnt, int tile, double (*tapd)[MAXTAPS][4],
+                                              float *fimg, float4 *s_rec, long long *s_idx, int *s_rc) {
     const ImageJob J = load_job(jobs, njobs, job0, tile);
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
@@ -1175,7 +1174,7 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
 #pragma unroll
     for (int ph = 0; ph < 4; ++ph) nph[ph] = min(tile_cnt[(long long)tile * 4 + ph], J.stage_cap);
     const int total = nph[0] + nph[1] + nph[2] + nph[3];
-    if (total == 0) return;
+    if (total == 0) return;    // the same for the whole workgroup
     if (J.qad) {
         // orientation mode 1: toed_qad_kernel has set it; scatter only
         for (int e = threadIdx.x; e < total; e += FIN_NT) {
@@ -1282,8 +1281,25 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
                 }
             }
         }
-        __syncthreads();   // the next chunk overwrites this chunk's slots only after they were read
+        __syncthreads();   // the next chunk (or tile) overwrites this chunk's slots only after they were read
     }
+}
+
+__global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *jobs, int njobs, ImageJob job0, int TH,
+                                                               const int *offsets, const int *chunk_pref,
+                                                               StageBuf stg, const int *tile_cnt, const int *tlist,
+                                                               int *ticket) {
+    __shared__ double tapd[3][MAXTAPS][4];
+    __shared__ float fimg[FROWS * FLDI];
+    __shared__ float4 s_rec[FIN_NT];       // a chunk of staged edgels: record,
+    __shared__ long long s_idx[FIN_NT];    // output row (-1: none),
+    __shared__ int s_rc[FIN_NT];           // tile-local interp row | col << 8
+    // the persistent tile kernels' tickets are re-armed once every tile is final
+    if (ticket && blockIdx.x == gridDim.x - 1 && threadIdx.x < NGROUPS) ticket[threadIdx.x] = 0;
+    const int nlist = tlist[0];
+    for (int k = blockIdx.x; k < nlist; k += gridDim.x)
+        finalize_tile(jobs, njobs, job0, TH, offsets, chunk_pref, stg, tile_cnt, tlist[1 + k], tapd, fimg, s_rec,
+                      s_idx, s_rc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1887,7 +1903,8 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
                    bool qad, const int *ccounts = nullptr, int *cflags = nullptr, long long ccap = 0,
                    long long *cbase = nullptr) {
     if (ntiles <= 0) return TOED_OK;
-    hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0);
+    int *tlist = tilecnt + 4 * ntiles;      // count, then the tiles with edgels
+    hipLaunchKernelGGL(toed_reset_kernel, dim3(ceil_div(njobs, 256)), dim3(256), 0, s, d_jobs, njobs, job0, tlist);
     if (ev) HIPCHK(hipEventRecord(ev[0], s));
     for (int gi = 0; gi < ngroups; ++gi) {
         const TileGroup &g = groups[gi];
@@ -1900,10 +1917,10 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
                 constexpr int R = decltype(Rc)::value;
                 if (in_type == TOED_U8)
                     hipLaunchKernelGGL((toed_tiles_kernel<uint8_t, T, R>), grid, dim3(NT), 0, s, d_jobs, njobs, job0,
-                                       counts, staging, tilecnt, g.tile_begin, g.tile_end, ticket);
+                                       counts, staging, tilecnt, tlist, g.tile_begin, g.tile_end, ticket);
                 else
                     hipLaunchKernelGGL((toed_tiles_kernel<float, T, R>), grid, dim3(NT), 0, s, d_jobs, njobs, job0,
-                                       counts, staging, tilecnt, g.tile_begin, g.tile_end, ticket);
+                                       counts, staging, tilecnt, tlist, g.tile_begin, g.tile_end, ticket);
             });
         });
     }
@@ -1916,9 +1933,9 @@ static int enqueue(hipStream_t s, int in_type, int TH, const ImageJob *d_jobs, i
     hipLaunchKernelGGL(toed_scan_images, dim3(njobs), dim3(1024), 0, s, d_jobs, njobs, job0, chunks);
     if (cbase)
         hipLaunchKernelGGL(toed_compact_base_kernel, dim3(1), dim3(1024), 0, s, ccounts, cflags, njobs, ccap, cbase);
-    // finalize: one workgroup per tile
-    hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)ntiles), dim3(FIN_NT), 0, s, d_jobs,
-                       njobs, job0, TH, offsets, chunks, staging, tilecnt, (int)ntiles, tickets);
+    // finalize: a grid that walks the tiles with edgels
+    hipLaunchKernelGGL(toed_finalize_kernel, dim3((unsigned)std::min<long long>(ntiles, FIN_GRID)), dim3(FIN_NT), 0, s,
+                       d_jobs, njobs, job0, TH, offsets, chunks, staging, tilecnt, tlist, tickets);
     if (ev) HIPCHK(hipEventRecord(ev[2], s));
     HIPCHK(hipGetLastError());
     return TOED_OK;
@@ -2050,7 +2067,7 @@ static int detect_impl(toed_ctx *c, const toed_image_t *im, int *d_count, int *d
     if ((st = grow(&c->d_counts, &c->n_counts, (size_t)ncounts))) return st;
     if ((st = grow(&c->d_offsets, &c->n_offsets, (size_t)ncounts))) return st;
     if ((st = grow(&c->d_chunks, &c->n_chunks, (size_t)J.nchunks))) return st;
-    if ((st = grow(&c->d_tilecnt, &c->n_tiles, (size_t)ntiles * 4))) return st;
+    if ((st = grow(&c->d_tilecnt, &c->n_tiles, (size_t)ntiles * 5 + 1))) return st;    // + the tile list
     if ((st = grow(&c->d_srec, &c->n_srec, (size_t)ntiles * 4 * J.stage_cap))) return st;
     if ((st = grow(&c->d_skey, &c->n_skey, (size_t)ntiles * 4 * J.stage_cap))) return st;
     J.counts_base = 0; J.chunk_base = 0; J.stage_base = 0; J.tile_begin = 0;
@@ -2113,7 +2130,7 @@ int toed_detect_noalloc(toed_ctx *c, const toed_image_t *im, int *d_count, int *
     const size_t o_offsets = o_counts + up(sizeof(int) * ncounts);
     const size_t o_chunks = o_offsets + up(sizeof(int) * ncounts);
     const size_t o_tilecnt = o_chunks + up(sizeof(int) * J.nchunks);
-    const size_t o_srec = o_tilecnt + up(sizeof(int) * ntiles * 4);
+    const size_t o_srec = o_tilecnt + up(sizeof(int) * (ntiles * 5 + 1));   // per-tile counts, the tile list
     const size_t o_skey = o_srec + up(sizeof(float4) * ntiles * 4 * J.stage_cap);
     const size_t end = o_skey + up(sizeof(uint32_t) * ntiles * 4 * J.stage_cap);
     if (end > NOALLOC_BYTES) {
@@ -2208,7 +2225,7 @@ int toed_batch_plan(toed_ctx *c, int n, const toed_image_t *imgs, toed_batch **o
               hipMalloc(&b->d_counts, sizeof(int) * b->ncounts) == hipSuccess &&
               hipMalloc(&b->d_offsets, sizeof(int) * b->ncounts) == hipSuccess &&
               hipMalloc(&b->d_chunks, sizeof(int) * std::max<long long>(chunks, 1)) == hipSuccess &&
-              hipMalloc(&b->d_tilecnt, sizeof(int) * tiles * 4) == hipSuccess &&
+              hipMalloc(&b->d_tilecnt, sizeof(int) * (tiles * 5 + 1)) == hipSuccess &&   // + the tile list
               hipMalloc(&b->d_ticket, sizeof(int) * NGROUPS) == hipSuccess &&
               hipMalloc(&b->d_srec, sizeof(float4) * std::max<long long>(b->nstaging, 1)) == hipSuccess &&
               hipMalloc(&b->d_skey, sizeof(uint32_t) * std::max<long long>(b->nstaging, 1)) == hipSuccess;
