@@ -83,6 +83,7 @@ struct SpJob {
     int nw, ntx, nty, tile_begin;
     int chunk_begin, nchunks, tpitch, pad;
     uint32_t *wpre;             // H x nw: rank of each kept word's first edgel
+    int *tlist;                 // tiles with kept edgels: their number, then their indices (any order)
     // differentiable form (null unless enabled on the plan)
     float4 *tq;                 // H x W: dirx, diry, dOxx/dtheta, dOyy/dtheta coefficients
     double2 *dirs;              // H x W: direction of survivors
@@ -569,6 +570,7 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
             if (sv[u]) posw[threadIdx.x + u * NT] = pv[u];
     }
     __syncthreads();
+    bool any_kept = false;
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
         const int p = threadIdx.x + k * NT;
@@ -600,12 +602,15 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
                 }
             }
         }
+        any_kept = any_kept || kp;
         const unsigned long long b = __ballot(kp);
         if ((lane & 31) == 0) {
             const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
             if (row < H) J.kmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
         }
     }
+    // a tile with kept edgels joins the list the correction and backward walk
+    if (__syncthreads_or(any_kept) && threadIdx.x == 0) J.tlist[1 + atomicAdd(J.tlist, 1)] = lt;
 }
 
 // B2: for tiles with kept edgels: the eight planes of the tile + HALO in LDS
@@ -613,7 +618,7 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
 // Three workgroups per CU: E, then Hx and Hy, share the input planes (LDS
 // 51.8 KB) and the registers are capped to fit (a couple of spills).
 template <int R>
-__global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+__device__ __forceinline__ void correct_tile(const SpJob &J, const SpTaps &tp, int lt) {
     using G = CGeo<R>;
     constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC;
     __shared__ float lds[G::FLOATS];
@@ -622,8 +627,6 @@ __global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, in
     __shared__ short elist[TH * TW];
     float *In = lds, *X = lds + G::IN, *Y = lds + G::IN + G::XN;
 
-    const SpJob J = load_job(jobs);
-    const int lt = blockIdx.x;
     if (lt >= J.ntx * J.nty) return;
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
@@ -776,6 +779,19 @@ __global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, in
                     if ((k >> 1) == sa) dst[k] = make_float4(r[4 * k], r[4 * k + 1], r[4 * k + 2], r[4 * k + 3]);
             }
         }
+    }
+}
+
+// A grid of (workgroups, maps) walks each map's list of tiles with kept edgels
+// (built by sp_isolate_kernel); the barrier between tiles keeps a tile's LDS
+// until every lane is done with it.
+template <int R>
+__global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+    const SpJob J = load_job(jobs);
+    const int nl = *J.tlist;
+    for (int k = blockIdx.x; k < nl; k += gridDim.x) {
+        if (k != (int)blockIdx.x) __syncthreads();
+        correct_tile<R>(J, tp, J.tlist[1 + k]);
     }
 }
 
@@ -1056,7 +1072,7 @@ __global__ __launch_bounds__(NT) void sp_bw_edgel_kernel(const SpJob *jobs, cons
 // the tile + HALO + R (the raw adjoints join dL/dHx, dL/dHy there), added to
 // HBM with float atomics (periodic wrap).
 template <int R>
-__global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const SpGrad *grads, SpTaps tp) {
+__device__ __forceinline__ void bw_tile(const SpJob &J, const SpGrad &Gd, const SpTaps &tp, int lt) {
     using G = CGeo<R>;
     constexpr int PR = G::PR, PC = G::PC, IR = G::IR, IC = G::IC, PP = PR * PC;
     __shared__ float lds[BGeo<R>::FLOATS];
@@ -1070,10 +1086,7 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     constexpr int NZ = BGeo<R>::NZ;
     float *Z = lds, *Xb = lds + NZ * PP, *St = lds + NZ * PP;
 
-    const SpJob J = load_job(jobs);
-    const int lt = blockIdx.x;
     if (lt >= J.ntx * J.nty) return;
-    const SpGrad Gd = load_grad(grads);
     const int ty = lt / J.ntx, tx = lt - ty * J.ntx;
     const int y0 = ty * TH, x0 = tx * TW;
     const int H = J.H, W = J.W;
@@ -1272,6 +1285,18 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     }
 }
 
+// Walks each map's list of tiles with kept edgels, as the correction does.
+template <int R>
+__global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const SpGrad *grads, SpTaps tp) {
+    const SpJob J = load_job(jobs);
+    const SpGrad Gd = load_grad(grads);
+    const int nl = *J.tlist;
+    for (int k = blockIdx.x; k < nl; k += gridDim.x) {
+        if (k != (int)blockIdx.x) __syncthreads();
+        bw_tile<R>(J, Gd, tp, J.tlist[1 + k]);
+    }
+}
+
 // SB2: dense, per tile.  dL/dtheta at every pixel (the edgel's own NMS term
 // plus Hx = E cos(theta), Hy = E sin(theta)), dL/dOxx and dL/dOyy through
 // theta = atan(Oyy sgn / (Oxx + 1e-5)), then the transposed central
@@ -1459,6 +1484,7 @@ __global__ void sp_reset_kernel(const SpJob *jobs, int njobs) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= njobs) return;
     *jobs[m].nsurv = 0;
+    *jobs[m].tlist = 0;
     if (jobs[m].flags_out) *jobs[m].flags_out = 0;
     if (jobs[m].count_out) *jobs[m].count_out = 0;
 }
@@ -1534,8 +1560,16 @@ struct toed_subpix {
         }                                                                                                \
     } while (0)
 
+// workgroups per map of the kernels that walk the kept-tile lists: about
+// SP_WALK_WGS in all, never more than a map's tiles
+constexpr int SP_WALK_WGS = 4096;
+static int walk_grid(const toed_subpix *p) {
+    return std::max(1, std::min(p->max_tiles, (SP_WALK_WGS + p->n - 1) / p->n));
+}
+
 template <int R> static void launch_bw_tile(hipStream_t s, const toed_subpix *p) {
-    hipLaunchKernelGGL(sp_bw_tile_kernel<R>, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads, p->taps);
+    hipLaunchKernelGGL(sp_bw_tile_kernel<R>, dim3(walk_grid(p), p->n), dim3(NT), 0, s, p->d_jobs, p->d_grads,
+                       p->taps);
 }
 
 static int launch_bw_tile_r(hipStream_t s, const toed_subpix *p) {
@@ -1550,7 +1584,7 @@ static int launch_bw_tile_r(hipStream_t s, const toed_subpix *p) {
 }
 
 template <int R> static void launch_correct(hipStream_t s, const toed_subpix *p) {
-    hipLaunchKernelGGL(sp_correct_kernel<R>, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n, p->taps);
+    hipLaunchKernelGGL(sp_correct_kernel<R>, dim3(walk_grid(p), p->n), dim3(NT), 0, s, p->d_jobs, p->n, p->taps);
 }
 
 static int launch_correct_r(hipStream_t s, const toed_subpix *p) {
@@ -1632,7 +1666,9 @@ int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, doub
         const size_t wh = (size_t)work(maps[m].H), ww = (size_t)work(maps[m].W);
         const size_t px = wh * ww, nw = (ww + 31) / 32;
         off[m] = total;
-        total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(wh * nw * 4) * 3;
+        const size_t ntile = nw * ((wh + TH - 1) / TH);
+        total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(wh * nw * 4) * 3 +
+                 align_up((ntile + 1) * 4);
         if (interp_n) total += align_up(px * 4) * (maps[m].d_orient ? 2 : 1);
     }
     if (hipSetDevice(p->device) != hipSuccess) { delete p; return TOED_ERR_HIP; }
@@ -1655,6 +1691,7 @@ int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, doub
         J.vmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
         J.kmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
         J.wpre = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
+        J.tlist = (int *)b; b += align_up(((size_t)J.nw * ((wh + TH - 1) / TH) + 1) * 4);
         J.nsurv = p->d_nsurv + 64 * m;
         J.src_map = M.d_map;
         J.src_orient = M.d_orient;
