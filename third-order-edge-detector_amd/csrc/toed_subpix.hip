@@ -92,7 +92,8 @@ struct SpJob {
     float4 *brec;               // 4 per edgel (by rank): the ten sample adjoints, the cubic
                                 // stencil's fraction (fx, fy) and base (int bits of xf, yf)
     int *gcount;                // kept edgels of the map (device int)
-    float *ghx, *ghy, *gth;     // H x W scratch: dL/dHx, dL/dHy, dL/dtheta at edgels
+    float *gth;                 // H x W scratch: dL/dtheta (cleared, then the edgels' own terms and
+                                // the tile kernel's E (dirx dL/dHy - diry dL/dHx))
     // work grid (interp_n = slog2 > 0, :92-105, :226-240): H x W above is the
     // work grid ((IH - 1) 2^n + 1) x ((IW - 1) 2^n + 1); map / orient point at
     // the plan's work planes, resized from the caller's IH x IW arrays src_map /
@@ -1069,8 +1070,9 @@ __global__ __launch_bounds__(NT) void sp_bw_edgel_kernel(const SpJob *jobs, cons
 // one pixel of the 4 x 4 stencil: the eight convolved-plane adjoints and the
 // raw Hx, Hy sample adjoints are scattered into Z (LDS atomics).  Then the
 // transposed circular convolutions pull Z back to dL/dE, dL/dHx, dL/dHy over
-// the tile + HALO + R (the raw adjoints join dL/dHx, dL/dHy there), added to
-// HBM with float atomics (periodic wrap).
+// the tile + HALO + R (the raw adjoints join dL/dHx, dL/dHy there); through
+// Hx = E dirx, Hy = E diry they become dL/dE and dL/dtheta at each pixel,
+// added to HBM with float atomics (periodic wrap).
 template <int R>
 __device__ __forceinline__ void bw_tile(const SpJob &J, const SpGrad &Gd, const SpTaps &tp, int lt) {
     using G = CGeo<R>;
@@ -1138,9 +1140,7 @@ __device__ __forceinline__ void bw_tile(const SpJob &J, const SpGrad &Gd, const 
                     atomicMax(&s_box[1], min(yfs + 2, H - 1) - (y0 - HALO));
                     atomicMin(&s_box[2], max(xfs - 1, 0) - (x0 - HALO));
                     atomicMax(&s_box[3], min(xfs + 2, W - 1) - (x0 - HALO));
-                } else {
-                    J.gth[(long long)gi * W + gj] = 0.f;   // not in the output: no gradient
-                }
+                }   // else not in the output: no gradient (its dL/dtheta stays cleared)
             }
             const float vb[10] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y};
 #pragma unroll
@@ -1170,7 +1170,7 @@ __device__ __forceinline__ void bw_tile(const SpJob &J, const SpGrad &Gd, const 
     }
 
     // transposed y pass (Z -> X bar), then x pass (X bar -> dL/dE, dL/dHx,
-    // dL/dHy added to HBM), per group and one X bar plane at a time: the x
+    // dL/dHy, then dL/dE and dL/dtheta to HBM), per group and one X bar plane at a time: the x
     // pass adds each plane's share into per-thread registers (the tasks of a
     // thread are the same for the three planes), so only one X bar plane is
     // held in LDS.  Output u of a 4-output block at o0 sums window[u + m] *
@@ -1191,6 +1191,7 @@ __device__ __forceinline__ void bw_tile(const SpJob &J, const SpGrad &Gd, const 
     constexpr int src[2][NSRC] = {{0, 4, 1, 3, 2}, {5, 7, 6, -1, -1}};
     constexpr int dst[2][NSRC] = {{0, 0, 1, 1, 2}, {0, 1, 2, -1, -1}};
     constexpr int fam[2][NSRC] = {{0, 1, 1, 2, 0}, {0, 1, 0, -1, -1}};
+    float ge[SLOTS][NB];      // group 0's dL/dE, added with group 1's terms at the end
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp) {
         float ex[SLOTS][2][NB];
@@ -1275,10 +1276,17 @@ __device__ __forceinline__ void bw_tile(const SpJob &J, const SpGrad &Gd, const 
                     }
                 }
                 if (grp == 0) {
-                    if (e0 != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], e0);
-                } else {
-                    if (e0 != 0.f) atomicAdd(&J.ghx[(long long)gy * W + gx], e0);
-                    if (e1 != 0.f) atomicAdd(&J.ghy[(long long)gy * W + gx], e1);
+                    ge[sl][u] = e0;
+                } else if (ge[sl][u] != 0.f || e0 != 0.f || e1 != 0.f) {
+                    // Hx = E dirx, Hy = E diry at this pixel: dL/dHx = e0 and
+                    // dL/dHy = e1 go to dL/dE (dirx e0 + diry e1) and to
+                    // dL/dtheta (E (dirx e1 - diry e0)) here, so only two
+                    // planes take atomics
+                    const float4 q = J.tq[(long long)gy * W + gx];
+                    const float ev = J.map[(long long)gy * J.pitch + gx];
+                    const float de = ge[sl][u] + q.x * e0 + q.y * e1, dt = ev * (q.x * e1 - q.y * e0);
+                    if (de != 0.f) atomicAdd(&Gd.gmap[(long long)gy * Gd.gpitch + gx], de);
+                    if (dt != 0.f) atomicAdd(&J.gth[(long long)gy * W + gx], dt);
                 }
             }
         }
@@ -1297,11 +1305,11 @@ __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const
     }
 }
 
-// SB2: dense, per tile.  dL/dtheta at every pixel (the edgel's own NMS term
-// plus Hx = E cos(theta), Hy = E sin(theta)), dL/dOxx and dL/dOyy through
-// theta = atan(Oyy sgn / (Oxx + 1e-5)), then the transposed central
-// differences (one-sided at the border) and the transposed triangle smoothing
-// (half-sample-symmetric border) into dL/dE; plus dL/dE of the products.
+// SB2: dense, per tile.  From dL/dtheta at every pixel (the edgel's own NMS
+// term and, added by SB1, that of Hx = E cos(theta), Hy = E sin(theta)),
+// dL/dOxx and dL/dOyy through theta = atan(Oyy sgn / (Oxx + 1e-5)), then the
+// transposed central differences (one-sided at the border) and the transposed
+// triangle smoothing (half-sample-symmetric border) into dL/dE.
 constexpr int DR = TH + 12, DC = TW + 12;    // tile +- 6
 
 __device__ __forceinline__ float d_transpose(const float *b, int stride, int j, int n, int jo) {
@@ -1356,7 +1364,7 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
     const int ry0 = y0 - 6, rx0 = x0 - 6;
     {
         // Every input of this tile is zero unless a kept edgel lies within
-        // 6 + HALO + R pixels of it: dL/dHx, dL/dHy reach HALO + R from an
+        // 6 + HALO + R pixels of it: dL/dtheta reaches HALO + R from an
         // edgel's pixel (cubic stencil, transposed filters), dL/dtheta of the
         // NMS sits at the edgel, and the transposed differences and smoothing
         // read 6 more.  Such interior tiles leave dL/dE unchanged; tiles near
@@ -1380,8 +1388,7 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
         // all of a thread's region loads are issued before any is used
         constexpr int NU = (DR * DC + NT - 1) / NT;
         float4 qv[NU];
-        float ev[NU], hxv[NU], hyv[NU], gtv[NU];
-        uint32_t kw[NU];
+        float gtv[NU];
         bool in[NU];
 #pragma unroll
         for (int u = 0; u < NU; ++u) {
@@ -1390,16 +1397,11 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
             const int gy = ry0 + r, gx = rx0 + c;
             in[u] = idx < DR * DC && gy >= 0 && gy < H && gx >= 0 && gx < W;
             qv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-            ev[u] = hxv[u] = hyv[u] = gtv[u] = 0.f;
-            kw[u] = 0u;
+            gtv[u] = 0.f;
             if (in[u]) {
                 const long long o = (long long)gy * W + gx;
                 qv[u] = J.tq[o];
-                ev[u] = J.map[(long long)gy * J.pitch + gx];
-                hxv[u] = J.ghx[o];
-                hyv[u] = J.ghy[o];
-                gtv[u] = J.gth[o];      // meaningful at kept edgels only (checked below)
-                kw[u] = J.kmask[(long long)gy * J.nw + (gx >> 5)];
+                gtv[u] = J.gth[o];
             }
         }
 #pragma unroll
@@ -1407,14 +1409,11 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
             const int idx = threadIdx.x + u * NT;
             if (idx >= DR * DC) continue;
             const int r = idx / DC, c = idx - r * DC;
-            const int gx = rx0 + c;
             float ox = 0.f, oy = 0.f;
             if (in[u]) {
                 const float4 q = qv[u];
-                float gt = ev[u] * (q.x * hyv[u] - q.y * hxv[u]);
-                if ((kw[u] >> (gx & 31)) & 1u) gt += gtv[u];
-                ox = gt * q.z;
-                oy = gt * q.w;
+                ox = gtv[u] * q.z;
+                oy = gtv[u] * q.w;
             }
             Ox[r][c] = ox;
             Oy[r][c] = oy;
@@ -1460,22 +1459,18 @@ __global__ __launch_bounds__(NT) void sp_bw_dense_kernel(const SpJob *jobs, cons
         const int li = p / TW, lj = p - li * TW;
         const int gi = y0 + li, gj = x0 + lj;
         if (gi >= H || gj >= W) continue;
-        const long long o = (long long)gi * W + gj;
-        const float4 q = J.tq[o];
-        float ge = tri_transpose(&Oy[li + 6][0], 1, gj, W, rx0);
-        ge += q.x * J.ghx[o] + q.y * J.ghy[o];
+        const float ge = tri_transpose(&Oy[li + 6][0], 1, gj, W, rx0);
         float *dst = &Gd.gmap[(long long)gi * Gd.gpitch + gj];
         *dst += ge;
     }
 }
 
-// zero dL/dHx, dL/dHy of every map (a kernel: the sequence stays capturable)
+// zero dL/dtheta of every map (a kernel: the sequence stays capturable)
 __global__ void sp_bw_clear_kernel(const SpJob *jobs) {
     const SpJob J = load_job(jobs);
     const long long n = (long long)J.H * J.W;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        J.ghx[i] = 0.f;
-        J.ghy[i] = 0.f;
+        J.gth[i] = 0.f;
     }
 }
 
@@ -1802,7 +1797,7 @@ int toed_subpix_set_maps(toed_subpix *p, const float *const *d_maps) {
 int toed_subpix_set_grad(toed_subpix *p, int on) {
     if (!p || (on && p->slog2)) return TOED_ERR_ARG;     // the adjoint covers interp_n = 0
     SPCHK(hipSetDevice(p->device));
-    // per map: tq, dirs (H x W x 16 B), ghx, ghy, gth (H x W x 4 B), the edgel
+    // per map: tq, dirs (H x W x 16 B), gth (H x W x 4 B), the edgel
     // count and the edgel records (min(capacity, H W) x (128 + 64) B: an edgel
     // per pixel at most)
     auto rec_rows = [](const SpJob &J) { return (size_t)std::min<long long>(J.capacity, (long long)J.H * J.W); };
@@ -1814,7 +1809,7 @@ int toed_subpix_set_grad(toed_subpix *p, int on) {
             if (J.H < 5 || J.W < 5) return TOED_ERR_ARG;   // single reflection of the smoothing
             const size_t px = (size_t)J.H * J.W;
             p->goff[m] = total;
-            total += align_up(px * 16) * 2 + align_up(px * 4) * 3 + align_up(4) + align_up(rec_rows(J) * 128) +
+            total += align_up(px * 16) * 2 + align_up(px * 4) + align_up(4) + align_up(rec_rows(J) * 128) +
                      align_up(rec_rows(J) * 64);
         }
         if (hipMalloc(&p->d_gwork, total) != hipSuccess) return TOED_ERR_ALLOC;
@@ -1824,14 +1819,12 @@ int toed_subpix_set_grad(toed_subpix *p, int on) {
     for (int m = 0; m < p->n; ++m) {
         SpJob &J = p->jobs[m];
         J.tq = nullptr; J.dirs = nullptr; J.rec = nullptr; J.brec = nullptr; J.gcount = nullptr;
-        J.ghx = J.ghy = J.gth = nullptr;
+        J.gth = nullptr;
         if (!on) continue;
         const size_t px = (size_t)J.H * J.W;
         char *b = p->d_gwork + p->goff[m];
         J.tq = (float4 *)b; b += align_up(px * 16);
         J.dirs = (double2 *)b; b += align_up(px * 16);
-        J.ghx = (float *)b; b += align_up(px * 4);
-        J.ghy = (float *)b; b += align_up(px * 4);
         J.gth = (float *)b; b += align_up(px * 4);
         J.gcount = (int *)b; b += align_up(4);
         J.rec = (float4 *)b; b += align_up(rec_rows(J) * 128);
