@@ -83,7 +83,8 @@ struct SpJob {
     int nw, ntx, nty, tile_begin;
     int chunk_begin, nchunks, tpitch, pad;
     uint32_t *wpre;             // H x nw: rank of each kept word's first edgel
-    int *tlist;                 // tiles with kept edgels: their number, then their indices (any order)
+    uint32_t *tbits;            // one bit per tile (raster order): the tile has kept edgels
+    int *tlist;                 // those tiles as a list: their number, then their indices
     // differentiable form (null unless enabled on the plan)
     float4 *tq;                 // H x W: dirx, diry, dOxx/dtheta, dOyy/dtheta coefficients
     double2 *dirs;              // H x W: direction of survivors
@@ -571,7 +572,6 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
             if (sv[u]) posw[threadIdx.x + u * NT] = pv[u];
     }
     __syncthreads();
-    bool any_kept = false;
 #pragma unroll 1
     for (int k = 0; k < 2; ++k) {
         const int p = threadIdx.x + k * NT;
@@ -603,15 +603,16 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
                 }
             }
         }
-        any_kept = any_kept || kp;
         const unsigned long long b = __ballot(kp);
         if ((lane & 31) == 0) {
             const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
-            if (row < H) J.kmask[(long long)row * J.nw + tx] = (uint32_t)(b >> lane);
+            const uint32_t w = (uint32_t)(b >> lane);
+            if (row < H) J.kmask[(long long)row * J.nw + tx] = w;
+            // the tile's bit in the set of tiles with kept edgels (no return
+            // value awaited; the scan turns the set into a list)
+            if (w) atomicOr(&J.tbits[lt >> 5], 1u << (lt & 31));
         }
     }
-    // a tile with kept edgels joins the list the correction and backward walk
-    if (__syncthreads_or(any_kept) && threadIdx.x == 0) J.tlist[1 + atomicAdd(J.tlist, 1)] = lt;
 }
 
 // B2: for tiles with kept edgels: the eight planes of the tile + HALO in LDS
@@ -783,17 +784,23 @@ __device__ __forceinline__ void correct_tile(const SpJob &J, const SpTaps &tp, i
     }
 }
 
-// A grid of (workgroups, maps) walks each map's list of tiles with kept edgels
-// (built by sp_isolate_kernel); the barrier between tiles keeps a tile's LDS
-// until every lane is done with it.
-template <int R>
-__global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
-    const SpJob J = load_job(jobs);
+// A grid of (workgroups, maps) walks each map's list of tiles with kept
+// edgels (sp_scan_maps builds it): entries blockIdx.x, blockIdx.x +
+// gridDim.x, ..., so every workgroup gets the same number of tiles to within
+// one; f(tile) for each, with a barrier between tiles so that a tile's LDS is
+// kept until every lane is done with it.
+template <class F> __device__ __forceinline__ void walk_kept_tiles(const SpJob &J, F f) {
     const int nl = *J.tlist;
     for (int k = blockIdx.x; k < nl; k += gridDim.x) {
         if (k != (int)blockIdx.x) __syncthreads();
-        correct_tile<R>(J, tp, J.tlist[1 + k]);
+        f(J.tlist[1 + k]);
     }
+}
+
+template <int R>
+__global__ __launch_bounds__(NT, 3) void sp_correct_kernel(const SpJob *jobs, int njobs, SpTaps tp) {
+    const SpJob J = load_job(jobs);
+    walk_kept_tiles(J, [&](int t) { correct_tile<R>(J, tp, t); });
 }
 
 // ---------------------------------------------------------------------------
@@ -818,9 +825,9 @@ __global__ __launch_bounds__(256) void sp_scan_chunks(const SpJob *jobs, int njo
 
 __global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chunk_sum) {
     __shared__ int wsum[16];
-    __shared__ int carry;
+    __shared__ int carry, tcarry;
     const SpJob &J = jobs[blockIdx.x];
-    if (threadIdx.x == 0) carry = 0;
+    if (threadIdx.x == 0) carry = tcarry = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int base = 0; base < J.nchunks; base += 1024) {
@@ -846,6 +853,33 @@ __global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chu
         if (J.gcount) *J.gcount = carry;
         if (carry > J.capacity && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_CAPACITY);
     }
+    // the set of tiles with kept edgels (bits set by the isolation kernel) as
+    // a list in raster order, for the kernels that walk it
+    const int nwords = (J.ntx * J.nty + 31) / 32;
+    for (int base = 0; base < nwords; base += 1024) {
+        const int k = base + threadIdx.x;
+        uint32_t bits = (k < nwords) ? J.tbits[k] : 0u;
+        const int x = __popc(bits);
+        int inc = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(inc, d);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int wb = tcarry;
+        for (int w = 0; w < wave; ++w) wb += wsum[w];
+        int pos = wb + inc - x;
+        while (bits) {
+            J.tlist[1 + pos++] = 32 * k + __ffs(bits) - 1;
+            bits &= bits - 1u;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) tcarry = wb + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) J.tlist[0] = tcarry;
 }
 
 // D: rank of the first kept edgel of every mask word (raster order), from
@@ -1293,16 +1327,12 @@ __device__ __forceinline__ void bw_tile(const SpJob &J, const SpGrad &Gd, const 
     }
 }
 
-// Walks each map's list of tiles with kept edgels, as the correction does.
+// Walks each map's set of tiles with kept edgels, as the correction does.
 template <int R>
 __global__ __launch_bounds__(NT) void sp_bw_tile_kernel(const SpJob *jobs, const SpGrad *grads, SpTaps tp) {
     const SpJob J = load_job(jobs);
     const SpGrad Gd = load_grad(grads);
-    const int nl = *J.tlist;
-    for (int k = blockIdx.x; k < nl; k += gridDim.x) {
-        if (k != (int)blockIdx.x) __syncthreads();
-        bw_tile<R>(J, Gd, tp, J.tlist[1 + k]);
-    }
+    walk_kept_tiles(J, [&](int t) { bw_tile<R>(J, Gd, tp, t); });
 }
 
 // SB2: dense, per tile.  From dL/dtheta at every pixel (the edgel's own NMS
@@ -1474,14 +1504,17 @@ __global__ void sp_bw_clear_kernel(const SpJob *jobs) {
     }
 }
 
-// zero per-map counters and flags (a kernel, so the sequence can be captured)
-__global__ void sp_reset_kernel(const SpJob *jobs, int njobs) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= njobs) return;
-    *jobs[m].nsurv = 0;
-    *jobs[m].tlist = 0;
-    if (jobs[m].flags_out) *jobs[m].flags_out = 0;
-    if (jobs[m].count_out) *jobs[m].count_out = 0;
+// zero per-map counters, flags and kept-tile bits (a kernel, so the sequence
+// can be captured); grid (bit words / 256, maps)
+__global__ void sp_reset_kernel(const SpJob *jobs) {
+    const SpJob J = load_job(jobs);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *J.nsurv = 0;
+        if (J.flags_out) *J.flags_out = 0;
+        if (J.count_out) *J.count_out = 0;
+    }
+    const int nwords = (J.ntx * J.nty + 31) / 32;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x) J.tbits[i] = 0u;
 }
 
 // R: interp_n > 0 only: the caller's IH x IW map (and given orientation) to
@@ -1663,7 +1696,7 @@ int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, doub
         off[m] = total;
         const size_t ntile = nw * ((wh + TH - 1) / TH);
         total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(wh * nw * 4) * 3 +
-                 align_up((ntile + 1) * 4);
+                 align_up((ntile + 31) / 32 * 4) + align_up((ntile + 1) * 4);
         if (interp_n) total += align_up(px * 4) * (maps[m].d_orient ? 2 : 1);
     }
     if (hipSetDevice(p->device) != hipSuccess) { delete p; return TOED_ERR_HIP; }
@@ -1686,6 +1719,7 @@ int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, doub
         J.vmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
         J.kmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
         J.wpre = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
+        J.tbits = (uint32_t *)b; b += align_up(((size_t)J.nw * ((wh + TH - 1) / TH) + 31) / 32 * 4);
         J.tlist = (int *)b; b += align_up(((size_t)J.nw * ((wh + TH - 1) / TH) + 1) * 4);
         J.nsurv = p->d_nsurv + 64 * m;
         J.src_map = M.d_map;
@@ -1755,7 +1789,8 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     if (changed || p->dirty)
         SPCHK(hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(SpJob) * p->n, hipMemcpyHostToDevice, s));
     p->dirty = false;
-    hipLaunchKernelGGL(sp_reset_kernel, dim3((p->n + 255) / 256), dim3(256), 0, s, p->d_jobs, p->n);
+    hipLaunchKernelGGL(sp_reset_kernel, dim3((p->max_tiles + 32 * 256 - 1) / (32 * 256), p->n), dim3(256), 0, s,
+                       p->d_jobs);
     if (p->timing) SPCHK(hipEventRecord(p->ev[0], s));
     if (p->slog2) hipLaunchKernelGGL(sp_resize_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
     hipLaunchKernelGGL(sp_orient_nms_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
