@@ -83,7 +83,7 @@ struct SpJob {
     int nw, ntx, nty, tile_begin;
     int chunk_begin, nchunks, tpitch, pad;
     uint32_t *wpre;             // H x nw: rank of each kept word's first edgel
-    uint32_t *tbits;            // one bit per tile (raster order): the tile has kept edgels
+    uint8_t *tflag;             // one byte per tile (raster order): nonzero when the tile has kept edgels
     int *tlist;                 // those tiles as a list: their number, then their indices
     // differentiable form (null unless enabled on the plan)
     float4 *tq;                 // H x W: dirx, diry, dOxx/dtheta, dOyy/dtheta coefficients
@@ -608,9 +608,9 @@ __global__ __launch_bounds__(NT) void sp_isolate_kernel(const SpJob *jobs, SpTap
             const int row = y0 + 2 * wave + 8 * k + (lane >> 5);
             const uint32_t w = (uint32_t)(b >> lane);
             if (row < H) J.kmask[(long long)row * J.nw + tx] = w;
-            // the tile's bit in the set of tiles with kept edgels (no return
-            // value awaited; the scan turns the set into a list)
-            if (w) atomicOr(&J.tbits[lt >> 5], 1u << (lt & 31));
+            // the tile's flag (plain byte stores of one value: no atomics;
+            // the scan turns the flags into a list)
+            if (w) J.tflag[lt] = 1;
         }
     }
 }
@@ -853,12 +853,18 @@ __global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chu
         if (J.gcount) *J.gcount = carry;
         if (carry > J.capacity && J.flags_out) atomicOr(J.flags_out, (int)TOED_FLAG_CAPACITY);
     }
-    // the set of tiles with kept edgels (bits set by the isolation kernel) as
-    // a list in raster order, for the kernels that walk it
-    const int nwords = (J.ntx * J.nty + 31) / 32;
+    // the tiles with kept edgels (flags set by the isolation kernel) as a
+    // list in raster order, for the kernels that walk it; four flags a thread
+    const int ntile = J.ntx * J.nty, nwords = (ntile + 3) / 4;
     for (int base = 0; base < nwords; base += 1024) {
         const int k = base + threadIdx.x;
-        uint32_t bits = (k < nwords) ? J.tbits[k] : 0u;
+        uint32_t bits = 0u;
+        if (k < nwords) {
+            const uint32_t v = reinterpret_cast<const uint32_t *>(J.tflag)[k];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (((v >> (8 * b)) & 0xffu) && 4 * k + b < ntile) bits |= 1u << b;
+        }
         const int x = __popc(bits);
         int inc = x;
 #pragma unroll
@@ -872,7 +878,7 @@ __global__ __launch_bounds__(1024) void sp_scan_maps(const SpJob *jobs, int *chu
         for (int w = 0; w < wave; ++w) wb += wsum[w];
         int pos = wb + inc - x;
         while (bits) {
-            J.tlist[1 + pos++] = 32 * k + __ffs(bits) - 1;
+            J.tlist[1 + pos++] = 4 * k + __ffs(bits) - 1;
             bits &= bits - 1u;
         }
         __syncthreads();
@@ -1513,8 +1519,9 @@ __global__ void sp_reset_kernel(const SpJob *jobs) {
         if (J.flags_out) *J.flags_out = 0;
         if (J.count_out) *J.count_out = 0;
     }
-    const int nwords = (J.ntx * J.nty + 31) / 32;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x) J.tbits[i] = 0u;
+    const int nwords = (J.ntx * J.nty + 3) / 4;      // the flag bytes, four at a time
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += gridDim.x * blockDim.x)
+        reinterpret_cast<uint32_t *>(J.tflag)[i] = 0u;
 }
 
 // R: interp_n > 0 only: the caller's IH x IW map (and given orientation) to
@@ -1696,7 +1703,7 @@ int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, doub
         off[m] = total;
         const size_t ntile = nw * ((wh + TH - 1) / TH);
         total += align_up(px * 4) * 2 + align_up(px * 16) + align_up(px * 8) + align_up(wh * nw * 4) * 3 +
-                 align_up((ntile + 31) / 32 * 4) + align_up((ntile + 1) * 4);
+                 align_up(ntile) + align_up((ntile + 1) * 4);
         if (interp_n) total += align_up(px * 4) * (maps[m].d_orient ? 2 : 1);
     }
     if (hipSetDevice(p->device) != hipSuccess) { delete p; return TOED_ERR_HIP; }
@@ -1719,7 +1726,7 @@ int toed_subpix_plan_n(toed_ctx *ctx, int n, const toed_subpix_map_t *maps, doub
         J.vmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
         J.kmask = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
         J.wpre = (uint32_t *)b; b += align_up((size_t)wh * J.nw * 4);
-        J.tbits = (uint32_t *)b; b += align_up(((size_t)J.nw * ((wh + TH - 1) / TH) + 31) / 32 * 4);
+        J.tflag = (uint8_t *)b; b += align_up((size_t)J.nw * ((wh + TH - 1) / TH));
         J.tlist = (int *)b; b += align_up(((size_t)J.nw * ((wh + TH - 1) / TH) + 1) * 4);
         J.nsurv = p->d_nsurv + 64 * m;
         J.src_map = M.d_map;
@@ -1789,7 +1796,7 @@ int toed_subpix_run(toed_subpix *p, int *d_counts, int *d_flags) {
     if (changed || p->dirty)
         SPCHK(hipMemcpyAsync(p->d_jobs, p->jobs.data(), sizeof(SpJob) * p->n, hipMemcpyHostToDevice, s));
     p->dirty = false;
-    hipLaunchKernelGGL(sp_reset_kernel, dim3((p->max_tiles + 32 * 256 - 1) / (32 * 256), p->n), dim3(256), 0, s,
+    hipLaunchKernelGGL(sp_reset_kernel, dim3((p->max_tiles + 4 * 256 - 1) / (4 * 256), p->n), dim3(256), 0, s,
                        p->d_jobs);
     if (p->timing) SPCHK(hipEventRecord(p->ev[0], s));
     if (p->slog2) hipLaunchKernelGGL(sp_resize_kernel, dim3(p->max_tiles, p->n), dim3(NT), 0, s, p->d_jobs, p->n);
