@@ -77,6 +77,13 @@ def _make_function():
             bd.run()
             plan.gen += 1
             ctx.gen = plan.gen
+            # the backward's zeroed gradient buffers, enqueued while the forward's
+            # kernels run (their host work then costs nothing on the way to the
+            # backward launch); only when a backward can follow
+            ctx.gbuf = None
+            if torch.is_grad_enabled() and (images.requires_grad or (torch.is_tensor(sigma_t) and sigma_t.requires_grad)):
+                ctx.gbuf = (torch.zeros(images.shape, dtype=torch.float32, device=images.device),
+                            torch.zeros(1, dtype=torch.float64, device=images.device))
             counts, _ = bd.counts_flags()          # the one host read of the forward
             counts = counts.astype(np.int64)
             offsets = np.concatenate([[0], np.cumsum(counts)])
@@ -103,8 +110,12 @@ def _make_function():
             plan, counts, offsets = ctx.plan, ctx.counts, ctx.offsets
             B, H, W = images.shape
             g_out = g_out.contiguous().float()
-            gimg = torch.zeros((B, H, W), dtype=torch.float32, device=images.device)
-            gsig = torch.zeros(1, dtype=torch.float64, device=images.device)
+            if ctx.gbuf is not None:
+                gimg, gsig = ctx.gbuf
+                ctx.gbuf = None
+            else:
+                gimg = torch.zeros((B, H, W), dtype=torch.float32, device=images.device)
+                gsig = torch.zeros(1, dtype=torch.float64, device=images.device)
             if plan.gen == ctx.gen and plan.ptr == images.data_ptr():
                 # the plan's staging still holds this forward's edgels: tile-driven backward,
                 # per-image addresses computed from the two batch tensors (an image with

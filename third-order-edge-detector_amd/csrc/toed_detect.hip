@@ -508,8 +508,9 @@ __device__ __forceinline__ float to_angle(double tx, double ty) {
 }
 
 // Third-order orientation (cpu_toed.cpp:227-232) of staged edgels, in FP64,
-// run by the scatter kernel on the tile's image region, which the wavefront
-// has staged in LDS (rows and columns from 1 + R1 before the tile, zero outside
+// run by the scatter kernel on the tile's image region, which the workgroup
+// has staged in LDS as FP64 (the window sums then read no conversions; rows
+// and columns from 1 + R1 before the tile, zero outside
 // the image, cpu_toed.cpp:207-208; row pitch FLDI): 8 lanes per edgel, lane l
 // the window rows p = -R1 + l + 8k (the fewest idle lanes for 2R1+1 = 11, 15,
 // 19 rows), the row sums for the four x families over the columns (unrolled by
@@ -517,11 +518,11 @@ __device__ __forceinline__ float to_angle(double tx, double ty) {
 // (DPP).  Phase (0,0) uses the R0 window through taps whose end values are
 // zero.  tapd: [family][tap][f] with families 0 unshifted, 1 unshifted with
 // zero end taps, 2 shifted.  (ci, cj): the edgel's image pixel in the region.
-constexpr int FLDI = (TW + 2 + 2 * MAXR1) | 1;            // region row pitch (floats)
+constexpr int FLDI = (TW + 2 + 2 * MAXR1) | 1;            // region row pitch (elements)
 constexpr int FROWS = 16 + 2 + 2 * MAXR1;                 // region rows at most (16-row tiles)
 
 template <int R1, int GT>
-__device__ __forceinline__ double theta_group(const float *img, int ci, int cj, int a, int b, int l,
+__device__ __forceinline__ double theta_group(const double *img, int ci, int cj, int a, int b, int l,
                                               const double (*tapd)[MAXTAPS][4]) {
     constexpr int N = 2 * R1 + 1;
     constexpr int NK = (N + GT - 1) / GT;
@@ -532,7 +533,7 @@ __device__ __forceinline__ double theta_group(const float *img, int ci, int cj, 
     // row sums of the lane's rows (rows beyond the window read a valid row and
     // are dropped below), each column's taps read once for all of them
     double h[NK][4];
-    const float *rows[NK];
+    const double *rows[NK];
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
         const int p = min(-R1 + l + GT * k, R1);
@@ -545,7 +546,7 @@ __device__ __forceinline__ double theta_group(const float *img, int ci, int cj, 
         const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
-            const double v = (double)rows[k][-qq];
+            const double v = rows[k][-qq];
             h[k][0] = fma(v, x0, h[k][0]);
             h[k][1] = fma(v, x1, h[k][1]);
             h[k][2] = fma(v, x2, h[k][2]);
@@ -1165,7 +1166,7 @@ constexpr int FIN_GRID = 2048;
 __device__ __forceinline__ void finalize_tile(const ImageJob *jobs, int njobs, const ImageJob &job0, int TH,
                                               const int *offsets, const int *chunk_pref, StageBuf stg,
                                               const int *tile_cnt, int tile, double (*tapd)[MAXTAPS][4],
-                                              float *fimg, float4 *s_rec, long long *s_idx, int *s_rc) {
+                                              double *fimg, float4 *s_rec, long long *s_idx, int *s_rc) {
     const ImageJob J = load_job(jobs, njobs, job0, tile);
     const long long tile_local = tile - J.tile_begin;
     const int tr = (int)(tile_local / J.ntc), tc = (int)(tile_local % J.ntc);
@@ -1237,7 +1238,7 @@ __device__ __forceinline__ void finalize_tile(const ImageJob *jobs, int njobs, c
 #pragma unroll
             for (int u = 0; u < NPASS; ++u) {
                 const int rr = r0 + RPP * u, gi = gi0 + rr;
-                if (rr < nri) fimg[rr * FLDI + col] = (colok && gi >= 0 && gi < J.H) ? vals[u] : 0.f;
+                if (rr < nri) fimg[rr * FLDI + col] = (colok && gi >= 0 && gi < J.H) ? (double)vals[u] : 0.0;
             }
         }
     }
@@ -1290,7 +1291,7 @@ __global__ __launch_bounds__(FIN_NT) void toed_finalize_kernel(const ImageJob *j
                                                                StageBuf stg, const int *tile_cnt, const int *tlist,
                                                                int *ticket) {
     __shared__ double tapd[3][MAXTAPS][4];
-    __shared__ float fimg[FROWS * FLDI];
+    __shared__ double fimg[FROWS * FLDI];
     __shared__ float4 s_rec[FIN_NT];       // a chunk of staged edgels: record,
     __shared__ long long s_idx[FIN_NT];    // output row (-1: none),
     __shared__ int s_rc[FIN_NT];           // tile-local interp row | col << 8
