@@ -139,7 +139,8 @@ def bench_fwd_bwd(torch, pkg, scene, steps, warmup, B=64, H=512, W=512, sigma=1.
     and sigma.  Reports forward-only and forward+backward throughput."""
     imgs = np.stack([scene.shapes_scene(9000 + b, H, W) for b in range(B)]).astype(np.float32)
     x = torch.from_numpy(imgs).cuda().requires_grad_(True)
-    sig = torch.tensor(sigma, dtype=torch.float64, requires_grad=True)
+    # sigma is a parameter in device memory, as a training loop holds it
+    sig = torch.tensor(sigma, dtype=torch.float64, device="cuda", requires_grad=True)
     w = torch.tensor(scene.loss_weights(12345), dtype=torch.float32, device="cuda")
     det = pkg.Detector(0, staging_fraction=0.25)
     state = {}

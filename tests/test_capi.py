@@ -55,3 +55,19 @@ def test_subpix_radius_host_only(pkg):
     for bad in [0.0, -1.0, 3.1, float("nan")]:
         with pytest.raises(pkg.ToedError):
             pkg.subpix_radius(bad)
+
+
+def test_autograd_wrapper_host_logic():
+    """The differentiable wrapper's host helpers (no GPU): sigma's host value
+    from a float or a host tensor (no autograd warning), and the plan cache
+    keyed by shape and parameters with at most two plans."""
+    import importlib
+    import warnings
+    torch = pytest.importorskip("torch")
+    ag = importlib.import_module("third-order-edge-detector_amd.autograd")
+    t = torch.tensor(1.5, dtype=torch.float64, requires_grad=True)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        assert ag._sigma_value(t) == 1.5
+    assert ag._sigma_value(2.0) == 2.0
+    assert callable(ag._plan_for) and ag._MAX_PLANS == 2

@@ -213,3 +213,28 @@ def test_in_place_edit_before_backward_is_rejected(pkg, detector, scene):
         y.add_(1.0)
     with pytest.raises(RuntimeError):
         e.sum().backward()
+
+
+def test_device_sigma_gradient_and_updates(pkg, detector, scene):
+    """sigma held in device memory: its gradient equals the host sigma's and
+    stays on the device, and an in-place update of it between calls is seen
+    by the next forward (the wrapper reads it anew when its version changes)."""
+    x = torch.from_numpy(np.stack([scene.shapes_scene(90, 96, 128)]).astype(np.float32)).cuda().requires_grad_(True)
+    w = torch.tensor([0.2, -0.6, 0.9, 0.4], device="cuda")
+    sig_h = torch.tensor(1.5, dtype=torch.float64, requires_grad=True)
+    e_h, _ = pkg.detect_differentiable(x, sig_h, detector=detector)
+    (e_h * w).sum().backward()
+    g_h, s_h = x.grad.clone(), float(sig_h.grad)
+    x.grad = None
+    sig_d = torch.tensor(1.5, dtype=torch.float64, device="cuda", requires_grad=True)
+    e_d, _ = pkg.detect_differentiable(x, sig_d, detector=detector)
+    (e_d * w).sum().backward()
+    assert sig_d.grad.is_cuda
+    assert torch.equal(e_d, e_h)
+    assert (x.grad - g_h).abs().max().item() <= 1e-5 * g_h.abs().max().item()
+    assert float(sig_d.grad) == pytest.approx(s_h, rel=1e-9)
+    with torch.no_grad():
+        sig_d.fill_(2.0)
+        e2, _ = pkg.detect_differentiable(x, sig_d, detector=detector)
+        e2_ref, _ = pkg.detect_differentiable(x, 2.0, detector=detector)
+    assert torch.equal(e2, e2_ref) and not torch.equal(e2[:5], e_d[:5])

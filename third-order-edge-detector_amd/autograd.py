@@ -3,7 +3,9 @@
     edges, offsets = detect_differentiable(images, sigma)
 
 ``images`` is a float32 [B, H, W] device tensor, ``sigma`` a float or a 0-dim
-tensor (which may require grad).  The forward is the batched FP32 detection
+tensor (which may require grad; on the images' device, its gradient stays
+there, with no synchronisation for it, and its value is read on the host once
+per in-place update).  The forward is the batched FP32 detection
 (one plan over the B images); ``edges`` is the concatenation of the B
 raster-ordered N_b x 4 lists and ``offsets`` (int64, B+1, on the host) marks
 where each image's rows start.  The backward holds the edgel set fixed (each
@@ -51,6 +53,24 @@ class _Plan:
 _PLANS = collections.OrderedDict()
 _MAX_PLANS = 2
 
+# sigma held in device memory: its value as last read on the host is kept on
+# the tensor object itself, with the storage address and version counter it
+# was read at.  An unchanged sigma is then not read again, so a forward does
+# not wait for the device's queue (e.g. the previous step's backward) to drain;
+# any in-place update bumps the version and is read anew.  (A dictionary keyed
+# by tensors would compare them with ==, an elementwise device operation.)
+def _sigma_value(sigma_t):
+    torch = _torch()
+    if not torch.is_tensor(sigma_t) or sigma_t.device.type == "cpu":
+        return float(sigma_t.detach()) if torch.is_tensor(sigma_t) else float(sigma_t)
+    key = (sigma_t.data_ptr(), sigma_t._version)
+    seen = getattr(sigma_t, "_toed_sigma_seen", None)
+    if seen is not None and seen[0] == key:
+        return seen[1]
+    value = float(sigma_t.detach())
+    sigma_t._toed_sigma_seen = (key, value)
+    return value
+
 
 def _plan_for(det, images, sigma, thresh, border, cap):
     key = (id(det), str(images.device), tuple(images.shape), tuple(images.stride()), float(sigma), float(thresh),
@@ -69,8 +89,7 @@ def _make_function():
 
     class _Detect(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, images, sigma_t, det, thresh, border, cap):
-            sigma = float(sigma_t)
+        def forward(ctx, images, sigma_t, det, thresh, border, cap, sigma):
             plan = _plan_for(det, images, sigma, thresh, border, cap)
             plan.point_at(images)
             bd = plan.bd
@@ -129,7 +148,7 @@ def _make_function():
                 gs = None
                 if ctx.sigma_shape is not None:
                     gs = gsig.reshape(()).to(ctx.sigma_dtype).reshape(ctx.sigma_shape)
-                return gimg, gs, None, None, None, None
+                return gimg, gs, None, None, None, None, None
             arr = (_lib.toed_grad_t * B)()
             for b in range(B):
                 it = arr[b]
@@ -149,7 +168,7 @@ def _make_function():
             gs = None
             if ctx.sigma_shape is not None:
                 gs = gsig.reshape(()).to(ctx.sigma_dtype).reshape(ctx.sigma_shape)
-            return gimg, gs, None, None, None, None
+            return gimg, gs, None, None, None, None, None
 
     return _Detect
 
@@ -173,7 +192,9 @@ def detect_differentiable(images, sigma, detector=None, thresh=2.0, border=10, c
     B, H, W = images.shape
     cap = int(capacity) if capacity else max(1024, H * W)
     sig = sigma if torch.is_tensor(sigma) else torch.tensor(float(sigma), dtype=torch.float64)
-    return _FN.apply(images, sig, det, float(thresh), int(border), cap)
+    # sigma's value on the host (read from the user's own tensor object, where
+    # a device sigma's last read is remembered)
+    return _FN.apply(images, sig, det, float(thresh), int(border), cap, _sigma_value(sig))
 
 
 _DET = {}
