@@ -54,7 +54,7 @@ constexpr int TW = 32;              // tile width in original pixels -> 64 inter
 constexpr int NT = 256;             // threads per tile workgroup (4 wavefronts)
 constexpr int MAXR1 = 9;            // largest supported R1 (sigma <= 2)
 constexpr int MAXTAPS = 2 * MAXR1 + 1;
-constexpr int CHUNK = 4096;         // scan chunk (entries)
+constexpr int CHUNK = 1024;         // scan chunk (entries)
 constexpr int NGROUPS = 3;          // supported window radii R1 = 5, 7, 9 (sigma 1, 1.5, 2)
 constexpr int BW_CHUNK = 64;        // edgels per pre-pass chunk of the tile backward
 constexpr int BW_G = 8;             // lanes per edgel in the tile backward
