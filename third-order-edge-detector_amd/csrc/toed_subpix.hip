@@ -56,7 +56,7 @@ constexpr int NT = 256;         // threads per workgroup
 constexpr int MAXR = 12;        // largest filter radius ceil(4 sigma)
 constexpr int MAXK = 8;         // isolation reach in work pixels of the default isolation kernel
 constexpr int MAXK_WIDE = 12;   // ... and of the wide one (radius 2^n r on an interp_n = n work grid)
-constexpr int CHUNKW = 4096;    // mask words per scan chunk
+constexpr int CHUNKW = 1024;    // mask words per scan chunk
 constexpr int HALO = 3;         // cubic stencil reach around an edgel pixel
 
 struct SpTaps {
