@@ -22,7 +22,7 @@
 //        survivors within floor(r + 2 sqrt 2) pixels (edgels lie within
 //        sqrt 2 of their pixel) -> kept mask words.
 //   C  sp_scan_chunks / sp_scan_maps / sp_prefix_kernel   exclusive scan of
-//        the kept-mask popcounts in raster order (chunks of 4096 words, then
+//        the kept-mask popcounts in raster order (chunks of 1024 words, then
 //        per map): the count, and the rank of each word's first kept edgel.
 //   B2 sp_correct_kernel<R>   one workgroup per tile with kept edgels: the
 //        eight circular separable Gaussian-derivative convolutions
