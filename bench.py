@@ -13,6 +13,7 @@ the frame's edgel list, and the reference CPU paths timed on this host.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload all|plain|batch|fwdbwd|subpix]
 """
 import argparse
+import gc
 import importlib
 import json
 import os
@@ -33,7 +34,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the plain-frame tile kernel (2160x3840 u8, sigma 2), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, FETCH_SIZE scaled by
 # 2.0 (what the counter reads for a 256 MiB copy); the summary is kept under profiles/.
-TRAFFIC_TILES_PLAIN_BYTES = 2.0 * 24216.8 * 1024 + 19422.8 * 1024
+TRAFFIC_TILES_PLAIN_BYTES = 2.0 * 24211.4 * 1024 + 19422.5 * 1024
 BATCH_SIZES = [(480, 640), (480, 752), (720, 1280), (1080, 1920), (2160, 3840)]
 BATCH_SIGMAS = [1.0, 1.5, 2.0]
 
@@ -51,14 +52,23 @@ def dense_flops_per_px(sigma, pkg):
 
 
 def time_loop(torch, fn, steps, warmup):
+    # Python's cyclic garbage collector is paused while a loop is timed (as
+    # timeit does): a collection pass over the objects this process holds is
+    # host noise unrelated to the work timed
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        fn()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+    finally:
+        if was:
+            gc.enable()
 
 
 def bench_plain(torch, pkg, scene, steps, warmup):
