@@ -185,7 +185,7 @@ __device__ __forceinline__ bool edgel_adjoint(int R, int C, float4 up, int l,
         for (int mm = 0; mm < 2; ++mm)
 #pragma unroll
             for (int rr = 0; rr < NSR; ++rr) { h0[mm][rr] = 0; h1[mm][rr] = 0; hs0[mm][rr] = 0; hs1[mm][rr] = 0; }
-#pragma unroll 2
+#pragma unroll 4
         for (int c = 0; c < P; ++c) {
             double px[NSR];
 #pragma unroll
