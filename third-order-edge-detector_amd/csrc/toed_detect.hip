@@ -513,8 +513,9 @@ __device__ __forceinline__ float to_angle(double tx, double ty) {
 // and columns from 1 + R1 before the tile, zero outside
 // the image, cpu_toed.cpp:207-208; row pitch FLDI): 8 lanes per edgel, lane l
 // the window rows p = -R1 + l + 8k (the fewest idle lanes for 2R1+1 = 11, 15,
-// 19 rows), the row sums for the four x families over the columns (unrolled by
-// 4), then the nine products with the row's y taps, summed across the 8 lanes
+// 19 rows), the row sums for the four x families over the columns (a column
+// per iteration, not unrolled: measured fastest), then the nine products with
+// the row's y taps, summed across the 8 lanes
 // (DPP).  Phase (0,0) uses the R0 window through taps whose end values are
 // zero.  tapd: [family][tap][f] with families 0 unshifted, 1 unshifted with
 // zero end taps, 2 shifted.  (ci, cj): the edgel's image pixel in the region.
@@ -540,7 +541,7 @@ __device__ __forceinline__ double theta_group(const double *img, int ci, int cj,
         rows[k] = img + (ci - p) * FLDI + cj;
         h[k][0] = h[k][1] = h[k][2] = h[k][3] = 0.0;
     }
-#pragma unroll 2
+#pragma unroll 1
     for (int qq = -R1; qq <= R1; ++qq) {
         const double *x = X[qq + R1];
         const double x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
