@@ -34,7 +34,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the plain-frame tile kernel (2160x3840 u8, sigma 2), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this workload, FETCH_SIZE scaled by
 # 2.0 (what the counter reads for a 256 MiB copy); the summary is kept under profiles/.
-TRAFFIC_TILES_PLAIN_BYTES = 2.0 * 24211.4 * 1024 + 19422.5 * 1024
+TRAFFIC_TILES_PLAIN_BYTES = 2.0 * 24211.2 * 1024 + 19422.8 * 1024
 BATCH_SIZES = [(480, 640), (480, 752), (720, 1280), (1080, 1920), (2160, 3840)]
 BATCH_SIGMAS = [1.0, 1.5, 2.0]
 
