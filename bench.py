@@ -330,7 +330,7 @@ def accuracy(gpu, ref):
 def host_cpu():
     """The host's CPU as lscpu reports it, and the threads this process may use:
     the CPUs in its affinity mask, limited by OMP_NUM_THREADS where the job
-    sets one (a GPU box allots each GPU a share of the host's cores)."""
+    sets one (a shared host may allot each GPU a share of its cores)."""
     info = {}
     try:
         out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
@@ -417,7 +417,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each image is independent; "
-              "see DESIGN.md for how the work would shard). Run with --gpus 1.", file=sys.stderr)
+              "it would shard by image, one process per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import torch
     pkg = importlib.import_module(PKG_NAME)

@@ -2,7 +2,7 @@
  * Deterministic synthetic inputs for TOED tests and benchmarks.
  *
  * Two generators, both driven by a SplitMix64 integer RNG so that any host
- * (this container or a GPU box) produces bit-identical images from a seed:
+ * produces bit-identical images from a seed:
  *
  *   toed_scene_u8      grey-level scene: uniform background ~U[40,200], then
  *                      24*k filled shapes painted in order (even index: disk,
